@@ -1,0 +1,25 @@
+"""gridoxide_amd: batched Newton-Raphson AC power flow on MI355X (HIP, gfx950).
+
+The numeric path lives in ``libgridox_hip.so`` (C ABI: ``include/gridox_hip.h``); this package
+holds the workload generator and the host mirror of the reference's solver interface.
+"""
+from .grid import PQ, PV, SLACK, Network, config_grid, generate_grid, scenario_specs  # noqa: F401
+from .solver import (  # noqa: F401
+    BatchError,
+    BatchSolver,
+    BusOutOfRange,
+    BusOverride,
+    BusType,
+    JacobianBackend,
+    Mode,
+    OutagesUnsupported,
+    PersistentSolver,
+    PowerFlowReport,
+    Scenario,
+    SolveStats,
+    SolveStatus,
+    Topology,
+    YBus,
+    build_ybus,
+    uniform_load_scaling,
+)

@@ -1,0 +1,96 @@
+// Device-resident topology and per-batch workspace shared by the host driver and the HIP kernels.
+//
+// Layout: every per-scenario array is "scenario-minor interleaved" in tiles of 64 scenarios,
+// element e of scenario s lives at   base[(tile * count + e) * 64 + lane],   tile = s / 64,
+// lane = s % 64, so one coalesced wavefront access touches the same element of 64 scenarios.
+// The topology arrays (Y-bus, Jacobian recipes, LU pattern and update schedule) are shared
+// read-only by every scenario.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace gx {
+
+constexpr int kLanes = 64;
+
+enum Status : int { kConverged = 0, kMaxIter = 1, kSingular = 2, kRunning = 3, kPad = 4 };
+
+// Jacobian entry kinds, in the reference's numbering (src/jacobian.rs:46-57); kFill marks an
+// L+U position that is not a Jacobian entry (structural fill, value 0 before elimination).
+enum Kind : int { kHii = 0, kNii = 1, kHik = 2, kNik = 3, kMii = 4, kLii = 5, kMik = 6, kLik = 7, kFill = 8 };
+
+struct DevTopo {
+  int n_bus = 0;   // buses
+  int n = 0;       // unknowns (rows/cols of the Jacobian)
+  int nnz_lu = 0;  // packed L+U entries (column k: U rows, diagonal, L rows)
+  int max_col = 0; // longest packed column
+  // Y-bus, CSR, rows sorted by column
+  const int* y_rowptr = nullptr;
+  const int* y_col = nullptr;
+  const double* y_g = nullptr;
+  const double* y_b = nullptr;
+  // per bus: unknown position in pivot-step space (-1 if none)
+  const int* rpos_p = nullptr;  // P-mismatch equation row -> pivot row
+  const int* rpos_q = nullptr;  // Q-mismatch equation row -> pivot row
+  const int* xpos_a = nullptr;  // angle unknown -> pivot column
+  const int* xpos_m = nullptr;  // magnitude unknown -> pivot column
+  const int* bus_type = nullptr;
+  // per packed L+U position: Jacobian recipe (kind, bus i, bus k, g, b)
+  const int* lu_kind = nullptr;
+  const int* lu_bi = nullptr;
+  const int* lu_bk = nullptr;
+  const double* lu_g = nullptr;
+  const double* lu_b = nullptr;
+  // LU pattern
+  const int* cp = nullptr;      // n+1
+  const int* ulen = nullptr;    // n
+  const int* lu_row = nullptr;  // nnz_lu, pivot-step row of each packed entry
+  // for every strictly-upper packed entry (j,k): offsets (relative to cp[k]) of the rows of
+  // L(:,j) inside column k, stored contiguously from upd_ptr[position]
+  const int* upd_ptr = nullptr;       // nnz_lu + 1 (only U positions own a range)
+  const uint16_t* upd_off = nullptr;  // total = sum over U entries of |L(:,j)|
+};
+
+// Per-batch device workspace (sized for n_tiles tiles).
+struct DevWork {
+  int n_scen = 0;
+  int n_tiles = 0;
+  int max_iter = 0;
+  double tol = 0.0;
+  // bus arrays, interleaved [tile][n_bus][64]
+  double* vm = nullptr;
+  double* va = nullptr;
+  double* vr = nullptr;
+  double* vi = nullptr;
+  double* ps = nullptr;
+  double* qs = nullptr;
+  double* pc = nullptr;
+  double* qc = nullptr;
+  // unknown arrays, interleaved [tile][n][64]
+  double* f = nullptr;    // FP64 mismatch in pivot-row order
+  double* dx = nullptr;   // FP64 Newton step in pivot-column order (mixed mode)
+  void* y = nullptr;      // LU-precision right-hand side / solution
+  void* lu = nullptr;     // packed L+U values, [tile][nnz_lu][64]
+  // per scenario
+  int* status = nullptr;
+  int* iters = nullptr;
+  unsigned long long* maxmis = nullptr;  // max |mismatch| as ordered bits of a non-negative double
+  double* history = nullptr;             // [n_scen][max_iter]
+};
+
+// Launchers (gx_kernels.hip).  All run on `stream`; none synchronizes.  `fp32` selects the
+// LU precision (float for the mixed-precision mode, double otherwise).
+void launch_to_tiles(const double* src, double* dst, int n_scen, int n_tiles, int count, hipStream_t stream);
+void launch_from_tiles(const double* src, double* dst, int n_scen, int count, hipStream_t stream);
+void launch_init(const DevTopo& t, const DevWork& w, hipStream_t stream);
+void launch_mismatch(const DevTopo& t, const DevWork& w, bool fp32, hipStream_t stream);
+void launch_decide(const DevTopo& t, const DevWork& w, int iter, hipStream_t stream);
+void launch_fill(const DevTopo& t, const DevWork& w, bool fp32, hipStream_t stream);
+void launch_lu(const DevTopo& t, const DevWork& w, bool fp32, bool factor, int iter, hipStream_t stream);
+void launch_refine(const DevTopo& t, const DevWork& w, double* dvr, double* dvi, hipStream_t stream);
+void launch_update(const DevTopo& t, const DevWork& w, bool fp32, bool mixed, hipStream_t stream);
+void launch_finish(const DevWork& w, hipStream_t stream);
+void launch_lane0(const double* src, double* dst, int count, bool to_tile, hipStream_t stream);
+
+}  // namespace gx

@@ -1,0 +1,784 @@
+// Host side of gridox_hip: Y-bus build, Jacobian pattern, symbolic phase, device upload and the
+// batched Newton-Raphson driver behind the C ABI in include/gridox_hip.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "gridox_hip.h"
+#include "gx_device.hpp"
+#include "gx_symbolic.hpp"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define GX_HIP(expr)                                                                 \
+  do {                                                                               \
+    hipError_t e_ = (expr);                                                          \
+    if (e_ != hipSuccess) return fail(GX_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+// Device allocation owned by a handle.
+struct DevMem {
+  void* p = nullptr;
+  size_t bytes = 0;
+  DevMem() = default;
+  DevMem(const DevMem&) = delete;
+  DevMem& operator=(const DevMem&) = delete;
+  ~DevMem() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  int reserve(size_t b) {
+    if (b <= bytes && p) return GX_OK;
+    release();
+    if (b == 0) b = 16;
+    hipError_t e = hipMalloc(&p, b);
+    if (e != hipSuccess) {
+      p = nullptr;
+      return fail(GX_ERR_DEVICE, std::string("hipMalloc: ") + hipGetErrorString(e));
+    }
+    bytes = b;
+    return GX_OK;
+  }
+  template <class T>
+  int upload(const std::vector<T>& v) {
+    int rc = reserve(sizeof(T) * v.size());
+    if (rc) return rc;
+    if (!v.empty()) GX_HIP(hipMemcpy(p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice));
+    return GX_OK;
+  }
+  template <class T>
+  T* as() const { return static_cast<T*>(p); }
+};
+
+struct cx {
+  double re, im;
+};
+inline cx cadd(cx a, cx b) { return {a.re + b.re, a.im + b.im}; }
+inline cx cmul(cx a, cx b) { return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re}; }
+inline cx cdiv(cx a, cx b) {
+  double ns = b.re * b.re + b.im * b.im;
+  return {(a.re * b.re + a.im * b.im) / ns, (a.im * b.re - a.re * b.im) / ns};
+}
+inline cx cneg(cx a) { return {-a.re, -a.im}; }
+inline cx cconj(cx a) { return {a.re, -a.im}; }
+
+// Packed L+U schedule derived from an LuPattern (shared by the topology and the plain system).
+struct LuSchedule {
+  std::vector<int> upd_ptr;
+  std::vector<uint16_t> upd_off;
+  int max_col = 0;
+  long long n_upd = 0;
+};
+
+int build_schedule(const gx::LuPattern& lu, LuSchedule& s) {
+  const int n = lu.n;
+  const int nnz = lu.nnz();
+  s.max_col = 0;
+  for (int k = 0; k < n; ++k) s.max_col = std::max(s.max_col, lu.cp[k + 1] - lu.cp[k]);
+  if (s.max_col >= 65536) return fail(GX_ERR_UNSUPPORTED, "an L+U column has 65536 or more entries");
+  s.upd_ptr.assign(nnz + 1, 0);
+  s.upd_off.clear();
+  std::vector<int> where(n, -1);
+  for (int k = 0; k < n; ++k) {
+    const int c0 = lu.cp[k], c1 = lu.cp[k + 1];
+    for (int p = c0; p < c1; ++p) where[lu.row[p]] = p - c0;
+    for (int p = c0; p < c1; ++p) {
+      s.upd_ptr[p] = (int)s.upd_off.size();
+      if (p - c0 < lu.ulen[k]) {
+        const int j = lu.row[p];
+        for (int q = lu.cp[j] + lu.ulen[j] + 1; q < lu.cp[j + 1]; ++q) {
+          int o = where[lu.row[q]];
+          if (o < 0) return fail(GX_ERR_ARG, "L+U pattern is not closed under elimination");
+          s.upd_off.push_back((uint16_t)o);
+        }
+      }
+    }
+    for (int p = c0; p < c1; ++p) where[lu.row[p]] = -1;
+  }
+  s.upd_ptr[nnz] = (int)s.upd_off.size();
+  if (s.upd_off.size() >= (size_t)0x7fffffff) return fail(GX_ERR_UNSUPPORTED, "update schedule too large");
+  s.n_upd = (long long)s.upd_off.size();
+  return GX_OK;
+}
+
+// Position of (pivot row r, pivot column k) inside the packed column k, or -1.
+int lu_position(const gx::LuPattern& lu, int r, int k) {
+  const int c0 = lu.cp[k], ul = lu.ulen[k];
+  if (r == k) return c0 + ul;
+  if (r < k) {
+    auto b = lu.row.begin() + c0, e = lu.row.begin() + c0 + ul;
+    auto it = std::lower_bound(b, e, r);
+    return (it != e && *it == r) ? (int)(it - lu.row.begin()) : -1;
+  }
+  auto b = lu.row.begin() + c0 + ul + 1, e = lu.row.begin() + lu.cp[k + 1];
+  auto it = std::lower_bound(b, e, r);
+  return (it != e && *it == r) ? (int)(it - lu.row.begin()) : -1;
+}
+
+// Triplets -> CSC with rows sorted per column; slot[e] = CSC position of triplet e
+// (build_csc_structure, src/klu_native/mod.rs:88-114).
+void triplets_to_csc(int n, const std::vector<int>& rows, const std::vector<int>& cols, gx::CscPattern& a,
+                     std::vector<int>& slot) {
+  const size_t nt = rows.size();
+  std::vector<int> ord(nt);
+  for (size_t i = 0; i < nt; ++i) ord[i] = (int)i;
+  std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) {
+    return cols[x] != cols[y] ? cols[x] < cols[y] : rows[x] < rows[y];
+  });
+  a.n = n;
+  a.ap.assign(n + 1, 0);
+  a.ai.clear();
+  slot.assign(nt, -1);
+  for (size_t t = 0; t < nt; ++t) {
+    int e = ord[t];
+    if (t > 0 && rows[e] == rows[ord[t - 1]] && cols[e] == cols[ord[t - 1]]) {
+      slot[e] = (int)a.ai.size() - 1;
+      continue;
+    }
+    a.ai.push_back(rows[e]);
+    a.ap[cols[e] + 1]++;
+    slot[e] = (int)a.ai.size() - 1;
+  }
+  for (int j = 0; j < n; ++j) a.ap[j + 1] += a.ap[j];
+}
+
+}  // namespace
+
+struct gx_topo {
+  int n_bus = 0;
+  std::vector<int> type;
+  std::vector<int> yrow, ycol;
+  std::vector<double> yg, yb;
+  std::vector<int> non_slack, pq, ns_pos, pq_pos;
+  int n_ang = 0, n = 0;
+  // Jacobian recipe in reference emission order
+  std::vector<int> jrow, jcol, jkind, jbi, jbk;
+  std::vector<double> jg, jb;
+  gx::CscPattern csc;
+  std::vector<int> slot;
+  gx::Ordering ord;
+  gx::LuPattern lu;
+  LuSchedule sched;
+  bool factored = false;
+  // host images of the device schedule (uploaded lazily by upload_topo)
+  std::vector<int> h_kind, h_bi, h_bk, h_rp, h_rq, h_xa, h_xm;
+  std::vector<double> h_g, h_b;
+  bool uploaded = false;
+  // device topology
+  DevMem d_yrow, d_ycol, d_yg, d_yb, d_type, d_rpos_p, d_rpos_q, d_xpos_a, d_xpos_m;
+  DevMem d_kind, d_bi, d_bk, d_g, d_b, d_cp, d_ulen, d_row, d_upd_ptr, d_upd_off;
+  gx::DevTopo dt;
+  // workspace
+  int ws_tiles = 0, ws_iter = 0;
+  int ws_fp32 = -1;
+  DevMem w_vm, w_va, w_vr, w_vi, w_ps, w_qs, w_pc, w_qc, w_f, w_dx, w_y, w_lu, w_status, w_iters, w_max, w_hist;
+  DevMem w_dvr, w_dvi, w_stage;
+};
+
+namespace {
+
+// Host FP64 injections for one scenario (used only to fix the pivot order).
+void host_injections(const gx_topo& t, const double* vm, const double* va, std::vector<double>& p,
+                     std::vector<double>& q) {
+  const int n = t.n_bus;
+  std::vector<cx> v(n);
+  for (int i = 0; i < n; ++i) v[i] = {vm[i] * std::cos(va[i]), vm[i] * std::sin(va[i])};
+  p.assign(n, 0.0);
+  q.assign(n, 0.0);
+  for (int i = 0; i < n; ++i) {
+    cx acc{0.0, 0.0};
+    for (int e = t.yrow[i]; e < t.yrow[i + 1]; ++e) acc = cadd(acc, cmul({t.yg[e], t.yb[e]}, v[t.ycol[e]]));
+    cx s = cmul(v[i], cconj(acc));
+    p[i] = s.re;
+    q[i] = s.im;
+  }
+}
+
+// Jacobian values in reference emission order at one operating point (src/jacobian.rs:210-250).
+void host_jacobian(const gx_topo& t, const double* vm, const double* va, const std::vector<double>& p,
+                   const std::vector<double>& q, std::vector<double>& vals) {
+  vals.resize(t.jkind.size());
+  for (size_t e = 0; e < t.jkind.size(); ++e) {
+    const int i = t.jbi[e], k = t.jbk[e];
+    const double g = t.jg[e], b = t.jb[e], vi = vm[i];
+    double v = 0.0;
+    switch (t.jkind[e]) {
+      case gx::kHii: v = -q[i] - vi * vi * b; break;
+      case gx::kNii: v = p[i] / vi + vi * g; break;
+      case gx::kMii: v = p[i] - vi * vi * g; break;
+      case gx::kLii: v = q[i] / vi - vi * b; break;
+      default: {
+        const double vk = vm[k], ang = va[i] - va[k], sn = std::sin(ang), cs = std::cos(ang);
+        switch (t.jkind[e]) {
+          case gx::kHik: v = vi * vk * (g * sn - b * cs); break;
+          case gx::kNik: v = vi * (g * cs + b * sn); break;
+          case gx::kMik: v = -vi * vk * (g * cs + b * sn); break;
+          default: v = vi * (g * sn - b * cs); break;
+        }
+      }
+    }
+    vals[e] = v;
+  }
+}
+
+// Derives the device schedule (host images) once the pivot order is known.
+int finalize_factor(gx_topo& t) {
+  int rc = build_schedule(t.lu, t.sched);
+  if (rc) return rc;
+  const gx::LuPattern& lu = t.lu;
+  const int nnz = lu.nnz();
+  std::vector<int> qinv(t.n);
+  for (int k = 0; k < t.n; ++k) qinv[lu.q[k]] = k;
+  std::vector<int> kind(nnz, gx::kFill), bi(nnz, 0), bk(nnz, 0);
+  std::vector<double> g(nnz, 0.0), b(nnz, 0.0);
+  for (size_t e = 0; e < t.jkind.size(); ++e) {
+    int pos = lu_position(lu, lu.pinv[t.jrow[e]], qinv[t.jcol[e]]);
+    if (pos < 0) return fail(GX_ERR_ARG, "Jacobian entry missing from the L+U pattern");
+    if (kind[pos] != gx::kFill) return fail(GX_ERR_UNSUPPORTED, "duplicate Jacobian entries (unmerged Y-bus)");
+    kind[pos] = t.jkind[e];
+    bi[pos] = t.jbi[e];
+    bk[pos] = t.jbk[e];
+    g[pos] = t.jg[e];
+    b[pos] = t.jb[e];
+  }
+  std::vector<int> rp(t.n_bus, -1), rq(t.n_bus, -1), xa(t.n_bus, -1), xm(t.n_bus, -1);
+  for (int i = 0; i < t.n_bus; ++i) {
+    if (t.ns_pos[i] >= 0) {
+      rp[i] = lu.pinv[t.ns_pos[i]];
+      xa[i] = qinv[t.ns_pos[i]];
+    }
+    if (t.pq_pos[i] >= 0) {
+      rq[i] = lu.pinv[t.n_ang + t.pq_pos[i]];
+      xm[i] = qinv[t.n_ang + t.pq_pos[i]];
+    }
+  }
+  t.h_kind.swap(kind);
+  t.h_bi.swap(bi);
+  t.h_bk.swap(bk);
+  t.h_g.swap(g);
+  t.h_b.swap(b);
+  t.h_rp.swap(rp);
+  t.h_rq.swap(rq);
+  t.h_xa.swap(xa);
+  t.h_xm.swap(xm);
+  t.factored = true;
+  t.uploaded = false;
+  t.ws_tiles = 0;  // workspace layout depends on nnz(L+U)
+  return GX_OK;
+}
+
+// Uploads the topology (Y-bus, recipes, LU pattern and schedule) once per factorization.
+int upload_topo(gx_topo& t) {
+  if (t.uploaded) return GX_OK;
+  int rc;
+  if ((rc = t.d_yrow.upload(t.yrow)) || (rc = t.d_ycol.upload(t.ycol)) || (rc = t.d_yg.upload(t.yg)) ||
+      (rc = t.d_yb.upload(t.yb)) || (rc = t.d_type.upload(t.type)) || (rc = t.d_rpos_p.upload(t.h_rp)) ||
+      (rc = t.d_rpos_q.upload(t.h_rq)) || (rc = t.d_xpos_a.upload(t.h_xa)) || (rc = t.d_xpos_m.upload(t.h_xm)) ||
+      (rc = t.d_kind.upload(t.h_kind)) || (rc = t.d_bi.upload(t.h_bi)) || (rc = t.d_bk.upload(t.h_bk)) ||
+      (rc = t.d_g.upload(t.h_g)) || (rc = t.d_b.upload(t.h_b)) || (rc = t.d_cp.upload(t.lu.cp)) ||
+      (rc = t.d_ulen.upload(t.lu.ulen)) || (rc = t.d_row.upload(t.lu.row)) ||
+      (rc = t.d_upd_ptr.upload(t.sched.upd_ptr)) || (rc = t.d_upd_off.upload(t.sched.upd_off)))
+    return rc;
+  gx::DevTopo& d = t.dt;
+  d.n_bus = t.n_bus;
+  d.n = t.n;
+  d.nnz_lu = t.lu.nnz();
+  d.max_col = t.sched.max_col;
+  d.y_rowptr = t.d_yrow.as<int>();
+  d.y_col = t.d_ycol.as<int>();
+  d.y_g = t.d_yg.as<double>();
+  d.y_b = t.d_yb.as<double>();
+  d.bus_type = t.d_type.as<int>();
+  d.rpos_p = t.d_rpos_p.as<int>();
+  d.rpos_q = t.d_rpos_q.as<int>();
+  d.xpos_a = t.d_xpos_a.as<int>();
+  d.xpos_m = t.d_xpos_m.as<int>();
+  d.lu_kind = t.d_kind.as<int>();
+  d.lu_bi = t.d_bi.as<int>();
+  d.lu_bk = t.d_bk.as<int>();
+  d.lu_g = t.d_g.as<double>();
+  d.lu_b = t.d_b.as<double>();
+  d.cp = t.d_cp.as<int>();
+  d.ulen = t.d_ulen.as<int>();
+  d.lu_row = t.d_row.as<int>();
+  d.upd_ptr = t.d_upd_ptr.as<int>();
+  d.upd_off = t.d_upd_off.as<uint16_t>();
+  t.uploaded = true;
+  return GX_OK;
+}
+
+int ensure_workspace(gx_topo& t, int n_tiles, int max_iter, bool fp32) {
+  if (n_tiles <= t.ws_tiles && max_iter <= t.ws_iter && (int)fp32 == t.ws_fp32) return GX_OK;
+  const size_t lanes = (size_t)n_tiles * gx::kLanes;
+  const size_t nb = lanes * t.n_bus * sizeof(double);
+  const size_t nu = lanes * t.n * sizeof(double);
+  const size_t es = fp32 ? sizeof(float) : sizeof(double);
+  int rc;
+  if ((rc = t.w_vm.reserve(nb)) || (rc = t.w_va.reserve(nb)) || (rc = t.w_vr.reserve(nb)) ||
+      (rc = t.w_vi.reserve(nb)) || (rc = t.w_ps.reserve(nb)) || (rc = t.w_qs.reserve(nb)) ||
+      (rc = t.w_pc.reserve(nb)) || (rc = t.w_qc.reserve(nb)) || (rc = t.w_f.reserve(nu)) ||
+      (rc = t.w_dx.reserve(nu)) || (rc = t.w_y.reserve(lanes * t.n * es)) ||
+      (rc = t.w_lu.reserve(lanes * (size_t)t.dt.nnz_lu * es)) || (rc = t.w_status.reserve(lanes * sizeof(int))) ||
+      (rc = t.w_iters.reserve(lanes * sizeof(int))) || (rc = t.w_max.reserve(lanes * sizeof(unsigned long long))) ||
+      (rc = t.w_hist.reserve(lanes * (size_t)max_iter * sizeof(double))))
+    return rc;
+  if (fp32 && ((rc = t.w_dvr.reserve(nb)) || (rc = t.w_dvi.reserve(nb)))) return rc;
+  t.ws_tiles = n_tiles;
+  t.ws_iter = max_iter;
+  t.ws_fp32 = (int)fp32;
+  return GX_OK;
+}
+
+int factor_at(gx_topo& t, const double* vm, const double* va, const double* ps, const double* qs) {
+  (void)ps;
+  (void)qs;
+  std::vector<double> p, q, vals;
+  host_injections(t, vm, va, p, q);
+  host_jacobian(t, vm, va, p, q, vals);
+  std::vector<double> ax(t.csc.ai.size(), 0.0);
+  for (size_t e = 0; e < vals.size(); ++e) ax[t.slot[e]] += vals[e];
+  if (!gx::factor_pivot(t.csc, ax.data(), t.ord, 1e-3, t.lu))
+    return fail(GX_ERR_SINGULAR, "first factorization found a zero or missing pivot");
+  return finalize_factor(t);
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* gx_version(void) { return "gridox_hip 0.1.0 (gfx950)"; }
+const char* gx_last_error(void) { return g_err.c_str(); }
+
+int gx_ybus_build(int n_bus, int n_line, const int* lf, const int* lt, const double* r, const double* x,
+                  const double* bsh, const double* gsh, int n_tr, const int* tf, const int* tt, const double* ys,
+                  const double* ysh, const double* tap, int cap, int* rowptr, int* col, double* y_re, double* y_im) {
+  if (n_bus <= 0 || n_line < 0 || n_tr < 0 || !rowptr) return fail(GX_ERR_ARG, "bad sizes");
+  // insertion-ordered accumulation per (row, col)
+  std::map<std::pair<int, int>, cx> acc;
+  auto add = [&](int i, int j, cx v) {
+    auto it = acc.find({i, j});
+    if (it == acc.end()) acc.emplace(std::make_pair(i, j), cadd({0.0, 0.0}, v));
+    else it->second = cadd(it->second, v);
+  };
+  for (int e = 0; e < n_line; ++e) {
+    if (lf[e] < 0 || lf[e] >= n_bus || lt[e] < 0 || lt[e] >= n_bus) return fail(GX_ERR_ARG, "line bus out of range");
+    if (lf[e] == lt[e]) {
+      add(lf[e], lf[e], {gsh ? gsh[e] : 0.0, bsh[e]});
+      continue;
+    }
+    cx yl = cdiv({1.0, 0.0}, {r[e], x[e]});
+    cx half{(gsh ? gsh[e] : 0.0) / 2.0, bsh[e] / 2.0};
+    add(lf[e], lf[e], cadd(yl, half));
+    add(lt[e], lt[e], cadd(yl, half));
+    add(lf[e], lt[e], cneg(yl));
+    add(lt[e], lf[e], cneg(yl));
+  }
+  for (int e = 0; e < n_tr; ++e) {
+    if (tf[e] < 0 || tf[e] >= n_bus || tt[e] < 0 || tt[e] >= n_bus) return fail(GX_ERR_ARG, "transformer bus out of range");
+    cx s{ys[2 * e], ys[2 * e + 1]}, sh{ysh[2 * e], ysh[2 * e + 1]}, a{tap[2 * e], tap[2 * e + 1]};
+    double k = std::hypot(a.re, a.im);
+    cx yd = cadd(s, {sh.re * 0.5, sh.im * 0.5});
+    add(tf[e], tf[e], cdiv(yd, {k * k, 0.0}));
+    add(tf[e], tt[e], cdiv(cneg(s), cconj(a)));
+    add(tt[e], tf[e], cdiv(cneg(s), a));
+    add(tt[e], tt[e], yd);
+  }
+  int nnz = (int)acc.size();
+  if (nnz > cap || !col || !y_re || !y_im) return -nnz;
+  for (int i = 0; i <= n_bus; ++i) rowptr[i] = 0;
+  int t = 0;
+  for (auto& kv : acc) {
+    rowptr[kv.first.first + 1]++;
+    col[t] = kv.first.second;
+    y_re[t] = kv.second.re;
+    y_im[t] = kv.second.im;
+    ++t;
+  }
+  for (int i = 0; i < n_bus; ++i) rowptr[i + 1] += rowptr[i];
+  return nnz;
+}
+
+gx_topo* gx_topo_create(int n_bus, const int* bus_type, const int* rowptr, const int* col, const double* y_re,
+                        const double* y_im, int* err) {
+  auto set = [&](int c) {
+    if (err) *err = c;
+  };
+  if (n_bus <= 0 || !bus_type || !rowptr || !col || !y_re || !y_im) {
+    set(fail(GX_ERR_ARG, "null or empty topology"));
+    return nullptr;
+  }
+  std::unique_ptr<gx_topo> t(new gx_topo());
+  t->n_bus = n_bus;
+  t->type.assign(bus_type, bus_type + n_bus);
+  const int nnz = rowptr[n_bus];
+  t->yrow.assign(rowptr, rowptr + n_bus + 1);
+  t->ycol.assign(col, col + nnz);
+  t->yg.assign(y_re, y_re + nnz);
+  t->yb.assign(y_im, y_im + nnz);
+  for (int i = 0; i < n_bus; ++i) {
+    if (t->type[i] < 0 || t->type[i] > 2) {
+      set(fail(GX_ERR_ARG, "bus type must be 0 (Slack), 1 (PV) or 2 (PQ)"));
+      return nullptr;
+    }
+    for (int e = rowptr[i]; e < rowptr[i + 1]; ++e)
+      if (col[e] < 0 || col[e] >= n_bus || (e > rowptr[i] && col[e] <= col[e - 1])) {
+        set(fail(GX_ERR_ARG, "Y-bus rows must be sorted, unique and in range"));
+        return nullptr;
+      }
+  }
+  // unknowns: angle of every non-slack bus, then magnitude of every PQ bus (bus order)
+  t->ns_pos.assign(n_bus, -1);
+  t->pq_pos.assign(n_bus, -1);
+  for (int i = 0; i < n_bus; ++i)
+    if (t->type[i] != 0) {
+      t->ns_pos[i] = (int)t->non_slack.size();
+      t->non_slack.push_back(i);
+    }
+  for (int i = 0; i < n_bus; ++i)
+    if (t->type[i] == 2) {
+      t->pq_pos[i] = (int)t->pq.size();
+      t->pq.push_back(i);
+    }
+  t->n_ang = (int)t->non_slack.size();
+  t->n = t->n_ang + (int)t->pq.size();
+  if (t->n == 0) {
+    set(fail(GX_ERR_ARG, "no unknowns (every bus is Slack)"));
+    return nullptr;
+  }
+  // JacobianPattern::analyze emission order (src/jacobian.rs:128-168)
+  auto push = [&](int r, int c, int kind, int i, int k, int e) {
+    t->jrow.push_back(r);
+    t->jcol.push_back(c);
+    t->jkind.push_back(kind);
+    t->jbi.push_back(i);
+    t->jbk.push_back(k);
+    t->jg.push_back(y_re[e]);
+    t->jb.push_back(y_im[e]);
+  };
+  for (int ri = 0; ri < t->n_ang; ++ri) {
+    int i = t->non_slack[ri];
+    for (int e = rowptr[i]; e < rowptr[i + 1]; ++e) {
+      int k = col[e];
+      if (k == i) {
+        push(ri, ri, gx::kHii, i, i, e);
+        if (t->pq_pos[i] >= 0) push(ri, t->n_ang + t->pq_pos[i], gx::kNii, i, i, e);
+        continue;
+      }
+      if (t->ns_pos[k] >= 0) push(ri, t->ns_pos[k], gx::kHik, i, k, e);
+      if (t->pq_pos[k] >= 0) push(ri, t->n_ang + t->pq_pos[k], gx::kNik, i, k, e);
+    }
+  }
+  for (int ri = 0; ri < (int)t->pq.size(); ++ri) {
+    int i = t->pq[ri];
+    for (int e = rowptr[i]; e < rowptr[i + 1]; ++e) {
+      int k = col[e];
+      if (k == i) {
+        push(t->n_ang + ri, t->ns_pos[i], gx::kMii, i, i, e);
+        push(t->n_ang + ri, t->n_ang + ri, gx::kLii, i, i, e);
+        continue;
+      }
+      if (t->ns_pos[k] >= 0) push(t->n_ang + ri, t->ns_pos[k], gx::kMik, i, k, e);
+      if (t->pq_pos[k] >= 0) push(t->n_ang + ri, t->n_ang + t->pq_pos[k], gx::kLik, i, k, e);
+    }
+  }
+  triplets_to_csc(t->n, t->jrow, t->jcol, t->csc, t->slot);
+  if (!gx::analyze(t->csc, t->ord)) {
+    set(fail(GX_ERR_SINGULAR, "Jacobian pattern is structurally singular"));
+    return nullptr;
+  }
+  set(GX_OK);
+  return t.release();
+}
+
+void gx_topo_destroy(gx_topo* t) { delete t; }
+
+int gx_topo_sizes(const gx_topo* t, long long* s) {
+  if (!t || !s) return fail(GX_ERR_ARG, "null");
+  s[0] = t->n_bus;
+  s[1] = (long long)t->ycol.size();
+  s[2] = t->n;
+  s[3] = (long long)t->jrow.size();
+  s[4] = t->factored ? t->lu.nnz() : 0;
+  s[5] = (long long)t->ord.r.size() - 1;
+  s[6] = t->factored ? 1 : 0;
+  s[7] = t->factored ? t->sched.n_upd : 0;
+  return GX_OK;
+}
+
+int gx_topo_jacobian_pattern(const gx_topo* t, int* rows, int* cols) {
+  if (!t || !rows || !cols) return fail(GX_ERR_ARG, "null");
+  std::memcpy(rows, t->jrow.data(), sizeof(int) * t->jrow.size());
+  std::memcpy(cols, t->jcol.data(), sizeof(int) * t->jcol.size());
+  return GX_OK;
+}
+
+int gx_topo_factor_at(gx_topo* t, const double* vm, const double* va, const double* p_spec, const double* q_spec) {
+  if (!t || !vm || !va) return fail(GX_ERR_ARG, "null");
+  return factor_at(*t, vm, va, p_spec, q_spec);
+}
+
+int gx_topo_set_pivot_order(gx_topo* t, const int* q, const int* pnum) {
+  if (!t || !q || !pnum) return fail(GX_ERR_ARG, "null");
+  std::vector<int> qq(q, q + t->n), pp(pnum, pnum + t->n);
+  if (!gx::pattern_from_order(t->csc, qq, pp, t->lu)) return fail(GX_ERR_ARG, "invalid or structurally singular pivot order");
+  return finalize_factor(*t);
+}
+
+int gx_topo_ordering(const gx_topo* t, int* p, int* q, int* r, int* pnum) {
+  if (!t) return fail(GX_ERR_ARG, "null");
+  if (p) std::memcpy(p, t->ord.p.data(), sizeof(int) * t->ord.p.size());
+  if (q) std::memcpy(q, t->ord.q.data(), sizeof(int) * t->ord.q.size());
+  if (r) std::memcpy(r, t->ord.r.data(), sizeof(int) * t->ord.r.size());
+  if (pnum) {
+    if (!t->factored) return fail(GX_ERR_STATE, "not factored yet");
+    std::memcpy(pnum, t->lu.pnum.data(), sizeof(int) * t->lu.pnum.size());
+  }
+  return GX_OK;
+}
+
+int gx_topo_lu_pattern(const gx_topo* t, int* cp, int* ulen, int* row) {
+  if (!t) return fail(GX_ERR_ARG, "null");
+  if (!t->factored) return fail(GX_ERR_STATE, "not factored yet");
+  if (cp) std::memcpy(cp, t->lu.cp.data(), sizeof(int) * t->lu.cp.size());
+  if (ulen) std::memcpy(ulen, t->lu.ulen.data(), sizeof(int) * t->lu.ulen.size());
+  if (row) std::memcpy(row, t->lu.row.data(), sizeof(int) * t->lu.row.size());
+  return GX_OK;
+}
+
+int gx_batch_solve(gx_topo* t, int n_scen, const double* p_spec, const double* q_spec, const double* vm0,
+                   const double* va0, double tol, int max_iter, int mode, int flags, double* vm, double* va,
+                   int* status, int* iters, double* history, void* stream_ptr) {
+  if (!t) return fail(GX_ERR_ARG, "null topology");
+  if (n_scen < 0 || max_iter < 0 || !(tol >= 0.0)) return fail(GX_ERR_ARG, "bad n_scen/max_iter/tol");
+  if (mode != GX_MODE_FP64 && mode != GX_MODE_MIXED) return fail(GX_ERR_ARG, "unknown mode");
+  if (n_scen == 0) return GX_OK;
+  if (!p_spec || !q_spec || !vm0 || !va0 || !vm || !va || !status || !iters) return fail(GX_ERR_ARG, "null array");
+  hipStream_t st = (hipStream_t)stream_ptr;
+  const bool dev = (flags & GX_DEVICE_PTRS) != 0;
+  const int nb = t->n_bus;
+  const size_t plane = (size_t)n_scen * nb * sizeof(double);
+  if (max_iter == 0) {
+    // no iteration: the starting point is returned as-is (MaxIterationsReached, empty history)
+    std::vector<int> st_h(n_scen, GX_MAX_ITER), it_h(n_scen, 0);
+    hipMemcpyKind k = dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToHost;
+    GX_HIP(hipMemcpyAsync(vm, vm0, plane, k, st));
+    GX_HIP(hipMemcpyAsync(va, va0, plane, k, st));
+    GX_HIP(hipMemcpyAsync(status, st_h.data(), sizeof(int) * n_scen, dev ? hipMemcpyHostToDevice : hipMemcpyHostToHost, st));
+    GX_HIP(hipMemcpyAsync(iters, it_h.data(), sizeof(int) * n_scen, dev ? hipMemcpyHostToDevice : hipMemcpyHostToHost, st));
+    GX_HIP(hipStreamSynchronize(st));
+    return GX_OK;
+  }
+  int rc;
+  if (!t->factored) {
+    // PersistentSolver's first factorization uses the first Jacobian it is asked to solve: the
+    // first scenario whose starting point is not already converged (src/solver.rs:699-713).
+    std::vector<double> hv(nb), ha(nb), hp(nb), hq(nb), pc, qc;
+    int chosen = -1;
+    for (int s = 0; s < n_scen && chosen < 0; ++s) {
+      const size_t off = (size_t)s * nb;
+      hipMemcpyKind k = dev ? hipMemcpyDeviceToHost : hipMemcpyHostToHost;
+      GX_HIP(hipMemcpy(hv.data(), vm0 + off, nb * sizeof(double), k));
+      GX_HIP(hipMemcpy(ha.data(), va0 + off, nb * sizeof(double), k));
+      GX_HIP(hipMemcpy(hp.data(), p_spec + off, nb * sizeof(double), k));
+      GX_HIP(hipMemcpy(hq.data(), q_spec + off, nb * sizeof(double), k));
+      host_injections(*t, hv.data(), ha.data(), pc, qc);
+      double mx = 0.0;
+      for (int i = 0; i < nb; ++i) {
+        if (t->ns_pos[i] >= 0) mx = std::fmax(mx, std::fabs(hp[i] - pc[i]));
+        if (t->pq_pos[i] >= 0) mx = std::fmax(mx, std::fabs(hq[i] - qc[i]));
+      }
+      if (!(mx < tol) || s == n_scen - 1) chosen = s;
+    }
+    if ((rc = factor_at(*t, hv.data(), ha.data(), hp.data(), hq.data()))) return rc;
+  }
+  if ((rc = upload_topo(*t))) return rc;
+  const bool fp32 = mode == GX_MODE_MIXED;
+  const int n_tiles = (n_scen + gx::kLanes - 1) / gx::kLanes;
+  if ((rc = ensure_workspace(*t, n_tiles, max_iter, fp32))) return rc;
+  gx::DevWork w;
+  w.n_scen = n_scen;
+  w.n_tiles = n_tiles;
+  w.max_iter = max_iter;
+  w.tol = tol;
+  w.vm = t->w_vm.as<double>();
+  w.va = t->w_va.as<double>();
+  w.vr = t->w_vr.as<double>();
+  w.vi = t->w_vi.as<double>();
+  w.ps = t->w_ps.as<double>();
+  w.qs = t->w_qs.as<double>();
+  w.pc = t->w_pc.as<double>();
+  w.qc = t->w_qc.as<double>();
+  w.f = t->w_f.as<double>();
+  w.dx = t->w_dx.as<double>();
+  w.y = t->w_y.p;
+  w.lu = t->w_lu.p;
+  w.status = t->w_status.as<int>();
+  w.iters = t->w_iters.as<int>();
+  w.maxmis = t->w_max.as<unsigned long long>();
+  w.history = t->w_hist.as<double>();
+
+  // inputs -> interleaved tiles
+  const double* src[4] = {p_spec, q_spec, vm0, va0};
+  double* dst[4] = {w.ps, w.qs, w.vm, w.va};
+  if (!dev && (rc = t->w_stage.reserve(plane))) return rc;
+  for (int a = 0; a < 4; ++a) {
+    const double* s = src[a];
+    if (!dev) {
+      GX_HIP(hipMemcpyAsync(t->w_stage.p, src[a], plane, hipMemcpyHostToDevice, st));
+      s = t->w_stage.as<double>();
+    }
+    gx::launch_to_tiles(s, dst[a], n_scen, n_tiles, nb, st);
+  }
+  gx::launch_init(t->dt, w, st);
+  for (int it = 0; it < max_iter; ++it) {
+    gx::launch_mismatch(t->dt, w, fp32, st);
+    gx::launch_decide(t->dt, w, it, st);
+    gx::launch_fill(t->dt, w, fp32, st);
+    gx::launch_lu(t->dt, w, fp32, true, it, st);
+    if (fp32) {
+      gx::launch_refine(t->dt, w, t->w_dvr.as<double>(), t->w_dvi.as<double>(), st);
+      gx::launch_lu(t->dt, w, fp32, false, it, st);
+    }
+    gx::launch_update(t->dt, w, fp32, fp32, st);
+  }
+  gx::launch_finish(w, st);
+  GX_HIP(hipGetLastError());
+
+  // outputs
+  double* outs[2] = {vm, va};
+  const double* tiles[2] = {w.vm, w.va};
+  for (int a = 0; a < 2; ++a) {
+    if (dev) {
+      gx::launch_from_tiles(tiles[a], outs[a], n_scen, nb, st);
+    } else {
+      gx::launch_from_tiles(tiles[a], t->w_stage.as<double>(), n_scen, nb, st);
+      GX_HIP(hipMemcpyAsync(outs[a], t->w_stage.p, plane, hipMemcpyDeviceToHost, st));
+    }
+  }
+  hipMemcpyKind k = dev ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+  GX_HIP(hipMemcpyAsync(status, w.status, sizeof(int) * n_scen, k, st));
+  GX_HIP(hipMemcpyAsync(iters, w.iters, sizeof(int) * n_scen, k, st));
+  if (history) GX_HIP(hipMemcpyAsync(history, w.history, sizeof(double) * n_scen * (size_t)max_iter, k, st));
+  GX_HIP(hipStreamSynchronize(st));
+  return GX_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------- LinearSolver-level system
+struct gx_sys {
+  int n = 0;
+  size_t n_trip = 0;
+  gx::CscPattern csc;
+  std::vector<int> slot, csc2lu, qinv;
+  gx::Ordering ord;
+  gx::LuPattern lu;
+  LuSchedule sched;
+  DevMem d_cp, d_ulen, d_row, d_upd_ptr, d_upd_off, d_lu, d_y, d_status, d_iters, d_vals, d_rhs;
+  gx::DevTopo dt;
+  gx::DevWork w;
+};
+
+extern "C" {
+
+gx_sys* gx_sys_create(int n, int nnz, const int* rows, const int* cols, const double* vals) {
+  if (n <= 0 || nnz <= 0 || !rows || !cols || !vals) {
+    fail(GX_ERR_ARG, "bad system");
+    return nullptr;
+  }
+  for (int e = 0; e < nnz; ++e)
+    if (rows[e] < 0 || rows[e] >= n || cols[e] < 0 || cols[e] >= n) {
+      fail(GX_ERR_ARG, "triplet out of range");
+      return nullptr;
+    }
+  std::unique_ptr<gx_sys> s(new gx_sys());
+  s->n = n;
+  s->n_trip = (size_t)nnz;
+  std::vector<int> r(rows, rows + nnz), c(cols, cols + nnz);
+  triplets_to_csc(n, r, c, s->csc, s->slot);
+  std::vector<double> ax(s->csc.ai.size(), 0.0);
+  for (int e = 0; e < nnz; ++e) ax[s->slot[e]] += vals[e];
+  if (!gx::analyze(s->csc, s->ord) || !gx::factor_pivot(s->csc, ax.data(), s->ord, 1e-3, s->lu)) {
+    fail(GX_ERR_SINGULAR, "singular system");
+    return nullptr;
+  }
+  if (build_schedule(s->lu, s->sched)) return nullptr;
+  s->qinv.assign(n, 0);
+  for (int k = 0; k < n; ++k) s->qinv[s->lu.q[k]] = k;
+  s->csc2lu.assign(s->csc.ai.size(), -1);
+  for (int j = 0; j < n; ++j)
+    for (int p = s->csc.ap[j]; p < s->csc.ap[j + 1]; ++p)
+      s->csc2lu[p] = lu_position(s->lu, s->lu.pinv[s->csc.ai[p]], s->qinv[j]);
+  const int nlu = s->lu.nnz();
+  if (s->d_cp.upload(s->lu.cp) || s->d_ulen.upload(s->lu.ulen) || s->d_row.upload(s->lu.row) ||
+      s->d_upd_ptr.upload(s->sched.upd_ptr) || s->d_upd_off.upload(s->sched.upd_off) ||
+      s->d_lu.reserve((size_t)nlu * gx::kLanes * sizeof(double)) ||
+      s->d_y.reserve((size_t)n * gx::kLanes * sizeof(double)) || s->d_status.reserve(gx::kLanes * sizeof(int)) ||
+      s->d_iters.reserve(gx::kLanes * sizeof(int)) || s->d_vals.reserve((size_t)nlu * sizeof(double)) ||
+      s->d_rhs.reserve((size_t)n * sizeof(double)))
+    return nullptr;
+  if (hipMemset(s->d_lu.p, 0, s->d_lu.bytes) != hipSuccess || hipMemset(s->d_y.p, 0, s->d_y.bytes) != hipSuccess) {
+    fail(GX_ERR_DEVICE, "hipMemset failed");
+    return nullptr;
+  }
+  gx::DevTopo& d = s->dt;
+  d.n = n;
+  d.nnz_lu = nlu;
+  d.max_col = s->sched.max_col;
+  d.cp = s->d_cp.as<int>();
+  d.ulen = s->d_ulen.as<int>();
+  d.lu_row = s->d_row.as<int>();
+  d.upd_ptr = s->d_upd_ptr.as<int>();
+  d.upd_off = s->d_upd_off.as<uint16_t>();
+  s->w.n_scen = 1;
+  s->w.n_tiles = 1;
+  s->w.lu = s->d_lu.p;
+  s->w.y = s->d_y.p;
+  s->w.status = s->d_status.as<int>();
+  s->w.iters = s->d_iters.as<int>();
+  return s.release();
+}
+
+int gx_sys_factor_and_solve_values(gx_sys* s, const double* vals, const double* rhs, double* x) {
+  if (!s || !vals || !rhs || !x) return fail(GX_ERR_ARG, "null");
+  const int n = s->n, nlu = s->lu.nnz();
+  std::vector<double> packed(nlu, 0.0), b(n), out(n);
+  for (size_t e = 0; e < s->n_trip; ++e) packed[s->csc2lu[s->slot[e]]] += vals[e];
+  for (int r = 0; r < n; ++r) b[s->lu.pinv[r]] = rhs[r];
+  std::vector<int> st(gx::kLanes, gx::kPad);
+  st[0] = gx::kRunning;
+  GX_HIP(hipMemcpy(s->d_vals.p, packed.data(), sizeof(double) * nlu, hipMemcpyHostToDevice));
+  GX_HIP(hipMemcpy(s->d_rhs.p, b.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+  GX_HIP(hipMemcpy(s->d_status.p, st.data(), sizeof(int) * gx::kLanes, hipMemcpyHostToDevice));
+  gx::launch_lane0(s->d_vals.as<double>(), s->d_lu.as<double>(), nlu, true, nullptr);
+  gx::launch_lane0(s->d_rhs.as<double>(), s->d_y.as<double>(), n, true, nullptr);
+  gx::launch_lu(s->dt, s->w, false, true, 0, nullptr);
+  gx::launch_lane0(s->d_y.as<double>(), s->d_rhs.as<double>(), n, false, nullptr);
+  GX_HIP(hipGetLastError());
+  GX_HIP(hipMemcpy(b.data(), s->d_rhs.p, sizeof(double) * n, hipMemcpyDeviceToHost));
+  GX_HIP(hipMemcpy(st.data(), s->d_status.p, sizeof(int), hipMemcpyDeviceToHost));
+  if (st[0] != gx::kRunning) return GX_SINGULAR;
+  for (int k = 0; k < n; ++k) x[s->lu.q[k]] = b[k];
+  return GX_OK;
+}
+
+void gx_sys_destroy(gx_sys* s) { delete s; }
+
+}  // extern "C"

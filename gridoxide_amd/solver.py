@@ -1,0 +1,317 @@
+"""Host-side mirror of the reference's solver / batch interface over the HIP backend.
+
+The names, argument meanings and error behaviour follow the reference so that a caller of
+``PersistentSolver`` / ``BatchSolver`` finds the same surface:
+
+* ``BusType``, ``SolveStatus``, ``SolveStats``            src/types.rs:5-9, src/solver.rs:118-171
+* ``JacobianBackend`` (adds ``HIP``)                      src/solver.rs:61-108
+* ``BusOverride``, ``Scenario``, ``uniform_load_scaling`` src/batch.rs:39-105
+* ``BatchError`` and its variants                         src/batch.rs:107-134
+* ``PersistentSolver`` (solve / solve_with_stats / reset) src/solver.rs:395-494
+* ``BatchSolver`` (solve)                                 src/batch.rs:152-289
+
+Every numeric step runs in ``libgridox_hip.so`` on the GPU; this module only marshals arrays.
+"""
+from __future__ import annotations
+
+import dataclasses
+import enum
+from typing import List, Optional, Sequence
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, f64, i32, ptr
+from .grid import PQ, PV, SLACK, Network
+
+
+class BusType(enum.IntEnum):
+    Slack = SLACK
+    PV = PV
+    PQ = PQ
+
+
+class SolveStatus(enum.IntEnum):
+    Converged = _lib.GX_CONVERGED
+    MaxIterationsReached = _lib.GX_MAX_ITER
+    Singular = _lib.GX_SINGULAR
+
+
+class JacobianBackend(enum.Enum):
+    """Backend selector; only ``HIP`` is implemented here (the reference's CPU backends are not)."""
+    HIP = "hip"
+
+
+class Mode(enum.IntEnum):
+    FP64 = _lib.GX_MODE_FP64
+    MIXED = _lib.GX_MODE_MIXED
+
+
+@dataclasses.dataclass
+class SolveStats:
+    status: SolveStatus
+    mismatch_history: List[float]
+
+    def iterations(self) -> int:
+        return len(self.mismatch_history)
+
+
+@dataclasses.dataclass
+class PowerFlowReport:
+    voltage_mag: np.ndarray
+    voltage_ang: np.ndarray
+    stats: SolveStats
+
+
+# ---------------------------------------------------------------- scenarios and errors
+@dataclasses.dataclass
+class BusOverride:
+    bus: int
+    p_spec: Optional[float] = None
+    q_spec: Optional[float] = None
+    voltage_mag: Optional[float] = None
+
+    def p(self, v: float) -> "BusOverride":
+        self.p_spec = v
+        return self
+
+    def q(self, v: float) -> "BusOverride":
+        self.q_spec = v
+        return self
+
+    def vm(self, v: float) -> "BusOverride":
+        self.voltage_mag = v
+        return self
+
+
+@dataclasses.dataclass
+class Scenario:
+    bus_overrides: List[BusOverride] = dataclasses.field(default_factory=list)
+    branch_outages: List[int] = dataclasses.field(default_factory=list)
+
+
+def uniform_load_scaling(template: Network, factor: float) -> Scenario:
+    """Scales every bus with a nonzero p_spec or q_spec by ``factor`` (src/batch.rs:98-105)."""
+    idx = np.nonzero((template.p_spec != 0.0) | (template.q_spec != 0.0))[0]
+    return Scenario([BusOverride(int(i), float(template.p_spec[i] * factor), float(template.q_spec[i] * factor))
+                     for i in idx])
+
+
+class BatchError(Exception):
+    pass
+
+
+class OutagesUnsupported(BatchError):
+    def __init__(self, scenario: int):
+        super().__init__(f"scenario {scenario} requests branch outages, which are not implemented yet "
+                         f"(see batch::Scenario::branch_outages)")
+        self.scenario = scenario
+
+
+class BusOutOfRange(BatchError):
+    def __init__(self, scenario: int, bus: int, n_buses: int):
+        super().__init__(f"scenario {scenario} overrides bus {bus}, but the template has only {n_buses} bus(es)")
+        self.scenario, self.bus, self.n_buses = scenario, bus, n_buses
+
+
+# ---------------------------------------------------------------- Y-bus and topology
+@dataclasses.dataclass
+class YBus:
+    n: int
+    rowptr: np.ndarray
+    col: np.ndarray
+    y: np.ndarray  # complex128
+
+
+def build_ybus(net: Network) -> YBus:
+    """Merged Y-bus of a network (gx_ybus_build; build_ybus + stamp_transformers + finish)."""
+    L = _lib.lib()
+    cap = 4 * (net.line_from.size + net.tr_from.size) + net.n_bus
+    rowptr = np.zeros(net.n_bus + 1, np.int32)
+    col = np.zeros(cap, np.int32)
+    yre = np.zeros(cap)
+    yim = np.zeros(cap)
+    cplx = lambda a: f64(np.stack([a.real, a.imag], 1).ravel()) if a.size else np.zeros(2)
+    lf, lt = i32(net.line_from), i32(net.line_to)
+    lr, lx, lb, lg = f64(net.line_r), f64(net.line_x), f64(net.line_b), f64(net.line_g)
+    tf, tt = i32(net.tr_from), i32(net.tr_to)
+    ys, ysh, tap = cplx(net.tr_ys), cplx(net.tr_ysh), cplx(net.tr_tap)
+    nnz = check(L.gx_ybus_build(net.n_bus, lf.size, ptr(lf, _lib._i32p), ptr(lt, _lib._i32p), ptr(lr), ptr(lx),
+                                ptr(lb), ptr(lg), tf.size, ptr(tf, _lib._i32p), ptr(tt, _lib._i32p), ptr(ys),
+                                ptr(ysh), ptr(tap), cap, ptr(rowptr, _lib._i32p), ptr(col, _lib._i32p), ptr(yre),
+                                ptr(yim)))
+    return YBus(net.n_bus, rowptr, col[:nnz].copy(), yre[:nnz] + 1j * yim[:nnz])
+
+
+class Topology:
+    """One topology + bus-type assignment on the device (gx_topo)."""
+
+    def __init__(self, bus_type: np.ndarray, ybus: YBus):
+        L = _lib.lib()
+        self.n_bus = int(ybus.n)
+        self.bus_type = i32(bus_type)
+        self.ybus = ybus
+        self._yre = f64(ybus.y.real)
+        self._yim = f64(ybus.y.imag)
+        err = np.zeros(1, np.int32)
+        h = L.gx_topo_create(self.n_bus, ptr(self.bus_type, _lib._i32p), ptr(i32(ybus.rowptr), _lib._i32p),
+                             ptr(i32(ybus.col), _lib._i32p), ptr(self._yre), ptr(self._yim), ptr(err, _lib._i32p))
+        if not h:
+            raise _lib.GxError(int(err[0]), L.gx_last_error().decode())
+        self._h = h
+
+    @classmethod
+    def from_network(cls, net: Network) -> "Topology":
+        return cls(net.bus_type, build_ybus(net))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _lib.lib().gx_topo_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def sizes(self) -> dict:
+        s = np.zeros(8, np.int64)
+        check(_lib.lib().gx_topo_sizes(self._h, ptr(s, _lib._i64p)))
+        keys = ["n_bus", "nnz_y", "n", "nnz_j", "nnz_lu", "n_blocks", "factored", "n_update"]
+        return {k: int(v) for k, v in zip(keys, s)}
+
+    def jacobian_pattern(self):
+        nj = self.sizes()["nnz_j"]
+        r = np.zeros(nj, np.int32)
+        c = np.zeros(nj, np.int32)
+        check(_lib.lib().gx_topo_jacobian_pattern(self._h, ptr(r, _lib._i32p), ptr(c, _lib._i32p)))
+        return r, c
+
+    def factor_at(self, vm, va, p_spec, q_spec):
+        check(_lib.lib().gx_topo_factor_at(self._h, ptr(f64(vm)), ptr(f64(va)), ptr(f64(p_spec)), ptr(f64(q_spec))))
+
+    def set_pivot_order(self, q, pnum):
+        check(_lib.lib().gx_topo_set_pivot_order(self._h, ptr(i32(q), _lib._i32p), ptr(i32(pnum), _lib._i32p)))
+
+    def ordering(self):
+        s = self.sizes()
+        n = s["n"]
+        p, q, r = np.zeros(n, np.int32), np.zeros(n, np.int32), np.zeros(n + 1, np.int32)
+        pnum = np.zeros(n, np.int32) if s["factored"] else None
+        check(_lib.lib().gx_topo_ordering(self._h, ptr(p, _lib._i32p), ptr(q, _lib._i32p), ptr(r, _lib._i32p),
+                                          ptr(pnum, _lib._i32p) if pnum is not None else None))
+        return p, q, r[:s["n_blocks"] + 1].copy(), pnum
+
+    def lu_pattern(self):
+        s = self.sizes()
+        cp = np.zeros(s["n"] + 1, np.int32)
+        ulen = np.zeros(s["n"], np.int32)
+        row = np.zeros(s["nnz_lu"], np.int32)
+        check(_lib.lib().gx_topo_lu_pattern(self._h, ptr(cp, _lib._i32p), ptr(ulen, _lib._i32p), ptr(row, _lib._i32p)))
+        return cp, ulen, row
+
+    def batch_solve(self, p_spec, q_spec, vm0, va0, tol=1e-8, max_iter=20, mode=Mode.FP64):
+        """Host arrays [n_scen, n_bus] -> (vm, va, status, iters, history)."""
+        p_spec = f64(np.atleast_2d(p_spec))
+        S = p_spec.shape[0]
+        q_spec = f64(np.broadcast_to(q_spec, (S, self.n_bus)))
+        vm0 = f64(np.broadcast_to(vm0, (S, self.n_bus)))
+        va0 = f64(np.broadcast_to(va0, (S, self.n_bus)))
+        vm = np.zeros((S, self.n_bus))
+        va = np.zeros((S, self.n_bus))
+        st = np.zeros(S, np.int32)
+        it = np.zeros(S, np.int32)
+        hist = np.zeros((S, max(max_iter, 1)))
+        check(_lib.lib().gx_batch_solve(self._h, S, p_spec.ctypes.data, q_spec.ctypes.data, vm0.ctypes.data,
+                                        va0.ctypes.data, float(tol), int(max_iter), int(mode), 0, vm.ctypes.data,
+                                        va.ctypes.data, st.ctypes.data, it.ctypes.data, hist.ctypes.data, None))
+        return vm, va, st, it, hist[:, :max_iter]
+
+    def batch_solve_device(self, p_spec, q_spec, vm0, va0, vm, va, status, iters, history, tol=1e-8, max_iter=20,
+                           mode=Mode.FP64, stream=0):
+        """Device-resident variant: every argument is a CUDA/HIP tensor (torch) already in HBM."""
+        S = int(p_spec.shape[0])
+        check(_lib.lib().gx_batch_solve(self._h, S, p_spec.data_ptr(), q_spec.data_ptr(), vm0.data_ptr(),
+                                        va0.data_ptr(), float(tol), int(max_iter), int(mode), _lib.GX_DEVICE_PTRS,
+                                        vm.data_ptr(), va.data_ptr(), status.data_ptr(), iters.data_ptr(),
+                                        history.data_ptr() if history is not None else None, stream or None))
+
+
+# ---------------------------------------------------------------- reference-shaped solvers
+def _stats(status: int, iters: int, hist_row: np.ndarray) -> SolveStats:
+    return SolveStats(SolveStatus(int(status)), [float(x) for x in hist_row[:int(iters)]])
+
+
+class PersistentSolver:
+    """One topology's cached pattern + pivot order, reused across solves (src/solver.rs:395-494)."""
+
+    def __init__(self, backend: JacobianBackend = JacobianBackend.HIP, mode: Mode = Mode.FP64):
+        if backend is not JacobianBackend.HIP:
+            raise ValueError("only the HIP backend is available")
+        self.mode = mode
+        self._topo: Optional[Topology] = None
+        self._key = None
+
+    def solve_with_stats(self, net: Network, tol: float, max_iter: int, ybus: Optional[YBus] = None):
+        """Solves ``net`` from its own vm/va; updates net.vm / net.va in place like the reference's
+        ``&mut [Bus]`` and returns ``SolveStats``."""
+        if self._topo is None:
+            self._topo = Topology(net.bus_type, ybus if ybus is not None else build_ybus(net))
+        vm, va, st, it, hist = self._topo.batch_solve(net.p_spec[None], net.q_spec[None], net.vm[None], net.va[None],
+                                                      tol, max_iter, self.mode)
+        net.vm[:] = vm[0]
+        net.va[:] = va[0]
+        return _stats(st[0], it[0], hist[0])
+
+    def solve(self, net: Network, tol: float, max_iter: int, ybus: Optional[YBus] = None) -> SolveStats:
+        return self.solve_with_stats(net, tol, max_iter, ybus)
+
+    def reset(self):
+        if self._topo is not None:
+            self._topo.close()
+        self._topo = None
+
+
+class BatchSolver:
+    """Scenario batch over one shared topology (src/batch.rs:152-289), all scenarios on the GPU."""
+
+    def __init__(self, backend: JacobianBackend = JacobianBackend.HIP, mode: Mode = Mode.FP64):
+        if backend is not JacobianBackend.HIP:
+            raise ValueError("only the HIP backend is available")
+        self.mode = mode
+
+    @staticmethod
+    def scenario_arrays(template: Network, scenarios: Sequence[Scenario]):
+        """Validates like the reference and expands overrides into dense [S, n_bus] arrays."""
+        n = template.n_bus
+        for i, sc in enumerate(scenarios):
+            if sc.branch_outages:
+                raise OutagesUnsupported(i)
+            for ov in sc.bus_overrides:
+                if ov.bus >= n or ov.bus < 0:
+                    raise BusOutOfRange(i, ov.bus, n)
+        S = len(scenarios)
+        p = np.repeat(template.p_spec[None], S, 0)
+        q = np.repeat(template.q_spec[None], S, 0)
+        vm = np.repeat(template.vm[None], S, 0)
+        for s, sc in enumerate(scenarios):
+            for ov in sc.bus_overrides:
+                if ov.p_spec is not None:
+                    p[s, ov.bus] = ov.p_spec
+                if ov.q_spec is not None:
+                    q[s, ov.bus] = ov.q_spec
+                if ov.voltage_mag is not None:
+                    vm[s, ov.bus] = ov.voltage_mag
+        va = np.repeat(template.va[None], S, 0)
+        return p, q, vm, va
+
+    def solve(self, template: Network, scenarios: Sequence[Scenario], tol: float, max_iter: int,
+              topology: Optional[Topology] = None) -> List[PowerFlowReport]:
+        """Starting point: each scenario's own template voltages with overrides applied."""
+        p, q, vm0, va0 = self.scenario_arrays(template, scenarios)
+        if len(scenarios) == 0:
+            return []
+        topo = topology or Topology.from_network(template)
+        vm, va, st, it, hist = topo.batch_solve(p, q, vm0, va0, tol, max_iter, self.mode)
+        return [PowerFlowReport(vm[s], va[s], _stats(st[s], it[s], hist[s])) for s in range(len(scenarios))]

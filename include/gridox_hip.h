@@ -1,0 +1,128 @@
+/* gridox_hip — batched Newton-Raphson AC power flow on one MI355X (gfx950).
+ *
+ * C ABI: plain pointers and sizes only.  Every entry point states the reference interface it
+ * replaces (file:line in m-mirz/gridoxide).  Host arrays are caller-owned; device memory is owned
+ * by the library (per topology handle) except where an argument says "device pointer".
+ *
+ * Bus types use the reference's BusType (src/types.rs:5-9): 0 = Slack, 1 = PV, 2 = PQ.
+ * Solve status uses SolveStatus (src/solver.rs:161-171): 0 = Converged, 1 = MaxIterationsReached,
+ * 2 = Singular.
+ *
+ * Handles are not thread-safe: one handle per calling thread, like the reference's per-worker
+ * PersistentSolver (src/batch.rs:233-246).
+ */
+#ifndef GRIDOX_HIP_H
+#define GRIDOX_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- return codes ---- */
+#define GX_OK 0
+#define GX_ERR_ARG (-1)          /* invalid argument (sizes, indices, null pointers) */
+#define GX_ERR_SINGULAR (-2)     /* structurally or numerically singular at symbolic/first factor */
+#define GX_ERR_DEVICE (-3)       /* HIP runtime error (allocation, launch) */
+#define GX_ERR_STATE (-4)        /* call out of order (e.g. pattern query before a factorization) */
+#define GX_ERR_CAPACITY (-5)     /* caller buffer too small; required size returned where stated */
+#define GX_ERR_UNSUPPORTED (-6)  /* topology exceeds a documented limit of the device path */
+
+/* ---- solve status (SolveStatus, src/solver.rs:161-171) ---- */
+#define GX_CONVERGED 0
+#define GX_MAX_ITER 1
+#define GX_SINGULAR 2
+
+/* ---- numeric modes ---- */
+#define GX_MODE_FP64 0   /* Jacobian, LU and solves in FP64 (the reference's arithmetic) */
+#define GX_MODE_MIXED 1  /* FP64 mismatch/update/convergence; FP32 Jacobian/LU/solves + one FP64
+                            residual refinement sweep per Newton step */
+
+/* ---- flags for gx_batch_solve ---- */
+#define GX_DEVICE_PTRS 1 /* every array argument is a device pointer (inputs already in HBM) */
+
+const char* gx_version(void);
+const char* gx_last_error(void);
+
+/* Y-bus build.  Replaces build_ybus (src/network.rs:225-247), stamp_transformers
+ * (src/network.rs:364-373) with branch_calc_param's closed (1,1) case (src/network.rs:339-360),
+ * and YBus::finish (src/network.rs:52-71): pi-model lines (series 1/(r+jx), half the shunt
+ * g + jb at each end; a line with from == to is a pure shunt), then transformers
+ * (yff = (ys + ysh/2)/|tap|^2, yft = -ys/conj(tap), ytf = -ys/tap, ytt = ys + ysh/2); duplicate
+ * entries summed in insertion order; CSR rows sorted by column.  Complex transformer arrays are
+ * interleaved (re, im).  Returns nnz, or -(nnz needed) when cap is too small. */
+int gx_ybus_build(int n_bus, int n_line, const int* line_from, const int* line_to, const double* r,
+                  const double* x, const double* b_shunt, const double* g_shunt, int n_tr, const int* tr_from,
+                  const int* tr_to, const double* y_series, const double* y_shunt, const double* tap, int cap,
+                  int* rowptr, int* col, double* y_re, double* y_im);
+
+/* ---- topology: Jacobian pattern + symbolic factorization, device-resident ---- */
+typedef struct gx_topo gx_topo;
+
+/* Replaces JacobianPattern::analyze (src/jacobian.rs:96-171) plus the symbolic half of
+ * KluNativeSystem::new (build_csc_structure + analyze, src/klu_native/mod.rs:88-114,159-169;
+ * analyze.rs:66-149) for one topology and bus-type assignment.  Uploads the Y-bus and pattern.
+ * Returns NULL on error (*err set). */
+gx_topo* gx_topo_create(int n_bus, const int* bus_type, const int* rowptr, const int* col, const double* y_re,
+                        const double* y_im, int* err);
+void gx_topo_destroy(gx_topo* t);
+
+/* sizes[0..7] = n_bus, nnz(Y), n unknowns, nnz(J), nnz(L+U) (0 before the first factor),
+ * BTF blocks, factored (0/1), sum over U entries of |L(:,j)| (the refactor's multiply-adds). */
+int gx_topo_sizes(const gx_topo* t, long long* sizes);
+
+/* Jacobian pattern in the reference's emission order (H/N rows, then M/L rows;
+ * src/jacobian.rs:128-168).  rows/cols sized nnz(J). */
+int gx_topo_jacobian_pattern(const gx_topo* t, int* rows, int* cols);
+
+/* The first numeric factorization that fixes the pivot order, done at one operating point:
+ * KluNativeSystem::new's factor (src/klu_native/factor.rs:47-131, kernel.rs:187-262: partial
+ * pivoting with diagonal preference, tol 1e-3, unscaled), evaluated on the Jacobian at
+ * (vm, va, p_spec, q_spec).  newton_raphson_cached does this on the first iteration that needs a
+ * solve (src/solver.rs:711-713); gx_batch_solve calls it implicitly with the first scenario's
+ * starting point when the topology has not been factored yet. */
+int gx_topo_factor_at(gx_topo* t, const double* vm, const double* va, const double* p_spec, const double* q_spec);
+
+/* Accepts an externally computed pivot order instead (the route for a host that keeps its own
+ * symbolic phase, e.g. KluNative's Symbolic.q and Numeric.pnum, src/klu_native/factor.rs:32-40):
+ * column order q[k] and final row order pnum[k] of the Jacobian in the reference's row/column
+ * numbering.  The L+U pattern follows from them. */
+int gx_topo_set_pivot_order(gx_topo* t, const int* q, const int* pnum);
+
+/* Orders after factoring: p/q (symbolic BTF+AMD row/column orders), r (block starts, nblocks+1
+ * entries, buffer sized n+1), pnum (final pivot rows).  Any pointer may be NULL. */
+int gx_topo_ordering(const gx_topo* t, int* p, int* q, int* r, int* pnum);
+
+/* Packed L+U pattern in pivot space: cp[n+1] column starts, ulen[n] strictly-upper counts,
+ * row[nnz(L+U)] row of each entry (column k: U rows ascending, diagonal, L rows ascending). */
+int gx_topo_lu_pattern(const gx_topo* t, int* cp, int* ulen, int* row);
+
+/* ---- batched Newton-Raphson ---- */
+/* Replaces BatchSolver::solve (src/batch.rs:198-289) driving PersistentSolver::solve_with_stats
+ * (src/solver.rs:444-475) / newton_raphson_cached (src/solver.rs:639-736) once per scenario.
+ * Inputs are scenario-major [n_scen][n_bus]: p_spec, q_spec (overrides already applied), and the
+ * starting point vm0/va0 (the caller applies linear_initial_guess or flat start).  Outputs:
+ * vm, va [n_scen][n_bus]; status, iters [n_scen]; history [n_scen][max_iter] (max |mismatch| at
+ * the top of each iteration; entries past iters are unspecified).  Non-convergence is a status,
+ * not an error (src/batch.rs:193-197).  mode: GX_MODE_FP64 / GX_MODE_MIXED.  flags:
+ * GX_DEVICE_PTRS.  stream: a hipStream_t (NULL = the null stream); the call returns after the
+ * results are complete on that stream. */
+int gx_batch_solve(gx_topo* t, int n_scen, const double* p_spec, const double* q_spec, const double* vm0,
+                   const double* va0, double tol, int max_iter, int mode, int flags, double* vm, double* va,
+                   int* status, int* iters, double* history, void* stream);
+
+/* ---- LinearSolver-level system ---- */
+typedef struct gx_sys gx_sys;
+/* Replaces LinearSolver::new for KluNative (src/solver.rs:35-57, src/klu_native/mod.rs:159-169):
+ * triplets (duplicates summed), analyze + first factorization with pivoting.  NULL when
+ * structurally or numerically singular (the reference returns None). */
+gx_sys* gx_sys_create(int n, int nnz, const int* rows, const int* cols, const double* vals);
+/* Replaces factor_and_solve_values (src/klu_native/mod.rs:185-210): values positionally matching
+ * the creation triplets; refactor on the device with the pivot order kept, then solve.
+ * Returns GX_OK, or GX_SINGULAR when a pivot is zero or the solution is not finite (None). */
+int gx_sys_factor_and_solve_values(gx_sys* s, const double* vals, const double* rhs, double* x);
+void gx_sys_destroy(gx_sys* s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
