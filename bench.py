@@ -1,0 +1,133 @@
+"""Benchmark: converged Newton-Raphson power-flow solves per second on one MI355X.
+
+A step is one batched Newton solve (to convergence or max_iter) of every scenario of the workload,
+with all inputs already resident in HBM.  Prints one JSON line.
+
+Workloads (the synthetic meshed grids of gridoxide_amd.grid):
+  config2  2,000 buses (1,000 chords), 4,096 scenarios, FP64 throughout (default)
+  config3  10,000 buses (5,000 chords), 8,192 scenarios, mixed precision (FP32 LU + FP64 refinement)
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+WORKLOADS = {
+    "config2": dict(config=2, n_scen=4096, mode="FP64"),
+    "config3": dict(config=3, n_scen=8192, mode="MIXED"),
+}
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(net, p, q, vm0, va0, tol, max_iter, budget_s=12.0):
+    """The CPU oracle (a port of BatchSolver + KluNative semantics) on the host cores, timed on a
+    bounded sample of the same scenarios."""
+    from oracle import oracle
+    o = oracle.Oracle(net)
+    cores = min(16, os.cpu_count() or 1)
+    # calibrate on a small sample, then size the sample to the time budget
+    n0 = min(cores * 2, p.shape[0])
+    t0 = time.perf_counter()
+    o.batch_solve(p[:n0], q[:n0], vm0[:n0], va0[:n0], tol=tol, max_iter=max_iter, threads=cores)
+    dt0 = time.perf_counter() - t0
+    n = int(min(p.shape[0], max(n0, budget_s / max(dt0 / n0, 1e-6))))
+    t0 = time.perf_counter()
+    _, _, st, _, _ = o.batch_solve(p[:n], q[:n], vm0[:n], va0[:n], tol=tol, max_iter=max_iter, threads=cores)
+    dt = time.perf_counter() - t0
+    conv = int((st == 0).sum())
+    return {"value": conv / dt, "unit": "converged solves/s", "cores": cores, "kind": "port",
+            "sample": f"first {n} of the workload's scenarios, {conv} converged, {dt:.1f} s wall on {cores} threads"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="config2", choices=sorted(WORKLOADS))
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (scenario sharding across "
+                                   "GPUs is described in DESIGN.md, not built)"}))
+        sys.exit(2)
+
+    import torch
+    from gridoxide_amd import Mode, Topology, config_grid, scenario_specs
+
+    wl = WORKLOADS[args.workload]
+    mode = Mode[wl["mode"]]
+    tol, max_iter = 1e-8, 20
+    net = config_grid(wl["config"])
+    S = wl["n_scen"]
+    log(f"workload {args.workload}: {net.n_bus} buses, {S} scenarios, mode {mode.name}")
+    p, q = scenario_specs(net, S, seed=2026)
+    vm0 = np.repeat(net.vm[None], S, 0)
+    va0 = np.repeat(net.va[None], S, 0)
+    topo = Topology.from_network(net)
+    dev = torch.device("cuda:0")
+    tp, tq, tvm0, tva0 = (torch.from_numpy(np.ascontiguousarray(a)).to(dev) for a in (p, q, vm0, va0))
+    vm = torch.empty_like(tvm0)
+    va = torch.empty_like(tva0)
+    st = torch.empty(S, dtype=torch.int32, device=dev)
+    it = torch.empty(S, dtype=torch.int32, device=dev)
+    hist = torch.empty((S, max_iter), dtype=torch.float64, device=dev)
+    stream = torch.cuda.current_stream().cuda_stream
+
+    def step():
+        topo.batch_solve_device(tp, tq, tvm0, tva0, vm, va, st, it, hist, tol=tol, max_iter=max_iter, mode=mode,
+                                stream=stream)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    log(f"warmup done; sizes {topo.sizes()}")
+    conv_total = 0
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        step()
+        conv_total += int((st == 0).sum().item())
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    ms = 1e3 * dt / args.steps
+    iters = it.cpu().numpy()
+    log(f"timed {args.steps} steps: {ms:.2f} ms/step, converged {conv_total // args.steps}/{S}, "
+        f"iterations max {iters.max()} mean {iters.mean():.2f}")
+    out = {
+        "metric": "converged NR solves/sec at 1e-8 pu mismatch",
+        "value": conv_total / dt,
+        "unit": "converged solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if mode == Mode.FP64 else "f64+f32",
+        "data": "synthetic",
+        "config": {"workload": f"{args.workload}: {net.n_bus}-bus meshed grid, {S} scenarios, flat start, "
+                               f"tol 1e-8, max 20 iterations, mode {mode.name}",
+                   "n_bus": net.n_bus, "n_scenarios": S, "mode": mode.name},
+        "iterations_mean": float(iters.mean()),
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(net, p, q, vm0, va0, tol, max_iter)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -685,6 +685,7 @@ int gx_batch_solve(gx_topo* t, int n_scen, const double* p_spec, const double* q
 // ---------------------------------------------------------------- LinearSolver-level system
 struct gx_sys {
   int n = 0;
+  bool on_device = false;
   size_t n_trip = 0;
   gx::CscPattern csc;
   std::vector<int> slot, csc2lu, qinv;
@@ -726,18 +727,27 @@ gx_sys* gx_sys_create(int n, int nnz, const int* rows, const int* cols, const do
   for (int j = 0; j < n; ++j)
     for (int p = s->csc.ap[j]; p < s->csc.ap[j + 1]; ++p)
       s->csc2lu[p] = lu_position(s->lu, s->lu.pinv[s->csc.ai[p]], s->qinv[j]);
-  const int nlu = s->lu.nnz();
-  if (s->d_cp.upload(s->lu.cp) || s->d_ulen.upload(s->lu.ulen) || s->d_row.upload(s->lu.row) ||
-      s->d_upd_ptr.upload(s->sched.upd_ptr) || s->d_upd_off.upload(s->sched.upd_off) ||
-      s->d_lu.reserve((size_t)nlu * gx::kLanes * sizeof(double)) ||
-      s->d_y.reserve((size_t)n * gx::kLanes * sizeof(double)) || s->d_status.reserve(gx::kLanes * sizeof(int)) ||
-      s->d_iters.reserve(gx::kLanes * sizeof(int)) || s->d_vals.reserve((size_t)nlu * sizeof(double)) ||
-      s->d_rhs.reserve((size_t)n * sizeof(double)))
-    return nullptr;
-  if (hipMemset(s->d_lu.p, 0, s->d_lu.bytes) != hipSuccess || hipMemset(s->d_y.p, 0, s->d_y.bytes) != hipSuccess) {
-    fail(GX_ERR_DEVICE, "hipMemset failed");
-    return nullptr;
-  }
+  return s.release();
+}
+
+}  // extern "C"
+
+namespace {
+
+// Device resources of a system, allocated at its first solve.
+int sys_to_device(gx_sys* s) {
+  if (s->on_device) return GX_OK;
+  const int n = s->n, nlu = s->lu.nnz();
+  int rc;
+  if ((rc = s->d_cp.upload(s->lu.cp)) || (rc = s->d_ulen.upload(s->lu.ulen)) || (rc = s->d_row.upload(s->lu.row)) ||
+      (rc = s->d_upd_ptr.upload(s->sched.upd_ptr)) || (rc = s->d_upd_off.upload(s->sched.upd_off)) ||
+      (rc = s->d_lu.reserve((size_t)nlu * gx::kLanes * sizeof(double))) ||
+      (rc = s->d_y.reserve((size_t)n * gx::kLanes * sizeof(double))) ||
+      (rc = s->d_status.reserve(gx::kLanes * sizeof(int))) || (rc = s->d_iters.reserve(gx::kLanes * sizeof(int))) ||
+      (rc = s->d_vals.reserve((size_t)nlu * sizeof(double))) || (rc = s->d_rhs.reserve((size_t)n * sizeof(double))))
+    return rc;
+  GX_HIP(hipMemset(s->d_lu.p, 0, s->d_lu.bytes));
+  GX_HIP(hipMemset(s->d_y.p, 0, s->d_y.bytes));
   gx::DevTopo& d = s->dt;
   d.n = n;
   d.nnz_lu = nlu;
@@ -753,11 +763,28 @@ gx_sys* gx_sys_create(int n, int nnz, const int* rows, const int* cols, const do
   s->w.y = s->d_y.p;
   s->w.status = s->d_status.as<int>();
   s->w.iters = s->d_iters.as<int>();
-  return s.release();
+  s->on_device = true;
+  return GX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int gx_sys_ordering(const gx_sys* s, int* p, int* q, int* r, int* pnum, int* nblocks) {
+  if (!s) return fail(GX_ERR_ARG, "null");
+  if (p) std::memcpy(p, s->ord.p.data(), sizeof(int) * s->n);
+  if (q) std::memcpy(q, s->ord.q.data(), sizeof(int) * s->n);
+  if (r) std::memcpy(r, s->ord.r.data(), sizeof(int) * s->ord.r.size());
+  if (pnum) std::memcpy(pnum, s->lu.pnum.data(), sizeof(int) * s->n);
+  if (nblocks) *nblocks = (int)s->ord.r.size() - 1;
+  return s->lu.nnz();
 }
 
 int gx_sys_factor_and_solve_values(gx_sys* s, const double* vals, const double* rhs, double* x) {
   if (!s || !vals || !rhs || !x) return fail(GX_ERR_ARG, "null");
+  int rc0 = sys_to_device(s);
+  if (rc0) return rc0;
   const int n = s->n, nlu = s->lu.nnz();
   std::vector<double> packed(nlu, 0.0), b(n), out(n);
   for (size_t e = 0; e < s->n_trip; ++e) packed[s->csc2lu[s->slot[e]]] += vals[e];

@@ -419,7 +419,7 @@ void launch_fill(const DevTopo& t, const DevWork& w, bool fp32, hipStream_t stre
 
 void launch_lu(const DevTopo& t, const DevWork& w, bool fp32, bool factor, int iter, hipStream_t stream) {
   const size_t es = fp32 ? sizeof(float) : sizeof(double);
-  int cap = (int)(65536 / (kLanes * es));
+  int cap = (int)(49152 / (kLanes * es));
   if (cap > t.max_col) cap = t.max_col;
   const size_t shmem = factor ? (size_t)cap * kLanes * es : 0;
   dim3 grid(w.n_tiles);

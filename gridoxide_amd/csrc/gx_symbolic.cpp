@@ -677,6 +677,11 @@ bool factor_pivot(const CscPattern& a, const double* ax, const Ordering& ord, do
   std::vector<std::vector<double>> lval(n);
   lu.q = ord.q;
   lu.pnum.assign(n, -1);
+  // Preferred ("diagonal") row of every step, and the step that currently prefers each row.
+  // When a step pivots on another step's preferred row, the displaced row becomes that other
+  // step's preference (KLU's diagonal bookkeeping, src/klu_native/kernel.rs:388-395).
+  std::vector<int> pref(ord.p), pref_of(n, -1);
+  for (int k = 0; k < n; ++k) pref_of[pref[k]] = k;
   for (int k = 0; k < n; ++k) {
     const int col = ord.q[k];
     reach.clear();
@@ -735,12 +740,20 @@ bool factor_pivot(const CscPattern& a, const double* ax, const Ordering& ord, do
         amax = av;
         piv = v;
       }
-      if (v == ord.p[k]) diag = v;
+      if (v == pref[k]) diag = v;
     }
     if (piv < 0) return false;
     if (diag >= 0 && std::fabs(x[diag]) >= tol * amax) piv = diag;
     double pv = x[piv];
     if (pv == 0.0 || !std::isfinite(pv)) return false;
+    const int diagrow = pref[k];
+    if (piv != diagrow && pinv[diagrow] < 0) {
+      const int kbar = pref_of[piv];
+      pref[kbar] = diagrow;
+      pref_of[diagrow] = kbar;
+    }
+    pref[k] = piv;
+    pref_of[piv] = k;
     pinv[piv] = k;
     lu.pnum[k] = piv;
     x[piv] = 0.0;

@@ -121,6 +121,11 @@ gx_sys* gx_sys_create(int n, int nnz, const int* rows, const int* cols, const do
  * Returns GX_OK, or GX_SINGULAR when a pivot is zero or the solution is not finite (None). */
 int gx_sys_factor_and_solve_values(gx_sys* s, const double* vals, const double* rhs, double* x);
 void gx_sys_destroy(gx_sys* s);
+/* Symbolic/pivot orders of a system (the Symbolic p/q/r and Numeric pnum of
+ * src/klu_native/analyze.rs:22-54 and factor.rs:32-40).  p, q, pnum sized n; r sized n+1;
+ * *nblocks receives the BTF block count.  Returns nnz(L+U) (strictly-triangular parts plus the
+ * diagonal, the whole matrix factored as one block in the given order) or an error code. */
+int gx_sys_ordering(const gx_sys* s, int* p, int* q, int* r, int* pnum, int* nblocks);
 
 #ifdef __cplusplus
 }

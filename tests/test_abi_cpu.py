@@ -1,0 +1,121 @@
+"""The C ABI library loads and exports every declared entry point; host-side logic that needs no
+GPU (Y-bus build, Jacobian pattern, argument validation, the reference-shaped error behaviour)."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+from gridoxide_amd import (BatchSolver, BusOutOfRange, BusOverride, OutagesUnsupported, Scenario, Topology,
+                           build_ybus, config_grid, uniform_load_scaling)
+from gridoxide_amd import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    names = []
+    for fn in os.listdir(os.path.join(ROOT, "include")):
+        if fn.endswith(".h"):
+            text = open(os.path.join(ROOT, "include", fn)).read()
+            text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+            names += re.findall(r"\b(gx_[a-z0-9_]+)\s*\(", text)
+    return sorted(set(names))
+
+
+def test_library_exports_every_declared_symbol():
+    L = _lib.lib()
+    names = declared_functions()
+    assert len(names) >= 15
+    for name in names:
+        assert hasattr(L, name), name
+    assert set(names) == set(_lib.SIGNATURES), "binding table and header disagree"
+    assert L.gx_version().decode().startswith("gridox_hip")
+
+
+def test_ybus_build_bit_exact_vs_oracle(oracle_mod):
+    for cfg in (1, 2):
+        net = config_grid(cfg)
+        a = build_ybus(net)
+        rowptr, col, y = oracle_mod.ybus(net)
+        assert np.array_equal(a.rowptr, rowptr) and np.array_equal(a.col, col)
+        assert np.array_equal(a.y, y)
+
+
+def test_ybus_capacity_error_reports_needed_size():
+    net = config_grid(1)
+    L = _lib.lib()
+    rowptr = np.zeros(net.n_bus + 1, np.int32)
+    lf, lt = _lib.i32(net.line_from), _lib.i32(net.line_to)
+    f = _lib.f64
+    rc = L.gx_ybus_build(net.n_bus, lf.size, _lib.ptr(lf, _lib._i32p), _lib.ptr(lt, _lib._i32p), _lib.ptr(f(net.line_r)),
+                         _lib.ptr(f(net.line_x)), _lib.ptr(f(net.line_b)), _lib.ptr(f(net.line_g)), 0, None, None,
+                         None, None, None, 1, _lib.ptr(rowptr, _lib._i32p), None, None, None)
+    assert rc < -1
+
+
+def test_jacobian_pattern_matches_reference_order(oracle_mod):
+    net = config_grid(1)
+    topo = Topology.from_network(net)
+    o = oracle_mod.Oracle(net)
+    p, q = o.injections(net.vm, net.va)
+    r, c, _ = o.jacobian(net.vm, net.va, p, q)
+    rr, cc = topo.jacobian_pattern()
+    assert np.array_equal(r, rr) and np.array_equal(c, cc)
+    s = topo.sizes()
+    assert s["nnz_j"] == r.size and s["factored"] == 0 and s["nnz_lu"] == 0
+
+
+def test_topology_rejects_bad_input():
+    net = config_grid(1)
+    yb = build_ybus(net)
+    bad = net.bus_type.copy()
+    bad[3] = 7
+    with pytest.raises(_lib.GxError) as e:
+        Topology(bad, yb)
+    assert e.value.code == _lib.GX_ERR_ARG
+    with pytest.raises(_lib.GxError):
+        Topology(np.zeros(net.n_bus, np.int32), yb)  # every bus slack: no unknowns
+
+
+def test_pattern_query_before_factor_is_a_state_error():
+    topo = Topology.from_network(config_grid(1))
+    with pytest.raises(_lib.GxError) as e:
+        topo.lu_pattern()
+    assert e.value.code == _lib.GX_ERR_STATE
+
+
+def test_set_pivot_order_validates():
+    topo = Topology.from_network(config_grid(1))
+    n = topo.sizes()["n"]
+    with pytest.raises(_lib.GxError):
+        topo.set_pivot_order(np.zeros(n, np.int32), np.arange(n, dtype=np.int32))
+
+
+def test_batch_errors_follow_the_reference():
+    """src: tests/batch_solver_test.rs:188-217."""
+    net = config_grid(1)
+    bs = BatchSolver()
+    with pytest.raises(OutagesUnsupported) as e:
+        bs.solve(net, [Scenario([BusOverride(2).p(-0.5)]), Scenario([], [0])], 1e-6, 20)
+    assert e.value.scenario == 1
+    with pytest.raises(BusOutOfRange) as e:
+        bs.solve(net, [Scenario([BusOverride(999).p(-0.5)])], 1e-6, 20)
+    assert (e.value.scenario, e.value.bus, e.value.n_buses) == (0, 999, net.n_bus)
+    assert bs.solve(net, [], 1e-6, 20) == []
+
+
+def test_uniform_load_scaling():
+    net = config_grid(1)
+    sc = uniform_load_scaling(net, 1.25)
+    nz = np.nonzero((net.p_spec != 0) | (net.q_spec != 0))[0]
+    assert [o.bus for o in sc.bus_overrides] == list(nz)
+    p, q, vm, va = BatchSolver.scenario_arrays(net, [sc])
+    assert np.allclose(p[0, nz], net.p_spec[nz] * 1.25) and np.allclose(q[0, nz], net.q_spec[nz] * 1.25)
+
+
+def test_no_cpu_fallback_when_library_missing(monkeypatch, tmp_path):
+    monkeypatch.setattr(_lib, "LIB_PATH", str(tmp_path / "missing.so"))
+    monkeypatch.setattr(_lib, "_LIB", None)
+    with pytest.raises(RuntimeError):
+        _lib.lib()

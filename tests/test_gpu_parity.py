@@ -1,0 +1,257 @@
+"""GPU parity: the HIP path (through the C ABI) against the CPU oracle and the reference's
+known answers.
+
+Bars (the BASELINE parity contract): identical convergence status; identical iteration count for
+the FP64 path and at most +1 for mixed precision; converged |V| and theta within 1e-8 pu / rad of
+the FP64 oracle on the same inputs.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from gridoxide_amd import (BatchSolver, BusOverride, Mode, Network, PersistentSolver, Scenario, SolveStatus,
+                           Topology, config_grid, scenario_specs)
+from gridoxide_amd import _lib
+
+pytestmark = pytest.mark.gpu
+
+V_TOL = 1e-8      # pu and rad, against the FP64 oracle
+MIS_TOL = 1e-8    # convergence tolerance used by the workloads
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+@pytest.fixture(scope="module")
+def known(golden_dir):
+    with open(os.path.join(golden_dir, "known_answers.json")) as f:
+        return json.load(f)
+
+
+@pytest.fixture(scope="module")
+def three_bus(golden_dir):
+    with open(os.path.join(golden_dir, "network.json")) as f:
+        return Network.from_json(f.read())
+
+
+def _oracle_batch(oracle_mod, net, p, q, vm0, va0, tol=MIS_TOL, max_iter=20):
+    o = oracle_mod.Oracle(net)
+    return o.batch_solve(p, q, vm0, va0, tol=tol, max_iter=max_iter, init=0, threads=4)
+
+
+def _compare(gpu, ref, mixed=False, check_hist=True):
+    vm, va, st, it, hist = gpu
+    rvm, rva, rst, rit, rhist = ref
+    assert np.array_equal(st, rst), f"status differs: {np.nonzero(st != rst)[0][:10]}"
+    if mixed:
+        assert np.all(it >= rit) and np.all(it <= rit + 1), f"iterations {it} vs {rit}"
+    else:
+        assert np.array_equal(it, rit), f"iterations differ at {np.nonzero(it != rit)[0][:10]}"
+    conv = st == 0
+    if conv.any():
+        dvm = np.abs(vm[conv] - rvm[conv]).max()
+        dva = np.abs(va[conv] - rva[conv]).max()
+        _say(f"  max|dVm|={dvm:.3e} max|dVa|={dva:.3e}")
+        assert dvm <= V_TOL and dva <= V_TOL
+    if check_hist and not mixed:
+        for s in range(len(st)):
+            n = it[s]
+            np.testing.assert_allclose(hist[s, :n], rhist[s, :n], rtol=1e-6, atol=1e-12)
+
+
+def test_three_bus_golden_voltages(three_bus, known):
+    _say("case: 3-bus network.json, flat start, FP64")
+    net = three_bus
+    topo = Topology.from_network(net)
+    vm, va, st, it, hist = topo.batch_solve(net.p_spec[None], net.q_spec[None], net.vm[None], net.va[None],
+                                            tol=1e-6, max_iter=20)
+    assert st[0] == 0
+    for i, (m, a) in enumerate(known["three_bus"]["golden_voltages"]):
+        assert abs(vm[0, i] - m) < known["three_bus"]["golden_tol"]
+        assert abs(va[0, i] - a) < known["three_bus"]["golden_tol"]
+
+
+def test_three_bus_linear_start_matches_doc_step(three_bus, known, oracle_mod):
+    """From the documented linear initial guess the first mismatch and step match klu.md:420-447."""
+    _say("case: 3-bus from the documented linear initial guess")
+    net = three_bus
+    o = oracle_mod.Oracle(net)
+    vm0, va0, rc = o.linear_initial_guess(net.p_spec, net.q_spec, net.vm, net.va)
+    assert rc == 0
+    topo = Topology.from_network(net)
+    vm, va, st, it, hist = topo.batch_solve(net.p_spec[None], net.q_spec[None], vm0[None], va0[None],
+                                            tol=1e-6, max_iter=1)
+    ka = known["three_bus"]
+    assert abs(hist[0, 0] - max(abs(x) for x in ka["mismatch_iter1"])) < ka["printed_tol"]
+    dva = va[0] - va0
+    dvm = vm[0] - vm0
+    assert abs(dva[1] - ka["dx_iter1"][0]) < ka["printed_tol"]
+    assert abs(dva[2] - ka["dx_iter1"][1]) < ka["printed_tol"]
+    assert abs(dvm[2] - ka["dx_iter1"][2]) < ka["printed_tol"]
+
+
+@pytest.mark.parametrize("mode", [Mode.FP64, Mode.MIXED])
+def test_config1_batch_vs_oracle(oracle_mod, mode):
+    _say(f"case: config-1 grid (200 buses), 70 scenarios, flat start, mode={mode.name}")
+    net = config_grid(1)
+    p, q = scenario_specs(net, 70, seed=7)
+    vm0 = np.repeat(net.vm[None], 70, 0)
+    va0 = np.repeat(net.va[None], 70, 0)
+    topo = Topology.from_network(net)
+    gpu = topo.batch_solve(p, q, vm0, va0, tol=MIS_TOL, max_iter=20, mode=mode)
+    ref = _oracle_batch(oracle_mod, net, p, q, vm0, va0)
+    assert np.all(ref[2] == 0)
+    _compare(gpu, ref, mixed=(mode == Mode.MIXED))
+
+
+@pytest.mark.parametrize("mode", [Mode.FP64, Mode.MIXED])
+def test_config2_subset_vs_oracle(oracle_mod, mode):
+    _say(f"case: config-2 grid (2000 buses), 128 scenarios, flat start, mode={mode.name}")
+    net = config_grid(2)
+    p, q = scenario_specs(net, 128, seed=11)
+    vm0 = np.repeat(net.vm[None], 128, 0)
+    va0 = np.repeat(net.va[None], 128, 0)
+    topo = Topology.from_network(net)
+    gpu = topo.batch_solve(p, q, vm0, va0, tol=MIS_TOL, max_iter=20, mode=mode)
+    ref = _oracle_batch(oracle_mod, net, p, q, vm0, va0)
+    _compare(gpu, ref, mixed=(mode == Mode.MIXED))
+
+
+def test_divergent_scenario_isolated():
+    """src: tests/batch_solver_test.rs:153-183 -- an unsolvable scenario does not perturb others."""
+    _say("case: config-1 grid, scenario 4 infeasible (P=Q=-1e6)")
+    net = config_grid(1)
+    p, q = scenario_specs(net, 8, seed=3)
+    vm0 = np.repeat(net.vm[None], 8, 0)
+    va0 = np.repeat(net.va[None], 8, 0)
+    topo = Topology.from_network(net)
+    clean = topo.batch_solve(p, q, vm0, va0, tol=MIS_TOL, max_iter=20)
+    bad_p, bad_q = p.copy(), q.copy()
+    bad_p[4, 5] = -1e6
+    bad_q[4, 5] = -1e6
+    got = topo.batch_solve(bad_p, bad_q, vm0, va0, tol=MIS_TOL, max_iter=20)
+    assert got[2][4] != 0
+    keep = [k for k in range(8) if k != 4]
+    assert np.all(got[2][keep] == 0)
+    assert np.array_equal(got[0][keep], clean[0][keep]) and np.array_equal(got[1][keep], clean[1][keep])
+
+
+def test_nan_start_is_singular_and_isolated():
+    """A non-finite solve is reported as Singular (src/klu_native/mod.rs:192, src/solver.rs:717-720)."""
+    _say("case: config-1 grid, scenario 2 starts with a NaN voltage")
+    net = config_grid(1)
+    p, q = scenario_specs(net, 5, seed=5)
+    vm0 = np.repeat(net.vm[None], 5, 0)
+    va0 = np.repeat(net.va[None], 5, 0)
+    vm0[2, 17] = np.nan
+    topo = Topology.from_network(net)
+    vm, va, st, it, hist = topo.batch_solve(p, q, vm0, va0, tol=MIS_TOL, max_iter=20)
+    assert st[2] == 2 and it[2] == 1
+    assert np.all(st[[0, 1, 3, 4]] == 0)
+
+
+def test_batch_equals_sequential_persistent_solver():
+    """src: tests/batch_solver_test.rs:68-97 -- batch == sequential solves, bit for bit."""
+    _say("case: config-1 grid, batch of 6 vs 6 sequential single-scenario solves")
+    net = config_grid(1)
+    scen = [Scenario([BusOverride(int(i)).p(float(net.p_spec[i] * f)).q(float(net.q_spec[i] * f))
+                      for i in np.nonzero(net.p_spec != 0)[0]]) for f in (0.6, 0.8, 0.9, 1.0, 1.1, 1.2)]
+    topo = Topology.from_network(net)
+    reports = BatchSolver().solve(net, scen, MIS_TOL, 20, topology=topo)
+    p, q, vm0, va0 = BatchSolver.scenario_arrays(net, scen)
+    for s in range(len(scen)):
+        one = topo.batch_solve(p[s:s + 1], q[s:s + 1], vm0[s:s + 1], va0[s:s + 1], tol=MIS_TOL, max_iter=20)
+        assert reports[s].stats.status == SolveStatus.Converged
+        assert np.array_equal(reports[s].voltage_mag, one[0][0])
+        assert np.array_equal(reports[s].voltage_ang, one[1][0])
+
+
+def test_persistent_solver_warm_equals_fresh(three_bus):
+    """src: tests/persistent_solver_test.rs:24-65 -- a warm solve matches a fresh one."""
+    _say("case: 3-bus persistent solver warm vs fresh")
+    ps = PersistentSolver()
+    a = three_bus.copy()
+    ps.solve(a, 1e-6, 20)
+    b = three_bus.copy()
+    b.p_spec[2] *= 1.5
+    b.q_spec[2] *= 1.5
+    ps.solve(b, 1e-6, 20)
+    c = three_bus.copy()
+    c.p_spec[2] *= 1.5
+    c.q_spec[2] *= 1.5
+    PersistentSolver().solve(c, 1e-6, 20)
+    assert np.abs(b.vm - c.vm).max() < 1e-9 and np.abs(b.va - c.va).max() < 1e-9
+
+
+def test_device_pointer_path_matches_host_path():
+    import torch
+    _say("case: config-1 grid, 100 scenarios, device-resident inputs")
+    net = config_grid(1)
+    S = 100
+    p, q = scenario_specs(net, S, seed=9)
+    vm0 = np.repeat(net.vm[None], S, 0)
+    va0 = np.repeat(net.va[None], S, 0)
+    topo = Topology.from_network(net)
+    host = topo.batch_solve(p, q, vm0, va0, tol=MIS_TOL, max_iter=20)
+    dev = torch.device("cuda:0")
+    tp, tq, tvm0, tva0 = (torch.from_numpy(np.ascontiguousarray(a)).to(dev) for a in (p, q, vm0, va0))
+    vm = torch.empty_like(tvm0)
+    va = torch.empty_like(tva0)
+    st = torch.empty(S, dtype=torch.int32, device=dev)
+    it = torch.empty(S, dtype=torch.int32, device=dev)
+    hist = torch.empty((S, 20), dtype=torch.float64, device=dev)
+    torch.cuda.synchronize()
+    topo.batch_solve_device(tp, tq, tvm0, tva0, vm, va, st, it, hist, tol=MIS_TOL, max_iter=20)
+    assert np.array_equal(vm.cpu().numpy(), host[0]) and np.array_equal(va.cpu().numpy(), host[1])
+    assert np.array_equal(st.cpu().numpy(), host[2]) and np.array_equal(it.cpu().numpy(), host[3])
+
+
+def _sys_solve(entries, n, vals_list, rhs):
+    L = _lib.lib()
+    r = np.array([e[0] for e in entries], np.int32)
+    c = np.array([e[1] for e in entries], np.int32)
+    v0 = np.array([e[2] for e in entries], np.float64)
+    h = L.gx_sys_create(n, len(entries), _lib.ptr(r, _lib._i32p), _lib.ptr(c, _lib._i32p), _lib.ptr(v0))
+    assert h
+    out = []
+    try:
+        for vals in vals_list:
+            x = np.zeros(n)
+            rc = L.gx_sys_factor_and_solve_values(h, _lib.ptr(np.asarray(vals, np.float64)),
+                                                  _lib.ptr(np.asarray(rhs, np.float64)), _lib.ptr(x))
+            out.append((rc, x))
+    finally:
+        L.gx_sys_destroy(h)
+    return out
+
+
+def test_sys_btf_example_refactor(known):
+    """docs/src/solvers/klu.md:336-405: factor, then refactor with new values on the same pattern."""
+    _say("case: gx_sys on the documented 4x4 BTF example")
+    ex = known["btf_4x4"]
+    before = ex["before"]
+    after = ex["after"]
+    res = _sys_solve(before, 4, [[e[2] for e in before], [e[2] for e in after]], ex["b"])
+    assert res[0][0] == 0 and res[1][0] == 0
+    np.testing.assert_allclose(res[0][1], ex["x_before"], rtol=0, atol=1e-14)
+    np.testing.assert_allclose(res[1][1], ex["x_after"], rtol=0, atol=1e-14)
+
+
+def test_sys_refactor_new_values_and_zero_pivot(known):
+    """src/klu_native/kernel.rs:741-812: refactor matches a dense solve; a new zero pivot is reported."""
+    _say("case: gx_sys refactor with new values, then a new zero pivot")
+    ex = known["refactor_new_values"]
+    n = ex["n"]
+    b = np.arange(1, n + 1, dtype=np.float64)
+    res = _sys_solve(ex["original"], n, [[e[2] for e in ex["updated"]]], b)
+    A = np.zeros((n, n))
+    for r, c, v in ex["updated"]:
+        A[r, c] += v
+    assert res[0][0] == 0
+    np.testing.assert_allclose(res[0][1], np.linalg.solve(A, b), rtol=1e-13, atol=1e-14)
+    ez = known["refactor_new_zero_pivot"]
+    res = _sys_solve(ez["entries"], 2, [[e[2] for e in ez["singular"]]], [1.0, 1.0])
+    assert res[0][0] == _lib.GX_SINGULAR
