@@ -94,18 +94,40 @@ def main():
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
-    log(f"warmup done; sizes {topo.sizes()}")
+    sizes = topo.sizes()
+    log(f"warmup done; sizes {sizes}")
+    topo.profile(True)  # HIP events around every launch on the solve's stream
     conv_total = 0
+    participations = {"lu": 0, "solve": 0}
     t0 = time.perf_counter()
     for k in range(args.steps):
         step()
-        conv_total += int((st == 0).sum().item())
+        stat = st.cpu().numpy()
+        iters = it.cpu().numpy()
+        conv_total += int((stat == 0).sum())
+        # Newton steps that needed a factorization: every iteration except the converged last one
+        lu_steps = int((iters - (stat == 0)).sum())
+        participations["lu"] += lu_steps
+        participations["solve"] += lu_steps
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
+    prof = topo.profile(False)
     ms = 1e3 * dt / args.steps
-    iters = it.cpu().numpy()
     log(f"timed {args.steps} steps: {ms:.2f} ms/step, converged {conv_total // args.steps}/{S}, "
         f"iterations max {iters.max()} mean {iters.mean():.2f}")
+    log("kernel time per step (ms): " + ", ".join(f"{k} {v[0] / args.steps:.3f} ({v[1] // args.steps} launches)"
+                                                   for k, v in prof.items() if v[1]))
+    # algorithmic bytes per scenario and Newton step (refactor reads A and writes L+U, the solve
+    # reads L+U and the right-hand side and writes the solution)
+    w = 8 if mode == Mode.FP64 else 4
+    n, nnz_j, nnz_lu = sizes["n"], sizes["nnz_j"], sizes["nnz_lu"]
+    bytes_lu = (nnz_j + 2 * nnz_lu) * w + 2 * n * w
+    bytes_solve = nnz_lu * w + 2 * n * w
+    dominant = max(prof, key=lambda k: prof[k][0])
+    kname = "lu" if dominant != "solve" else "solve"
+    b = bytes_lu if kname == "lu" else bytes_solve
+    kms, klaunch = prof[kname]
+    achieved = participations[kname] * b / (kms / 1e3) / 1e9 if kms > 0 else 0.0
     out = {
         "metric": "converged NR solves/sec at 1e-8 pu mismatch",
         "value": conv_total / dt,
@@ -123,6 +145,13 @@ def main():
                                f"tol 1e-8, max 20 iterations, mode {mode.name}",
                    "n_bus": net.n_bus, "n_scenarios": S, "mode": mode.name},
         "iterations_mean": float(iters.mean()),
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": f"k_lu ({'refactor + solve' if kname == 'lu' else 'correction solve'}, "
+                               f"{'FP64' if w == 8 else 'FP32'})",
+                     "bytes_per_scenario_step": b, "avg_launch_ms": kms / max(klaunch, 1),
+                     "launches": klaunch, "scenario_steps": participations[kname]},
+        "kernel_ms_per_step": {k: v[0] / args.steps for k, v in prof.items() if v[1]},
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(net, p, q, vm0, va0, tol, max_iter)

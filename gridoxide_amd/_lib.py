@@ -22,6 +22,7 @@ GX_ERR_UNSUPPORTED = -6
 GX_CONVERGED, GX_MAX_ITER, GX_SINGULAR = 0, 1, 2
 GX_MODE_FP64, GX_MODE_MIXED = 0, 1
 GX_DEVICE_PTRS = 1
+KERNEL_CLASSES = ["mismatch", "fill", "lu", "refine", "solve", "update"]
 
 _i32p = ctypes.POINTER(ctypes.c_int)
 _i64p = ctypes.POINTER(ctypes.c_longlong)
@@ -45,6 +46,7 @@ SIGNATURES = {
     "gx_topo_lu_pattern": (ctypes.c_int, [_vp, _i32p, _i32p, _i32p]),
     "gx_batch_solve": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, _vp, _vp, ctypes.c_double, ctypes.c_int,
                                       ctypes.c_int, ctypes.c_int, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "gx_topo_profile": (ctypes.c_int, [_vp, ctypes.c_int, _f64p, _i32p]),
     "gx_sys_create": (_vp, [ctypes.c_int, ctypes.c_int, _i32p, _i32p, _f64p]),
     "gx_sys_factor_and_solve_values": (ctypes.c_int, [_vp, _f64p, _f64p, _f64p]),
     "gx_sys_destroy": (None, [_vp]),

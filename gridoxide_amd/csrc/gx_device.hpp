@@ -1,8 +1,8 @@
 // Device-resident topology and per-batch workspace shared by the host driver and the HIP kernels.
 //
-// Layout: every per-scenario array is "scenario-minor interleaved" in tiles of 64 scenarios,
-// element e of scenario s lives at   base[(tile * count + e) * 64 + lane],   tile = s / 64,
-// lane = s % 64, so one coalesced wavefront access touches the same element of 64 scenarios.
+// Layout: every per-scenario array is "scenario-minor interleaved" in tiles of kTile scenarios:
+// element e of scenario s lives at   base[(tile * count + e) * kTile + s % kTile],
+// tile = s / kTile, so the same element of a tile's scenarios is one contiguous segment.
 // The topology arrays (Y-bus, Jacobian recipes, LU pattern and update schedule) are shared
 // read-only by every scenario.
 #pragma once
@@ -12,9 +12,10 @@
 
 namespace gx {
 
-constexpr int kLanes = 64;
+constexpr int kLanes = 64;  // wavefront width
+constexpr int kTile = 16;   // scenarios per tile
 
-enum Status : int { kConverged = 0, kMaxIter = 1, kSingular = 2, kRunning = 3, kPad = 4 };
+enum Status : int { kConverged = 0, kMaxIter = 1, kSingular = 2, kRunning = 3, kPadLane = 4 };
 
 // Jacobian entry kinds, in the reference's numbering (src/jacobian.rs:46-57); kFill marks an
 // L+U position that is not a Jacobian entry (structural fill, value 0 before elimination).
@@ -58,7 +59,7 @@ struct DevWork {
   int n_tiles = 0;
   int max_iter = 0;
   double tol = 0.0;
-  // bus arrays, interleaved [tile][n_bus][64]
+  // bus arrays, interleaved [tile][n_bus][kTile]
   double* vm = nullptr;
   double* va = nullptr;
   double* vr = nullptr;
@@ -67,16 +68,17 @@ struct DevWork {
   double* qs = nullptr;
   double* pc = nullptr;
   double* qc = nullptr;
-  // unknown arrays, interleaved [tile][n][64]
+  // unknown arrays, interleaved [tile][n][kTile]
   double* f = nullptr;    // FP64 mismatch in pivot-row order
   double* dx = nullptr;   // FP64 Newton step in pivot-column order (mixed mode)
   void* y = nullptr;      // LU-precision right-hand side / solution
-  void* lu = nullptr;     // packed L+U values, [tile][nnz_lu][64]
+  void* lu = nullptr;     // packed L+U values, [tile][nnz_lu][kTile]
   // per scenario
   int* status = nullptr;
   int* iters = nullptr;
   unsigned long long* maxmis = nullptr;  // max |mismatch| as ordered bits of a non-negative double
   double* history = nullptr;             // [n_scen][max_iter]
+  int* running = nullptr;                // [max_iter] scenarios still running after each decision
 };
 
 // Launchers (gx_kernels.hip).  All run on `stream`; none synchronizes.  `fp32` selects the
@@ -92,5 +94,7 @@ void launch_refine(const DevTopo& t, const DevWork& w, double* dvr, double* dvi,
 void launch_update(const DevTopo& t, const DevWork& w, bool fp32, bool mixed, hipStream_t stream);
 void launch_finish(const DevWork& w, hipStream_t stream);
 void launch_lane0(const double* src, double* dst, int count, bool to_tile, hipStream_t stream);
+// Dynamic LDS bytes the factoring LU kernel needs for this topology.
+size_t lu_shmem(const DevTopo& t, bool fp32);
 
 }  // namespace gx

@@ -9,6 +9,7 @@
 #include <map>
 #include <memory>
 #include <string>
+#include <tuple>
 #include <utility>
 #include <vector>
 
@@ -189,7 +190,19 @@ struct gx_topo {
   int ws_tiles = 0, ws_iter = 0;
   int ws_fp32 = -1;
   DevMem w_vm, w_va, w_vr, w_vi, w_ps, w_qs, w_pc, w_qc, w_f, w_dx, w_y, w_lu, w_status, w_iters, w_max, w_hist;
-  DevMem w_dvr, w_dvi, w_stage;
+  DevMem w_dvr, w_dvi, w_stage, w_running;
+  int* h_running = nullptr;  // pinned host mirror of the running counters
+  int h_running_len = 0;
+  // opt-in per-launch timing: (kernel class, start, stop)
+  bool profiling = false;
+  std::vector<std::tuple<int, hipEvent_t, hipEvent_t>> prof;
+  ~gx_topo() {
+    for (auto& e : prof) {
+      (void)hipEventDestroy(std::get<1>(e));
+      (void)hipEventDestroy(std::get<2>(e));
+    }
+    if (h_running) (void)hipHostFree(h_running);
+  }
 };
 
 namespace {
@@ -326,7 +339,7 @@ int upload_topo(gx_topo& t) {
 
 int ensure_workspace(gx_topo& t, int n_tiles, int max_iter, bool fp32) {
   if (n_tiles <= t.ws_tiles && max_iter <= t.ws_iter && (int)fp32 == t.ws_fp32) return GX_OK;
-  const size_t lanes = (size_t)n_tiles * gx::kLanes;
+  const size_t lanes = (size_t)n_tiles * gx::kTile;
   const size_t nb = lanes * t.n_bus * sizeof(double);
   const size_t nu = lanes * t.n * sizeof(double);
   const size_t es = fp32 ? sizeof(float) : sizeof(double);
@@ -337,8 +350,16 @@ int ensure_workspace(gx_topo& t, int n_tiles, int max_iter, bool fp32) {
       (rc = t.w_dx.reserve(nu)) || (rc = t.w_y.reserve(lanes * t.n * es)) ||
       (rc = t.w_lu.reserve(lanes * (size_t)t.dt.nnz_lu * es)) || (rc = t.w_status.reserve(lanes * sizeof(int))) ||
       (rc = t.w_iters.reserve(lanes * sizeof(int))) || (rc = t.w_max.reserve(lanes * sizeof(unsigned long long))) ||
-      (rc = t.w_hist.reserve(lanes * (size_t)max_iter * sizeof(double))))
+      (rc = t.w_hist.reserve(lanes * (size_t)max_iter * sizeof(double))) ||
+      (rc = t.w_running.reserve((size_t)max_iter * sizeof(int))))
     return rc;
+  if (t.h_running_len < max_iter) {
+    if (t.h_running) (void)hipHostFree(t.h_running);
+    t.h_running = nullptr;
+    t.h_running_len = 0;
+    GX_HIP(hipHostMalloc((void**)&t.h_running, sizeof(int) * max_iter, hipHostMallocDefault));
+    t.h_running_len = max_iter;
+  }
   if (fp32 && ((rc = t.w_dvr.reserve(nb)) || (rc = t.w_dvi.reserve(nb)))) return rc;
   t.ws_tiles = n_tiles;
   t.ws_iter = max_iter;
@@ -364,6 +385,30 @@ int factor_at(gx_topo& t, const double* vm, const double* va, const double* ps, 
 extern "C" {
 
 const char* gx_version(void) { return "gridox_hip 0.1.0 (gfx950)"; }
+
+int gx_topo_profile(gx_topo* t, int enable, double* ms, int* launches) {
+  if (!t) return fail(GX_ERR_ARG, "null");
+  if (ms || launches) {
+    for (int c = 0; c < GX_K_COUNT; ++c) {
+      if (ms) ms[c] = 0.0;
+      if (launches) launches[c] = 0;
+    }
+    for (auto& e : t->prof) {
+      float v = 0.0f;
+      GX_HIP(hipEventSynchronize(std::get<2>(e)));
+      GX_HIP(hipEventElapsedTime(&v, std::get<1>(e), std::get<2>(e)));
+      if (ms) ms[std::get<0>(e)] += v;
+      if (launches) launches[std::get<0>(e)] += 1;
+    }
+  }
+  for (auto& e : t->prof) {
+    (void)hipEventDestroy(std::get<1>(e));
+    (void)hipEventDestroy(std::get<2>(e));
+  }
+  t->prof.clear();
+  t->profiling = enable != 0;
+  return GX_OK;
+}
 const char* gx_last_error(void) { return g_err.c_str(); }
 
 int gx_ybus_build(int n_bus, int n_line, const int* lf, const int* lt, const double* r, const double* x,
@@ -610,7 +655,9 @@ int gx_batch_solve(gx_topo* t, int n_scen, const double* p_spec, const double* q
   }
   if ((rc = upload_topo(*t))) return rc;
   const bool fp32 = mode == GX_MODE_MIXED;
-  const int n_tiles = (n_scen + gx::kLanes - 1) / gx::kLanes;
+  if (gx::lu_shmem(t->dt, fp32) > 65536)
+    return fail(GX_ERR_UNSUPPORTED, "longest L+U column exceeds the LDS staging limit of the device refactor");
+  const int n_tiles = (n_scen + gx::kTile - 1) / gx::kTile;
   if ((rc = ensure_workspace(*t, n_tiles, max_iter, fp32))) return rc;
   gx::DevWork w;
   w.n_scen = n_scen;
@@ -633,6 +680,7 @@ int gx_batch_solve(gx_topo* t, int n_scen, const double* p_spec, const double* q
   w.iters = t->w_iters.as<int>();
   w.maxmis = t->w_max.as<unsigned long long>();
   w.history = t->w_hist.as<double>();
+  w.running = t->w_running.as<int>();
 
   // inputs -> interleaved tiles
   const double* src[4] = {p_spec, q_spec, vm0, va0};
@@ -647,16 +695,38 @@ int gx_batch_solve(gx_topo* t, int n_scen, const double* p_spec, const double* q
     gx::launch_to_tiles(s, dst[a], n_scen, n_tiles, nb, st);
   }
   gx::launch_init(t->dt, w, st);
-  for (int it = 0; it < max_iter; ++it) {
-    gx::launch_mismatch(t->dt, w, fp32, st);
-    gx::launch_decide(t->dt, w, it, st);
-    gx::launch_fill(t->dt, w, fp32, st);
-    gx::launch_lu(t->dt, w, fp32, true, it, st);
-    if (fp32) {
-      gx::launch_refine(t->dt, w, t->w_dvr.as<double>(), t->w_dvi.as<double>(), st);
-      gx::launch_lu(t->dt, w, fp32, false, it, st);
+  // optional per-launch timing of each kernel class on this stream
+  auto timed = [&](int cls, auto&& launch) -> int {
+    if (!t->profiling) {
+      launch();
+      return GX_OK;
     }
-    gx::launch_update(t->dt, w, fp32, fp32, st);
+    hipEvent_t a, b;
+    GX_HIP(hipEventCreate(&a));
+    GX_HIP(hipEventCreate(&b));
+    GX_HIP(hipEventRecord(a, st));
+    launch();
+    GX_HIP(hipEventRecord(b, st));
+    t->prof.emplace_back(cls, a, b);
+    return GX_OK;
+  };
+  for (int it = 0; it < max_iter; ++it) {
+    if ((rc = timed(GX_K_MISMATCH, [&] { gx::launch_mismatch(t->dt, w, fp32, st); }))) return rc;
+    gx::launch_decide(t->dt, w, it, st);
+    // stop as soon as no scenario is left running (no empty launches after convergence)
+    GX_HIP(hipMemcpyAsync(t->h_running + it, w.running + it, sizeof(int), hipMemcpyDeviceToHost, st));
+    GX_HIP(hipStreamSynchronize(st));
+    if (t->h_running[it] == 0) break;
+    if ((rc = timed(GX_K_FILL, [&] { gx::launch_fill(t->dt, w, fp32, st); }))) return rc;
+    if ((rc = timed(GX_K_LU, [&] { gx::launch_lu(t->dt, w, fp32, true, it, st); }))) return rc;
+    if (fp32) {
+      if ((rc = timed(GX_K_REFINE, [&] {
+             gx::launch_refine(t->dt, w, t->w_dvr.as<double>(), t->w_dvi.as<double>(), st);
+           })))
+        return rc;
+      if ((rc = timed(GX_K_SOLVE, [&] { gx::launch_lu(t->dt, w, fp32, false, it, st); }))) return rc;
+    }
+    if ((rc = timed(GX_K_UPDATE, [&] { gx::launch_update(t->dt, w, fp32, fp32, st); }))) return rc;
   }
   gx::launch_finish(w, st);
   GX_HIP(hipGetLastError());
@@ -741,9 +811,9 @@ int sys_to_device(gx_sys* s) {
   int rc;
   if ((rc = s->d_cp.upload(s->lu.cp)) || (rc = s->d_ulen.upload(s->lu.ulen)) || (rc = s->d_row.upload(s->lu.row)) ||
       (rc = s->d_upd_ptr.upload(s->sched.upd_ptr)) || (rc = s->d_upd_off.upload(s->sched.upd_off)) ||
-      (rc = s->d_lu.reserve((size_t)nlu * gx::kLanes * sizeof(double))) ||
-      (rc = s->d_y.reserve((size_t)n * gx::kLanes * sizeof(double))) ||
-      (rc = s->d_status.reserve(gx::kLanes * sizeof(int))) || (rc = s->d_iters.reserve(gx::kLanes * sizeof(int))) ||
+      (rc = s->d_lu.reserve((size_t)nlu * gx::kTile * sizeof(double))) ||
+      (rc = s->d_y.reserve((size_t)n * gx::kTile * sizeof(double))) ||
+      (rc = s->d_status.reserve(gx::kTile * sizeof(int))) || (rc = s->d_iters.reserve(gx::kTile * sizeof(int))) ||
       (rc = s->d_vals.reserve((size_t)nlu * sizeof(double))) || (rc = s->d_rhs.reserve((size_t)n * sizeof(double))))
     return rc;
   GX_HIP(hipMemset(s->d_lu.p, 0, s->d_lu.bytes));
@@ -789,11 +859,11 @@ int gx_sys_factor_and_solve_values(gx_sys* s, const double* vals, const double* 
   std::vector<double> packed(nlu, 0.0), b(n), out(n);
   for (size_t e = 0; e < s->n_trip; ++e) packed[s->csc2lu[s->slot[e]]] += vals[e];
   for (int r = 0; r < n; ++r) b[s->lu.pinv[r]] = rhs[r];
-  std::vector<int> st(gx::kLanes, gx::kPad);
+  std::vector<int> st(gx::kTile, gx::kPadLane);
   st[0] = gx::kRunning;
   GX_HIP(hipMemcpy(s->d_vals.p, packed.data(), sizeof(double) * nlu, hipMemcpyHostToDevice));
   GX_HIP(hipMemcpy(s->d_rhs.p, b.data(), sizeof(double) * n, hipMemcpyHostToDevice));
-  GX_HIP(hipMemcpy(s->d_status.p, st.data(), sizeof(int) * gx::kLanes, hipMemcpyHostToDevice));
+  GX_HIP(hipMemcpy(s->d_status.p, st.data(), sizeof(int) * gx::kTile, hipMemcpyHostToDevice));
   gx::launch_lane0(s->d_vals.as<double>(), s->d_lu.as<double>(), nlu, true, nullptr);
   gx::launch_lane0(s->d_rhs.as<double>(), s->d_y.as<double>(), n, true, nullptr);
   gx::launch_lu(s->dt, s->w, false, true, 0, nullptr);

@@ -211,6 +211,14 @@ class Topology:
         check(_lib.lib().gx_topo_lu_pattern(self._h, ptr(cp, _lib._i32p), ptr(ulen, _lib._i32p), ptr(row, _lib._i32p)))
         return cp, ulen, row
 
+    def profile(self, enable: bool = True):
+        """Returns {kernel class: (total ms, launches)} accumulated since the last call and turns
+        per-launch event timing on/off for later solves."""
+        ms = np.zeros(len(_lib.KERNEL_CLASSES))
+        n = np.zeros(len(_lib.KERNEL_CLASSES), np.int32)
+        check(_lib.lib().gx_topo_profile(self._h, int(enable), ptr(ms), ptr(n, _lib._i32p)))
+        return {k: (float(a), int(b)) for k, a, b in zip(_lib.KERNEL_CLASSES, ms, n)}
+
     def batch_solve(self, p_spec, q_spec, vm0, va0, tol=1e-8, max_iter=20, mode=Mode.FP64):
         """Host arrays [n_scen, n_bus] -> (vm, va, status, iters, history)."""
         p_spec = f64(np.atleast_2d(p_spec))

@@ -110,6 +110,20 @@ int gx_batch_solve(gx_topo* t, int n_scen, const double* p_spec, const double* q
                    const double* va0, double tol, int max_iter, int mode, int flags, double* vm, double* va,
                    int* status, int* iters, double* history, void* stream);
 
+/* ---- per-kernel timing (no counterpart in the reference: measurement support) ---- */
+#define GX_K_MISMATCH 0 /* injections + mismatch */
+#define GX_K_FILL 1     /* Jacobian values into the L+U storage */
+#define GX_K_LU 2       /* numeric refactor + forward/back substitution */
+#define GX_K_REFINE 3   /* mixed precision: FP64 residual of the step */
+#define GX_K_SOLVE 4    /* mixed precision: correction solve with the FP32 factors */
+#define GX_K_UPDATE 5   /* voltage update */
+#define GX_K_COUNT 6
+/* Reads and clears the accumulated HIP-event timings of t's launches since the last call
+ * (ms[GX_K_COUNT] milliseconds summed per kernel class, launches[GX_K_COUNT]; either may be NULL),
+ * then turns timing on or off for later gx_batch_solve calls.  Timing brackets each launch with
+ * events on the solve's own stream. */
+int gx_topo_profile(gx_topo* t, int enable, double* ms, int* launches);
+
 /* ---- LinearSolver-level system ---- */
 typedef struct gx_sys gx_sys;
 /* Replaces LinearSolver::new for KluNative (src/solver.rs:35-57, src/klu_native/mod.rs:159-169):
