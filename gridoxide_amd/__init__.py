@@ -10,6 +10,8 @@ from .solver import (  # noqa: F401
     BusOutOfRange,
     BusOverride,
     BusType,
+    IslandReport,
+    IslandStatus,
     JacobianBackend,
     Mode,
     OutagesUnsupported,

@@ -17,6 +17,10 @@ constexpr int kTile = 16;   // scenarios per tile
 
 enum Status : int { kConverged = 0, kMaxIter = 1, kSingular = 2, kRunning = 3, kPadLane = 4 };
 
+// device bus codes: the reference's BusType (0 Slack, 1 PV, 2 PQ) plus kDeadBus for a member of
+// an island without a Slack bus (a Slack bus pinned to V = 0 with no injection)
+constexpr int kDeadBus = 3;
+
 // Jacobian entry kinds, in the reference's numbering (src/jacobian.rs:46-57); kFill marks an
 // L+U position that is not a Jacobian entry (structural fill, value 0 before elimination).
 enum Kind : int { kHii = 0, kNii = 1, kHik = 2, kNik = 3, kMii = 4, kLii = 5, kMik = 6, kLik = 7, kFill = 8 };
@@ -94,6 +98,8 @@ void launch_refine(const DevTopo& t, const DevWork& w, double* dvr, double* dvi,
 void launch_update(const DevTopo& t, const DevWork& w, bool fp32, bool mixed, hipStream_t stream);
 void launch_finish(const DevWork& w, hipStream_t stream);
 void launch_lane0(const double* src, double* dst, int count, bool to_tile, hipStream_t stream);
+void launch_island_max(const DevTopo& t, const DevWork& w, const int* comp, int n_comp, unsigned long long* out,
+                       hipStream_t stream);
 // Dynamic LDS bytes the factoring LU kernel needs for this topology.
 size_t lu_shmem(const DevTopo& t, bool fp32);
 

@@ -164,7 +164,13 @@ void triplets_to_csc(int n, const std::vector<int>& rows, const std::vector<int>
 
 struct gx_topo {
   int n_bus = 0;
-  std::vector<int> type;
+  std::vector<int> type;        // effective bus types (unreferenced islands marked Slack)
+  std::vector<int> given_type;  // bus types as passed in
+  // islands (connected components of the Y-bus graph, numbered by lowest bus)
+  std::vector<int> comp;        // bus -> component
+  std::vector<int> verdict;     // component -> GX_ISLAND_*
+  std::vector<int> dev_type;    // device bus codes: 0/1/2, 3 = member of an unreferenced island
+  DevMem d_comp, w_island;
   std::vector<int> yrow, ycol;
   std::vector<double> yg, yb;
   std::vector<int> non_slack, pq, ns_pos, pq_pos;
@@ -193,6 +199,9 @@ struct gx_topo {
   DevMem w_dvr, w_dvi, w_stage, w_running;
   int* h_running = nullptr;  // pinned host mirror of the running counters
   int h_running_len = 0;
+  // the last gx_batch_solve (for island reports)
+  gx::DevWork last;
+  bool has_last = false;
   // opt-in per-launch timing: (kernel class, start, stop)
   bool profiling = false;
   std::vector<std::tuple<int, hipEvent_t, hipEvent_t>> prof;
@@ -302,7 +311,8 @@ int upload_topo(gx_topo& t) {
   if (t.uploaded) return GX_OK;
   int rc;
   if ((rc = t.d_yrow.upload(t.yrow)) || (rc = t.d_ycol.upload(t.ycol)) || (rc = t.d_yg.upload(t.yg)) ||
-      (rc = t.d_yb.upload(t.yb)) || (rc = t.d_type.upload(t.type)) || (rc = t.d_rpos_p.upload(t.h_rp)) ||
+      (rc = t.d_yb.upload(t.yb)) || (rc = t.d_type.upload(t.dev_type)) ||
+      (rc = t.d_comp.upload(t.comp)) || (rc = t.d_rpos_p.upload(t.h_rp)) ||
       (rc = t.d_rpos_q.upload(t.h_rq)) || (rc = t.d_xpos_a.upload(t.h_xa)) || (rc = t.d_xpos_m.upload(t.h_xm)) ||
       (rc = t.d_kind.upload(t.h_kind)) || (rc = t.d_bi.upload(t.h_bi)) || (rc = t.d_bk.upload(t.h_bk)) ||
       (rc = t.d_g.upload(t.h_g)) || (rc = t.d_b.upload(t.h_b)) || (rc = t.d_cp.upload(t.lu.cp)) ||
@@ -488,6 +498,43 @@ gx_topo* gx_topo_create(int n_bus, const int* bus_type, const int* rowptr, const
         return nullptr;
       }
   }
+  // islands: connected_components / classify / mark_unreferenced_islands (src/network.rs:121-223)
+  t->given_type = t->type;
+  t->comp.assign(n_bus, -1);
+  {
+    std::vector<int> stack;
+    int nc = 0;
+    for (int s0 = 0; s0 < n_bus; ++s0) {
+      if (t->comp[s0] >= 0) continue;
+      t->comp[s0] = nc;
+      stack.assign(1, s0);
+      while (!stack.empty()) {
+        int i = stack.back();
+        stack.pop_back();
+        for (int e = rowptr[i]; e < rowptr[i + 1]; ++e) {
+          int j = col[e];
+          if (j != i && t->comp[j] < 0) {
+            t->comp[j] = nc;
+            stack.push_back(j);
+          }
+        }
+      }
+      ++nc;
+    }
+    std::vector<int> slacks(nc, 0);
+    for (int i = 0; i < n_bus; ++i)
+      if (t->type[i] == 0) slacks[t->comp[i]]++;
+    t->verdict.assign(nc, GX_ISLAND_SOLVABLE);
+    for (int c = 0; c < nc; ++c)
+      t->verdict[c] = slacks[c] == 0 ? GX_ISLAND_NO_REFERENCE
+                                     : (slacks[c] == 1 ? GX_ISLAND_SOLVABLE : GX_ISLAND_AMBIGUOUS);
+    t->dev_type = t->type;
+    for (int i = 0; i < n_bus; ++i)
+      if (t->verdict[t->comp[i]] == GX_ISLAND_NO_REFERENCE) {
+        t->type[i] = 0;
+        t->dev_type[i] = 3;
+      }
+  }
   // unknowns: angle of every non-slack bus, then magnitude of every PQ bus (bus order)
   t->ns_pos.assign(n_bus, -1);
   t->pq_pos.assign(n_bus, -1);
@@ -553,6 +600,43 @@ gx_topo* gx_topo_create(int n_bus, const int* bus_type, const int* rowptr, const
 }
 
 void gx_topo_destroy(gx_topo* t) { delete t; }
+
+int gx_topo_islands(const gx_topo* t, int* comp, int* verdict) {
+  if (!t) return fail(GX_ERR_ARG, "null");
+  if (comp) std::memcpy(comp, t->comp.data(), sizeof(int) * t->comp.size());
+  if (verdict) std::memcpy(verdict, t->verdict.data(), sizeof(int) * t->verdict.size());
+  return (int)t->verdict.size();
+}
+
+int gx_batch_island_status(gx_topo* t, int* out) {
+  if (!t || !out) return fail(GX_ERR_ARG, "null");
+  if (!t->has_last) return fail(GX_ERR_STATE, "no batch solved yet");
+  const gx::DevWork& w = t->last;
+  const int nc = (int)t->verdict.size();
+  const size_t cells = (size_t)w.n_scen * nc;
+  int rc;
+  if ((rc = t->w_island.reserve(cells * sizeof(unsigned long long)))) return rc;
+  GX_HIP(hipMemset(t->w_island.p, 0, cells * sizeof(unsigned long long)));
+  gx::launch_island_max(t->dt, w, t->d_comp.as<int>(), nc, t->w_island.as<unsigned long long>(), nullptr);
+  GX_HIP(hipGetLastError());
+  std::vector<unsigned long long> mx(cells);
+  std::vector<int> st(w.n_scen);
+  GX_HIP(hipMemcpy(mx.data(), t->w_island.p, cells * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+  GX_HIP(hipMemcpy(st.data(), w.status, sizeof(int) * w.n_scen, hipMemcpyDeviceToHost));
+  for (int s = 0; s < w.n_scen; ++s)
+    for (int c = 0; c < nc; ++c) {
+      int v;
+      if (t->verdict[c] == GX_ISLAND_NO_REFERENCE) v = GX_ISLAND_STATUS_NO_REFERENCE;
+      else if (t->verdict[c] == GX_ISLAND_AMBIGUOUS) v = GX_ISLAND_STATUS_AMBIGUOUS;
+      else {
+        double m;
+        std::memcpy(&m, &mx[(size_t)s * nc + c], sizeof(double));
+        v = m < w.tol ? GX_CONVERGED : (st[s] == GX_SINGULAR ? GX_SINGULAR : GX_MAX_ITER);
+      }
+      out[(size_t)s * nc + c] = v;
+    }
+  return GX_OK;
+}
 
 int gx_topo_sizes(const gx_topo* t, long long* s) {
   if (!t || !s) return fail(GX_ERR_ARG, "null");
@@ -730,6 +814,8 @@ int gx_batch_solve(gx_topo* t, int n_scen, const double* p_spec, const double* q
   }
   gx::launch_finish(w, st);
   GX_HIP(hipGetLastError());
+  t->last = w;
+  t->has_last = true;
 
   // outputs
   double* outs[2] = {vm, va};

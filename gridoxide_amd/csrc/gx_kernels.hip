@@ -117,6 +117,12 @@ __global__ __launch_bounds__(256) void k_init(DevTopo t, DevWork w) {
   const size_t bb = at(tile, t.n_bus, 0) + s;
   for (int i = elem_first(); i < t.n_bus; i += elem_stride()) {
     const size_t o = bb + (size_t)i * kTile;
+    if (t.bus_type[i] == kDeadBus) {  // member of an island without a Slack bus: V = 0, no injection
+      w.vm[o] = 0.0;
+      w.va[o] = 0.0;
+      w.ps[o] = 0.0;
+      w.qs[o] = 0.0;
+    }
     const double m = w.vm[o], a = w.va[o];
     double sn, cs;
     sincos(a, &sn, &cs);
@@ -412,6 +418,34 @@ __global__ __launch_bounds__(256) void k_update(DevTopo t, DevWork w) {
   }
 }
 
+// Max |mismatch| of every island at the final voltages (resolve_pending, src/solver.rs:234-265):
+// slack buses skipped, P mismatch for PV and PQ buses, Q mismatch for PQ buses.
+__global__ __launch_bounds__(256) void k_island_max(DevTopo t, DevWork w, const int* __restrict__ comp, int n_comp,
+                                                    unsigned long long* __restrict__ out) {
+  const int tile = blockIdx.y, s = sub();
+  const int sc = tile * kTile + s;
+  const size_t bb = at(tile, t.n_bus, 0) + s;
+  for (int i = elem_first(); i < t.n_bus; i += elem_stride()) {
+    const int ty = t.bus_type[i];
+    if (ty != 1 && ty != 2) continue;
+    double ir = 0.0, ii = 0.0;
+    const int p2 = t.y_rowptr[i + 1];
+    for (int p = t.y_rowptr[i]; p < p2; ++p) {
+      const size_t oj = bb + (size_t)t.y_col[p] * kTile;
+      const double g = t.y_g[p], b = t.y_b[p];
+      const double vr = w.vr[oj], vi = w.vi[oj];
+      ir += g * vr - b * vi;
+      ii += g * vi + b * vr;
+    }
+    const size_t o = bb + (size_t)i * kTile;
+    const double vr = w.vr[o], vi = w.vi[o];
+    double mx = fabs(w.ps[o] - (vr * ir + vi * ii));
+    if (ty == 2) mx = fmax(mx, fabs(w.qs[o] - (vi * ir - vr * ii)));
+    if (sc < w.n_scen && mx > 0.0)
+      atomicMax(&out[(size_t)sc * n_comp + comp[i]], (unsigned long long)__double_as_longlong(mx));
+  }
+}
+
 __global__ void k_finish(DevWork w) {
   const int sc = blockIdx.x * blockDim.x + threadIdx.x;
   if (sc >= w.n_tiles * kTile) return;
@@ -506,6 +540,12 @@ void launch_update(const DevTopo& t, const DevWork& w, bool fp32, bool mixed, hi
 void launch_finish(const DevWork& w, hipStream_t stream) {
   const int threads = w.n_tiles * kTile;
   hipLaunchKernelGGL(k_finish, dim3((threads + 255) / 256), dim3(256), 0, stream, w);
+}
+
+void launch_island_max(const DevTopo& t, const DevWork& w, const int* comp, int n_comp, unsigned long long* out,
+                       hipStream_t stream) {
+  hipLaunchKernelGGL(k_island_max, dim3(elem_blocks(t.n_bus), w.n_tiles), dim3(256), 0, stream, t, w, comp, n_comp,
+                     out);
 }
 
 void launch_lane0(const double* src, double* dst, int count, bool to_tile, hipStream_t stream) {

@@ -37,6 +37,22 @@ class SolveStatus(enum.IntEnum):
     Singular = _lib.GX_SINGULAR
 
 
+class IslandStatus(enum.IntEnum):
+    """src/solver.rs:179-212"""
+    Converged = _lib.GX_CONVERGED
+    MaxIterationsReached = _lib.GX_MAX_ITER
+    Singular = _lib.GX_SINGULAR
+    NoReferenceBus = 3
+    AmbiguousReferenceBus = 4
+
+
+@dataclasses.dataclass
+class IslandReport:
+    bus_indices: List[int]
+    slack_indices: List[int]
+    status: IslandStatus
+
+
 class JacobianBackend(enum.Enum):
     """Backend selector; only ``HIP`` is implemented here (the reference's CPU backends are not)."""
     HIP = "hip"
@@ -61,6 +77,7 @@ class PowerFlowReport:
     voltage_mag: np.ndarray
     voltage_ang: np.ndarray
     stats: SolveStats
+    islands: List[IslandReport] = dataclasses.field(default_factory=list)
 
 
 # ---------------------------------------------------------------- scenarios and errors
@@ -211,6 +228,28 @@ class Topology:
         check(_lib.lib().gx_topo_lu_pattern(self._h, ptr(cp, _lib._i32p), ptr(ulen, _lib._i32p), ptr(row, _lib._i32p)))
         return cp, ulen, row
 
+    def islands(self):
+        """(component of each bus, verdict of each component: 0 solvable, 1 no reference, 2 ambiguous)."""
+        comp = np.zeros(self.n_bus, np.int32)
+        verdict = np.zeros(self.n_bus, np.int32)
+        nc = check(_lib.lib().gx_topo_islands(self._h, ptr(comp, _lib._i32p), ptr(verdict, _lib._i32p)))
+        return comp, verdict[:nc].copy()
+
+    def island_status(self, n_scen: int) -> np.ndarray:
+        """[n_scen, n_islands] IslandStatus codes of the last batch solve."""
+        nc = check(_lib.lib().gx_topo_islands(self._h, None, None))
+        out = np.zeros((n_scen, nc), np.int32)
+        check(_lib.lib().gx_batch_island_status(self._h, ptr(out, _lib._i32p)))
+        return out
+
+    def island_reports(self, n_scen: int) -> List[List[IslandReport]]:
+        comp, verdict = self.islands()
+        st = self.island_status(n_scen)
+        members = [np.nonzero(comp == c)[0] for c in range(verdict.size)]
+        slacks = [[int(i) for i in m if self.bus_type[i] == SLACK] for m in members]
+        return [[IslandReport([int(i) for i in members[c]], slacks[c], IslandStatus(int(st[s, c])))
+                 for c in range(verdict.size)] for s in range(n_scen)]
+
     def profile(self, enable: bool = True):
         """Returns {kernel class: (total ms, launches)} accumulated since the last call and turns
         per-launch event timing on/off for later solves."""
@@ -322,4 +361,5 @@ class BatchSolver:
             return []
         topo = topology or Topology.from_network(template)
         vm, va, st, it, hist = topo.batch_solve(p, q, vm0, va0, tol, max_iter, self.mode)
-        return [PowerFlowReport(vm[s], va[s], _stats(st[s], it[s], hist[s])) for s in range(len(scenarios))]
+        isl = topo.island_reports(len(scenarios))
+        return [PowerFlowReport(vm[s], va[s], _stats(st[s], it[s], hist[s]), isl[s]) for s in range(len(scenarios))]

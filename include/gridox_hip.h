@@ -110,6 +110,26 @@ int gx_batch_solve(gx_topo* t, int n_scen, const double* p_spec, const double* q
                    const double* va0, double tol, int max_iter, int mode, int flags, double* vm, double* va,
                    int* status, int* iters, double* history, void* stream);
 
+/* ---- islands ---- */
+/* Island verdicts (IslandStatus, src/solver.rs:179-212). */
+#define GX_ISLAND_SOLVABLE 0      /* exactly one Slack bus: solved; per-scenario status below */
+#define GX_ISLAND_NO_REFERENCE 1  /* no Slack bus: every member pinned to V = 0, P = Q = 0 */
+#define GX_ISLAND_AMBIGUOUS 2     /* two or more Slack buses: solved as is, reported as such */
+/* Connected components of the Y-bus graph (connected_components + classify,
+ * src/network.rs:121-193): comp[n_bus] = component of each bus (components numbered by their
+ * lowest bus), verdict[n_comp] = GX_ISLAND_*.  Returns the number of components.  Members of an
+ * unreferenced island become Slack buses at V = 0 with no injection in every solve
+ * (mark_unreferenced_islands, src/network.rs:211-223). */
+int gx_topo_islands(const gx_topo* t, int* comp, int* verdict);
+/* Per-scenario, per-island outcome of the last gx_batch_solve on t (finish_island_reports +
+ * resolve_pending, src/solver.rs:234-312): out[s][c] is GX_CONVERGED / GX_MAX_ITER / GX_SINGULAR
+ * for a solvable island (its own max mismatch at the final voltages against tol, else the
+ * scenario's status), or GX_ISLAND_STATUS_NO_REFERENCE / GX_ISLAND_STATUS_AMBIGUOUS.  out is a
+ * host array [n_scen][n_comp]. */
+#define GX_ISLAND_STATUS_NO_REFERENCE 3
+#define GX_ISLAND_STATUS_AMBIGUOUS 4
+int gx_batch_island_status(gx_topo* t, int* out);
+
 /* ---- per-kernel timing (no counterpart in the reference: measurement support) ---- */
 #define GX_K_MISMATCH 0 /* injections + mismatch */
 #define GX_K_FILL 1     /* Jacobian values into the L+U storage */

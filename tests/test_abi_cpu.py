@@ -119,3 +119,37 @@ def test_no_cpu_fallback_when_library_missing(monkeypatch, tmp_path):
     monkeypatch.setattr(_lib, "_LIB", None)
     with pytest.raises(RuntimeError):
         _lib.lib()
+
+
+def _with_islands():
+    """The 200-bus grid plus two extra islands: one with no Slack bus, one with two."""
+    import dataclasses
+    net = config_grid(1)
+    n0 = net.n_bus
+    extra_types = np.array([2, 1, 2, 0, 2, 0], np.int32)  # island A: 3 buses no slack; island B: 3 buses two slacks
+    lf = np.r_[net.line_from, [n0, n0 + 1, n0 + 3, n0 + 4]].astype(np.int32)
+    lt = np.r_[net.line_to, [n0 + 1, n0 + 2, n0 + 4, n0 + 5]].astype(np.int32)
+    k = 4
+    net = dataclasses.replace(
+        net, bus_type=np.r_[net.bus_type, extra_types].astype(np.int32), vm=np.r_[net.vm, np.ones(6)],
+        va=np.r_[net.va, np.zeros(6)], p_spec=np.r_[net.p_spec, [-0.1, 0.1, -0.1, 0, -0.1, 0]],
+        q_spec=np.r_[net.q_spec, [-0.05, 0, -0.05, 0, -0.05, 0]], line_from=lf, line_to=lt,
+        line_r=np.r_[net.line_r, np.full(k, 0.01)], line_x=np.r_[net.line_x, np.full(k, 0.05)],
+        line_b=np.r_[net.line_b, np.zeros(k)], line_g=np.r_[net.line_g, np.zeros(k)])
+    return net, n0
+
+
+def test_island_classification(oracle_mod):
+    net, n0 = _with_islands()
+    topo = Topology.from_network(net)
+    comp, verdict = topo.islands()
+    assert list(verdict) == [0, 1, 2]
+    assert np.all(comp[:n0] == 0) and np.all(comp[n0:n0 + 3] == 1) and np.all(comp[n0 + 3:] == 2)
+    # reference numbering of components by their lowest bus
+    ocomp = np.zeros(net.n_bus, np.int32)
+    assert oracle_mod.lib().orc_components(net.n_bus, _lib.ptr(topo.ybus.rowptr, _lib._i32p),
+                                           _lib.ptr(topo.ybus.col, _lib._i32p), _lib.ptr(ocomp, _lib._i32p)) == 3
+    assert np.array_equal(ocomp, comp)
+    # unreferenced island: its buses carry no unknowns (marked Slack)
+    plain = Topology.from_network(config_grid(1))
+    assert topo.sizes()["n"] == plain.sizes()["n"] + 2  # island B: its one PQ bus (angle + magnitude)

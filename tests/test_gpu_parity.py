@@ -255,3 +255,22 @@ def test_sys_refactor_new_values_and_zero_pivot(known):
     ez = known["refactor_new_zero_pivot"]
     res = _sys_solve(ez["entries"], 2, [[e[2] for e in ez["singular"]]], [1.0, 1.0])
     assert res[0][0] == _lib.GX_SINGULAR
+
+
+def test_islands_marked_and_reported(oracle_mod):
+    """src/network.rs:121-223, src/solver.rs:234-312: an island without a Slack bus is pinned to
+    V = 0; an island with two is solved and reported Ambiguous; the main island is unaffected."""
+    _say("case: config-1 grid plus an unreferenced and an ambiguous island")
+    from test_abi_cpu import _with_islands
+    from gridoxide_amd import IslandStatus
+    net, n0 = _with_islands()
+    plain = config_grid(1)
+    rep = BatchSolver().solve(net, [Scenario()], MIS_TOL, 20)[0]
+    ref = _oracle_batch(oracle_mod, plain, plain.p_spec[None], plain.q_spec[None], plain.vm[None], plain.va[None])
+    assert rep.stats.status == SolveStatus.Converged
+    assert np.abs(rep.voltage_mag[:n0] - ref[0][0]).max() <= V_TOL
+    assert np.abs(rep.voltage_ang[:n0] - ref[1][0]).max() <= V_TOL
+    assert np.all(rep.voltage_mag[n0:n0 + 3] == 0) and np.all(rep.voltage_ang[n0:n0 + 3] == 0)
+    assert [r.status for r in rep.islands] == [IslandStatus.Converged, IslandStatus.NoReferenceBus,
+                                               IslandStatus.AmbiguousReferenceBus]
+    assert rep.islands[1].slack_indices == [] and len(rep.islands[2].slack_indices) == 2
