@@ -22,6 +22,7 @@ GX_ERR_UNSUPPORTED = -6
 GX_CONVERGED, GX_MAX_ITER, GX_SINGULAR = 0, 1, 2
 GX_MODE_FP64, GX_MODE_MIXED = 0, 1
 GX_DEVICE_PTRS = 1
+GX_INIT_LINEAR = 2
 KERNEL_CLASSES = ["mismatch", "fill", "lu", "refine", "solve", "update"]
 
 _i32p = ctypes.POINTER(ctypes.c_int)

@@ -57,6 +57,21 @@ struct DevTopo {
   const uint16_t* upd_off = nullptr;  // total = sum over U entries of |L(:,j)|
 };
 
+// The linear initial guess system of a topology (complex, m = number of PQ buses): recipes
+// for its packed L+U values and right-hand side.  Its LU pattern lives in a second DevTopo.
+struct DevLg {
+  int m = 0;
+  int nnz_lu = 0;
+  const int* src = nullptr;      // nnz_lu: Y-bus entry index of the value, or -1 (fill)
+  const int* dbus = nullptr;     // nnz_lu: bus whose load admittance joins this diagonal, or -1
+  const int* row_bus = nullptr;  // m: bus of unknown r
+  const int* row_piv = nullptr;  // m: pivot row of equation r
+  const int* row_x = nullptr;    // m: pivot column holding unknown r
+  const int* rhs_ptr = nullptr;  // m + 1
+  const int* rhs_e = nullptr;    // Y-bus entries coupling r to a bus of known voltage
+  const int* rhs_j = nullptr;    // that bus
+};
+
 // Per-batch device workspace (sized for n_tiles tiles).
 struct DevWork {
   int n_scen = 0;
@@ -100,6 +115,9 @@ void launch_finish(const DevWork& w, hipStream_t stream);
 void launch_lane0(const double* src, double* dst, int count, bool to_tile, hipStream_t stream);
 void launch_island_max(const DevTopo& t, const DevWork& w, const int* comp, int n_comp, unsigned long long* out,
                        hipStream_t stream);
+void launch_linear_guess(const DevTopo& t, const DevLg& lg, const DevTopo& lgt, const DevWork& w, void* lu,
+                         void* y, int* lg_status, int* lg_iters, hipStream_t stream);
+size_t lg_shmem(const DevTopo& lgt);
 // Dynamic LDS bytes the factoring LU kernel needs for this topology.
 size_t lu_shmem(const DevTopo& t, bool fp32);
 

@@ -199,6 +199,17 @@ struct gx_topo {
   DevMem w_dvr, w_dvi, w_stage, w_running;
   int* h_running = nullptr;  // pinned host mirror of the running counters
   int h_running_len = 0;
+  // linear initial guess system (built on first use)
+  bool lg_built = false;
+  gx::LuPattern lg_lu;
+  LuSchedule lg_sched;
+  std::vector<int> lg_src, lg_dbus, lg_row_bus, lg_row_piv, lg_row_x, lg_rhs_ptr, lg_rhs_e, lg_rhs_j;
+  DevMem d_lg_src, d_lg_dbus, d_lg_row_bus, d_lg_row_piv, d_lg_row_x, d_lg_rhs_ptr, d_lg_rhs_e, d_lg_rhs_j;
+  DevMem d_lg_cp, d_lg_ulen, d_lg_row, d_lg_upd_ptr, d_lg_upd_off;
+  DevMem w_lg_lu, w_lg_y, w_lg_status, w_lg_iters;
+  gx::DevLg dlg;
+  gx::DevTopo dlgt;
+  bool ybus_uploaded = false;
   // the last gx_batch_solve (for island reports)
   gx::DevWork last;
   bool has_last = false;
@@ -306,13 +317,13 @@ int finalize_factor(gx_topo& t) {
   return GX_OK;
 }
 
+int upload_ybus(gx_topo& t);
+
 // Uploads the topology (Y-bus, recipes, LU pattern and schedule) once per factorization.
 int upload_topo(gx_topo& t) {
   if (t.uploaded) return GX_OK;
   int rc;
-  if ((rc = t.d_yrow.upload(t.yrow)) || (rc = t.d_ycol.upload(t.ycol)) || (rc = t.d_yg.upload(t.yg)) ||
-      (rc = t.d_yb.upload(t.yb)) || (rc = t.d_type.upload(t.dev_type)) ||
-      (rc = t.d_comp.upload(t.comp)) || (rc = t.d_rpos_p.upload(t.h_rp)) ||
+  if ((rc = upload_ybus(t)) || (rc = t.d_rpos_p.upload(t.h_rp)) ||
       (rc = t.d_rpos_q.upload(t.h_rq)) || (rc = t.d_xpos_a.upload(t.h_xa)) || (rc = t.d_xpos_m.upload(t.h_xm)) ||
       (rc = t.d_kind.upload(t.h_kind)) || (rc = t.d_bi.upload(t.h_bi)) || (rc = t.d_bk.upload(t.h_bk)) ||
       (rc = t.d_g.upload(t.h_g)) || (rc = t.d_b.upload(t.h_b)) || (rc = t.d_cp.upload(t.lu.cp)) ||
@@ -320,15 +331,8 @@ int upload_topo(gx_topo& t) {
       (rc = t.d_upd_ptr.upload(t.sched.upd_ptr)) || (rc = t.d_upd_off.upload(t.sched.upd_off)))
     return rc;
   gx::DevTopo& d = t.dt;
-  d.n_bus = t.n_bus;
-  d.n = t.n;
   d.nnz_lu = t.lu.nnz();
   d.max_col = t.sched.max_col;
-  d.y_rowptr = t.d_yrow.as<int>();
-  d.y_col = t.d_ycol.as<int>();
-  d.y_g = t.d_yg.as<double>();
-  d.y_b = t.d_yb.as<double>();
-  d.bus_type = t.d_type.as<int>();
   d.rpos_p = t.d_rpos_p.as<int>();
   d.rpos_q = t.d_rpos_q.as<int>();
   d.xpos_a = t.d_xpos_a.as<int>();
@@ -347,6 +351,7 @@ int upload_topo(gx_topo& t) {
   return GX_OK;
 }
 
+// Per-batch buffers that do not depend on the factorization (the L+U buffer is reserved after it).
 int ensure_workspace(gx_topo& t, int n_tiles, int max_iter, bool fp32) {
   if (n_tiles <= t.ws_tiles && max_iter <= t.ws_iter && (int)fp32 == t.ws_fp32) return GX_OK;
   const size_t lanes = (size_t)n_tiles * gx::kTile;
@@ -358,7 +363,7 @@ int ensure_workspace(gx_topo& t, int n_tiles, int max_iter, bool fp32) {
       (rc = t.w_vi.reserve(nb)) || (rc = t.w_ps.reserve(nb)) || (rc = t.w_qs.reserve(nb)) ||
       (rc = t.w_pc.reserve(nb)) || (rc = t.w_qc.reserve(nb)) || (rc = t.w_f.reserve(nu)) ||
       (rc = t.w_dx.reserve(nu)) || (rc = t.w_y.reserve(lanes * t.n * es)) ||
-      (rc = t.w_lu.reserve(lanes * (size_t)t.dt.nnz_lu * es)) || (rc = t.w_status.reserve(lanes * sizeof(int))) ||
+      (rc = t.w_status.reserve(lanes * sizeof(int))) ||
       (rc = t.w_iters.reserve(lanes * sizeof(int))) || (rc = t.w_max.reserve(lanes * sizeof(unsigned long long))) ||
       (rc = t.w_hist.reserve(lanes * (size_t)max_iter * sizeof(double))) ||
       (rc = t.w_running.reserve((size_t)max_iter * sizeof(int))))
@@ -374,6 +379,132 @@ int ensure_workspace(gx_topo& t, int n_tiles, int max_iter, bool fp32) {
   t.ws_tiles = n_tiles;
   t.ws_iter = max_iter;
   t.ws_fp32 = (int)fp32;
+  return GX_OK;
+}
+
+int upload_ybus(gx_topo& t) {
+  if (t.ybus_uploaded) return GX_OK;
+  int rc;
+  if ((rc = t.d_yrow.upload(t.yrow)) || (rc = t.d_ycol.upload(t.ycol)) || (rc = t.d_yg.upload(t.yg)) ||
+      (rc = t.d_yb.upload(t.yb)) || (rc = t.d_type.upload(t.dev_type)) || (rc = t.d_comp.upload(t.comp)))
+    return rc;
+  gx::DevTopo& d = t.dt;
+  d.n_bus = t.n_bus;
+  d.n = t.n;
+  d.y_rowptr = t.d_yrow.as<int>();
+  d.y_col = t.d_ycol.as<int>();
+  d.y_g = t.d_yg.as<double>();
+  d.y_b = t.d_yb.as<double>();
+  d.bus_type = t.d_type.as<int>();
+  t.ybus_uploaded = true;
+  return GX_OK;
+}
+
+// The linear initial guess system (src/network.rs:627-678) for the bus types as given: the
+// reference runs the guess before islands are marked (src/batch.rs:268, src/solver.rs:452-454).
+int build_lg(gx_topo& t, int n_tiles) {
+  int rc;
+  if (!t.lg_built) {
+    std::vector<int> pq, pos(t.n_bus, -1);
+    for (int i = 0; i < t.n_bus; ++i)
+      if (t.given_type[i] == 2) {
+        pos[i] = (int)pq.size();
+        pq.push_back(i);
+      }
+    const int m = (int)pq.size();
+    std::vector<int> tr, tc, tsrc, tdbus;
+    t.lg_rhs_ptr.assign(m + 1, 0);
+    t.lg_rhs_e.clear();
+    t.lg_rhs_j.clear();
+    for (int r = 0; r < m; ++r) {
+      const int i = pq[r];
+      bool diag = false;
+      for (int e = t.yrow[i]; e < t.yrow[i + 1]; ++e) {
+        const int j = t.ycol[e];
+        if (j == i) diag = true;
+        if (pos[j] >= 0) {
+          tr.push_back(r);
+          tc.push_back(pos[j]);
+          tsrc.push_back(e);
+          tdbus.push_back(j == i ? i : -1);
+        } else {
+          t.lg_rhs_e.push_back(e);
+          t.lg_rhs_j.push_back(j);
+        }
+      }
+      if (!diag) {
+        tr.push_back(r);
+        tc.push_back(r);
+        tsrc.push_back(-1);
+        tdbus.push_back(i);
+      }
+      t.lg_rhs_ptr[r + 1] = (int)t.lg_rhs_e.size();
+    }
+    t.dlg.m = m;
+    if (m > 0) {
+      gx::CscPattern a;
+      std::vector<int> slot;
+      triplets_to_csc(m, tr, tc, a, slot);
+      gx::Ordering ord;
+      if (!gx::analyze(a, ord) || !gx::pattern_from_order(a, ord.q, ord.p, t.lg_lu))
+        return fail(GX_ERR_SINGULAR, "linear initial guess system is structurally singular");
+      if ((rc = build_schedule(t.lg_lu, t.lg_sched))) return rc;
+      std::vector<int> qinv(m);
+      for (int k = 0; k < m; ++k) qinv[t.lg_lu.q[k]] = k;
+      const int nlu = t.lg_lu.nnz();
+      t.lg_src.assign(nlu, -1);
+      t.lg_dbus.assign(nlu, -1);
+      for (size_t e = 0; e < tr.size(); ++e) {
+        int p = lu_position(t.lg_lu, t.lg_lu.pinv[tr[e]], qinv[tc[e]]);
+        if (p < 0) return fail(GX_ERR_ARG, "linear guess entry missing from its L+U pattern");
+        t.lg_src[p] = tsrc[e];
+        t.lg_dbus[p] = tdbus[e];
+      }
+      t.lg_row_bus = pq;
+      t.lg_row_piv.assign(m, 0);
+      t.lg_row_x.assign(m, 0);
+      for (int r = 0; r < m; ++r) {
+        t.lg_row_piv[r] = t.lg_lu.pinv[r];
+        t.lg_row_x[r] = qinv[r];
+      }
+      if ((rc = t.d_lg_src.upload(t.lg_src)) || (rc = t.d_lg_dbus.upload(t.lg_dbus)) ||
+          (rc = t.d_lg_row_bus.upload(t.lg_row_bus)) || (rc = t.d_lg_row_piv.upload(t.lg_row_piv)) ||
+          (rc = t.d_lg_row_x.upload(t.lg_row_x)) || (rc = t.d_lg_rhs_ptr.upload(t.lg_rhs_ptr)) ||
+          (rc = t.d_lg_rhs_e.upload(t.lg_rhs_e)) || (rc = t.d_lg_rhs_j.upload(t.lg_rhs_j)) ||
+          (rc = t.d_lg_cp.upload(t.lg_lu.cp)) || (rc = t.d_lg_ulen.upload(t.lg_lu.ulen)) ||
+          (rc = t.d_lg_row.upload(t.lg_lu.row)) || (rc = t.d_lg_upd_ptr.upload(t.lg_sched.upd_ptr)) ||
+          (rc = t.d_lg_upd_off.upload(t.lg_sched.upd_off)))
+        return rc;
+      gx::DevLg& l = t.dlg;
+      l.nnz_lu = nlu;
+      l.src = t.d_lg_src.as<int>();
+      l.dbus = t.d_lg_dbus.as<int>();
+      l.row_bus = t.d_lg_row_bus.as<int>();
+      l.row_piv = t.d_lg_row_piv.as<int>();
+      l.row_x = t.d_lg_row_x.as<int>();
+      l.rhs_ptr = t.d_lg_rhs_ptr.as<int>();
+      l.rhs_e = t.d_lg_rhs_e.as<int>();
+      l.rhs_j = t.d_lg_rhs_j.as<int>();
+      gx::DevTopo& d = t.dlgt;
+      d.n = m;
+      d.nnz_lu = nlu;
+      d.max_col = t.lg_sched.max_col;
+      d.cp = t.d_lg_cp.as<int>();
+      d.ulen = t.d_lg_ulen.as<int>();
+      d.lu_row = t.d_lg_row.as<int>();
+      d.upd_ptr = t.d_lg_upd_ptr.as<int>();
+      d.upd_off = t.d_lg_upd_off.as<uint16_t>();
+      if (gx::lg_shmem(d) > 65536)
+        return fail(GX_ERR_UNSUPPORTED, "linear guess L+U column exceeds the LDS staging limit");
+    }
+    t.lg_built = true;
+  }
+  if (t.dlg.m > 0) {
+    const size_t lanes = (size_t)n_tiles * gx::kTile;
+    if ((rc = t.w_lg_lu.reserve(lanes * (size_t)t.dlg.nnz_lu * 16)) || (rc = t.w_lg_y.reserve(lanes * t.dlg.m * 16)) ||
+        (rc = t.w_lg_status.reserve(lanes * sizeof(int))) || (rc = t.w_lg_iters.reserve(lanes * sizeof(int))))
+      return rc;
+  }
   return GX_OK;
 }
 
@@ -703,46 +834,10 @@ int gx_batch_solve(gx_topo* t, int n_scen, const double* p_spec, const double* q
   const bool dev = (flags & GX_DEVICE_PTRS) != 0;
   const int nb = t->n_bus;
   const size_t plane = (size_t)n_scen * nb * sizeof(double);
-  if (max_iter == 0) {
-    // no iteration: the starting point is returned as-is (MaxIterationsReached, empty history)
-    std::vector<int> st_h(n_scen, GX_MAX_ITER), it_h(n_scen, 0);
-    hipMemcpyKind k = dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToHost;
-    GX_HIP(hipMemcpyAsync(vm, vm0, plane, k, st));
-    GX_HIP(hipMemcpyAsync(va, va0, plane, k, st));
-    GX_HIP(hipMemcpyAsync(status, st_h.data(), sizeof(int) * n_scen, dev ? hipMemcpyHostToDevice : hipMemcpyHostToHost, st));
-    GX_HIP(hipMemcpyAsync(iters, it_h.data(), sizeof(int) * n_scen, dev ? hipMemcpyHostToDevice : hipMemcpyHostToHost, st));
-    GX_HIP(hipStreamSynchronize(st));
-    return GX_OK;
-  }
   int rc;
-  if (!t->factored) {
-    // PersistentSolver's first factorization uses the first Jacobian it is asked to solve: the
-    // first scenario whose starting point is not already converged (src/solver.rs:699-713).
-    std::vector<double> hv(nb), ha(nb), hp(nb), hq(nb), pc, qc;
-    int chosen = -1;
-    for (int s = 0; s < n_scen && chosen < 0; ++s) {
-      const size_t off = (size_t)s * nb;
-      hipMemcpyKind k = dev ? hipMemcpyDeviceToHost : hipMemcpyHostToHost;
-      GX_HIP(hipMemcpy(hv.data(), vm0 + off, nb * sizeof(double), k));
-      GX_HIP(hipMemcpy(ha.data(), va0 + off, nb * sizeof(double), k));
-      GX_HIP(hipMemcpy(hp.data(), p_spec + off, nb * sizeof(double), k));
-      GX_HIP(hipMemcpy(hq.data(), q_spec + off, nb * sizeof(double), k));
-      host_injections(*t, hv.data(), ha.data(), pc, qc);
-      double mx = 0.0;
-      for (int i = 0; i < nb; ++i) {
-        if (t->ns_pos[i] >= 0) mx = std::fmax(mx, std::fabs(hp[i] - pc[i]));
-        if (t->pq_pos[i] >= 0) mx = std::fmax(mx, std::fabs(hq[i] - qc[i]));
-      }
-      if (!(mx < tol) || s == n_scen - 1) chosen = s;
-    }
-    if ((rc = factor_at(*t, hv.data(), ha.data(), hp.data(), hq.data()))) return rc;
-  }
-  if ((rc = upload_topo(*t))) return rc;
   const bool fp32 = mode == GX_MODE_MIXED;
-  if (gx::lu_shmem(t->dt, fp32) > 65536)
-    return fail(GX_ERR_UNSUPPORTED, "longest L+U column exceeds the LDS staging limit of the device refactor");
   const int n_tiles = (n_scen + gx::kTile - 1) / gx::kTile;
-  if ((rc = ensure_workspace(*t, n_tiles, max_iter, fp32))) return rc;
+  if ((rc = upload_ybus(*t)) || (rc = ensure_workspace(*t, n_tiles, max_iter, fp32))) return rc;
   gx::DevWork w;
   w.n_scen = n_scen;
   w.n_tiles = n_tiles;
@@ -759,7 +854,6 @@ int gx_batch_solve(gx_topo* t, int n_scen, const double* p_spec, const double* q
   w.f = t->w_f.as<double>();
   w.dx = t->w_dx.as<double>();
   w.y = t->w_y.p;
-  w.lu = t->w_lu.p;
   w.status = t->w_status.as<int>();
   w.iters = t->w_iters.as<int>();
   w.maxmis = t->w_max.as<unsigned long long>();
@@ -769,7 +863,7 @@ int gx_batch_solve(gx_topo* t, int n_scen, const double* p_spec, const double* q
   // inputs -> interleaved tiles
   const double* src[4] = {p_spec, q_spec, vm0, va0};
   double* dst[4] = {w.ps, w.qs, w.vm, w.va};
-  if (!dev && (rc = t->w_stage.reserve(plane))) return rc;
+  if ((rc = t->w_stage.reserve(plane))) return rc;
   for (int a = 0; a < 4; ++a) {
     const double* s = src[a];
     if (!dev) {
@@ -778,6 +872,49 @@ int gx_batch_solve(gx_topo* t, int n_scen, const double* p_spec, const double* q
     }
     gx::launch_to_tiles(s, dst[a], n_scen, n_tiles, nb, st);
   }
+  if (flags & GX_INIT_LINEAR) {
+    // BatchSolver runs linear_initial_guess on every scenario first (src/batch.rs:268)
+    if ((rc = build_lg(*t, n_tiles))) return rc;
+    gx::launch_linear_guess(t->dt, t->dlg, t->dlgt, w, t->w_lg_lu.p, t->w_lg_y.p, t->w_lg_status.as<int>(),
+                            t->w_lg_iters.as<int>(), st);
+    GX_HIP(hipGetLastError());
+  }
+  if (!t->factored) {
+    // PersistentSolver's first factorization uses the first Jacobian it is asked to solve: the
+    // first scenario whose starting point is not already converged (src/solver.rs:699-713).
+    gx::launch_from_tiles(w.vm, t->w_stage.as<double>(), n_scen, nb, st);
+    std::vector<double> hvm((size_t)n_scen * nb);
+    GX_HIP(hipMemcpyAsync(hvm.data(), t->w_stage.p, plane, hipMemcpyDeviceToHost, st));
+    GX_HIP(hipStreamSynchronize(st));
+    gx::launch_from_tiles(w.va, t->w_stage.as<double>(), n_scen, nb, st);
+    std::vector<double> hva((size_t)n_scen * nb);
+    GX_HIP(hipMemcpyAsync(hva.data(), t->w_stage.p, plane, hipMemcpyDeviceToHost, st));
+    GX_HIP(hipStreamSynchronize(st));
+    std::vector<double> hp(nb), hq(nb), pc, qc;
+    int chosen = -1;
+    for (int s = 0; s < n_scen && chosen < 0; ++s) {
+      const size_t off = (size_t)s * nb;
+      hipMemcpyKind k = dev ? hipMemcpyDeviceToHost : hipMemcpyHostToHost;
+      GX_HIP(hipMemcpy(hp.data(), p_spec + off, nb * sizeof(double), k));
+      GX_HIP(hipMemcpy(hq.data(), q_spec + off, nb * sizeof(double), k));
+      for (int i = 0; i < nb; ++i)
+        if (t->dev_type[i] == gx::kDeadBus) hvm[off + i] = hva[off + i] = hp[i] = hq[i] = 0.0;
+      host_injections(*t, hvm.data() + off, hva.data() + off, pc, qc);
+      double mx = 0.0;
+      for (int i = 0; i < nb; ++i) {
+        if (t->ns_pos[i] >= 0) mx = std::fmax(mx, std::fabs(hp[i] - pc[i]));
+        if (t->pq_pos[i] >= 0) mx = std::fmax(mx, std::fabs(hq[i] - qc[i]));
+      }
+      if (!(mx < tol) || s == n_scen - 1) chosen = s;
+    }
+    const size_t off = (size_t)chosen * nb;
+    if ((rc = factor_at(*t, hvm.data() + off, hva.data() + off, hp.data(), hq.data()))) return rc;
+  }
+  if ((rc = upload_topo(*t))) return rc;
+  if (gx::lu_shmem(t->dt, fp32) > 65536)
+    return fail(GX_ERR_UNSUPPORTED, "longest L+U column exceeds the LDS staging limit of the device refactor");
+  if ((rc = t->w_lu.reserve((size_t)n_tiles * gx::kTile * (size_t)t->dt.nnz_lu * (fp32 ? 4 : 8)))) return rc;
+  w.lu = t->w_lu.p;
   gx::launch_init(t->dt, w, st);
   // optional per-launch timing of each kernel class on this stream
   auto timed = [&](int cls, auto&& launch) -> int {
@@ -831,7 +968,8 @@ int gx_batch_solve(gx_topo* t, int n_scen, const double* p_spec, const double* q
   hipMemcpyKind k = dev ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
   GX_HIP(hipMemcpyAsync(status, w.status, sizeof(int) * n_scen, k, st));
   GX_HIP(hipMemcpyAsync(iters, w.iters, sizeof(int) * n_scen, k, st));
-  if (history) GX_HIP(hipMemcpyAsync(history, w.history, sizeof(double) * n_scen * (size_t)max_iter, k, st));
+  if (history && max_iter > 0)
+    GX_HIP(hipMemcpyAsync(history, w.history, sizeof(double) * n_scen * (size_t)max_iter, k, st));
   GX_HIP(hipStreamSynchronize(st));
   return GX_OK;
 }

@@ -232,6 +232,24 @@ __global__ __launch_bounds__(256) void k_fill(DevTopo t, DevWork w) {
   }
 }
 
+// Complex double for the linear initial guess, with num-complex 0.4 arithmetic (mul: (ac-bd,
+// ad+bc); div: by the squared norm).
+struct zd {
+  double re, im;
+};
+__device__ __forceinline__ zd operator-(zd a, zd b) { return {a.re - b.re, a.im - b.im}; }
+__device__ __forceinline__ zd operator*(zd a, zd b) { return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re}; }
+__device__ __forceinline__ zd operator/(zd a, zd b) {
+  const double ns = b.re * b.re + b.im * b.im;
+  return {(a.re * b.re + a.im * b.im) / ns, (a.im * b.re - a.re * b.im) / ns};
+}
+__device__ __forceinline__ bool is_zero(float v) { return v == 0.0f; }
+__device__ __forceinline__ bool is_zero(double v) { return v == 0.0; }
+__device__ __forceinline__ bool is_zero(zd v) { return v.re == 0.0 && v.im == 0.0; }
+__device__ __forceinline__ bool finite_value(float v) { return isfinite(v); }
+__device__ __forceinline__ bool finite_value(double v) { return isfinite(v); }
+__device__ __forceinline__ bool finite_value(zd v) { return isfinite(v.re) && isfinite(v.im); }
+
 // Refactor (optional) + forward/back substitution; one wavefront per tile of 16 scenarios.
 // Column k is staged in LDS (x[o][s], o < len(k) <= cap), updated there by the 4 lane groups
 // (distinct target rows within one update), divided by its pivot and written back.
@@ -277,7 +295,7 @@ __global__ __launch_bounds__(64) void k_lu(DevTopo t, DevWork w, int cap, int it
         }
       }
       const T ukk = x[ul * kStride + s];
-      if (ukk == (T)0) ok = false;
+      if (is_zero(ukk)) ok = false;
       for (int o = ul + 1 + first_of_group(ul + 1, g); o < len; o += kGroups) x[o * kStride + s] = x[o * kStride + s] / ukk;
       for (int o = o0; o < len; o += kGroups) lu[(size_t)(c0 + o) * kTile] = x[o * kStride + s];
       // forward substitution with column k of L (from LDS); the right-hand side belongs to group 0
@@ -306,7 +324,7 @@ __global__ __launch_bounds__(64) void k_lu(DevTopo t, DevWork w, int cap, int it
       const int c0 = t.cp[k], ul = t.ulen[k];
       const T xk = y[(size_t)k * kTile] / lu[(size_t)(c0 + ul) * kTile];
       y[(size_t)k * kTile] = xk;
-      if (!isfinite((double)xk)) ok = false;
+      if (!finite_value(xk)) ok = false;
       for (int u = 0; u < ul; ++u) {
         const size_t yj = (size_t)t.lu_row[c0 + u] * kTile;
         y[yj] = y[yj] - lu[(size_t)(c0 + u) * kTile] * xk;
@@ -446,6 +464,66 @@ __global__ __launch_bounds__(256) void k_island_max(DevTopo t, DevWork w, const 
   }
 }
 
+// ---------------------------------------------------------------- linear initial guess
+// linear_initial_guess (src/network.rs:627-678): every PQ load becomes the constant admittance
+// -conj(S) on its Y-bus diagonal and Y'_nn U_n = -Y'_ns U_s is solved for the PQ voltages; a
+// singular or non-finite solve leaves the starting point unchanged.
+__global__ __launch_bounds__(256) void k_lg_fill(DevTopo t, DevLg lg, DevWork w, zd* __restrict__ lu,
+                                                 int* __restrict__ lg_status) {
+  const int tile = blockIdx.y, s = sub();
+  if (blockIdx.x == 0 && threadIdx.x < kTile) {
+    const int sc = tile * kTile + s;
+    lg_status[sc] = sc < w.n_scen ? kRunning : kPadLane;
+  }
+  const size_t bb = at(tile, t.n_bus, 0) + s;
+  zd* out = lu + at(tile, lg.nnz_lu, 0) + s;
+  for (int p = elem_first(); p < lg.nnz_lu; p += elem_stride()) {
+    zd v{0.0, 0.0};
+    const int e = lg.src[p];
+    if (e >= 0) v = zd{t.y_g[e], t.y_b[e]};
+    const int d = lg.dbus[p];
+    if (d >= 0) {
+      const size_t o = bb + (size_t)d * kTile;
+      v.re += -w.ps[o];  // -conj(P + jQ) = -P + jQ
+      v.im += w.qs[o];
+    }
+    out[(size_t)p * kTile] = v;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_lg_rhs(DevTopo t, DevLg lg, DevWork w, zd* __restrict__ y) {
+  const int tile = blockIdx.y, s = sub();
+  const size_t bb = at(tile, t.n_bus, 0) + s;
+  zd* out = y + at(tile, lg.m, 0) + s;
+  for (int r = elem_first(); r < lg.m; r += elem_stride()) {
+    zd b{0.0, 0.0};
+    for (int q = lg.rhs_ptr[r]; q < lg.rhs_ptr[r + 1]; ++q) {
+      const int e = lg.rhs_e[q];
+      const size_t oj = bb + (size_t)lg.rhs_j[q] * kTile;
+      double sn, cs;
+      sincos(w.va[oj], &sn, &cs);
+      const zd u{w.vm[oj] * cs, w.vm[oj] * sn};
+      b = b - zd{t.y_g[e], t.y_b[e]} * u;
+    }
+    out[(size_t)lg.row_piv[r] * kTile] = b;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_lg_apply(DevTopo t, DevLg lg, DevWork w, const zd* __restrict__ y,
+                                                  const int* __restrict__ lg_status) {
+  const int tile = blockIdx.y, s = sub();
+  const int sc = tile * kTile + s;
+  if (lg_status[sc] != kRunning) return;  // singular or padding: keep the starting point
+  const size_t bb = at(tile, t.n_bus, 0) + s;
+  const zd* in = y + at(tile, lg.m, 0) + s;
+  for (int r = elem_first(); r < lg.m; r += elem_stride()) {
+    const zd u = in[(size_t)lg.row_x[r] * kTile];
+    const size_t o = bb + (size_t)lg.row_bus[r] * kTile;
+    w.vm[o] = hypot(u.re, u.im);
+    w.va[o] = atan2(u.im, u.re);
+  }
+}
+
 __global__ void k_finish(DevWork w) {
   const int sc = blockIdx.x * blockDim.x + threadIdx.x;
   if (sc >= w.n_tiles * kTile) return;
@@ -546,6 +624,30 @@ void launch_island_max(const DevTopo& t, const DevWork& w, const int* comp, int 
                        hipStream_t stream) {
   hipLaunchKernelGGL(k_island_max, dim3(elem_blocks(t.n_bus), w.n_tiles), dim3(256), 0, stream, t, w, comp, n_comp,
                      out);
+}
+
+void launch_linear_guess(const DevTopo& t, const DevLg& lg, const DevTopo& lgt, const DevWork& w, void* lu,
+                         void* y, int* lg_status, int* lg_iters, hipStream_t stream) {
+  if (lg.m <= 0) return;
+  hipLaunchKernelGGL(k_lg_fill, dim3(elem_blocks(lg.nnz_lu), w.n_tiles), dim3(256), 0, stream, t, lg, w,
+                     reinterpret_cast<zd*>(lu), lg_status);
+  hipLaunchKernelGGL(k_lg_rhs, dim3(elem_blocks(lg.m), w.n_tiles), dim3(256), 0, stream, t, lg, w,
+                     reinterpret_cast<zd*>(y));
+  DevWork lw = w;
+  lw.lu = lu;
+  lw.y = y;
+  lw.status = lg_status;
+  lw.iters = lg_iters;
+  const int cap = lgt.max_col < 1 ? 1 : lgt.max_col;
+  const size_t shmem = (size_t)cap * kStride * sizeof(zd);
+  hipLaunchKernelGGL((k_lu<zd, true>), dim3(w.n_tiles), dim3(64), shmem, stream, lgt, lw, cap, 0);
+  hipLaunchKernelGGL(k_lg_apply, dim3(elem_blocks(lg.m), w.n_tiles), dim3(256), 0, stream, t, lg, w,
+                     reinterpret_cast<const zd*>(y), lg_status);
+}
+
+size_t lg_shmem(const DevTopo& lgt) {
+  const int cap = lgt.max_col < 1 ? 1 : lgt.max_col;
+  return (size_t)cap * kStride * 2 * sizeof(double);
 }
 
 void launch_lane0(const double* src, double* dst, int count, bool to_tile, hipStream_t stream) {

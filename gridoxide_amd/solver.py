@@ -258,8 +258,9 @@ class Topology:
         check(_lib.lib().gx_topo_profile(self._h, int(enable), ptr(ms), ptr(n, _lib._i32p)))
         return {k: (float(a), int(b)) for k, a, b in zip(_lib.KERNEL_CLASSES, ms, n)}
 
-    def batch_solve(self, p_spec, q_spec, vm0, va0, tol=1e-8, max_iter=20, mode=Mode.FP64):
-        """Host arrays [n_scen, n_bus] -> (vm, va, status, iters, history)."""
+    def batch_solve(self, p_spec, q_spec, vm0, va0, tol=1e-8, max_iter=20, mode=Mode.FP64, init="given"):
+        """Host arrays [n_scen, n_bus] -> (vm, va, status, iters, history).  init: "given" starts
+        from vm0/va0; "linear" first applies linear_initial_guess on the device."""
         p_spec = f64(np.atleast_2d(p_spec))
         S = p_spec.shape[0]
         q_spec = f64(np.broadcast_to(q_spec, (S, self.n_bus)))
@@ -271,18 +272,28 @@ class Topology:
         it = np.zeros(S, np.int32)
         hist = np.zeros((S, max(max_iter, 1)))
         check(_lib.lib().gx_batch_solve(self._h, S, p_spec.ctypes.data, q_spec.ctypes.data, vm0.ctypes.data,
-                                        va0.ctypes.data, float(tol), int(max_iter), int(mode), 0, vm.ctypes.data,
+                                        va0.ctypes.data, float(tol), int(max_iter), int(mode), _init_flag(init),
+                                        vm.ctypes.data,
                                         va.ctypes.data, st.ctypes.data, it.ctypes.data, hist.ctypes.data, None))
         return vm, va, st, it, hist[:, :max_iter]
 
     def batch_solve_device(self, p_spec, q_spec, vm0, va0, vm, va, status, iters, history, tol=1e-8, max_iter=20,
-                           mode=Mode.FP64, stream=0):
+                           mode=Mode.FP64, stream=0, init="given"):
         """Device-resident variant: every argument is a CUDA/HIP tensor (torch) already in HBM."""
         S = int(p_spec.shape[0])
         check(_lib.lib().gx_batch_solve(self._h, S, p_spec.data_ptr(), q_spec.data_ptr(), vm0.data_ptr(),
-                                        va0.data_ptr(), float(tol), int(max_iter), int(mode), _lib.GX_DEVICE_PTRS,
+                                        va0.data_ptr(), float(tol), int(max_iter), int(mode),
+                                        _lib.GX_DEVICE_PTRS | _init_flag(init),
                                         vm.data_ptr(), va.data_ptr(), status.data_ptr(), iters.data_ptr(),
                                         history.data_ptr() if history is not None else None, stream or None))
+
+
+def _init_flag(init: str) -> int:
+    if init == "given":
+        return 0
+    if init == "linear":
+        return _lib.GX_INIT_LINEAR
+    raise ValueError(f"unknown init {init!r} (expected 'given' or 'linear')")
 
 
 # ---------------------------------------------------------------- reference-shaped solvers
@@ -321,12 +332,17 @@ class PersistentSolver:
 
 
 class BatchSolver:
-    """Scenario batch over one shared topology (src/batch.rs:152-289), all scenarios on the GPU."""
+    """Scenario batch over one shared topology (src/batch.rs:152-289), all scenarios on the GPU.
 
-    def __init__(self, backend: JacobianBackend = JacobianBackend.HIP, mode: Mode = Mode.FP64):
+    Like the reference, every scenario starts from linear_initial_guess (src/batch.rs:268) unless
+    ``init="given"`` asks for the template/override voltages as they are (flat start)."""
+
+    def __init__(self, backend: JacobianBackend = JacobianBackend.HIP, mode: Mode = Mode.FP64, init: str = "linear"):
         if backend is not JacobianBackend.HIP:
             raise ValueError("only the HIP backend is available")
+        _init_flag(init)
         self.mode = mode
+        self.init = init
 
     @staticmethod
     def scenario_arrays(template: Network, scenarios: Sequence[Scenario]):
@@ -355,11 +371,12 @@ class BatchSolver:
 
     def solve(self, template: Network, scenarios: Sequence[Scenario], tol: float, max_iter: int,
               topology: Optional[Topology] = None) -> List[PowerFlowReport]:
-        """Starting point: each scenario's own template voltages with overrides applied."""
+        """Starting point: the template voltages with overrides applied, then (by default) the
+        linear initial guess."""
         p, q, vm0, va0 = self.scenario_arrays(template, scenarios)
         if len(scenarios) == 0:
             return []
         topo = topology or Topology.from_network(template)
-        vm, va, st, it, hist = topo.batch_solve(p, q, vm0, va0, tol, max_iter, self.mode)
+        vm, va, st, it, hist = topo.batch_solve(p, q, vm0, va0, tol, max_iter, self.mode, init=self.init)
         isl = topo.island_reports(len(scenarios))
         return [PowerFlowReport(vm[s], va[s], _stats(st[s], it[s], hist[s]), isl[s]) for s in range(len(scenarios))]

@@ -39,6 +39,9 @@ extern "C" {
 
 /* ---- flags for gx_batch_solve ---- */
 #define GX_DEVICE_PTRS 1 /* every array argument is a device pointer (inputs already in HBM) */
+#define GX_INIT_LINEAR 2 /* start each scenario from linear_initial_guess (src/network.rs:627-678),
+                            computed on the device from vm0/va0, as BatchSolver does
+                            (src/batch.rs:268); without it vm0/va0 are used as given */
 
 const char* gx_version(void);
 const char* gx_last_error(void);
@@ -60,8 +63,8 @@ typedef struct gx_topo gx_topo;
 
 /* Replaces JacobianPattern::analyze (src/jacobian.rs:96-171) plus the symbolic half of
  * KluNativeSystem::new (build_csc_structure + analyze, src/klu_native/mod.rs:88-114,159-169;
- * analyze.rs:66-149) for one topology and bus-type assignment.  Uploads the Y-bus and pattern.
- * Returns NULL on error (*err set). */
+ * analyze.rs:66-149) for one topology and bus-type assignment.  Host-only: device buffers are
+ * allocated at the first solve.  Returns NULL on error (*err set). */
 gx_topo* gx_topo_create(int n_bus, const int* bus_type, const int* rowptr, const int* col, const double* y_re,
                         const double* y_im, int* err);
 void gx_topo_destroy(gx_topo* t);
@@ -100,7 +103,7 @@ int gx_topo_lu_pattern(const gx_topo* t, int* cp, int* ulen, int* row);
 /* Replaces BatchSolver::solve (src/batch.rs:198-289) driving PersistentSolver::solve_with_stats
  * (src/solver.rs:444-475) / newton_raphson_cached (src/solver.rs:639-736) once per scenario.
  * Inputs are scenario-major [n_scen][n_bus]: p_spec, q_spec (overrides already applied), and the
- * starting point vm0/va0 (the caller applies linear_initial_guess or flat start).  Outputs:
+ * starting point vm0/va0 (used as given, or refined by GX_INIT_LINEAR).  Outputs:
  * vm, va [n_scen][n_bus]; status, iters [n_scen]; history [n_scen][max_iter] (max |mismatch| at
  * the top of each iteration; entries past iters are unspecified).  Non-convergence is a status,
  * not an error (src/batch.rs:193-197).  mode: GX_MODE_FP64 / GX_MODE_MIXED.  flags:

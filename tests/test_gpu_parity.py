@@ -274,3 +274,37 @@ def test_islands_marked_and_reported(oracle_mod):
     assert [r.status for r in rep.islands] == [IslandStatus.Converged, IslandStatus.NoReferenceBus,
                                                IslandStatus.AmbiguousReferenceBus]
     assert rep.islands[1].slack_indices == [] and len(rep.islands[2].slack_indices) == 2
+
+
+def test_linear_initial_guess_three_bus_doc_values(three_bus, known):
+    """docs/src/solvers/klu.md:420-422: the linear guess of network.json (max_iter 0 returns it)."""
+    _say("case: 3-bus device linear initial guess")
+    net = three_bus
+    topo = Topology.from_network(net)
+    vm, va, st, it, _ = topo.batch_solve(net.p_spec[None], net.q_spec[None], net.vm[None], net.va[None],
+                                         tol=1e-6, max_iter=0, init="linear")
+    ka = known["three_bus"]
+    assert st[0] == 1 and it[0] == 0
+    np.testing.assert_allclose(vm[0], ka["linear_guess_vm"], atol=ka["linear_guess_tol"])
+    np.testing.assert_allclose(va[0], ka["linear_guess_va"], atol=ka["linear_guess_tol"])
+
+
+@pytest.mark.parametrize("mode", [Mode.FP64, Mode.MIXED])
+def test_linear_initial_guess_batch_vs_oracle(oracle_mod, mode):
+    """BatchSolver semantics (src/batch.rs:268): linear guess per scenario, then Newton."""
+    _say(f"case: config-1 grid, 40 scenarios, linear initial guess, mode={mode.name}")
+    net = config_grid(1)
+    p, q = scenario_specs(net, 40, seed=21)
+    vm0 = np.repeat(net.vm[None], 40, 0)
+    va0 = np.repeat(net.va[None], 40, 0)
+    topo = Topology.from_network(net)
+    # the guess alone
+    g = topo.batch_solve(p, q, vm0, va0, tol=MIS_TOL, max_iter=0, init="linear")
+    o = oracle_mod.Oracle(net)
+    for s in (0, 17, 39):
+        gvm, gva, rc = o.linear_initial_guess(p[s], q[s], vm0[s], va0[s])
+        assert rc == 0
+        assert np.abs(g[0][s] - gvm).max() < 1e-12 and np.abs(g[1][s] - gva).max() < 1e-12
+    gpu = topo.batch_solve(p, q, vm0, va0, tol=MIS_TOL, max_iter=20, mode=mode, init="linear")
+    ref = o.batch_solve(p, q, vm0, va0, tol=MIS_TOL, max_iter=20, init=1, threads=4)
+    _compare(gpu, ref, mixed=(mode == Mode.MIXED))
