@@ -47,6 +47,7 @@ SIGNATURES = {
     "gx_topo_lu_pattern": (ctypes.c_int, [_vp, _i32p, _i32p, _i32p]),
     "gx_batch_solve": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, _vp, _vp, ctypes.c_double, ctypes.c_int,
                                       ctypes.c_int, ctypes.c_int, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "gx_batch_fallbacks": (ctypes.c_int, [_vp]),
     "gx_topo_islands": (ctypes.c_int, [_vp, _i32p, _i32p]),
     "gx_batch_island_status": (ctypes.c_int, [_vp, _i32p]),
     "gx_topo_profile": (ctypes.c_int, [_vp, ctypes.c_int, _f64p, _i32p]),

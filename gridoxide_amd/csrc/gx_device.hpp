@@ -15,7 +15,7 @@ namespace gx {
 constexpr int kLanes = 64;  // wavefront width
 constexpr int kTile = 16;   // scenarios per tile
 
-enum Status : int { kConverged = 0, kMaxIter = 1, kSingular = 2, kRunning = 3, kPadLane = 4 };
+enum Status : int { kConverged = 0, kMaxIter = 1, kSingular = 2, kRunning = 3, kPadLane = 4, kStalled = 5 };
 
 // device bus codes: the reference's BusType (0 Slack, 1 PV, 2 PQ) plus kDeadBus for a member of
 // an island without a Slack bus (a Slack bus pinned to V = 0 with no injection)
@@ -98,6 +98,10 @@ struct DevWork {
   unsigned long long* maxmis = nullptr;  // max |mismatch| as ordered bits of a non-negative double
   double* history = nullptr;             // [n_scen][max_iter]
   int* running = nullptr;                // [max_iter] scenarios still running after each decision
+  int stall_check = 0;                   // mixed precision: stop a scenario whose mismatch stops falling
+  double* vm0 = nullptr;                 // starting point kept for the FP64 re-solve (mixed mode)
+  double* va0 = nullptr;
+  int* fallback = nullptr;               // [1] scenarios handed to the FP64 re-solve
 };
 
 // Launchers (gx_kernels.hip).  All run on `stream`; none synchronizes.  `fp32` selects the
@@ -112,6 +116,9 @@ void launch_lu(const DevTopo& t, const DevWork& w, bool fp32, bool factor, int i
 void launch_refine(const DevTopo& t, const DevWork& w, double* dvr, double* dvi, hipStream_t stream);
 void launch_update(const DevTopo& t, const DevWork& w, bool fp32, bool mixed, hipStream_t stream);
 void launch_finish(const DevWork& w, hipStream_t stream);
+// Mixed precision: restarts every scenario that did not converge from its starting point, for an
+// FP64 re-solve (status back to running, count in w.fallback[0]).
+void launch_fallback(const DevTopo& t, const DevWork& w, hipStream_t stream);
 void launch_lane0(const double* src, double* dst, int count, bool to_tile, hipStream_t stream);
 void launch_island_max(const DevTopo& t, const DevWork& w, const int* comp, int n_comp, unsigned long long* out,
                        hipStream_t stream);

@@ -196,7 +196,8 @@ struct gx_topo {
   int ws_tiles = 0, ws_iter = 0;
   int ws_fp32 = -1;
   DevMem w_vm, w_va, w_vr, w_vi, w_ps, w_qs, w_pc, w_qc, w_f, w_dx, w_y, w_lu, w_status, w_iters, w_max, w_hist;
-  DevMem w_dvr, w_dvi, w_stage, w_running;
+  DevMem w_dvr, w_dvi, w_stage, w_running, w_fallback, w_vm0, w_va0;
+  int last_fallback = 0;
   int* h_running = nullptr;  // pinned host mirror of the running counters
   int h_running_len = 0;
   // linear initial guess system (built on first use)
@@ -362,18 +363,18 @@ int ensure_workspace(gx_topo& t, int n_tiles, int max_iter, bool fp32) {
   if ((rc = t.w_vm.reserve(nb)) || (rc = t.w_va.reserve(nb)) || (rc = t.w_vr.reserve(nb)) ||
       (rc = t.w_vi.reserve(nb)) || (rc = t.w_ps.reserve(nb)) || (rc = t.w_qs.reserve(nb)) ||
       (rc = t.w_pc.reserve(nb)) || (rc = t.w_qc.reserve(nb)) || (rc = t.w_f.reserve(nu)) ||
-      (rc = t.w_dx.reserve(nu)) || (rc = t.w_y.reserve(lanes * t.n * es)) ||
+      (rc = t.w_dx.reserve(nu)) || (rc = t.w_y.reserve(lanes * t.n * sizeof(double))) ||
       (rc = t.w_status.reserve(lanes * sizeof(int))) ||
       (rc = t.w_iters.reserve(lanes * sizeof(int))) || (rc = t.w_max.reserve(lanes * sizeof(unsigned long long))) ||
       (rc = t.w_hist.reserve(lanes * (size_t)max_iter * sizeof(double))) ||
       (rc = t.w_running.reserve((size_t)max_iter * sizeof(int))))
     return rc;
-  if (t.h_running_len < max_iter) {
+  if (t.h_running_len < std::max(max_iter, 1)) {
     if (t.h_running) (void)hipHostFree(t.h_running);
     t.h_running = nullptr;
     t.h_running_len = 0;
-    GX_HIP(hipHostMalloc((void**)&t.h_running, sizeof(int) * max_iter, hipHostMallocDefault));
-    t.h_running_len = max_iter;
+    GX_HIP(hipHostMalloc((void**)&t.h_running, sizeof(int) * std::max(max_iter, 1), hipHostMallocDefault));
+    t.h_running_len = std::max(max_iter, 1);
   }
   if (fp32 && ((rc = t.w_dvr.reserve(nb)) || (rc = t.w_dvi.reserve(nb)))) return rc;
   t.ws_tiles = n_tiles;
@@ -732,6 +733,11 @@ gx_topo* gx_topo_create(int n_bus, const int* bus_type, const int* rowptr, const
 
 void gx_topo_destroy(gx_topo* t) { delete t; }
 
+int gx_batch_fallbacks(const gx_topo* t) {
+  if (!t) return fail(GX_ERR_ARG, "null");
+  return t->last_fallback;
+}
+
 int gx_topo_islands(const gx_topo* t, int* comp, int* verdict) {
   if (!t) return fail(GX_ERR_ARG, "null");
   if (comp) std::memcpy(comp, t->comp.data(), sizeof(int) * t->comp.size());
@@ -911,10 +917,21 @@ int gx_batch_solve(gx_topo* t, int n_scen, const double* p_spec, const double* q
     if ((rc = factor_at(*t, hvm.data() + off, hva.data() + off, hp.data(), hq.data()))) return rc;
   }
   if ((rc = upload_topo(*t))) return rc;
-  if (gx::lu_shmem(t->dt, fp32) > 65536)
+  if (gx::lu_shmem(t->dt, false) > 65536)
     return fail(GX_ERR_UNSUPPORTED, "longest L+U column exceeds the LDS staging limit of the device refactor");
-  if ((rc = t->w_lu.reserve((size_t)n_tiles * gx::kTile * (size_t)t->dt.nnz_lu * (fp32 ? 4 : 8)))) return rc;
+  // FP64-sized in both modes: the mixed mode's FP64 re-solve reuses the same buffer
+  if ((rc = t->w_lu.reserve((size_t)n_tiles * gx::kTile * (size_t)t->dt.nnz_lu * 8))) return rc;
   w.lu = t->w_lu.p;
+  if ((rc = t->w_fallback.reserve(sizeof(int)))) return rc;
+  w.fallback = t->w_fallback.as<int>();
+  if (fp32) {
+    const size_t nbytes = (size_t)n_tiles * gx::kTile * nb * sizeof(double);
+    if ((rc = t->w_vm0.reserve(nbytes)) || (rc = t->w_va0.reserve(nbytes))) return rc;
+    w.vm0 = t->w_vm0.as<double>();
+    w.va0 = t->w_va0.as<double>();
+    GX_HIP(hipMemcpyAsync(w.vm0, w.vm, nbytes, hipMemcpyDeviceToDevice, st));
+    GX_HIP(hipMemcpyAsync(w.va0, w.va, nbytes, hipMemcpyDeviceToDevice, st));
+  }
   gx::launch_init(t->dt, w, st);
   // optional per-launch timing of each kernel class on this stream
   auto timed = [&](int cls, auto&& launch) -> int {
@@ -931,26 +948,46 @@ int gx_batch_solve(gx_topo* t, int n_scen, const double* p_spec, const double* q
     t->prof.emplace_back(cls, a, b);
     return GX_OK;
   };
-  for (int it = 0; it < max_iter; ++it) {
-    if ((rc = timed(GX_K_MISMATCH, [&] { gx::launch_mismatch(t->dt, w, fp32, st); }))) return rc;
-    gx::launch_decide(t->dt, w, it, st);
-    // stop as soon as no scenario is left running (no empty launches after convergence)
-    GX_HIP(hipMemcpyAsync(t->h_running + it, w.running + it, sizeof(int), hipMemcpyDeviceToHost, st));
-    GX_HIP(hipStreamSynchronize(st));
-    if (t->h_running[it] == 0) break;
-    if ((rc = timed(GX_K_FILL, [&] { gx::launch_fill(t->dt, w, fp32, st); }))) return rc;
-    if ((rc = timed(GX_K_LU, [&] { gx::launch_lu(t->dt, w, fp32, true, it, st); }))) return rc;
-    if (fp32) {
-      if ((rc = timed(GX_K_REFINE, [&] {
-             gx::launch_refine(t->dt, w, t->w_dvr.as<double>(), t->w_dvi.as<double>(), st);
-           })))
-        return rc;
-      if ((rc = timed(GX_K_SOLVE, [&] { gx::launch_lu(t->dt, w, fp32, false, it, st); }))) return rc;
+  // the Newton loop (newton_raphson_cached, src/solver.rs:663-736) for every running scenario
+  auto newton = [&](bool lu32) -> int {
+    for (int it = 0; it < max_iter; ++it) {
+      int rc2;
+      if ((rc2 = timed(GX_K_MISMATCH, [&] { gx::launch_mismatch(t->dt, w, lu32, st); }))) return rc2;
+      gx::launch_decide(t->dt, w, it, st);
+      // stop as soon as no scenario is left running (no empty launches after convergence)
+      GX_HIP(hipMemcpyAsync(t->h_running + it, w.running + it, sizeof(int), hipMemcpyDeviceToHost, st));
+      GX_HIP(hipStreamSynchronize(st));
+      if (t->h_running[it] == 0) break;
+      if ((rc2 = timed(GX_K_FILL, [&] { gx::launch_fill(t->dt, w, lu32, st); }))) return rc2;
+      if ((rc2 = timed(GX_K_LU, [&] { gx::launch_lu(t->dt, w, lu32, true, it, st); }))) return rc2;
+      if (lu32) {
+        if ((rc2 = timed(GX_K_REFINE, [&] {
+               gx::launch_refine(t->dt, w, t->w_dvr.as<double>(), t->w_dvi.as<double>(), st);
+             })))
+          return rc2;
+        if ((rc2 = timed(GX_K_SOLVE, [&] { gx::launch_lu(t->dt, w, lu32, false, it, st); }))) return rc2;
+      }
+      if ((rc2 = timed(GX_K_UPDATE, [&] { gx::launch_update(t->dt, w, lu32, lu32, st); }))) return rc2;
     }
-    if ((rc = timed(GX_K_UPDATE, [&] { gx::launch_update(t->dt, w, fp32, fp32, st); }))) return rc;
+    gx::launch_finish(w, st);
+    GX_HIP(hipGetLastError());
+    return GX_OK;
+  };
+  t->last_fallback = 0;
+  if (fp32) {
+    w.stall_check = 1;
+    if ((rc = newton(true))) return rc;
+    // per-instance FP64 fallback: every scenario that stalled, hit max_iter or a singular FP32
+    // factor is solved again from its starting point with the FP64 path
+    w.stall_check = 0;
+    gx::launch_fallback(t->dt, w, st);
+    GX_HIP(hipMemcpyAsync(&t->h_running[0], w.fallback, sizeof(int), hipMemcpyDeviceToHost, st));
+    GX_HIP(hipStreamSynchronize(st));
+    t->last_fallback = t->h_running[0];
+    if (t->last_fallback > 0 && (rc = newton(false))) return rc;
+  } else {
+    if ((rc = newton(false))) return rc;
   }
-  gx::launch_finish(w, st);
-  GX_HIP(hipGetLastError());
   t->last = w;
   t->has_last = true;
 

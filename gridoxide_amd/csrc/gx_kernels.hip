@@ -112,8 +112,10 @@ __global__ __launch_bounds__(256) void k_init(DevTopo t, DevWork w) {
     w.iters[sc] = 0;
     w.maxmis[sc] = 0ull;
   }
-  if (blockIdx.x == 0 && tile == 0)
+  if (blockIdx.x == 0 && tile == 0) {
     for (int it = threadIdx.x; it < w.max_iter; it += blockDim.x) w.running[it] = 0;
+    if (threadIdx.x == 0 && w.fallback) *w.fallback = 0;
+  }
   const size_t bb = at(tile, t.n_bus, 0) + s;
   for (int i = elem_first(); i < t.n_bus; i += elem_stride()) {
     const size_t o = bb + (size_t)i * kTile;
@@ -187,9 +189,46 @@ __global__ void k_decide(DevWork w, int iter) {
   if (m < w.tol) {
     w.status[sc] = kConverged;
     w.iters[sc] = iter + 1;
+  } else if (w.stall_check && iter >= 1 && !(m < w.history[(size_t)sc * w.max_iter + iter - 1])) {
+    // mixed precision: the mismatch stopped falling; leave the scenario to the FP64 re-solve
+    w.status[sc] = kStalled;
+    w.iters[sc] = iter + 1;
   } else {
     atomicAdd(&w.running[iter], 1);
   }
+}
+
+// FP64 re-solve, step 1: every scenario that did not converge gets its starting point back.
+__global__ __launch_bounds__(256) void k_fallback_restore(DevTopo t, DevWork w) {
+  const int tile = blockIdx.y, s = sub();
+  const int sc = tile * kTile + s;
+  const int st = w.status[sc];
+  if (st == kConverged || st == kPadLane) return;  // no barrier follows
+  const size_t bb = at(tile, t.n_bus, 0) + s;
+  for (int i = elem_first(); i < t.n_bus; i += elem_stride()) {
+    const size_t o = bb + (size_t)i * kTile;
+    const double m = w.vm0[o], a = w.va0[o];
+    w.vm[o] = m;
+    w.va[o] = a;
+    double sn, cs;
+    sincos(a, &sn, &cs);
+    w.vr[o] = m * cs;
+    w.vi[o] = m * sn;
+  }
+}
+
+// FP64 re-solve, step 2 (after step 1 completed): those scenarios run again.
+__global__ void k_fallback_reset(DevWork w) {
+  const int sc = blockIdx.x * blockDim.x + threadIdx.x;
+  if (sc == 0)
+    for (int it = 0; it < w.max_iter; ++it) w.running[it] = 0;
+  if (sc >= w.n_tiles * kTile) return;
+  const int st = w.status[sc];
+  if (st == kConverged || st == kPadLane) return;
+  w.status[sc] = kRunning;
+  w.iters[sc] = 0;
+  w.maxmis[sc] = 0ull;
+  atomicAdd(w.fallback, 1);
 }
 
 // Jacobian values into the packed L+U storage, in a branch-free form of the eight formulas: with
@@ -613,6 +652,12 @@ void launch_update(const DevTopo& t, const DevWork& w, bool fp32, bool mixed, hi
   } else {
     hipLaunchKernelGGL((k_update<double, false>), grid, dim3(256), 0, stream, t, w);
   }
+}
+
+void launch_fallback(const DevTopo& t, const DevWork& w, hipStream_t stream) {
+  hipLaunchKernelGGL(k_fallback_restore, dim3(elem_blocks(t.n_bus), w.n_tiles), dim3(256), 0, stream, t, w);
+  const int threads = w.n_tiles * kTile;
+  hipLaunchKernelGGL(k_fallback_reset, dim3((threads + 255) / 256), dim3(256), 0, stream, w);
 }
 
 void launch_finish(const DevWork& w, hipStream_t stream) {

@@ -228,6 +228,10 @@ class Topology:
         check(_lib.lib().gx_topo_lu_pattern(self._h, ptr(cp, _lib._i32p), ptr(ulen, _lib._i32p), ptr(row, _lib._i32p)))
         return cp, ulen, row
 
+    def fallbacks(self) -> int:
+        """Scenarios of the last mixed-precision solve that were re-solved with the FP64 path."""
+        return check(_lib.lib().gx_batch_fallbacks(self._h))
+
     def islands(self):
         """(component of each bus, verdict of each component: 0 solvable, 1 no reference, 2 ambiguous)."""
         comp = np.zeros(self.n_bus, np.int32)

@@ -147,6 +147,11 @@ int gx_batch_island_status(gx_topo* t, int* out);
  * events on the solve's own stream. */
 int gx_topo_profile(gx_topo* t, int enable, double* ms, int* launches);
 
+/* Number of scenarios of the last GX_MODE_MIXED gx_batch_solve on t that were solved again with
+ * the FP64 path (their mismatch stopped falling, or they reached max_iter or a singular FP32
+ * factor); their reported status, iterations and history are the FP64 solve's. */
+int gx_batch_fallbacks(const gx_topo* t);
+
 /* ---- LinearSolver-level system ---- */
 typedef struct gx_sys gx_sys;
 /* Replaces LinearSolver::new for KluNative (src/solver.rs:35-57, src/klu_native/mod.rs:159-169):

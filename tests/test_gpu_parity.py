@@ -308,3 +308,24 @@ def test_linear_initial_guess_batch_vs_oracle(oracle_mod, mode):
     gpu = topo.batch_solve(p, q, vm0, va0, tol=MIS_TOL, max_iter=20, mode=mode, init="linear")
     ref = o.batch_solve(p, q, vm0, va0, tol=MIS_TOL, max_iter=20, init=1, threads=4)
     _compare(gpu, ref, mixed=(mode == Mode.MIXED))
+
+
+def test_mixed_fallback_reproduces_fp64_for_failing_scenarios():
+    """A scenario the mixed path cannot converge is re-solved with the FP64 path: its reported
+    result is then exactly the FP64 mode's, and the others keep their mixed-precision results."""
+    _say("case: config-1 grid, 16 scenarios, scenario 3 infeasible, mixed vs FP64")
+    net = config_grid(1)
+    p, q = scenario_specs(net, 16, seed=13)
+    p[3, 9] = -1e6
+    q[3, 9] = -1e6
+    vm0 = np.repeat(net.vm[None], 16, 0)
+    va0 = np.repeat(net.va[None], 16, 0)
+    topo = Topology.from_network(net)
+    mixed = topo.batch_solve(p, q, vm0, va0, tol=MIS_TOL, max_iter=20, mode=Mode.MIXED)
+    assert topo.fallbacks() >= 1
+    fp64 = topo.batch_solve(p, q, vm0, va0, tol=MIS_TOL, max_iter=20, mode=Mode.FP64)
+    assert mixed[2][3] == fp64[2][3] != 0 and mixed[3][3] == fp64[3][3]
+    assert np.array_equal(mixed[0][3], fp64[0][3]) and np.array_equal(mixed[1][3], fp64[1][3])
+    ok = [s for s in range(16) if s != 3]
+    assert np.all(mixed[2][ok] == 0)
+    assert np.abs(mixed[0][ok] - fp64[0][ok]).max() <= V_TOL
