@@ -72,6 +72,13 @@ def lib():
     if _LIB is None:
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(f"{LIB_PATH} is missing: build the HIP library first (python -m gridoxide_amd.build)")
+        # One HIP runtime per process: PyTorch ships its own libamdhip64 (same soname, other file
+        # name).  Loading torch first lets this library bind to that already-loaded runtime
+        # instead of pulling in a second one, which would leave torch without a usable device.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = ctypes.CDLL(LIB_PATH)
         for name, (res, args) in SIGNATURES.items():
             fn = getattr(L, name)

@@ -33,6 +33,8 @@ namespace gx {
 namespace {
 
 constexpr int kGroups = kLanes / kTile;
+// Mixed precision: stagnation below this mismatch (pu) is attributed to the FP32 factors.
+constexpr double kStallFloor = 1e-4;
 constexpr int kStride = kTile + 1;  // LDS row stride of a staged column (bank-conflict padding)
 
 __device__ __forceinline__ size_t at(int tile, int count, int e) {
@@ -189,8 +191,10 @@ __global__ void k_decide(DevWork w, int iter) {
   if (m < w.tol) {
     w.status[sc] = kConverged;
     w.iters[sc] = iter + 1;
-  } else if (w.stall_check && iter >= 1 && !(m < w.history[(size_t)sc * w.max_iter + iter - 1])) {
-    // mixed precision: the mismatch stopped falling; leave the scenario to the FP64 re-solve
+  } else if (w.stall_check && iter >= 1 && w.history[(size_t)sc * w.max_iter + iter - 1] < kStallFloor &&
+             !(m < w.history[(size_t)sc * w.max_iter + iter - 1])) {
+    // mixed precision: the mismatch stopped falling although it was already small (the FP32
+    // factors limit the step); leave the scenario to the FP64 re-solve
     w.status[sc] = kStalled;
     w.iters[sc] = iter + 1;
   } else {
@@ -203,7 +207,7 @@ __global__ __launch_bounds__(256) void k_fallback_restore(DevTopo t, DevWork w) 
   const int tile = blockIdx.y, s = sub();
   const int sc = tile * kTile + s;
   const int st = w.status[sc];
-  if (st == kConverged || st == kPadLane) return;  // no barrier follows
+  if (st != kStalled && st != kSingular) return;  // no barrier follows
   const size_t bb = at(tile, t.n_bus, 0) + s;
   for (int i = elem_first(); i < t.n_bus; i += elem_stride()) {
     const size_t o = bb + (size_t)i * kTile;
@@ -224,7 +228,7 @@ __global__ void k_fallback_reset(DevWork w) {
     for (int it = 0; it < w.max_iter; ++it) w.running[it] = 0;
   if (sc >= w.n_tiles * kTile) return;
   const int st = w.status[sc];
-  if (st == kConverged || st == kPadLane) return;
+  if (st != kStalled && st != kSingular) return;
   w.status[sc] = kRunning;
   w.iters[sc] = 0;
   w.maxmis[sc] = 0ull;

@@ -147,9 +147,11 @@ int gx_batch_island_status(gx_topo* t, int* out);
  * events on the solve's own stream. */
 int gx_topo_profile(gx_topo* t, int enable, double* ms, int* launches);
 
-/* Number of scenarios of the last GX_MODE_MIXED gx_batch_solve on t that were solved again with
- * the FP64 path (their mismatch stopped falling, or they reached max_iter or a singular FP32
- * factor); their reported status, iterations and history are the FP64 solve's. */
+/* Number of scenarios of the last GX_MODE_MIXED gx_batch_solve on t that were solved again from
+ * their starting point with the FP64 path: those whose mismatch stopped falling once below
+ * 1e-4 pu (the FP32 factors limit the step) and those with a singular FP32 factor.  Their
+ * reported status, iterations and history are the FP64 solve's.  A mixed-precision run that
+ * neither converges nor stalls keeps its MaxIterationsReached result. */
 int gx_batch_fallbacks(const gx_topo* t);
 
 /* ---- LinearSolver-level system ---- */

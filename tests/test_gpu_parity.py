@@ -57,7 +57,9 @@ def _compare(gpu, ref, mixed=False, check_hist=True):
         _say(f"  max|dVm|={dvm:.3e} max|dVa|={dva:.3e}")
         assert dvm <= V_TOL and dva <= V_TOL
     if check_hist and not mixed:
-        for s in range(len(st)):
+        # a diverging Newton run amplifies rounding differences chaotically, so the mismatch
+        # history is compared where the iteration converged
+        for s in np.nonzero(conv)[0]:
             n = it[s]
             np.testing.assert_allclose(hist[s, :n], rhist[s, :n], rtol=1e-6, atol=1e-12)
 
@@ -163,7 +165,8 @@ def test_batch_equals_sequential_persistent_solver():
     reports = BatchSolver().solve(net, scen, MIS_TOL, 20, topology=topo)
     p, q, vm0, va0 = BatchSolver.scenario_arrays(net, scen)
     for s in range(len(scen)):
-        one = topo.batch_solve(p[s:s + 1], q[s:s + 1], vm0[s:s + 1], va0[s:s + 1], tol=MIS_TOL, max_iter=20)
+        one = topo.batch_solve(p[s:s + 1], q[s:s + 1], vm0[s:s + 1], va0[s:s + 1], tol=MIS_TOL, max_iter=20,
+                               init="linear")
         assert reports[s].stats.status == SolveStatus.Converged
         assert np.array_equal(reports[s].voltage_mag, one[0][0])
         assert np.array_equal(reports[s].voltage_ang, one[1][0])
@@ -265,7 +268,7 @@ def test_islands_marked_and_reported(oracle_mod):
     from gridoxide_amd import IslandStatus
     net, n0 = _with_islands()
     plain = config_grid(1)
-    rep = BatchSolver().solve(net, [Scenario()], MIS_TOL, 20)[0]
+    rep = BatchSolver(init="given").solve(net, [Scenario()], MIS_TOL, 20)[0]
     ref = _oracle_batch(oracle_mod, plain, plain.p_spec[None], plain.q_spec[None], plain.vm[None], plain.va[None])
     assert rep.stats.status == SolveStatus.Converged
     assert np.abs(rep.voltage_mag[:n0] - ref[0][0]).max() <= V_TOL
