@@ -59,6 +59,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="config2", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--tile", type=int, default=0, help="scenarios per tile (4, 8, 16; 0 = automatic)")
+    ap.add_argument("--ring", type=int, default=0, help="LDS solve-window rows (power of two; 0 = default)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (scenario sharding across "
@@ -78,6 +80,7 @@ def main():
     vm0 = np.repeat(net.vm[None], S, 0)
     va0 = np.repeat(net.va[None], S, 0)
     topo = Topology.from_network(net)
+    topo.set_launch(args.tile, args.ring)
     dev = torch.device("cuda:0")
     tp, tq, tvm0, tva0 = (torch.from_numpy(np.ascontiguousarray(a)).to(dev) for a in (p, q, vm0, va0))
     vm = torch.empty_like(tvm0)

@@ -51,6 +51,7 @@ SIGNATURES = {
     "gx_topo_islands": (ctypes.c_int, [_vp, _i32p, _i32p]),
     "gx_batch_island_status": (ctypes.c_int, [_vp, _i32p]),
     "gx_topo_profile": (ctypes.c_int, [_vp, ctypes.c_int, _f64p, _i32p]),
+    "gx_topo_set_launch": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
     "gx_sys_create": (_vp, [ctypes.c_int, ctypes.c_int, _i32p, _i32p, _f64p]),
     "gx_sys_factor_and_solve_values": (ctypes.c_int, [_vp, _f64p, _f64p, _f64p]),
     "gx_sys_destroy": (None, [_vp]),

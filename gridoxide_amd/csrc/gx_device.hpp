@@ -1,8 +1,9 @@
 // Device-resident topology and per-batch workspace shared by the host driver and the HIP kernels.
 //
-// Layout: every per-scenario array is "scenario-minor interleaved" in tiles of kTile scenarios:
-// element e of scenario s lives at   base[(tile * count + e) * kTile + s % kTile],
-// tile = s / kTile, so the same element of a tile's scenarios is one contiguous segment.
+// Layout: every per-scenario array is "scenario-minor interleaved" in tiles of TW scenarios
+// (TW = DevWork::tile, 4, 8 or 16): element e of scenario s lives at
+//   base[(tile * count + e) * TW + s % TW],   tile = s / TW,
+// so the same element of a tile's scenarios is one contiguous segment.
 // The topology arrays (Y-bus, Jacobian recipes, LU pattern and update schedule) are shared
 // read-only by every scenario.
 #pragma once
@@ -12,8 +13,17 @@
 
 namespace gx {
 
-constexpr int kLanes = 64;  // wavefront width
-constexpr int kTile = 16;   // scenarios per tile
+constexpr int kLanes = 64;    // wavefront width
+constexpr int kMaxTile = 16;  // widest tile (scenarios per tile): 4, 8 or 16 are compiled
+constexpr int kRingGap = 32;  // solve ring: columns between fences / rows kept behind the front
+
+// Default rows of the LDS ring that holds the moving window of the right-hand side during the
+// triangular solves, for a tile width (a power of two; ring plus staged column stay within about
+// 48 KB of LDS for FP64).
+constexpr int ring_rows(int tile) { return tile <= 4 ? 1024 : (tile <= 8 ? 512 : 256); }
+// Pushes of a triangular solve that reach farther than this many rows are gathered instead.
+constexpr int ring_window_of(int rows) { return rows - kRingGap - 1; }
+constexpr int kMaxLds = 160 * 1024;  // LDS bytes per workgroup on gfx950
 
 enum Status : int { kConverged = 0, kMaxIter = 1, kSingular = 2, kRunning = 3, kPadLane = 4, kStalled = 5 };
 
@@ -55,6 +65,17 @@ struct DevTopo {
   // L(:,j) inside column k, stored contiguously from upd_ptr[position]
   const int* upd_ptr = nullptr;       // nnz_lu + 1 (only U positions own a range)
   const uint16_t* upd_off = nullptr;  // total = sum over U entries of |L(:,j)|
+  // per strictly-upper packed entry (j,k): {first L position of column j, |L(:,j)|, upd_ptr, j}
+  const int* rec = nullptr;           // 4 * nnz_lu
+  // triangular solves with an LDS window of `window` rows: the entries of L (by row) and
+  // of U (by row) that reach farther are gathered when their row enters the window
+  const int* fwd_ptr = nullptr;  // n + 1
+  const int* fwd_pos = nullptr;  // packed position of L(r, j)
+  const int* fwd_col = nullptr;  // j
+  const int* bwd_ptr = nullptr;  // n + 1
+  const int* bwd_pos = nullptr;  // packed position of U(r, j)
+  const int* bwd_col = nullptr;  // j
+  int window = 0;                // window W the far lists were built for (ring rows W + kRingGap + 1)
 };
 
 // The linear initial guess system of a topology (complex, m = number of PQ buses): recipes
@@ -74,11 +95,12 @@ struct DevLg {
 
 // Per-batch device workspace (sized for n_tiles tiles).
 struct DevWork {
+  int tile = 16;  // scenarios per tile (4, 8 or 16)
   int n_scen = 0;
   int n_tiles = 0;
   int max_iter = 0;
   double tol = 0.0;
-  // bus arrays, interleaved [tile][n_bus][kTile]
+  // bus arrays, interleaved [tile][n_bus][TW]
   double* vm = nullptr;
   double* va = nullptr;
   double* vr = nullptr;
@@ -87,11 +109,11 @@ struct DevWork {
   double* qs = nullptr;
   double* pc = nullptr;
   double* qc = nullptr;
-  // unknown arrays, interleaved [tile][n][kTile]
+  // unknown arrays, interleaved [tile][n][TW]
   double* f = nullptr;    // FP64 mismatch in pivot-row order
   double* dx = nullptr;   // FP64 Newton step in pivot-column order (mixed mode)
   void* y = nullptr;      // LU-precision right-hand side / solution
-  void* lu = nullptr;     // packed L+U values, [tile][nnz_lu][kTile]
+  void* lu = nullptr;     // packed L+U values, [tile][nnz_lu][TW]
   // per scenario
   int* status = nullptr;
   int* iters = nullptr;
@@ -106,8 +128,9 @@ struct DevWork {
 
 // Launchers (gx_kernels.hip).  All run on `stream`; none synchronizes.  `fp32` selects the
 // LU precision (float for the mixed-precision mode, double otherwise).
-void launch_to_tiles(const double* src, double* dst, int n_scen, int n_tiles, int count, hipStream_t stream);
-void launch_from_tiles(const double* src, double* dst, int n_scen, int count, hipStream_t stream);
+void launch_to_tiles(const double* src, double* dst, int n_scen, int n_tiles, int count, int tile,
+                     hipStream_t stream);
+void launch_from_tiles(const double* src, double* dst, int n_scen, int count, int tile, hipStream_t stream);
 void launch_init(const DevTopo& t, const DevWork& w, hipStream_t stream);
 void launch_mismatch(const DevTopo& t, const DevWork& w, bool fp32, hipStream_t stream);
 void launch_decide(const DevTopo& t, const DevWork& w, int iter, hipStream_t stream);
@@ -119,13 +142,14 @@ void launch_finish(const DevWork& w, hipStream_t stream);
 // Mixed precision: restarts every scenario that did not converge from its starting point, for an
 // FP64 re-solve (status back to running, count in w.fallback[0]).
 void launch_fallback(const DevTopo& t, const DevWork& w, hipStream_t stream);
-void launch_lane0(const double* src, double* dst, int count, bool to_tile, hipStream_t stream);
+void launch_lane0(const double* src, double* dst, int count, bool to_tile, int tile, hipStream_t stream);
 void launch_island_max(const DevTopo& t, const DevWork& w, const int* comp, int n_comp, unsigned long long* out,
                        hipStream_t stream);
 void launch_linear_guess(const DevTopo& t, const DevLg& lg, const DevTopo& lgt, const DevWork& w, void* lu,
                          void* y, int* lg_status, int* lg_iters, hipStream_t stream);
-size_t lg_shmem(const DevTopo& lgt);
-// Dynamic LDS bytes the factoring LU kernel needs for this topology.
-size_t lu_shmem(const DevTopo& t, bool fp32);
+// Dynamic LDS bytes of the factoring LU kernel for a topology (its window) and tile width:
+// complex (linear guess), FP32 or FP64 values.
+size_t lg_shmem(const DevTopo& lgt, int tile);
+size_t lu_shmem(const DevTopo& t, bool fp32, int tile);
 
 }  // namespace gx

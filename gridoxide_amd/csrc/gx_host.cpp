@@ -84,6 +84,7 @@ inline cx cconj(cx a) { return {a.re, -a.im}; }
 struct LuSchedule {
   std::vector<int> upd_ptr;
   std::vector<uint16_t> upd_off;
+  std::vector<int> rec;  // per U position (j,k): {first L position of column j, |L(:,j)|, upd_ptr, j}
   int max_col = 0;
   long long n_upd = 0;
 };
@@ -116,7 +117,128 @@ int build_schedule(const gx::LuPattern& lu, LuSchedule& s) {
   s.upd_ptr[nnz] = (int)s.upd_off.size();
   if (s.upd_off.size() >= (size_t)0x7fffffff) return fail(GX_ERR_UNSUPPORTED, "update schedule too large");
   s.n_upd = (long long)s.upd_off.size();
+  s.rec.assign(4 * (size_t)nnz, 0);
+  for (int k = 0; k < n; ++k)
+    for (int p = lu.cp[k]; p < lu.cp[k] + lu.ulen[k]; ++p) {
+      const int j = lu.row[p], l0 = lu.cp[j] + lu.ulen[j] + 1;
+      int* r = &s.rec[4 * (size_t)p];
+      r[0] = l0;
+      r[1] = lu.cp[j + 1] - l0;
+      r[2] = s.upd_ptr[p];
+      r[3] = j;
+    }
   return GX_OK;
+}
+
+// The entries the triangular solves gather instead of pushing, for an LDS window of W rows:
+// L(r, j) with r - j > W and U(r, j) with j - r > W, grouped by row r (ascending j).
+struct FarLists {
+  std::vector<int> fptr, fpos, fcol, bptr, bpos, bcol;
+};
+
+void build_far(const gx::LuPattern& lu, int W, FarLists& f) {
+  const int n = lu.n;
+  f.fptr.assign(n + 1, 0);
+  f.bptr.assign(n + 1, 0);
+  for (int j = 0; j < n; ++j) {
+    const int c0 = lu.cp[j], d = c0 + lu.ulen[j];
+    for (int p = c0; p < d; ++p)
+      if (j - lu.row[p] > W) f.bptr[lu.row[p] + 1]++;
+    for (int p = d + 1; p < lu.cp[j + 1]; ++p)
+      if (lu.row[p] - j > W) f.fptr[lu.row[p] + 1]++;
+  }
+  for (int r = 0; r < n; ++r) {
+    f.fptr[r + 1] += f.fptr[r];
+    f.bptr[r + 1] += f.bptr[r];
+  }
+  f.fpos.assign(f.fptr[n], 0);
+  f.fcol.assign(f.fptr[n], 0);
+  f.bpos.assign(f.bptr[n], 0);
+  f.bcol.assign(f.bptr[n], 0);
+  std::vector<int> fn(f.fptr.begin(), f.fptr.end() - 1), bn(f.bptr.begin(), f.bptr.end() - 1);
+  for (int j = 0; j < n; ++j) {
+    const int c0 = lu.cp[j], d = c0 + lu.ulen[j];
+    for (int p = c0; p < d; ++p)
+      if (j - lu.row[p] > W) {
+        const int q = bn[lu.row[p]]++;
+        f.bpos[q] = p;
+        f.bcol[q] = j;
+      }
+    for (int p = d + 1; p < lu.cp[j + 1]; ++p)
+      if (lu.row[p] - j > W) {
+        const int q = fn[lu.row[p]]++;
+        f.fpos[q] = p;
+        f.fcol[q] = j;
+      }
+  }
+}
+
+// Device copy of a pattern's numeric schedule, pointed to by the schedule fields of a DevTopo.
+struct DevSched {
+  DevMem cp, ulen, row, upd_ptr, upd_off, rec, fptr, fpos, fcol, bptr, bpos, bcol;
+  bool uploaded = false;
+  int window = -1;
+  void reset() {
+    uploaded = false;
+    window = -1;
+  }
+};
+
+// Uploads the pattern and update schedule once and the far lists whenever the window changes.
+int upload_sched(const gx::LuPattern& lu, const LuSchedule& s, int window, DevSched& m, gx::DevTopo& d) {
+  int rc;
+  if (!m.uploaded) {
+    if ((rc = m.cp.upload(lu.cp)) || (rc = m.ulen.upload(lu.ulen)) || (rc = m.row.upload(lu.row)) ||
+        (rc = m.upd_ptr.upload(s.upd_ptr)) || (rc = m.upd_off.upload(s.upd_off)) || (rc = m.rec.upload(s.rec)))
+      return rc;
+    m.uploaded = true;
+    m.window = -1;
+  }
+  if (m.window != window) {
+    FarLists f;
+    build_far(lu, window, f);
+    if ((rc = m.fptr.upload(f.fptr)) || (rc = m.fpos.upload(f.fpos)) || (rc = m.fcol.upload(f.fcol)) ||
+        (rc = m.bptr.upload(f.bptr)) || (rc = m.bpos.upload(f.bpos)) || (rc = m.bcol.upload(f.bcol)))
+      return rc;
+    m.window = window;
+  }
+  d.n = lu.n;
+  d.nnz_lu = lu.nnz();
+  d.max_col = s.max_col;
+  d.cp = m.cp.as<int>();
+  d.ulen = m.ulen.as<int>();
+  d.lu_row = m.row.as<int>();
+  d.upd_ptr = m.upd_ptr.as<int>();
+  d.upd_off = m.upd_off.as<uint16_t>();
+  d.rec = m.rec.as<int>();
+  d.fwd_ptr = m.fptr.as<int>();
+  d.fwd_pos = m.fpos.as<int>();
+  d.fwd_col = m.fcol.as<int>();
+  d.bwd_ptr = m.bptr.as<int>();
+  d.bwd_pos = m.bpos.as<int>();
+  d.bwd_col = m.bcol.as<int>();
+  d.window = window;
+  return GX_OK;
+}
+
+// Rows of the solve ring for a launch: the requested (or default) power of two, halved while the
+// LU kernel's LDS (staged column + ring, `elem` bytes per value) does not fit; -1 if nothing fits.
+int pick_ring_rows(int max_col, int tile, size_t elem, int requested) {
+  const size_t cap = std::max(max_col, 1);
+  auto bytes = [&](int rows) { return (cap + (size_t)rows) * (size_t)(tile + 1) * elem; };
+  int rows = requested > 0 ? requested : gx::ring_rows(tile);
+  while (rows > 64 && bytes(rows) > (size_t)gx::kMaxLds) rows /= 2;
+  return bytes(rows) <= (size_t)gx::kMaxLds ? rows : -1;
+}
+
+bool valid_tile(int tile) { return tile == 4 || tile == 8 || tile == 16; }
+bool valid_ring(int rows) { return rows == 0 || (rows >= 64 && rows <= 4096 && (rows & (rows - 1)) == 0); }
+
+// Scenarios per tile when the caller leaves it open: the widest tile that still gives at least
+// 512 wavefronts (two per CU), else the narrowest.
+int auto_tile(int n_scen) {
+  for (int tw : {16, 8}) if ((n_scen + tw - 1) / tw >= 512) return tw;
+  return 4;
 }
 
 // Position of (pivot row r, pivot column k) inside the packed column k, or -1.
@@ -190,10 +312,14 @@ struct gx_topo {
   bool uploaded = false;
   // device topology
   DevMem d_yrow, d_ycol, d_yg, d_yb, d_type, d_rpos_p, d_rpos_q, d_xpos_a, d_xpos_m;
-  DevMem d_kind, d_bi, d_bk, d_g, d_b, d_cp, d_ulen, d_row, d_upd_ptr, d_upd_off;
+  DevMem d_kind, d_bi, d_bk, d_g, d_b;
+  DevSched ds;
   gx::DevTopo dt;
+  // launch shape: scenarios per tile and solve ring rows (0 = automatic)
+  int tile_req = 0, ring_req = 0;
   // workspace
-  int ws_tiles = 0, ws_iter = 0;
+  size_t ws_lanes = 0;
+  int ws_iter = 0;
   int ws_fp32 = -1;
   DevMem w_vm, w_va, w_vr, w_vi, w_ps, w_qs, w_pc, w_qc, w_f, w_dx, w_y, w_lu, w_status, w_iters, w_max, w_hist;
   DevMem w_dvr, w_dvi, w_stage, w_running, w_fallback, w_vm0, w_va0;
@@ -206,7 +332,7 @@ struct gx_topo {
   LuSchedule lg_sched;
   std::vector<int> lg_src, lg_dbus, lg_row_bus, lg_row_piv, lg_row_x, lg_rhs_ptr, lg_rhs_e, lg_rhs_j;
   DevMem d_lg_src, d_lg_dbus, d_lg_row_bus, d_lg_row_piv, d_lg_row_x, d_lg_rhs_ptr, d_lg_rhs_e, d_lg_rhs_j;
-  DevMem d_lg_cp, d_lg_ulen, d_lg_row, d_lg_upd_ptr, d_lg_upd_off;
+  DevSched lg_ds;
   DevMem w_lg_lu, w_lg_y, w_lg_status, w_lg_iters;
   gx::DevLg dlg;
   gx::DevTopo dlgt;
@@ -314,26 +440,24 @@ int finalize_factor(gx_topo& t) {
   t.h_xm.swap(xm);
   t.factored = true;
   t.uploaded = false;
-  t.ws_tiles = 0;  // workspace layout depends on nnz(L+U)
+  t.ds.reset();
   return GX_OK;
 }
 
 int upload_ybus(gx_topo& t);
 
-// Uploads the topology (Y-bus, recipes, LU pattern and schedule) once per factorization.
-int upload_topo(gx_topo& t) {
-  if (t.uploaded) return GX_OK;
+// Uploads the topology (Y-bus, recipes, LU pattern and schedule) once per factorization, and the
+// far lists of the solves for the given window.
+int upload_topo(gx_topo& t, int window) {
   int rc;
+  if ((rc = upload_sched(t.lu, t.sched, window, t.ds, t.dt))) return rc;
+  if (t.uploaded) return GX_OK;
   if ((rc = upload_ybus(t)) || (rc = t.d_rpos_p.upload(t.h_rp)) ||
       (rc = t.d_rpos_q.upload(t.h_rq)) || (rc = t.d_xpos_a.upload(t.h_xa)) || (rc = t.d_xpos_m.upload(t.h_xm)) ||
       (rc = t.d_kind.upload(t.h_kind)) || (rc = t.d_bi.upload(t.h_bi)) || (rc = t.d_bk.upload(t.h_bk)) ||
-      (rc = t.d_g.upload(t.h_g)) || (rc = t.d_b.upload(t.h_b)) || (rc = t.d_cp.upload(t.lu.cp)) ||
-      (rc = t.d_ulen.upload(t.lu.ulen)) || (rc = t.d_row.upload(t.lu.row)) ||
-      (rc = t.d_upd_ptr.upload(t.sched.upd_ptr)) || (rc = t.d_upd_off.upload(t.sched.upd_off)))
+      (rc = t.d_g.upload(t.h_g)) || (rc = t.d_b.upload(t.h_b)))
     return rc;
   gx::DevTopo& d = t.dt;
-  d.nnz_lu = t.lu.nnz();
-  d.max_col = t.sched.max_col;
   d.rpos_p = t.d_rpos_p.as<int>();
   d.rpos_q = t.d_rpos_q.as<int>();
   d.xpos_a = t.d_xpos_a.as<int>();
@@ -343,19 +467,13 @@ int upload_topo(gx_topo& t) {
   d.lu_bk = t.d_bk.as<int>();
   d.lu_g = t.d_g.as<double>();
   d.lu_b = t.d_b.as<double>();
-  d.cp = t.d_cp.as<int>();
-  d.ulen = t.d_ulen.as<int>();
-  d.lu_row = t.d_row.as<int>();
-  d.upd_ptr = t.d_upd_ptr.as<int>();
-  d.upd_off = t.d_upd_off.as<uint16_t>();
   t.uploaded = true;
   return GX_OK;
 }
 
 // Per-batch buffers that do not depend on the factorization (the L+U buffer is reserved after it).
-int ensure_workspace(gx_topo& t, int n_tiles, int max_iter, bool fp32) {
-  if (n_tiles <= t.ws_tiles && max_iter <= t.ws_iter && (int)fp32 == t.ws_fp32) return GX_OK;
-  const size_t lanes = (size_t)n_tiles * gx::kTile;
+int ensure_workspace(gx_topo& t, size_t lanes, int max_iter, bool fp32) {
+  if (lanes <= t.ws_lanes && max_iter <= t.ws_iter && (int)fp32 == t.ws_fp32) return GX_OK;
   const size_t nb = lanes * t.n_bus * sizeof(double);
   const size_t nu = lanes * t.n * sizeof(double);
   const size_t es = fp32 ? sizeof(float) : sizeof(double);
@@ -377,7 +495,7 @@ int ensure_workspace(gx_topo& t, int n_tiles, int max_iter, bool fp32) {
     t.h_running_len = std::max(max_iter, 1);
   }
   if (fp32 && ((rc = t.w_dvr.reserve(nb)) || (rc = t.w_dvi.reserve(nb)))) return rc;
-  t.ws_tiles = n_tiles;
+  t.ws_lanes = lanes;
   t.ws_iter = max_iter;
   t.ws_fp32 = (int)fp32;
   return GX_OK;
@@ -403,7 +521,7 @@ int upload_ybus(gx_topo& t) {
 
 // The linear initial guess system (src/network.rs:627-678) for the bus types as given: the
 // reference runs the guess before islands are marked (src/batch.rs:268, src/solver.rs:452-454).
-int build_lg(gx_topo& t, int n_tiles) {
+int build_lg(gx_topo& t, size_t lanes, int tile) {
   int rc;
   if (!t.lg_built) {
     std::vector<int> pq, pos(t.n_bus, -1);
@@ -471,10 +589,7 @@ int build_lg(gx_topo& t, int n_tiles) {
       if ((rc = t.d_lg_src.upload(t.lg_src)) || (rc = t.d_lg_dbus.upload(t.lg_dbus)) ||
           (rc = t.d_lg_row_bus.upload(t.lg_row_bus)) || (rc = t.d_lg_row_piv.upload(t.lg_row_piv)) ||
           (rc = t.d_lg_row_x.upload(t.lg_row_x)) || (rc = t.d_lg_rhs_ptr.upload(t.lg_rhs_ptr)) ||
-          (rc = t.d_lg_rhs_e.upload(t.lg_rhs_e)) || (rc = t.d_lg_rhs_j.upload(t.lg_rhs_j)) ||
-          (rc = t.d_lg_cp.upload(t.lg_lu.cp)) || (rc = t.d_lg_ulen.upload(t.lg_lu.ulen)) ||
-          (rc = t.d_lg_row.upload(t.lg_lu.row)) || (rc = t.d_lg_upd_ptr.upload(t.lg_sched.upd_ptr)) ||
-          (rc = t.d_lg_upd_off.upload(t.lg_sched.upd_off)))
+          (rc = t.d_lg_rhs_e.upload(t.lg_rhs_e)) || (rc = t.d_lg_rhs_j.upload(t.lg_rhs_j)))
         return rc;
       gx::DevLg& l = t.dlg;
       l.nnz_lu = nlu;
@@ -486,22 +601,13 @@ int build_lg(gx_topo& t, int n_tiles) {
       l.rhs_ptr = t.d_lg_rhs_ptr.as<int>();
       l.rhs_e = t.d_lg_rhs_e.as<int>();
       l.rhs_j = t.d_lg_rhs_j.as<int>();
-      gx::DevTopo& d = t.dlgt;
-      d.n = m;
-      d.nnz_lu = nlu;
-      d.max_col = t.lg_sched.max_col;
-      d.cp = t.d_lg_cp.as<int>();
-      d.ulen = t.d_lg_ulen.as<int>();
-      d.lu_row = t.d_lg_row.as<int>();
-      d.upd_ptr = t.d_lg_upd_ptr.as<int>();
-      d.upd_off = t.d_lg_upd_off.as<uint16_t>();
-      if (gx::lg_shmem(d) > 65536)
-        return fail(GX_ERR_UNSUPPORTED, "linear guess L+U column exceeds the LDS staging limit");
     }
     t.lg_built = true;
   }
   if (t.dlg.m > 0) {
-    const size_t lanes = (size_t)n_tiles * gx::kTile;
+    const int rows = pick_ring_rows(t.lg_sched.max_col, tile, 16, 0);
+    if (rows < 0) return fail(GX_ERR_UNSUPPORTED, "linear guess L+U column exceeds the LDS staging limit");
+    if ((rc = upload_sched(t.lg_lu, t.lg_sched, gx::ring_window_of(rows), t.lg_ds, t.dlgt))) return rc;
     if ((rc = t.w_lg_lu.reserve(lanes * (size_t)t.dlg.nnz_lu * 16)) || (rc = t.w_lg_y.reserve(lanes * t.dlg.m * 16)) ||
         (rc = t.w_lg_status.reserve(lanes * sizeof(int))) || (rc = t.w_lg_iters.reserve(lanes * sizeof(int))))
       return rc;
@@ -552,6 +658,15 @@ int gx_topo_profile(gx_topo* t, int enable, double* ms, int* launches) {
   return GX_OK;
 }
 const char* gx_last_error(void) { return g_err.c_str(); }
+
+int gx_topo_set_launch(gx_topo* t, int tile, int ring_rows) {
+  if (!t) return fail(GX_ERR_ARG, "null");
+  if (!(tile == 0 || valid_tile(tile))) return fail(GX_ERR_ARG, "tile must be 0, 4, 8 or 16");
+  if (!valid_ring(ring_rows)) return fail(GX_ERR_ARG, "ring_rows must be 0 or a power of two in [64, 4096]");
+  t->tile_req = tile;
+  t->ring_req = ring_rows;
+  return GX_OK;
+}
 
 int gx_ybus_build(int n_bus, int n_line, const int* lf, const int* lt, const double* r, const double* x,
                   const double* bsh, const double* gsh, int n_tr, const int* tf, const int* tt, const double* ys,
@@ -842,9 +957,12 @@ int gx_batch_solve(gx_topo* t, int n_scen, const double* p_spec, const double* q
   const size_t plane = (size_t)n_scen * nb * sizeof(double);
   int rc;
   const bool fp32 = mode == GX_MODE_MIXED;
-  const int n_tiles = (n_scen + gx::kTile - 1) / gx::kTile;
-  if ((rc = upload_ybus(*t)) || (rc = ensure_workspace(*t, n_tiles, max_iter, fp32))) return rc;
+  const int tile = t->tile_req ? t->tile_req : auto_tile(n_scen);
+  const int n_tiles = (n_scen + tile - 1) / tile;
+  const size_t lanes = (size_t)n_tiles * tile;
+  if ((rc = upload_ybus(*t)) || (rc = ensure_workspace(*t, lanes, max_iter, fp32))) return rc;
   gx::DevWork w;
+  w.tile = tile;
   w.n_scen = n_scen;
   w.n_tiles = n_tiles;
   w.max_iter = max_iter;
@@ -876,11 +994,11 @@ int gx_batch_solve(gx_topo* t, int n_scen, const double* p_spec, const double* q
       GX_HIP(hipMemcpyAsync(t->w_stage.p, src[a], plane, hipMemcpyHostToDevice, st));
       s = t->w_stage.as<double>();
     }
-    gx::launch_to_tiles(s, dst[a], n_scen, n_tiles, nb, st);
+    gx::launch_to_tiles(s, dst[a], n_scen, n_tiles, nb, tile, st);
   }
   if (flags & GX_INIT_LINEAR) {
     // BatchSolver runs linear_initial_guess on every scenario first (src/batch.rs:268)
-    if ((rc = build_lg(*t, n_tiles))) return rc;
+    if ((rc = build_lg(*t, lanes, tile))) return rc;
     gx::launch_linear_guess(t->dt, t->dlg, t->dlgt, w, t->w_lg_lu.p, t->w_lg_y.p, t->w_lg_status.as<int>(),
                             t->w_lg_iters.as<int>(), st);
     GX_HIP(hipGetLastError());
@@ -888,11 +1006,11 @@ int gx_batch_solve(gx_topo* t, int n_scen, const double* p_spec, const double* q
   if (!t->factored) {
     // PersistentSolver's first factorization uses the first Jacobian it is asked to solve: the
     // first scenario whose starting point is not already converged (src/solver.rs:699-713).
-    gx::launch_from_tiles(w.vm, t->w_stage.as<double>(), n_scen, nb, st);
+    gx::launch_from_tiles(w.vm, t->w_stage.as<double>(), n_scen, nb, tile, st);
     std::vector<double> hvm((size_t)n_scen * nb);
     GX_HIP(hipMemcpyAsync(hvm.data(), t->w_stage.p, plane, hipMemcpyDeviceToHost, st));
     GX_HIP(hipStreamSynchronize(st));
-    gx::launch_from_tiles(w.va, t->w_stage.as<double>(), n_scen, nb, st);
+    gx::launch_from_tiles(w.va, t->w_stage.as<double>(), n_scen, nb, tile, st);
     std::vector<double> hva((size_t)n_scen * nb);
     GX_HIP(hipMemcpyAsync(hva.data(), t->w_stage.p, plane, hipMemcpyDeviceToHost, st));
     GX_HIP(hipStreamSynchronize(st));
@@ -916,16 +1034,18 @@ int gx_batch_solve(gx_topo* t, int n_scen, const double* p_spec, const double* q
     const size_t off = (size_t)chosen * nb;
     if ((rc = factor_at(*t, hvm.data() + off, hva.data() + off, hp.data(), hq.data()))) return rc;
   }
-  if ((rc = upload_topo(*t))) return rc;
-  if (gx::lu_shmem(t->dt, false) > 65536)
+  // the solve ring is sized for FP64 values: the mixed mode's FP64 re-solve uses the same lists
+  const int rows = pick_ring_rows(t->sched.max_col, tile, sizeof(double), t->ring_req);
+  if (rows < 0)
     return fail(GX_ERR_UNSUPPORTED, "longest L+U column exceeds the LDS staging limit of the device refactor");
+  if ((rc = upload_topo(*t, gx::ring_window_of(rows)))) return rc;
   // FP64-sized in both modes: the mixed mode's FP64 re-solve reuses the same buffer
-  if ((rc = t->w_lu.reserve((size_t)n_tiles * gx::kTile * (size_t)t->dt.nnz_lu * 8))) return rc;
+  if ((rc = t->w_lu.reserve(lanes * (size_t)t->dt.nnz_lu * 8))) return rc;
   w.lu = t->w_lu.p;
   if ((rc = t->w_fallback.reserve(sizeof(int)))) return rc;
   w.fallback = t->w_fallback.as<int>();
   if (fp32) {
-    const size_t nbytes = (size_t)n_tiles * gx::kTile * nb * sizeof(double);
+    const size_t nbytes = lanes * nb * sizeof(double);
     if ((rc = t->w_vm0.reserve(nbytes)) || (rc = t->w_va0.reserve(nbytes))) return rc;
     w.vm0 = t->w_vm0.as<double>();
     w.va0 = t->w_va0.as<double>();
@@ -996,9 +1116,9 @@ int gx_batch_solve(gx_topo* t, int n_scen, const double* p_spec, const double* q
   const double* tiles[2] = {w.vm, w.va};
   for (int a = 0; a < 2; ++a) {
     if (dev) {
-      gx::launch_from_tiles(tiles[a], outs[a], n_scen, nb, st);
+      gx::launch_from_tiles(tiles[a], outs[a], n_scen, nb, tile, st);
     } else {
-      gx::launch_from_tiles(tiles[a], t->w_stage.as<double>(), n_scen, nb, st);
+      gx::launch_from_tiles(tiles[a], t->w_stage.as<double>(), n_scen, nb, tile, st);
       GX_HIP(hipMemcpyAsync(outs[a], t->w_stage.p, plane, hipMemcpyDeviceToHost, st));
     }
   }
@@ -1023,7 +1143,8 @@ struct gx_sys {
   gx::Ordering ord;
   gx::LuPattern lu;
   LuSchedule sched;
-  DevMem d_cp, d_ulen, d_row, d_upd_ptr, d_upd_off, d_lu, d_y, d_status, d_iters, d_vals, d_rhs;
+  DevSched ds;
+  DevMem d_lu, d_y, d_status, d_iters, d_vals, d_rhs;
   gx::DevTopo dt;
   gx::DevWork w;
 };
@@ -1066,28 +1187,24 @@ gx_sys* gx_sys_create(int n, int nnz, const int* rows, const int* cols, const do
 namespace {
 
 // Device resources of a system, allocated at its first solve.
+// One system is solved as lane 0 of a single tile of kSysTile scenarios.
+constexpr int kSysTile = 4;
+
 int sys_to_device(gx_sys* s) {
   if (s->on_device) return GX_OK;
   const int n = s->n, nlu = s->lu.nnz();
+  const int rows = pick_ring_rows(s->sched.max_col, kSysTile, sizeof(double), 0);
+  if (rows < 0) return fail(GX_ERR_UNSUPPORTED, "longest L+U column exceeds the LDS staging limit");
   int rc;
-  if ((rc = s->d_cp.upload(s->lu.cp)) || (rc = s->d_ulen.upload(s->lu.ulen)) || (rc = s->d_row.upload(s->lu.row)) ||
-      (rc = s->d_upd_ptr.upload(s->sched.upd_ptr)) || (rc = s->d_upd_off.upload(s->sched.upd_off)) ||
-      (rc = s->d_lu.reserve((size_t)nlu * gx::kTile * sizeof(double))) ||
-      (rc = s->d_y.reserve((size_t)n * gx::kTile * sizeof(double))) ||
-      (rc = s->d_status.reserve(gx::kTile * sizeof(int))) || (rc = s->d_iters.reserve(gx::kTile * sizeof(int))) ||
+  if ((rc = upload_sched(s->lu, s->sched, gx::ring_window_of(rows), s->ds, s->dt)) ||
+      (rc = s->d_lu.reserve((size_t)nlu * kSysTile * sizeof(double))) ||
+      (rc = s->d_y.reserve((size_t)n * kSysTile * sizeof(double))) ||
+      (rc = s->d_status.reserve(kSysTile * sizeof(int))) || (rc = s->d_iters.reserve(kSysTile * sizeof(int))) ||
       (rc = s->d_vals.reserve((size_t)nlu * sizeof(double))) || (rc = s->d_rhs.reserve((size_t)n * sizeof(double))))
     return rc;
   GX_HIP(hipMemset(s->d_lu.p, 0, s->d_lu.bytes));
   GX_HIP(hipMemset(s->d_y.p, 0, s->d_y.bytes));
-  gx::DevTopo& d = s->dt;
-  d.n = n;
-  d.nnz_lu = nlu;
-  d.max_col = s->sched.max_col;
-  d.cp = s->d_cp.as<int>();
-  d.ulen = s->d_ulen.as<int>();
-  d.lu_row = s->d_row.as<int>();
-  d.upd_ptr = s->d_upd_ptr.as<int>();
-  d.upd_off = s->d_upd_off.as<uint16_t>();
+  s->w.tile = kSysTile;
   s->w.n_scen = 1;
   s->w.n_tiles = 1;
   s->w.lu = s->d_lu.p;
@@ -1120,15 +1237,15 @@ int gx_sys_factor_and_solve_values(gx_sys* s, const double* vals, const double* 
   std::vector<double> packed(nlu, 0.0), b(n), out(n);
   for (size_t e = 0; e < s->n_trip; ++e) packed[s->csc2lu[s->slot[e]]] += vals[e];
   for (int r = 0; r < n; ++r) b[s->lu.pinv[r]] = rhs[r];
-  std::vector<int> st(gx::kTile, gx::kPadLane);
+  std::vector<int> st(kSysTile, gx::kPadLane);
   st[0] = gx::kRunning;
   GX_HIP(hipMemcpy(s->d_vals.p, packed.data(), sizeof(double) * nlu, hipMemcpyHostToDevice));
   GX_HIP(hipMemcpy(s->d_rhs.p, b.data(), sizeof(double) * n, hipMemcpyHostToDevice));
-  GX_HIP(hipMemcpy(s->d_status.p, st.data(), sizeof(int) * gx::kTile, hipMemcpyHostToDevice));
-  gx::launch_lane0(s->d_vals.as<double>(), s->d_lu.as<double>(), nlu, true, nullptr);
-  gx::launch_lane0(s->d_rhs.as<double>(), s->d_y.as<double>(), n, true, nullptr);
+  GX_HIP(hipMemcpy(s->d_status.p, st.data(), sizeof(int) * kSysTile, hipMemcpyHostToDevice));
+  gx::launch_lane0(s->d_vals.as<double>(), s->d_lu.as<double>(), nlu, true, kSysTile, nullptr);
+  gx::launch_lane0(s->d_rhs.as<double>(), s->d_y.as<double>(), n, true, kSysTile, nullptr);
   gx::launch_lu(s->dt, s->w, false, true, 0, nullptr);
-  gx::launch_lane0(s->d_y.as<double>(), s->d_rhs.as<double>(), n, false, nullptr);
+  gx::launch_lane0(s->d_y.as<double>(), s->d_rhs.as<double>(), n, false, kSysTile, nullptr);
   GX_HIP(hipGetLastError());
   GX_HIP(hipMemcpy(b.data(), s->d_rhs.p, sizeof(double) * n, hipMemcpyDeviceToHost));
   GX_HIP(hipMemcpy(st.data(), s->d_status.p, sizeof(int), hipMemcpyDeviceToHost));

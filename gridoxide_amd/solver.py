@@ -262,6 +262,11 @@ class Topology:
         check(_lib.lib().gx_topo_profile(self._h, int(enable), ptr(ms), ptr(n, _lib._i32p)))
         return {k: (float(a), int(b)) for k, a, b in zip(_lib.KERNEL_CLASSES, ms, n)}
 
+    def set_launch(self, tile: int = 0, ring_rows: int = 0):
+        """Scenarios per tile (4, 8, 16; 0 = automatic) and LDS solve-window rows (power of two,
+        0 = default) for later solves."""
+        check(_lib.lib().gx_topo_set_launch(self._h, int(tile), int(ring_rows)))
+
     def batch_solve(self, p_spec, q_spec, vm0, va0, tol=1e-8, max_iter=20, mode=Mode.FP64, init="given"):
         """Host arrays [n_scen, n_bus] -> (vm, va, status, iters, history).  init: "given" starts
         from vm0/va0; "linear" first applies linear_initial_guess on the device."""

@@ -147,6 +147,15 @@ int gx_batch_island_status(gx_topo* t, int* out);
  * events on the solve's own stream. */
 int gx_topo_profile(gx_topo* t, int enable, double* ms, int* launches);
 
+/* Launch shape of later gx_batch_solve calls on t (no counterpart in the reference, whose CPU
+ * workers have no such knob; this is the setting a tuner sweeps).  tile: scenarios per
+ * wavefront tile, 4, 8 or 16 (0 = automatic: the widest tile that still yields 512 tiles).
+ * ring_rows: rows of the LDS window that holds the right-hand side during the triangular solves,
+ * a power of two in [64, 4096] (0 = the default for the tile); it is halved while the kernel's
+ * LDS would exceed 160 KB.  Results do not depend on either setting beyond rounding of the
+ * triangular-solve sums.  GX_ERR_ARG for other values. */
+int gx_topo_set_launch(gx_topo* t, int tile, int ring_rows);
+
 /* Number of scenarios of the last GX_MODE_MIXED gx_batch_solve on t that were solved again from
  * their starting point with the FP64 path: those whose mismatch stopped falling once below
  * 1e-4 pu (the FP32 factors limit the step) and those with a singular FP32 factor.  Their

@@ -153,3 +153,13 @@ def test_island_classification(oracle_mod):
     # unreferenced island: its buses carry no unknowns (marked Slack)
     plain = Topology.from_network(config_grid(1))
     assert topo.sizes()["n"] == plain.sizes()["n"] + 2  # island B: its one PQ bus (angle + magnitude)
+
+
+def test_set_launch_validates():
+    topo = Topology.from_network(config_grid(1))
+    topo.set_launch(8, 256)
+    topo.set_launch(0, 0)
+    for bad in ((5, 0), (32, 0), (4, 100), (4, 32), (16, 8192)):
+        with pytest.raises(_lib.GxError) as e:
+            topo.set_launch(*bad)
+        assert e.value.code == _lib.GX_ERR_ARG

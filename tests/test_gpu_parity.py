@@ -95,28 +95,38 @@ def test_three_bus_linear_start_matches_doc_step(three_bus, known, oracle_mod):
     assert abs(dvm[2] - ka["dx_iter1"][2]) < ka["printed_tol"]
 
 
+# launch shapes: (scenarios per tile, solve ring rows); 0 = automatic.  A 64-row ring gathers
+# every triangular-solve entry that reaches farther than 31 rows.
+SHAPES_C1 = [(0, 0), (8, 0), (16, 64)]
+SHAPES_C2 = [(0, 0), (16, 0), (4, 64)]
+
+
+@pytest.mark.parametrize("shape", SHAPES_C1)
 @pytest.mark.parametrize("mode", [Mode.FP64, Mode.MIXED])
-def test_config1_batch_vs_oracle(oracle_mod, mode):
-    _say(f"case: config-1 grid (200 buses), 70 scenarios, flat start, mode={mode.name}")
+def test_config1_batch_vs_oracle(oracle_mod, mode, shape):
+    _say(f"case: config-1 grid (200 buses), 70 scenarios, flat start, mode={mode.name}, tile/ring={shape}")
     net = config_grid(1)
     p, q = scenario_specs(net, 70, seed=7)
     vm0 = np.repeat(net.vm[None], 70, 0)
     va0 = np.repeat(net.va[None], 70, 0)
     topo = Topology.from_network(net)
+    topo.set_launch(*shape)
     gpu = topo.batch_solve(p, q, vm0, va0, tol=MIS_TOL, max_iter=20, mode=mode)
     ref = _oracle_batch(oracle_mod, net, p, q, vm0, va0)
     assert np.all(ref[2] == 0)
     _compare(gpu, ref, mixed=(mode == Mode.MIXED))
 
 
+@pytest.mark.parametrize("shape", SHAPES_C2)
 @pytest.mark.parametrize("mode", [Mode.FP64, Mode.MIXED])
-def test_config2_subset_vs_oracle(oracle_mod, mode):
-    _say(f"case: config-2 grid (2000 buses), 128 scenarios, flat start, mode={mode.name}")
+def test_config2_subset_vs_oracle(oracle_mod, mode, shape):
+    _say(f"case: config-2 grid (2000 buses), 128 scenarios, flat start, mode={mode.name}, tile/ring={shape}")
     net = config_grid(2)
     p, q = scenario_specs(net, 128, seed=11)
     vm0 = np.repeat(net.vm[None], 128, 0)
     va0 = np.repeat(net.va[None], 128, 0)
     topo = Topology.from_network(net)
+    topo.set_launch(*shape)
     gpu = topo.batch_solve(p, q, vm0, va0, tol=MIS_TOL, max_iter=20, mode=mode)
     ref = _oracle_batch(oracle_mod, net, p, q, vm0, va0)
     _compare(gpu, ref, mixed=(mode == Mode.MIXED))
@@ -314,15 +324,17 @@ def test_linear_initial_guess_batch_vs_oracle(oracle_mod, mode):
 
 
 def test_mixed_fallback_reproduces_fp64_for_failing_scenarios():
-    """A scenario the mixed path cannot converge is re-solved with the FP64 path: its reported
-    result is then exactly the FP64 mode's, and the others keep their mixed-precision results."""
-    _say("case: config-1 grid, 16 scenarios, scenario 3 infeasible, mixed vs FP64")
+    """A scenario whose FP32 factorization fails is re-solved with the FP64 path: its reported
+    result is then exactly the FP64 mode's, and the others keep their mixed-precision results.
+    Scenario 3 starts with |V| = 1e20 at one PQ bus: its Jacobian entries (~1e41) overflow FP32,
+    so the FP32 factor is non-finite (Singular), while FP64 represents them."""
+    _say("case: config-1 grid, 16 scenarios, scenario 3 overflows FP32, mixed vs FP64")
     net = config_grid(1)
     p, q = scenario_specs(net, 16, seed=13)
-    p[3, 9] = -1e6
-    q[3, 9] = -1e6
     vm0 = np.repeat(net.vm[None], 16, 0)
     va0 = np.repeat(net.va[None], 16, 0)
+    bus = int(np.nonzero(net.bus_type == 2)[0][0])
+    vm0[3, bus] = 1e20
     topo = Topology.from_network(net)
     mixed = topo.batch_solve(p, q, vm0, va0, tol=MIS_TOL, max_iter=20, mode=Mode.MIXED)
     assert topo.fallbacks() >= 1
