@@ -13,17 +13,31 @@
 
 namespace gx {
 
-constexpr int kLanes = 64;    // wavefront width
-constexpr int kMaxTile = 16;  // widest tile (scenarios per tile): 4, 8 or 16 are compiled
-constexpr int kRingGap = 32;  // solve ring: columns between fences / rows kept behind the front
+constexpr int kLanes = 64;        // wavefront width
+constexpr int kMaxTile = 16;      // widest tile (scenarios per tile): 4, 8 or 16 are compiled
+constexpr int kRingGap = 32;      // solve ring: columns between fences / rows kept behind the front
+constexpr int kChunkRounds = 16;  // refactor update rounds fetched into registers at a time
+constexpr int kMaxLds = 160 * 1024;  // LDS bytes per workgroup on gfx950
+
+constexpr int kWideTile = 64;     // one lane per scenario: every index of the LU kernel is uniform
+constexpr int kRoundItems = 16;   // update items per round of the wide-tile LU kernel
+constexpr int kWideStage = 16;    // entries of a column the wide-tile LU kernel prefetches
+constexpr int kWideFar = 4;       // far entries of an entering row the wide-tile LU kernel prefetches
 
 // Default rows of the LDS ring that holds the moving window of the right-hand side during the
-// triangular solves, for a tile width (a power of two; ring plus staged column stay within about
-// 48 KB of LDS for FP64).
-constexpr int ring_rows(int tile) { return tile <= 4 ? 1024 : (tile <= 8 ? 512 : 256); }
-// Pushes of a triangular solve that reach farther than this many rows are gathered instead.
-constexpr int ring_window_of(int rows) { return rows - kRingGap - 1; }
-constexpr int kMaxLds = 160 * 1024;  // LDS bytes per workgroup on gfx950
+// triangular solves, for a tile width (a power of two).
+constexpr int ring_rows(int tile) { return tile == kWideTile ? 256 : (tile <= 8 ? 512 : 256); }
+// Pushes of a triangular solve that reach farther than this many rows are gathered instead.  The
+// wide tile keeps no rows behind the front (a lane only reads its own scenario's values).
+constexpr int ring_window_of(int rows, int tile) { return tile == kWideTile ? rows - 1 : rows - kRingGap - 1; }
+// LDS bytes of the LU kernel for a tile width: the staged column(s) of max_col + 1 rows (factoring
+// only; two for the narrow tiles) and the solve ring, each row holding tile (+1 padding on narrow
+// tiles) values of elem bytes.
+constexpr size_t lu_lds_bytes(int max_col, int rows, int tile, size_t elem, bool factor) {
+  return tile == kWideTile
+             ? ((factor ? (size_t)(max_col < 1 ? 1 : max_col) + 1 : 0) + (size_t)rows) * (size_t)tile * elem
+             : ((factor ? 2 * ((size_t)(max_col < 1 ? 1 : max_col) + 1) : 0) + (size_t)rows) * (size_t)(tile + 1) * elem;
+}
 
 enum Status : int { kConverged = 0, kMaxIter = 1, kSingular = 2, kRunning = 3, kPadLane = 4, kStalled = 5 };
 
@@ -61,12 +75,39 @@ struct DevTopo {
   const int* cp = nullptr;      // n+1
   const int* ulen = nullptr;    // n
   const int* lu_row = nullptr;  // nnz_lu, pivot-step row of each packed entry
-  // for every strictly-upper packed entry (j,k): offsets (relative to cp[k]) of the rows of
-  // L(:,j) inside column k, stored contiguously from upd_ptr[position]
-  const int* upd_ptr = nullptr;       // nnz_lu + 1 (only U positions own a range)
-  const uint16_t* upd_off = nullptr;  // total = sum over U entries of |L(:,j)|
-  // per strictly-upper packed entry (j,k): {first L position of column j, |L(:,j)|, upd_ptr, j}
-  const int* rec = nullptr;           // 4 * nnz_lu
+  // refactor updates, scheduled for G = 64 / tile lane groups: the updates of column k
+  // (x_i -= L(i,j) * U(j,k)) in rounds[k] rounds of G slots, one per lane group, stored in chunks of
+  // kChunkRounds rounds; every column owns at least one chunk.  Slot: {x, word}, word = target
+  // offset (bits 0-15) | source U offset (bits 16-30) | bit 31 set when L(i,j) is read from the
+  // previous column's LDS copy at offset x (else x is its packed position; -1 for an empty slot)
+  const int* chunk0 = nullptr;  // n + 1: first chunk of column k
+  const int* rounds = nullptr;  // n
+  const int2* slots = nullptr;  // chunk0[n] * kChunkRounds * G
+  int groups = 0;               // G the rounds were built for
+  // per column k, for the window and G above:
+  //   col_a = {cp[k], column length, ulen[k], chunk0[k]}
+  //   col_f = {rounds[k], range [q0, q1) of fo_pos/fo_col: the far L entries of the row
+  //            r = k + W + 1 entering the forward window after column k that fslot does not hold, 0}
+  //   col_b = {far U range [bwd_ptr[r], bwd_ptr[r+1]) of the row r = k - W - 1 entering the
+  //            backward window after column k (empty if r < 0), 0, 0}
+  const int4* col_a = nullptr;
+  const int4* col_f = nullptr;
+  const int4* col_b = nullptr;
+  // far L entries of the row entering the forward window after column k: fslot[k * G + g] = the
+  // first entry {packed position, column} whose position is g mod G ({-1, -1} if none)
+  const int2* fslot = nullptr;
+  const int* fo_pos = nullptr;
+  const int* fo_col = nullptr;
+  // wide tile: the updates of column k in rounds [uround[k], uround[k+1]) of kRoundItems items
+  // {target offset | source U offset << 16, packed position of L(i,j)} (position -1: empty item
+  // aimed at the unused row just past the column); items of a round have distinct targets and
+  // read no target of the same round; every column owns at least one round, and the first round
+  // of column k reads no L(:,k-1)
+  const int* uround = nullptr;
+  const int2* uitems = nullptr;
+  // wide tile: first kWideFar far entries {position, column} of the rows entering the backward
+  // window after column k (forward: fslot); col_f / col_b then hold the ranges of the others
+  const int2* bslot = nullptr;
   // triangular solves with an LDS window of `window` rows: the entries of L (by row) and
   // of U (by row) that reach farther are gathered when their row enters the window
   const int* fwd_ptr = nullptr;  // n + 1

@@ -84,7 +84,6 @@ inline cx cconj(cx a) { return {a.re, -a.im}; }
 struct LuSchedule {
   std::vector<int> upd_ptr;
   std::vector<uint16_t> upd_off;
-  std::vector<int> rec;  // per U position (j,k): {first L position of column j, |L(:,j)|, upd_ptr, j}
   int max_col = 0;
   long long n_upd = 0;
 };
@@ -117,16 +116,86 @@ int build_schedule(const gx::LuPattern& lu, LuSchedule& s) {
   s.upd_ptr[nnz] = (int)s.upd_off.size();
   if (s.upd_off.size() >= (size_t)0x7fffffff) return fail(GX_ERR_UNSUPPORTED, "update schedule too large");
   s.n_upd = (long long)s.upd_off.size();
-  s.rec.assign(4 * (size_t)nnz, 0);
-  for (int k = 0; k < n; ++k)
-    for (int p = lu.cp[k]; p < lu.cp[k] + lu.ulen[k]; ++p) {
-      const int j = lu.row[p], l0 = lu.cp[j] + lu.ulen[j] + 1;
-      int* r = &s.rec[4 * (size_t)p];
-      r[0] = l0;
-      r[1] = lu.cp[j + 1] - l0;
-      r[2] = s.upd_ptr[p];
-      r[3] = j;
+  return GX_OK;
+}
+
+// Refactor update rounds for G lane groups (the layout is described with DevTopo::slots).
+// Column k's update items (target row offset t, source U offset u, L(i,j) at position p) are taken
+// in the order of the sequential left-looking refactor (j ascending, then i ascending); each goes to
+// the earliest round after every earlier item that targets t or u (so every entry receives its
+// updates in that same order, and U(j,k) is final before it is used), into slot p mod G -- the
+// lane group that wrote L at p is the one that reads it -- or, when L(i,j) comes from the previous
+// column (read from LDS), into any free slot; no other item of the round may target t or read t.
+struct RoundPlan {
+  std::vector<int> chunk0, rounds;
+  std::vector<int> slots;  // pairs {x, word}
+};
+
+int build_rounds(const gx::LuPattern& lu, const LuSchedule& s, int G, RoundPlan& out) {
+  const int n = lu.n, NB = gx::kChunkRounds;
+  if (s.max_col >= 32768) return fail(GX_ERR_UNSUPPORTED, "an L+U column has 32768 or more entries");
+  out.chunk0.assign(n + 1, 0);
+  out.rounds.assign(n, 0);
+  out.slots.clear();
+  std::vector<int> last;
+  std::vector<int> sx, sw;                  // per round * G slot
+  std::vector<std::vector<int>> rt, rs;     // per round: targets, sources
+  for (int k = 0; k < n; ++k) {
+    const int c0 = lu.cp[k], len = lu.cp[k + 1] - c0, ul = lu.ulen[k];
+    const int noop_w = len;  // empty slot: target the unused row just past the column
+    last.assign(len, -1);
+    sx.clear();
+    sw.clear();
+    int R = 0;
+    auto add_round = [&]() {
+      sx.insert(sx.end(), G, -1);
+      sw.insert(sw.end(), G, noop_w);
+      if ((int)rt.size() <= R) {
+        rt.emplace_back();
+        rs.emplace_back();
+      }
+      rt[R].clear();
+      rs[R].clear();
+      ++R;
+    };
+    for (int u = 0; u < ul; ++u) {
+      const int j = lu.row[c0 + u], l0 = lu.cp[j] + lu.ulen[j] + 1, ll = lu.cp[j + 1] - l0;
+      const int q0 = s.upd_ptr[c0 + u];
+      const bool prev = j == k - 1;
+      for (int e = 0; e < ll; ++e) {
+        const int p = l0 + e, tg = s.upd_off[q0 + e];
+        for (int r = std::max(last[tg], last[u]) + 1;; ++r) {
+          while (r >= R) add_round();
+          int slot = -1;
+          if (prev) {
+            for (int gg = 0; gg < G && slot < 0; ++gg)
+              if (sx[(size_t)r * G + gg] == -1) slot = gg;
+          } else if (sx[(size_t)r * G + p % G] == -1) {
+            slot = p % G;
+          }
+          if (slot < 0) continue;
+          if (std::find(rt[r].begin(), rt[r].end(), tg) != rt[r].end()) continue;
+          if (std::find(rs[r].begin(), rs[r].end(), tg) != rs[r].end()) continue;
+          sx[(size_t)r * G + slot] = prev ? p - lu.cp[j] : p;
+          sw[(size_t)r * G + slot] = tg | (u << 16) | (prev ? (int)0x80000000u : 0);
+          rt[r].push_back(tg);
+          rs[r].push_back(u);
+          last[tg] = r;
+          break;
+        }
+      }
     }
+    const int nch = std::max(1, (R + NB - 1) / NB);
+    out.rounds[k] = R;
+    out.chunk0[k + 1] = out.chunk0[k] + nch;
+    for (int r = 0; r < nch * NB; ++r)
+      for (int gg = 0; gg < G; ++gg) {
+        const bool real = r < R;
+        out.slots.push_back(real ? sx[(size_t)r * G + gg] : -1);
+        out.slots.push_back(real ? sw[(size_t)r * G + gg] : noop_w);
+      }
+    if (out.slots.size() >= (size_t)0x7fffffff) return fail(GX_ERR_UNSUPPORTED, "refactor round schedule too large");
+  }
   return GX_OK;
 }
 
@@ -173,34 +242,208 @@ void build_far(const gx::LuPattern& lu, int W, FarLists& f) {
   }
 }
 
+// Update rounds of the wide tile (the layout is described with DevTopo::uitems): items taken in
+// the order of the sequential left-looking refactor, each into the earliest round after every
+// earlier item that targets its target or its source, where no other item targets or reads its
+// target; every entry therefore receives its updates in the sequential order.
+int build_uniform_rounds(const gx::LuPattern& lu, const LuSchedule& s, std::vector<int>& ur,
+                         std::vector<int>& items) {
+  const int n = lu.n, IPR = gx::kRoundItems;
+  if (s.max_col >= 32768) return fail(GX_ERR_UNSUPPORTED, "an L+U column has 32768 or more entries");
+  ur.assign(n + 1, 0);
+  items.clear();
+  std::vector<int> last, rx, rw, cnt;
+  std::vector<std::vector<int>> rt, rs;
+  for (int k = 0; k < n; ++k) {
+    const int c0 = lu.cp[k], len = lu.cp[k + 1] - c0, ul = lu.ulen[k];
+    last.assign(len, -1);
+    rx.clear();
+    rw.clear();
+    cnt.clear();
+    int R = 0;
+    for (int u = 0; u < ul; ++u) {
+      const int j = lu.row[c0 + u], l0 = lu.cp[j] + lu.ulen[j] + 1, ll = lu.cp[j + 1] - l0;
+      const int q0 = s.upd_ptr[c0 + u];
+      // L(:,k-1) is written back while the first round of column k is being fetched: its items
+      // start at round 1
+      const int r_min = j == k - 1 ? 1 : 0;
+      for (int e = 0; e < ll; ++e) {
+        const int tg = s.upd_off[q0 + e];
+        for (int r = std::max(std::max(last[tg], last[u]) + 1, r_min);; ++r) {
+          while (r >= R) {
+            rx.insert(rx.end(), IPR, -1);
+            rw.insert(rw.end(), IPR, len);
+            cnt.push_back(0);
+            if ((int)rt.size() <= R) {
+              rt.emplace_back();
+              rs.emplace_back();
+            }
+            rt[R].clear();
+            rs[R].clear();
+            ++R;
+          }
+          if (cnt[r] == IPR) continue;
+          if (std::find(rt[r].begin(), rt[r].end(), tg) != rt[r].end()) continue;
+          if (std::find(rs[r].begin(), rs[r].end(), tg) != rs[r].end()) continue;
+          rx[(size_t)r * IPR + cnt[r]] = l0 + e;
+          rw[(size_t)r * IPR + cnt[r]] = tg | (u << 16);
+          ++cnt[r];
+          rt[r].push_back(tg);
+          rs[r].push_back(u);
+          last[tg] = r;
+          break;
+        }
+      }
+    }
+    if (R == 0) {  // every column owns at least one round (the kernel fetches rounds as one stream)
+      rx.insert(rx.end(), IPR, -1);
+      rw.insert(rw.end(), IPR, len);
+      R = 1;
+    }
+    ur[k + 1] = ur[k] + R;
+    for (size_t q = 0; q < rx.size(); ++q) {
+      items.push_back(rw[q]);
+      items.push_back(rx[q]);
+    }
+    if (items.size() >= (size_t)0x7fffffff) return fail(GX_ERR_UNSUPPORTED, "refactor round schedule too large");
+  }
+  return GX_OK;
+}
+
 // Device copy of a pattern's numeric schedule, pointed to by the schedule fields of a DevTopo.
 struct DevSched {
-  DevMem cp, ulen, row, upd_ptr, upd_off, rec, fptr, fpos, fcol, bptr, bpos, bcol;
+  DevMem cp, ulen, row, chunk0, rounds, slots, fptr, fpos, fcol, bptr, bpos, bcol, col_a, col_f, col_b, fslot, opos, ocol;
+  DevMem uround, uitems, bslot;
+  std::vector<int> h_chunk0, h_rounds, h_fptr, h_bptr;
   bool uploaded = false;
   int window = -1;
+  int groups = -1;
   void reset() {
     uploaded = false;
     window = -1;
+    groups = -1;
   }
 };
 
-// Uploads the pattern and update schedule once and the far lists whenever the window changes.
-int upload_sched(const gx::LuPattern& lu, const LuSchedule& s, int window, DevSched& m, gx::DevTopo& d) {
+// Uploads the pattern once, the update rounds whenever the lane-group count changes and the far
+// lists whenever the window changes.
+int upload_sched(const gx::LuPattern& lu, const LuSchedule& s, int window, int tile, DevSched& m,
+                 gx::DevTopo& d) {
   int rc;
+  const int G = gx::kLanes / tile;
   if (!m.uploaded) {
-    if ((rc = m.cp.upload(lu.cp)) || (rc = m.ulen.upload(lu.ulen)) || (rc = m.row.upload(lu.row)) ||
-        (rc = m.upd_ptr.upload(s.upd_ptr)) || (rc = m.upd_off.upload(s.upd_off)) || (rc = m.rec.upload(s.rec)))
-      return rc;
+    if ((rc = m.cp.upload(lu.cp)) || (rc = m.ulen.upload(lu.ulen)) || (rc = m.row.upload(lu.row))) return rc;
     m.uploaded = true;
     m.window = -1;
+    m.groups = -1;
   }
-  if (m.window != window) {
+  const bool regroup = m.groups != G, rewindow = m.window != window;
+  if (regroup && tile == gx::kWideTile) {
+    std::vector<int> ur, items;
+    if ((rc = build_uniform_rounds(lu, s, ur, items)) || (rc = m.uround.upload(ur)) || (rc = m.uitems.upload(items)))
+      return rc;
+    m.h_chunk0.swap(ur);  // col_a.w = first round of the column
+    m.h_rounds.assign(lu.n, 0);
+    m.groups = G;
+  } else if (regroup) {
+    RoundPlan plan;
+    if ((rc = build_rounds(lu, s, G, plan)) || (rc = m.chunk0.upload(plan.chunk0)) ||
+        (rc = m.rounds.upload(plan.rounds)) || (rc = m.slots.upload(plan.slots)))
+      return rc;
+    m.h_chunk0.swap(plan.chunk0);
+    m.h_rounds.swap(plan.rounds);
+    m.groups = G;
+  }
+  if (rewindow) {
     FarLists f;
     build_far(lu, window, f);
     if ((rc = m.fptr.upload(f.fptr)) || (rc = m.fpos.upload(f.fpos)) || (rc = m.fcol.upload(f.fcol)) ||
         (rc = m.bptr.upload(f.bptr)) || (rc = m.bpos.upload(f.bpos)) || (rc = m.bcol.upload(f.bcol)))
       return rc;
+    m.h_fptr.swap(f.fptr);
+    m.h_bptr.swap(f.bptr);
     m.window = window;
+  }
+  if ((regroup || rewindow) && tile == gx::kWideTile) {
+    // wide tile: the first kWideFar far entries of the rows entering the forward / backward window
+    // after column k in fslot[k][i] / bslot[k][i] ({position, column}, ascending columns), the rest
+    // in the ranges col_f = {0, q0, q1, 0} of fwd_pos/fwd_col and col_b = {q0, q1, 0, 0} of
+    // bwd_pos/bwd_col
+    const int n = lu.n, MF = gx::kWideFar;
+    FarLists f;
+    build_far(lu, window, f);
+    std::vector<int> a(4 * (size_t)n), fw(4 * (size_t)n, 0), bw(4 * (size_t)n, 0);
+    std::vector<int> fslot(2 * (size_t)n * MF, -1), bslot(2 * (size_t)n * MF, -1);
+    for (int k = 0; k < n; ++k) {
+      a[4 * k] = lu.cp[k];
+      a[4 * k + 1] = lu.cp[k + 1] - lu.cp[k];
+      a[4 * k + 2] = lu.ulen[k];
+      a[4 * k + 3] = m.h_chunk0[k];
+      const long long rf = (long long)k + window + 1, rb = (long long)k - window - 1;
+      if (rf < n) {
+        const int q0 = f.fptr[rf], q1 = f.fptr[rf + 1];
+        for (int i = 0; i < MF && q0 + i < q1; ++i) {
+          fslot[2 * ((size_t)k * MF + i)] = f.fpos[q0 + i];
+          fslot[2 * ((size_t)k * MF + i) + 1] = f.fcol[q0 + i];
+        }
+        fw[4 * k + 1] = std::min(q0 + MF, q1);
+        fw[4 * k + 2] = q1;
+      }
+      if (rb >= 0) {
+        const int q0 = f.bptr[rb], q1 = f.bptr[rb + 1];
+        for (int i = 0; i < MF && q0 + i < q1; ++i) {
+          bslot[2 * ((size_t)k * MF + i)] = f.bpos[q0 + i];
+          bslot[2 * ((size_t)k * MF + i) + 1] = f.bcol[q0 + i];
+        }
+        bw[4 * k] = std::min(q0 + MF, q1);
+        bw[4 * k + 1] = q1;
+      }
+    }
+    if ((rc = m.col_a.upload(a)) || (rc = m.col_f.upload(fw)) || (rc = m.col_b.upload(bw)) ||
+        (rc = m.fslot.upload(fslot)) || (rc = m.bslot.upload(bslot)))
+      return rc;
+  } else if (regroup || rewindow) {
+    // far L entries of the row entering the forward window after column k: the first one of each
+    // lane group (position mod G) in fslot[k][g], the others in an overflow list
+    const int n = lu.n;
+    std::vector<int> fpos, fcol;
+    {
+      FarLists f;
+      build_far(lu, window, f);
+      fpos.swap(f.fpos);
+      fcol.swap(f.fcol);
+    }
+    std::vector<int> a(4 * (size_t)n), fw(4 * (size_t)n, 0), bw(4 * (size_t)n, 0);
+    std::vector<int> fslot(2 * (size_t)n * G, -1), opos, ocol;
+    for (int k = 0; k < n; ++k) {
+      a[4 * k] = lu.cp[k];
+      a[4 * k + 1] = lu.cp[k + 1] - lu.cp[k];
+      a[4 * k + 2] = lu.ulen[k];
+      a[4 * k + 3] = m.h_chunk0[k];
+      fw[4 * k] = m.h_rounds[k];
+      const long long rf = (long long)k + window + 1, rb = (long long)k - window - 1;
+      fw[4 * k + 1] = fw[4 * k + 2] = (int)opos.size();
+      if (rf < n) {
+        for (int q = m.h_fptr[rf]; q < m.h_fptr[rf + 1]; ++q) {
+          int* sl = &fslot[2 * ((size_t)k * G + fpos[q] % G)];
+          if (sl[0] < 0) {
+            sl[0] = fpos[q];
+            sl[1] = fcol[q];
+          } else {
+            opos.push_back(fpos[q]);
+            ocol.push_back(fcol[q]);
+          }
+        }
+        fw[4 * k + 2] = (int)opos.size();
+      }
+      if (rb >= 0) {
+        bw[4 * k] = m.h_bptr[rb];
+        bw[4 * k + 1] = m.h_bptr[rb + 1];
+      }
+    }
+    if ((rc = m.col_a.upload(a)) || (rc = m.col_f.upload(fw)) || (rc = m.col_b.upload(bw)) ||
+        (rc = m.fslot.upload(fslot)) || (rc = m.opos.upload(opos)) || (rc = m.ocol.upload(ocol)))
+      return rc;
   }
   d.n = lu.n;
   d.nnz_lu = lu.nnz();
@@ -208,9 +451,19 @@ int upload_sched(const gx::LuPattern& lu, const LuSchedule& s, int window, DevSc
   d.cp = m.cp.as<int>();
   d.ulen = m.ulen.as<int>();
   d.lu_row = m.row.as<int>();
-  d.upd_ptr = m.upd_ptr.as<int>();
-  d.upd_off = m.upd_off.as<uint16_t>();
-  d.rec = m.rec.as<int>();
+  d.chunk0 = m.chunk0.as<int>();
+  d.rounds = m.rounds.as<int>();
+  d.slots = m.slots.as<int2>();
+  d.groups = G;
+  d.col_a = m.col_a.as<int4>();
+  d.col_f = m.col_f.as<int4>();
+  d.col_b = m.col_b.as<int4>();
+  d.fslot = m.fslot.as<int2>();
+  d.bslot = m.bslot.as<int2>();
+  d.fo_pos = m.opos.as<int>();
+  d.fo_col = m.ocol.as<int>();
+  d.uround = m.uround.as<int>();
+  d.uitems = m.uitems.as<int2>();
   d.fwd_ptr = m.fptr.as<int>();
   d.fwd_pos = m.fpos.as<int>();
   d.fwd_col = m.fcol.as<int>();
@@ -224,14 +477,13 @@ int upload_sched(const gx::LuPattern& lu, const LuSchedule& s, int window, DevSc
 // Rows of the solve ring for a launch: the requested (or default) power of two, halved while the
 // LU kernel's LDS (staged column + ring, `elem` bytes per value) does not fit; -1 if nothing fits.
 int pick_ring_rows(int max_col, int tile, size_t elem, int requested) {
-  const size_t cap = std::max(max_col, 1);
-  auto bytes = [&](int rows) { return (cap + (size_t)rows) * (size_t)(tile + 1) * elem; };
+  auto bytes = [&](int rows) { return gx::lu_lds_bytes(max_col, rows, tile, elem, true); };
   int rows = requested > 0 ? requested : gx::ring_rows(tile);
   while (rows > 64 && bytes(rows) > (size_t)gx::kMaxLds) rows /= 2;
   return bytes(rows) <= (size_t)gx::kMaxLds ? rows : -1;
 }
 
-bool valid_tile(int tile) { return tile == 4 || tile == 8 || tile == 16; }
+bool valid_tile(int tile) { return tile == 4 || tile == 8 || tile == 16 || tile == gx::kWideTile; }
 bool valid_ring(int rows) { return rows == 0 || (rows >= 64 && rows <= 4096 && (rows & (rows - 1)) == 0); }
 
 // Scenarios per tile when the caller leaves it open: the widest tile that still gives at least
@@ -448,9 +700,9 @@ int upload_ybus(gx_topo& t);
 
 // Uploads the topology (Y-bus, recipes, LU pattern and schedule) once per factorization, and the
 // far lists of the solves for the given window.
-int upload_topo(gx_topo& t, int window) {
+int upload_topo(gx_topo& t, int window, int tile) {
   int rc;
-  if ((rc = upload_sched(t.lu, t.sched, window, t.ds, t.dt))) return rc;
+  if ((rc = upload_sched(t.lu, t.sched, window, tile, t.ds, t.dt))) return rc;
   if (t.uploaded) return GX_OK;
   if ((rc = upload_ybus(t)) || (rc = t.d_rpos_p.upload(t.h_rp)) ||
       (rc = t.d_rpos_q.upload(t.h_rq)) || (rc = t.d_xpos_a.upload(t.h_xa)) || (rc = t.d_xpos_m.upload(t.h_xm)) ||
@@ -607,7 +859,7 @@ int build_lg(gx_topo& t, size_t lanes, int tile) {
   if (t.dlg.m > 0) {
     const int rows = pick_ring_rows(t.lg_sched.max_col, tile, 16, 0);
     if (rows < 0) return fail(GX_ERR_UNSUPPORTED, "linear guess L+U column exceeds the LDS staging limit");
-    if ((rc = upload_sched(t.lg_lu, t.lg_sched, gx::ring_window_of(rows), t.lg_ds, t.dlgt))) return rc;
+    if ((rc = upload_sched(t.lg_lu, t.lg_sched, gx::ring_window_of(rows, tile), tile, t.lg_ds, t.dlgt))) return rc;
     if ((rc = t.w_lg_lu.reserve(lanes * (size_t)t.dlg.nnz_lu * 16)) || (rc = t.w_lg_y.reserve(lanes * t.dlg.m * 16)) ||
         (rc = t.w_lg_status.reserve(lanes * sizeof(int))) || (rc = t.w_lg_iters.reserve(lanes * sizeof(int))))
       return rc;
@@ -1038,7 +1290,7 @@ int gx_batch_solve(gx_topo* t, int n_scen, const double* p_spec, const double* q
   const int rows = pick_ring_rows(t->sched.max_col, tile, sizeof(double), t->ring_req);
   if (rows < 0)
     return fail(GX_ERR_UNSUPPORTED, "longest L+U column exceeds the LDS staging limit of the device refactor");
-  if ((rc = upload_topo(*t, gx::ring_window_of(rows)))) return rc;
+  if ((rc = upload_topo(*t, gx::ring_window_of(rows, tile), tile))) return rc;
   // FP64-sized in both modes: the mixed mode's FP64 re-solve reuses the same buffer
   if ((rc = t->w_lu.reserve(lanes * (size_t)t->dt.nnz_lu * 8))) return rc;
   w.lu = t->w_lu.p;
@@ -1196,7 +1448,7 @@ int sys_to_device(gx_sys* s) {
   const int rows = pick_ring_rows(s->sched.max_col, kSysTile, sizeof(double), 0);
   if (rows < 0) return fail(GX_ERR_UNSUPPORTED, "longest L+U column exceeds the LDS staging limit");
   int rc;
-  if ((rc = upload_sched(s->lu, s->sched, gx::ring_window_of(rows), s->ds, s->dt)) ||
+  if ((rc = upload_sched(s->lu, s->sched, gx::ring_window_of(rows, kSysTile), kSysTile, s->ds, s->dt)) ||
       (rc = s->d_lu.reserve((size_t)nlu * kSysTile * sizeof(double))) ||
       (rc = s->d_y.reserve((size_t)n * kSysTile * sizeof(double))) ||
       (rc = s->d_status.reserve(kSysTile * sizeof(int))) || (rc = s->d_iters.reserve(kSysTile * sizeof(int))) ||

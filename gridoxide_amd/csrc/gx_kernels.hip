@@ -331,126 +331,266 @@ __global__ __launch_bounds__(256) void k_fill(DevTopo t, DevWork w) {
   }
 }
 
+// Entries of a column a lane holds in registers (longer columns take a slower loop).
+template <int G>
+constexpr int stage_regs() {
+  return 32 / G + 1;
+}
+
+__device__ __forceinline__ int rl(int v, int lane) { return __builtin_amdgcn_readlane(v, lane); }
+
 // Refactor (optional) + forward/back substitution; one wavefront per tile.
-//   LDS: xc[cap][TW+1] (the column being factored) and ring[R][TW+1] (the right-hand side rows
-//   inside the moving window; row r lives in slot r mod R, R = W + kRingGap + 1 a power of two).
+//   LDS: two staged columns xc[2][cap+1][TW+1] (column k in buffer k & 1; the previous column stays
+//   readable for the updates that use it) and ring[R][TW+1] (the right-hand side rows inside the
+//   moving window; row r lives in slot r mod R, R = W + kRingGap + 1 a power of two).
+// Refactor of column k: stage A(:,k), run its update rounds (host-scheduled, one item per lane
+// group and round: x_t -= L(i,j) * x_u), divide the L part by the pivot, write back.
 // Forward: at column k the ring holds rows [k - B, k + W] (B = kRingGap); y_k = ring[k]; L(:,k)
 // pushes into the ring rows i <= k + W; row k + W + 1 then enters the ring (taking the slot of
 // row k - B) as f minus the contributions of its L entries farther than W, gathered at that
 // moment.  Back substitution mirrors this with U: the ring holds rows [k - W, k + B].
+// Memory: every global load a column needs is issued at the top of an earlier column -- column
+// metadata a batch of G columns ahead (lane group g holds column batch start + g, read with
+// readlane), the staged values and rows, the first kChunkRounds update rounds' L values and the
+// far entries one column ahead, the slots those loads index two columns ahead -- so each column
+// waits once, for loads a column old.  Loads are issued unconditionally at clamped addresses and
+// unused results discarded.  L values of column k-1 (not yet written back when column k's loads
+// are issued) are read from LDS instead.
 template <typename T, int TW, bool kFactor>
 __global__ __launch_bounds__(64) void k_lu(DevTopo t, DevWork w, int cap, int iter) {
   constexpr int G = kLanes / TW;
   constexpr int S = TW + 1;
   constexpr int B = kRingGap;
+  constexpr int NB = kChunkRounds;
+  constexpr int MS = stage_regs<G>();
   const int W = t.window;
   const int R = W + B + 1;
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  T* xc = reinterpret_cast<T*>(smem_raw);
-  T* ring = xc + (kFactor ? (size_t)cap * S : 0);
+  T* xcb = reinterpret_cast<T*>(smem_raw);
+  const int xstride = (cap + 1) * S;
+  T* ring = xcb + (kFactor ? 2 * (size_t)xstride : 0);
   const int tile = blockIdx.x, g = grp<TW>(), s = sub<TW>();
   if (!tile_running<TW>(w.status, tile)) return;
   const int sc = tile * TW + s;
   T* __restrict__ lu = reinterpret_cast<T*>(w.lu) + at<TW>(tile, t.nnz_lu, 0) + s;
   T* __restrict__ y = reinterpret_cast<T*>(w.y) + at<TW>(tile, t.n, 0) + s;
   const int n = t.n;
+  const int nl = t.nnz_lu;
   bool ok = true;        // per lane: no zero pivot, finite solution
   bool overflow = false; // a column longer than the staging area (the same for every lane)
   auto slot = [&](int r) { return (r & (R - 1)) * S + s; };
+  auto cl = [](int v, int hi) { return v < 0 ? 0 : (v > hi ? hi : v); };
+  const T zero = zero_value<T>();
 
   // ------------------------------------------------ forward pass (+ refactor)
   for (int r = g; r <= W && r < n; r += G) ring[slot(r)] = y[(size_t)r * TW];
   // f of the rows entering the window: within a batch of G columns, group g holds the row that
   // enters after column (batch start + g); the next batch is loaded one batch ahead
-  auto f_of = [&](int kk) -> T {
-    const int r = kk + W + 1;
-    return r < n ? y[(size_t)r * TW] : zero_value<T>();
-  };
+  auto f_of = [&](int kk) -> T { return y[(size_t)cl(kk + W + 1, n - 1) * TW]; };
   T fcur = f_of(g), fnext = f_of(G + g);
+  int4 ca = t.col_a[cl(g, n - 1)], cf = t.col_f[cl(g, n - 1)];
+  int4 na = t.col_a[cl(G + g, n - 1)], nf = t.col_f[cl(G + g, n - 1)];
+  int kb = 0;  // first column of the batch in ca/cf
+  // metadata of column kk in [kb, kb + 2G)
+  auto m_c0 = [&](int kk) { return rl(kk - kb >= G ? na.x : ca.x, ((kk - kb) % G) * TW); };
+  auto m_len = [&](int kk) { return rl(kk - kb >= G ? na.y : ca.y, ((kk - kb) % G) * TW); };
+  auto m_ul = [&](int kk) { return rl(kk - kb >= G ? na.z : ca.z, ((kk - kb) % G) * TW); };
+  auto m_ch0 = [&](int kk) { return rl(kk - kb >= G ? na.w : ca.w, ((kk - kb) % G) * TW); };
+
+  // staged entries: factor -> entries o = first_of_group(c0, g) + G*i of the whole column (owned
+  // positions); solve only -> L entries o = ul + 1 + g + G*i
+  T sv[MS], cv[MS];
+  int sr[MS], cr[MS];
+  auto stage_issue = [&](int kk) {
+    const int c0 = m_c0(kk), len = m_len(kk), ul = m_ul(kk);
+    const int f0 = kFactor ? first_of_group<G>(c0, g) : ul + 1 + g;
+#pragma unroll
+    for (int i = 0; i < MS; ++i) {
+      const int o = f0 + G * i;
+      const int p = o < len ? c0 + o : c0;
+      sv[i] = lu[(size_t)p * TW];
+      sr[i] = t.lu_row[p];
+    }
+  };
+  // update rounds: s0/v0 = column k (slots, L values), s1/v1 = column k+1, s2 = slots of k+2
+  int2 s0[NB], s1[NB], s2[NB];
+  T v0[NB], v1[NB];
+  const int2* __restrict__ slots = t.slots + g;
+  const int n_slot_rows = kFactor ? t.chunk0[n] * NB : 0;
+  auto slots_issue = [&](int kk, int2 (&d)[NB]) {
+    const int base = m_ch0(kk) * NB;
+#pragma unroll
+    for (int r = 0; r < NB; ++r) d[r] = slots[(size_t)cl(base + r, n_slot_rows - 1) * G];
+  };
+  auto vals_issue = [&](const int2 (&sm)[NB], T (&d)[NB]) {
+#pragma unroll
+    for (int r = 0; r < NB; ++r) d[r] = lu[(size_t)cl(sm[r].x, nl - 1) * TW];
+  };
+  // far entries of the row entering after a column: f0s/f0l/f0y = column k (slot, L value, y value),
+  // f1s/f1l/f1y = column k+1, f2s = slot of column k+2
+  int2 f0s, f1s, f2s;
+  T f0l, f0y, f1l, f1y;
+  auto fslot_issue = [&](int kk) { return t.fslot[(size_t)cl(kk, n - 1) * G + g]; };
+  auto fval_issue = [&](int2 fs, T& lv, T& yv) {
+    lv = lu[(size_t)cl(fs.x, nl - 1) * TW];
+    yv = y[(size_t)cl(fs.y, n - 1) * TW];
+  };
+#pragma unroll
+  for (int i = 0; i < MS; ++i) {
+    sv[i] = zero;
+    sr[i] = 0;
+  }
+  stage_issue(0);
+  if (kFactor) {
+    slots_issue(0, s1);
+    slots_issue(n > 1 ? 1 : 0, s2);
+    vals_issue(s1, v1);
+  }
+  f1s = fslot_issue(0);
+  f2s = fslot_issue(1);
+  fval_issue(f1s, f1l, f1y);
+  int prev_c0 = 0;  // first packed position of column k-1
   for (int k = 0; k < n; ++k) {
-    if (k > 0 && k % G == 0) {
+    const int kg = k % G;
+    if (k > 0 && kg == 0) {
+      kb = k;
       fcur = fnext;
       fnext = f_of(k + G + g);
+      ca = na;
+      cf = nf;
+      na = t.col_a[cl(k + G + g, n - 1)];
+      nf = t.col_f[cl(k + G + g, n - 1)];
     }
-    const int c0 = t.cp[k], len = t.cp[k + 1] - c0, ul = t.ulen[k];
+    const int lk = kg * TW;
+    const int c0 = rl(ca.x, lk), len = rl(ca.y, lk), ul = rl(ca.z, lk), ch0 = rl(ca.w, lk);
+    const int nr = rl(cf.x, lk), oq0 = rl(cf.y, lk), oq1 = rl(cf.z, lk);
+    // take this column's data, issue the next columns' loads
+#pragma unroll
+    for (int i = 0; i < MS; ++i) {
+      cv[i] = sv[i];
+      cr[i] = sr[i];
+    }
+    if (kFactor) {
+#pragma unroll
+      for (int r = 0; r < NB; ++r) {
+        s0[r] = s1[r];
+        v0[r] = v1[r];
+        s1[r] = s2[r];
+      }
+    }
+    f0s = f1s;
+    f0l = f1l;
+    f0y = f1y;
+    f1s = f2s;
+    {
+      const int k1 = k + 1 < n ? k + 1 : k, k2 = k + 2 < n ? k + 2 : k1;
+      stage_issue(k1);
+      if (kFactor) {
+        vals_issue(s1, v1);
+        slots_issue(k2, s2);
+      }
+      fval_issue(f1s, f1l, f1y);
+      f2s = fslot_issue(k2);
+    }
+    const int f0 = kFactor ? first_of_group<G>(c0, g) : ul + 1 + g;
+    T* xc = xcb + (k & 1) * xstride;
+    const T* xprev = xcb + ((k & 1) ^ 1) * xstride;
     if (kFactor) {
       if (len > cap) {  // the host sizes cap to the longest column; kept as a guard
         overflow = true;
         break;
       }
-      for (int o = first_of_group<G>(c0, g); o < len; o += G) xc[o * S + s] = lu[(size_t)(c0 + o) * TW];
-      if (ul > 0) {
-        // updates by the columns j of U(:,k), in ascending order; the L rows of the next update
-        // are loaded while the current one runs
-        const int* __restrict__ rp = t.rec + 4 * (size_t)c0;
-        int4 rc = *reinterpret_cast<const int4*>(rp);
-        int e = first_of_group<G>(rc.x, g);
-        bool pv = e < rc.y;
-        T pl = pv ? lu[(size_t)(rc.x + e) * TW] : zero_value<T>();
-        int po = pv ? t.upd_off[rc.z + e] : 0;
-        for (int u = 0; u < ul; ++u) {
-          const int4 cur = rc;
-          const T cl = pl;
-          const int co = po, ce = e;
-          const bool cv = pv;
-          if (u + 1 < ul) {
-            rc = *reinterpret_cast<const int4*>(rp + 4 * (u + 1));
-            e = first_of_group<G>(rc.x, g);
-            pv = e < rc.y;
-            pl = pv ? lu[(size_t)(rc.x + e) * TW] : zero_value<T>();
-            po = pv ? t.upd_off[rc.z + e] : 0;
-          }
-          const T ujk = xc[u * S + s];
-          if (cv) xc[co * S + s] = xc[co * S + s] - cl * ujk;
-          // longer L columns: further rows, four per lane at a time (distinct target rows)
-          int ee = ce + G;
-          for (; ee + 3 * G < cur.y; ee += 4 * G) {
-            T lv[4];
-            int oo[4];
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-              lv[q] = lu[(size_t)(cur.x + ee + q * G) * TW];
-              oo[q] = t.upd_off[cur.z + ee + q * G];
-            }
-            T xv[4];
+      for (int i = 0; i < MS; ++i) {
+        const int o = f0 + G * i;
+        if (o < len) xc[o * S + s] = cv[i];
+      }
+      for (int o = f0 + G * MS; o < len; o += G) xc[o * S + s] = lu[(size_t)(c0 + o) * TW];
+      // first chunk of rounds from registers, later chunks loaded here
+      const int nch = nr > NB ? (nr + NB - 1) / NB : 1;
+      for (int c = 0; c < nch; ++c) {
+        if (c > 0) {
 #pragma unroll
-            for (int q = 0; q < 4; ++q) xv[q] = xc[oo[q] * S + s];
+          for (int r = 0; r < NB; ++r) s0[r] = slots[((size_t)(ch0 + c) * NB + r) * G];
 #pragma unroll
-            for (int q = 0; q < 4; ++q) xc[oo[q] * S + s] = xv[q] - lv[q] * ujk;
-          }
-          for (; ee < cur.y; ee += G) {
-            const int oo = t.upd_off[cur.z + ee];
-            xc[oo * S + s] = xc[oo * S + s] - lu[(size_t)(cur.x + ee) * TW] * ujk;
+          for (int r = 0; r < NB; ++r) v0[r] = lu[(size_t)cl(s0[r].x, nl - 1) * TW];
+        }
+        const int rleft = nr - c * NB;
+#pragma unroll
+        for (int r = 0; r < NB; ++r) {
+          if (r < rleft) {
+            const int wd = s0[r].y;
+            const int tg = wd & 0xffff, su = (wd >> 16) & 0x7fff;
+            const T lv = wd < 0 ? xprev[s0[r].x * S + s] : (s0[r].x >= 0 ? v0[r] : zero);
+            const T xt = xc[tg * S + s], xu = xc[su * S + s];
+            xc[tg * S + s] = xt - lv * xu;
           }
         }
       }
       const T ukk = xc[ul * S + s];
       if (is_zero(ukk)) ok = false;
-      for (int o = ul + 1 + first_of_group<G>(ul + 1, g); o < len; o += G) xc[o * S + s] = xc[o * S + s] / ukk;
-      for (int o = first_of_group<G>(c0, g); o < len; o += G) lu[(size_t)(c0 + o) * TW] = xc[o * S + s];
+#pragma unroll
+      for (int i = 0; i < MS; ++i) {
+        const int o = f0 + G * i;
+        if (o < len) {
+          T v = xc[o * S + s];
+          if (o > ul) {
+            v = v / ukk;
+            xc[o * S + s] = v;
+          }
+          lu[(size_t)(c0 + o) * TW] = v;
+          cv[i] = v;
+        }
+      }
+      for (int o = f0 + G * MS; o < len; o += G) {
+        T v = xc[o * S + s];
+        if (o > ul) {
+          v = v / ukk;
+          xc[o * S + s] = v;
+        }
+        lu[(size_t)(c0 + o) * TW] = v;
+      }
     }
-    // forward substitution: y_k is final; push L(:,k) into the window rows
+    // forward substitution: y_k is final
     const T yk = ring[slot(k)];
-    for (int o = ul + 1 + g; o < len; o += G) {
+    if (g == 0) y[(size_t)k * TW] = yk;
+    // the row entering the window after column k: f minus its far L entries
+    const int rin = k + W + 1;
+    if (rin < n) {
+      T acc = zero;
+      if (f0s.x >= 0) {
+        const int j = f0s.y;
+        T lv = f0l;
+        if (kFactor && j == k) lv = xc[(f0s.x - c0) * S + s];
+        if (kFactor && j == k - 1) lv = xprev[(f0s.x - prev_c0) * S + s];
+        const T yj = j >= k - B ? ring[slot(j)] : f0y;
+        acc = lv * yj;
+      }
+      for (int q = oq0; q < oq1; ++q) {  // entries beyond one per lane group
+        const int p = t.fo_pos[q];
+        if (p % G != g) continue;  // the lane group that wrote L at p reads it
+        const int j = t.fo_col[q];
+        const T yj = j > k - B ? ring[slot(j)] : y[(size_t)j * TW];
+        const T lv = kFactor && j == k ? xc[(p - c0) * S + s] : lu[(size_t)p * TW];
+        acc = acc + lv * yj;
+      }
+      acc = group_sum<TW>(acc);
+      if (g == kg) ring[slot(rin)] = fcur - acc;
+    }
+    // push L(:,k) into the window rows (rows farther than W are gathered when they enter)
+#pragma unroll
+    for (int i = 0; i < MS; ++i) {
+      const int o = f0 + G * i;
+      if (o > ul && o < len && cr[i] - k <= W) ring[slot(cr[i])] = ring[slot(cr[i])] - cv[i] * yk;
+    }
+    for (int o = f0 + G * MS; o < len; o += G) {
+      if (o <= ul) continue;
       const int i = t.lu_row[c0 + o];
-      if (i - k > W) continue;  // gathered when row i enters the window
+      if (i - k > W) continue;
       const T lv = kFactor ? xc[o * S + s] : lu[(size_t)(c0 + o) * TW];
       ring[slot(i)] = ring[slot(i)] - lv * yk;
     }
-    if (g == 0) y[(size_t)k * TW] = yk;
-    const int r = k + W + 1;
-    if (r < n) {
-      T acc = zero_value<T>();
-      for (int q = t.fwd_ptr[r]; q < t.fwd_ptr[r + 1]; ++q) {
-        const int p = t.fwd_pos[q];
-        if (p % G != g) continue;  // the lane group that wrote L at p reads it
-        const int j = t.fwd_col[q];
-        const T yj = j > k - B ? ring[slot(j)] : y[(size_t)j * TW];
-        acc = acc + lu[(size_t)p * TW] * yj;
-      }
-      acc = group_sum<TW>(acc);
-      if (g == k % G) ring[slot(r)] = fcur - acc;
-    }
+    prev_c0 = c0;
     if ((k + 1) % B == 0) workgroup_fence();
   }
   workgroup_fence();
@@ -459,49 +599,75 @@ __global__ __launch_bounds__(64) void k_lu(DevTopo t, DevWork w, int cap, int it
   if (!overflow) {
     // ring rows [max(0, n-1-W), n-1] <- y
     for (int r = n - 1 - g; r >= 0 && r >= n - 1 - W; r -= G) ring[slot(r)] = y[(size_t)r * TW];
-    // within a batch of G columns (counted from the end), group g holds y of the row entering
-    // after column n-1-(batch start + g) and that column's pivot
-    auto y_of = [&](int m) -> T {
-      const int r = n - 1 - m - W - 1;
-      return r >= 0 ? y[(size_t)r * TW] : zero_value<T>();
-    };
-    auto piv_of = [&](int m) -> T {
-      const int k = n - 1 - m;
-      return k >= 0 ? lu[(size_t)(t.cp[k] + t.ulen[k]) * TW] : zero_value<T>();
-    };
+    // column k = n-1-m: metadata in batches of G (lane group g: m = batch start + g), the y of
+    // the row entering after it likewise, its U entries and pivot loaded one column ahead
+    auto y_of = [&](int m) -> T { return y[(size_t)cl(n - 1 - m - W - 1, n - 1) * TW]; };
     T ycur = y_of(g), ynext = y_of(G + g);
-    T pcur = piv_of(g), pnext = piv_of(G + g);
-    for (int k = n - 1; k >= 0; --k) {
-      const int m = n - 1 - k;
-      if (m > 0 && m % G == 0) {
+    int4 ba = t.col_a[n - 1 - cl(g, n - 1)], bb = t.col_b[n - 1 - cl(g, n - 1)];
+    int4 nba = t.col_a[n - 1 - cl(G + g, n - 1)], nbb = t.col_b[n - 1 - cl(G + g, n - 1)];
+    int mb = 0;
+    T uv[MS], cu[MS], pv, cpv;
+    int ur[MS], cur_[MS];
+    auto u_issue = [&](int mm) {
+      const bool second = mm - mb >= G;
+      const int lane = ((mm - mb) % G) * TW;
+      const int c0 = rl(second ? nba.x : ba.x, lane), ul = rl(second ? nba.z : ba.z, lane);
+#pragma unroll
+      for (int i = 0; i < MS; ++i) {
+        const int u = g + G * i;
+        const int p = u < ul ? c0 + u : c0;
+        uv[i] = lu[(size_t)p * TW];
+        ur[i] = t.lu_row[p];
+      }
+      pv = lu[(size_t)(c0 + ul) * TW];
+    };
+    u_issue(0);
+    for (int m = 0; m < n; ++m) {
+      const int k = n - 1 - m, mg = m % G;
+      if (m > 0 && mg == 0) {
+        mb = m;
         ycur = ynext;
         ynext = y_of(m + G + g);
-        pcur = pnext;
-        pnext = piv_of(m + G + g);
+        ba = nba;
+        bb = nbb;
+        nba = t.col_a[n - 1 - cl(m + G + g, n - 1)];
+        nbb = t.col_b[n - 1 - cl(m + G + g, n - 1)];
       }
-      const T ukk = shfl_v(pcur, (m % G) * TW + s);
-      const T xk = ring[slot(k)] / ukk;
+      const int lk = mg * TW;
+      const int c0 = rl(ba.x, lk), ul = rl(ba.z, lk), bq0 = rl(bb.x, lk), bq1 = rl(bb.y, lk);
+#pragma unroll
+      for (int i = 0; i < MS; ++i) {
+        cu[i] = uv[i];
+        cur_[i] = ur[i];
+      }
+      cpv = pv;
+      u_issue(m + 1 < n ? m + 1 : m);
+      const T xk = ring[slot(k)] / cpv;
       if (!finite_value(xk)) ok = false;
       if (g == 0) {
         ring[slot(k)] = xk;
         y[(size_t)k * TW] = xk;
       }
-      const int c0 = t.cp[k], ul = t.ulen[k];
-      for (int u = g; u < ul; u += G) {
-        const int j = t.lu_row[c0 + u];
-        if (k - j > W) continue;  // gathered when row j enters the window
-        ring[slot(j)] = ring[slot(j)] - lu[(size_t)(c0 + u) * TW] * xk;
-      }
       const int r = k - W - 1;
       if (r >= 0) {
-        T acc = zero_value<T>();
-        for (int q = t.bwd_ptr[r] + g; q < t.bwd_ptr[r + 1]; q += G) {
+        T acc = zero;
+        for (int q = bq0 + g; q < bq1; q += G) {
           const int j = t.bwd_col[q];
           const T xj = j < k + B ? ring[slot(j)] : y[(size_t)j * TW];
           acc = acc + lu[(size_t)t.bwd_pos[q] * TW] * xj;
         }
-        acc = group_sum<TW>(acc);
-        if (g == m % G) ring[slot(r)] = ycur - acc;
+        if (bq1 > bq0) acc = group_sum<TW>(acc);
+        if (g == mg) ring[slot(r)] = ycur - acc;
+      }
+#pragma unroll
+      for (int i = 0; i < MS; ++i) {
+        const int u = g + G * i;
+        if (u < ul && k - cur_[i] <= W) ring[slot(cur_[i])] = ring[slot(cur_[i])] - cu[i] * xk;
+      }
+      for (int u = g + G * MS; u < ul; u += G) {
+        const int j = t.lu_row[c0 + u];
+        if (k - j > W) continue;  // gathered when row j enters the window
+        ring[slot(j)] = ring[slot(j)] - lu[(size_t)(c0 + u) * TW] * xk;
       }
       if ((m + 1) % B == 0) workgroup_fence();
     }
@@ -509,6 +675,322 @@ __global__ __launch_bounds__(64) void k_lu(DevTopo t, DevWork w, int cap, int it
   // a zero pivot seen by any group of the scenario, or a non-finite solution, makes it Singular
   ok = all_groups<TW>(ok && !overflow);
   if (g == 0 && !ok && w.status[sc] == kRunning) {
+    w.status[sc] = kSingular;
+    w.iters[sc] = iter + 1;
+  }
+}
+
+// Wide tile: one wavefront per tile of 64 scenarios, lane = scenario.  Every index is the same
+// for all lanes, every access is one contiguous 64-value row, and a lane only ever reads values of
+// its own scenario, so no fences are needed.
+//   LDS: xc[cap+1][64] (the column being factored) and ring[R][64] with R = W + 1: at column k the
+//   forward ring holds rows [k, k + W]; row k + W + 1 enters in the slot of row k once y_k has
+//   been read, as f minus its far L entries taken in ascending column order (the order of the
+//   column-oriented substitution).  Back substitution mirrors this with U.
+// Refactor of column k: stage A(:,k), run its update rounds (kRoundItems independent items
+// x_t -= L(i,j) * x_u each), divide the L part by the pivot, write back.
+// Memory: loads are issued ahead of use so that each waits for data that has been in flight for a
+// column or a round: column metadata and far-entry slots two columns ahead; staged values, row
+// indices, the entering row's f and its first kWideFar far values one column ahead; round items
+// two rounds ahead and their L values one round ahead (rounds form one stream over all columns;
+// the host keeps L(:,k-1) out of column k's first round, so those loads never read a column
+// before it is written).
+template <typename T, bool kFactor>
+__global__ __launch_bounds__(64) void k_lu64(DevTopo t, DevWork w, int cap, int iter) {
+  constexpr int TW = kWideTile;
+  constexpr int IPR = kRoundItems;
+  // complex values (the linear initial guess, run once per batch) take the shallow pipeline
+  constexpr bool kDeep = sizeof(T) <= 8;
+  constexpr int MS = kDeep ? kWideStage : 4;
+  constexpr int MF = kWideFar;
+  const int tile = blockIdx.x, s = threadIdx.x;
+  if (!tile_running<TW>(w.status, tile)) return;
+  const int sc = tile * TW + s;
+  T* __restrict__ lu = reinterpret_cast<T*>(w.lu) + at<TW>(tile, t.nnz_lu, 0) + s;
+  T* __restrict__ y = reinterpret_cast<T*>(w.y) + at<TW>(tile, t.n, 0) + s;
+  const int* __restrict__ lrow = t.lu_row;
+  const int2* __restrict__ uitems = t.uitems;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T* xc = reinterpret_cast<T*>(smem_raw);
+  T* ring = xc + (kFactor ? (size_t)(cap + 1) * TW : 0);
+  const int n = t.n, W = t.window, R = W + 1, nl = t.nnz_lu;
+  auto rs = [&](int r) { return (r & (R - 1)) * TW + s; };
+  auto xs = [&](int o) { return o * TW + s; };
+  auto cl = [](int v, int hi) { return v < 0 ? 0 : (v > hi ? hi : v); };
+  bool ok = true;
+  const T zero = zero_value<T>();
+
+  // ------------------------------------------------ forward pass (+ refactor)
+  for (int r = 0; r <= W && r < n; ++r) ring[rs(r)] = y[(size_t)r * TW];
+  // level 1 (two columns ahead): metadata {c0, len, ul, first round}, overflow range, far slots
+  int4 m1 = t.col_a[0], m2 = t.col_a[cl(1, n - 1)];
+  int4 q1 = t.col_f[0], q2 = t.col_f[cl(1, n - 1)];
+  int2 fs1[MF], fs2[MF];
+#pragma unroll
+  for (int i = 0; i < MF; ++i) {
+    fs1[i] = t.fslot[i];
+    fs2[i] = t.fslot[(size_t)cl(1, n - 1) * MF + i];
+  }
+  // level 2 (one column ahead): staged values (factor: A(:,k) from offset 0; solve: L(:,k)),
+  // L-part row indices, the entering row's f, far values
+  T nsv[MS], nfl[MF], nfy[MF], nfr;
+  int nlr[MS];
+  // (column kk's metadata in m1, its far slots in fs1)
+  auto lvl2_issue = [&](int kk) {
+    const int c0 = m1.x, len = m1.y, ul = m1.z;
+    const int f0 = kFactor ? 0 : ul + 1;
+#pragma unroll
+    for (int i = 0; i < MS; ++i) {
+      const int o = f0 + i;
+      nsv[i] = lu[(size_t)(c0 + (o < len ? o : 0)) * TW];
+    }
+#pragma unroll
+    for (int i = 0; i < MS; ++i) {
+      const int o = ul + 1 + i;
+      nlr[i] = lrow[c0 + (o < len ? o : 0)];
+    }
+#pragma unroll
+    for (int i = 0; i < MF; ++i) {
+      nfl[i] = lu[(size_t)cl(fs1[i].x, nl - 1) * TW];
+      nfy[i] = y[(size_t)cl(fs1[i].y, n - 1) * TW];
+    }
+    nfr = y[(size_t)cl(kk + W + 1, n - 1) * TW];
+  };
+  lvl2_issue(0);
+  // round stream: items two rounds ahead, L values one round ahead
+  const int n_rounds = kFactor ? t.uround[n] : 0;
+  int2 i1[kDeep ? IPR : 1], i2[kDeep ? IPR : 1];
+  T v1[kDeep ? IPR : 1];
+  if (kFactor && kDeep) {
+#pragma unroll
+    for (int i = 0; i < IPR; ++i) {
+      i1[i] = uitems[i];
+      i2[i] = uitems[(size_t)cl(1, n_rounds - 1) * IPR + i];
+    }
+#pragma unroll
+    for (int i = 0; i < IPR; ++i) v1[i] = lu[(size_t)(i1[i].y < 0 ? 0 : i1[i].y) * TW];
+  }
+  for (int k = 0; k < n; ++k) {
+    // this column's data; the next columns' loads
+    const int4 mk = m1, qk = q1;
+    int2 cfs[MF];
+    T cvs[MS], cfl[MF], cfy[MF];
+    int clr[MS];
+#pragma unroll
+    for (int i = 0; i < MS; ++i) {
+      cvs[i] = nsv[i];
+      clr[i] = nlr[i];
+    }
+#pragma unroll
+    for (int i = 0; i < MF; ++i) {
+      cfs[i] = fs1[i];
+      cfl[i] = nfl[i];
+      cfy[i] = nfy[i];
+      fs1[i] = fs2[i];
+    }
+    const T cfr = nfr;
+    m1 = m2;
+    q1 = q2;
+    {
+      const int k2 = cl(k + 2, n - 1);
+      m2 = t.col_a[k2];
+      q2 = t.col_f[k2];
+#pragma unroll
+      for (int i = 0; i < MF; ++i) fs2[i] = t.fslot[(size_t)k2 * MF + i];
+    }
+    if (k + 1 < n) lvl2_issue(k + 1);
+    const int c0 = mk.x, len = mk.y, ul = mk.z;
+    if (kFactor) {
+      if (len > cap) {  // the host sizes cap to the longest column; kept as a guard
+        ok = false;
+        break;
+      }
+#pragma unroll
+      for (int i = 0; i < MS; ++i)
+        if (i < len) xc[xs(i)] = cvs[i];
+      for (int o = MS; o < len; ++o) xc[xs(o)] = lu[(size_t)(c0 + o) * TW];
+      const int r0 = mk.w, r1 = k + 1 < n ? m1.w : n_rounds;
+      for (int rho = r0; rho < r1; ++rho) {
+        int2 it[IPR];
+        T lv[IPR];
+        if constexpr (kDeep) {
+#pragma unroll
+          for (int i = 0; i < IPR; ++i) {
+            it[i] = i1[i];
+            lv[i] = i1[i].y < 0 ? zero : v1[i];
+            i1[i] = i2[i];
+          }
+#pragma unroll
+          for (int i = 0; i < IPR; ++i) v1[i] = lu[(size_t)(i1[i].y < 0 ? 0 : i1[i].y) * TW];
+          const int rn = cl(rho + 2, n_rounds - 1);
+#pragma unroll
+          for (int i = 0; i < IPR; ++i) i2[i] = uitems[(size_t)rn * IPR + i];
+        } else {
+#pragma unroll
+          for (int i = 0; i < IPR; ++i) {
+            it[i] = uitems[(size_t)rho * IPR + i];
+            lv[i] = it[i].y < 0 ? zero : lu[(size_t)(it[i].y < 0 ? 0 : it[i].y) * TW];
+          }
+        }
+        if (it[0].y >= 0) {  // empty rounds (columns without updates) do nothing
+          T xt[IPR], xu[IPR];
+#pragma unroll
+          for (int i = 0; i < IPR; ++i) {
+            xt[i] = xc[xs(it[i].x & 0xffff)];
+            xu[i] = xc[xs(it[i].x >> 16)];
+          }
+#pragma unroll
+          for (int i = 0; i < IPR; ++i) xc[xs(it[i].x & 0xffff)] = xt[i] - lv[i] * xu[i];
+        }
+      }
+      const T ukk = xc[xs(ul)];
+      if (is_zero(ukk)) ok = false;
+      for (int o = ul + 1; o < len; ++o) xc[xs(o)] = xc[xs(o)] / ukk;
+      for (int o = 0; o < len; ++o) lu[(size_t)(c0 + o) * TW] = xc[xs(o)];
+    }
+    // forward substitution: y_k is final
+    const T yk = ring[rs(k)];
+    y[(size_t)k * TW] = yk;
+    // the row entering the window after column k: f minus its far L entries, ascending columns
+    const int rin = k + W + 1;
+    if (rin < n) {
+      T v = cfr;
+#pragma unroll
+      for (int i = 0; i < MF; ++i) {
+        if (cfs[i].x >= 0) {
+          const int p = cfs[i].x, j = cfs[i].y;
+          T lj = cfl[i], yj = cfy[i];
+          if (kFactor && j == k) lj = xc[xs(p - c0)];
+          if (kFactor && j == k - 1) lj = lu[(size_t)p * TW];
+          if (j == k) yj = yk;
+          if (j == k - 1) yj = y[(size_t)j * TW];
+          v = v - lj * yj;
+        }
+      }
+      for (int q = qk.y; q < qk.z; ++q) {
+        const int p = t.fwd_pos[q], j = t.fwd_col[q];
+        const T lj = (kFactor && j == k) ? xc[xs(p - c0)] : lu[(size_t)p * TW];
+        const T yj = j == k ? yk : y[(size_t)j * TW];
+        v = v - lj * yj;
+      }
+      ring[rs(rin)] = v;
+    }
+    // push L(:,k) into the window rows (farther rows are gathered when they enter)
+#pragma unroll
+    for (int i = 0; i < MS; ++i) {
+      const int o = ul + 1 + i;
+      if (o < len) {
+        const int r = clr[i];
+        const T lv = kFactor ? xc[xs(o)] : cvs[i];
+        if (r - k <= W) ring[rs(r)] = ring[rs(r)] - lv * yk;
+      }
+    }
+    for (int o = ul + 1 + MS; o < len; ++o) {
+      const int r = lrow[c0 + o];
+      const T lv = kFactor ? xc[xs(o)] : lu[(size_t)(c0 + o) * TW];
+      if (r - k <= W) ring[rs(r)] = ring[rs(r)] - lv * yk;
+    }
+  }
+
+  // ------------------------------------------------ back substitution
+  if (ok) {
+    for (int r = n - 1; r >= 0 && r >= n - 1 - W; --r) ring[rs(r)] = y[(size_t)r * TW];
+    // level 1: metadata, overflow range and far slots of column k - 2; level 2: U values and rows,
+    // pivot, the entering row's y and far values of column k - 1
+    int4 b1 = t.col_a[n - 1], b2 = t.col_a[cl(n - 2, n - 1)];
+    int4 e1 = t.col_b[n - 1], e2 = t.col_b[cl(n - 2, n - 1)];
+    int2 bs1[MF], bs2[MF];
+#pragma unroll
+    for (int i = 0; i < MF; ++i) {
+      bs1[i] = t.bslot[(size_t)(n - 1) * MF + i];
+      bs2[i] = t.bslot[(size_t)cl(n - 2, n - 1) * MF + i];
+    }
+    T nuv[MS], npv, nbl[MF], nbx[MF], nyr;
+    int nur[MS];
+    // (column kk's metadata in b1, its far slots in bs1)
+    auto b2_issue = [&](int kk) {
+      const int c0 = b1.x, ul = b1.z;
+#pragma unroll
+      for (int i = 0; i < MS; ++i) {
+        const int p = c0 + (i < ul ? i : 0);
+        nuv[i] = lu[(size_t)p * TW];
+        nur[i] = lrow[p];
+      }
+      npv = lu[(size_t)(c0 + ul) * TW];
+#pragma unroll
+      for (int i = 0; i < MF; ++i) {
+        nbl[i] = lu[(size_t)cl(bs1[i].x, nl - 1) * TW];
+        nbx[i] = y[(size_t)cl(bs1[i].y, n - 1) * TW];
+      }
+      nyr = y[(size_t)cl(kk - W - 1, n - 1) * TW];
+    };
+    b2_issue(n - 1);
+    for (int k = n - 1; k >= 0; --k) {
+      const int4 mk = b1, ek = e1;
+      int2 cbs[MF];
+      T cuv[MS], cbl[MF], cbx[MF];
+      int cur_[MS];
+#pragma unroll
+      for (int i = 0; i < MS; ++i) {
+        cuv[i] = nuv[i];
+        cur_[i] = nur[i];
+      }
+#pragma unroll
+      for (int i = 0; i < MF; ++i) {
+        cbs[i] = bs1[i];
+        cbl[i] = nbl[i];
+        cbx[i] = nbx[i];
+        bs1[i] = bs2[i];
+      }
+      const T cpv = npv, cyr = nyr;
+      b1 = b2;
+      e1 = e2;
+      {
+        const int k2 = cl(k - 2, n - 1);
+        b2 = t.col_a[k2];
+        e2 = t.col_b[k2];
+#pragma unroll
+        for (int i = 0; i < MF; ++i) bs2[i] = t.bslot[(size_t)k2 * MF + i];
+      }
+      if (k > 0) b2_issue(k - 1);
+      const int c0 = mk.x, ul = mk.z;
+      const T xk = ring[rs(k)] / cpv;
+      if (!finite_value(xk)) ok = false;
+      y[(size_t)k * TW] = xk;
+      const int rin = k - W - 1;
+      if (rin >= 0) {
+        T v = cyr;
+#pragma unroll
+        for (int i = 0; i < MF; ++i) {
+          if (cbs[i].x >= 0) {
+            const int j = cbs[i].y;
+            T xj = cbx[i];
+            if (j == k) xj = xk;
+            if (j == k + 1) xj = y[(size_t)j * TW];
+            v = v - cbl[i] * xj;
+          }
+        }
+        for (int q = ek.x; q < ek.y; ++q) {
+          const int j = t.bwd_col[q];
+          const T xj = j == k ? xk : y[(size_t)j * TW];
+          v = v - lu[(size_t)t.bwd_pos[q] * TW] * xj;
+        }
+        ring[rs(rin)] = v;
+      }
+#pragma unroll
+      for (int i = 0; i < MS; ++i) {
+        if (i < ul) {
+          const int r = cur_[i];
+          if (k - r <= W) ring[rs(r)] = ring[rs(r)] - cuv[i] * xk;  // farther rows: gathered on entry
+        }
+      }
+      for (int u = MS; u < ul; ++u) {
+        const int r = lrow[c0 + u];
+        if (k - r <= W) ring[rs(r)] = ring[rs(r)] - lu[(size_t)(c0 + u) * TW] * xk;
+      }
+    }
+  }
+  if (!ok && w.status[sc] == kRunning) {
     w.status[sc] = kSingular;
     w.iters[sc] = iter + 1;
   }
@@ -732,15 +1214,14 @@ void with_tile(int tile, F&& f) {
   switch (tile) {
     case 4: f(std::integral_constant<int, 4>{}); break;
     case 8: f(std::integral_constant<int, 8>{}); break;
+    case kWideTile: f(std::integral_constant<int, kWideTile>{}); break;
     default: f(std::integral_constant<int, 16>{}); break;
   }
 }
 
 template <typename T>
 size_t lu_bytes(const DevTopo& t, int tile, bool factor) {
-  const size_t cap = t.max_col < 1 ? 1 : t.max_col;
-  const size_t rows = (size_t)t.window + kRingGap + 1;
-  return ((factor ? cap : 0) + rows) * (size_t)(tile + 1) * sizeof(T);
+  return lu_lds_bytes(t.max_col, t.window + kRingGap + 1, tile, sizeof(T), factor);
 }
 
 // Raises a kernel's dynamic-LDS limit to what this launch needs (once per larger size).
@@ -756,7 +1237,16 @@ void run_lu(const DevTopo& t, const DevWork& w, bool factor, int iter, hipStream
   static size_t granted_f = 65536, granted_s = 65536;
   const int cap = t.max_col < 1 ? 1 : t.max_col;
   const size_t shmem = lu_bytes<T>(t, TW, factor);
-  if (factor) {
+  if constexpr (TW == kWideTile) {
+    if (factor) {
+      allow_lds(k_lu64<T, true>, shmem, granted_f);
+      hipLaunchKernelGGL((k_lu64<T, true>), dim3(w.n_tiles), dim3(64), shmem, stream, t, w, cap, iter);
+    } else {
+      allow_lds(k_lu64<T, false>, shmem, granted_s);
+      hipLaunchKernelGGL((k_lu64<T, false>), dim3(w.n_tiles), dim3(64), shmem, stream, t, w, cap, iter);
+    }
+    return;
+  } else if (factor) {
     allow_lds(k_lu<T, TW, true>, shmem, granted_f);
     hipLaunchKernelGGL((k_lu<T, TW, true>), dim3(w.n_tiles), dim3(64), shmem, stream, t, w, cap, iter);
   } else {

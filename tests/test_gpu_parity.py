@@ -97,8 +97,8 @@ def test_three_bus_linear_start_matches_doc_step(three_bus, known, oracle_mod):
 
 # launch shapes: (scenarios per tile, solve ring rows); 0 = automatic.  A 64-row ring gathers
 # every triangular-solve entry that reaches farther than 31 rows.
-SHAPES_C1 = [(0, 0), (8, 0), (16, 64)]
-SHAPES_C2 = [(0, 0), (16, 0), (4, 64)]
+SHAPES_C1 = [(0, 0), (8, 0), (16, 64), (64, 0), (64, 64)]
+SHAPES_C2 = [(0, 0), (16, 0), (4, 64), (64, 0)]
 
 
 @pytest.mark.parametrize("shape", SHAPES_C1)
