@@ -486,11 +486,13 @@ int pick_ring_rows(int max_col, int tile, size_t elem, int requested) {
 bool valid_tile(int tile) { return tile == 4 || tile == 8 || tile == 16 || tile == gx::kWideTile; }
 bool valid_ring(int rows) { return rows == 0 || (rows >= 64 && rows <= 4096 && (rows & (rows - 1)) == 0); }
 
-// Scenarios per tile when the caller leaves it open: the widest tile that still gives at least
-// 512 wavefronts (two per CU), else the narrowest.
-int auto_tile(int n_scen) {
-  for (int tw : {16, 8}) if ((n_scen + tw - 1) / tw >= 512) return tw;
-  return 4;
+// Scenarios per tile when the caller leaves it open: the narrowest tile whose wavefronts can all
+// be resident at once (1024 at FP64, 2048 at FP32: four or eight per CU within the LU kernel's LDS
+// and register budget), else the next wider one.
+int auto_tile(int n_scen, bool fp32) {
+  const int resident = fp32 ? 2048 : 1024;
+  for (int tw : {4, 8}) if ((n_scen + tw - 1) / tw <= resident) return tw;
+  return 16;
 }
 
 // Position of (pivot row r, pivot column k) inside the packed column k, or -1.
@@ -1209,7 +1211,7 @@ int gx_batch_solve(gx_topo* t, int n_scen, const double* p_spec, const double* q
   const size_t plane = (size_t)n_scen * nb * sizeof(double);
   int rc;
   const bool fp32 = mode == GX_MODE_MIXED;
-  const int tile = t->tile_req ? t->tile_req : auto_tile(n_scen);
+  const int tile = t->tile_req ? t->tile_req : auto_tile(n_scen, fp32);
   const int n_tiles = (n_scen + tile - 1) / tile;
   const size_t lanes = (size_t)n_tiles * tile;
   if ((rc = upload_ybus(*t)) || (rc = ensure_workspace(*t, lanes, max_iter, fp32))) return rc;

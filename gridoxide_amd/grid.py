@@ -222,6 +222,28 @@ def scenario_specs(net: Network, n_scen: int, seed: int):
     return p, q
 
 
+def scenario_loads(net: Network, n_scen: int, seed: int):
+    """Per-scenario ``(p_spec, q_spec)`` drawn from the grid generator's own load distribution
+    (PQ loads P~U[0.05,0.5] pu, Q = P*U[0.1,0.4]; PV generation 0.95 of the total load split in
+    proportion to fresh U[0.5,1.5] weights), the scenario model of the mixed-precision, uneven-batch
+    and serving workloads.  Scenario ``s`` draws from its own generator seeded ``(seed, s)``."""
+    n = net.n_bus
+    pq = np.nonzero(net.bus_type == PQ)[0]
+    pv = np.nonzero(net.bus_type == PV)[0]
+    p = np.tile(net.p_spec, (n_scen, 1))
+    q = np.tile(net.q_spec, (n_scen, 1))
+    for s in range(n_scen):
+        rng = np.random.default_rng([seed, s])
+        lp = rng.uniform(0.05, 0.5, pq.size)
+        lq = lp * rng.uniform(0.1, 0.4, pq.size)
+        p[s, pq] = -lp
+        q[s, pq] = -lq
+        if pv.size:
+            w = rng.uniform(0.5, 1.5, pv.size)
+            p[s, pv] = 0.95 * lp.sum() * w / w.sum()
+    return p, q
+
+
 def config_grid(config: int, seed: int = 20261203) -> Network:
     """The grid of workload ``config`` (1: 200 buses, 2: 2,000 buses, 3: 10,000 buses)."""
     sizes = {1: (200, 100), 2: (2000, 1000), 3: (10000, 5000)}

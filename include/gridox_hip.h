@@ -149,7 +149,8 @@ int gx_topo_profile(gx_topo* t, int enable, double* ms, int* launches);
 
 /* Launch shape of later gx_batch_solve calls on t (no counterpart in the reference, whose CPU
  * workers have no such knob; this is the setting a tuner sweeps).  tile: scenarios per
- * wavefront tile, 4, 8 or 16 (0 = automatic: the widest tile that still yields 512 tiles).
+ * wavefront tile, 4, 8, 16 (0 = automatic: the narrowest tile whose wavefronts can all be resident
+ * at once), or 64 (one lane per scenario).
  * ring_rows: rows of the LDS window that holds the right-hand side during the triangular solves,
  * a power of two in [64, 4096] (0 = the default for the tile); it is halved while the kernel's
  * LDS would exceed 160 KB.  Results do not depend on either setting beyond rounding of the
