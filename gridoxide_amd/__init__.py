@@ -3,7 +3,7 @@
 The numeric path lives in ``libgridox_hip.so`` (C ABI: ``include/gridox_hip.h``); this package
 holds the workload generator and the host mirror of the reference's solver interface.
 """
-from .grid import PQ, PV, SLACK, Network, config_grid, generate_grid, scenario_loads, scenario_specs  # noqa: F401
+from .grid import PQ, PV, SLACK, Network, config_grid, generate_grid, scenario_loads, scenario_specs, without_branch  # noqa: F401
 from .solver import (  # noqa: F401
     BatchError,
     BatchSolver,

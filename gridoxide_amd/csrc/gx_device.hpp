@@ -71,6 +71,11 @@ struct DevTopo {
   const int* lu_bk = nullptr;
   const double* lu_g = nullptr;
   const double* lu_b = nullptr;
+  const int* lu_yp = nullptr;   // per packed position: the Y-bus position its recipe reads (-1: fill)
+  // branches (per-scenario outages): Y-bus positions of the ff, ft, tf, tt stamps (-1: none) and
+  // their values (re, im per stamp)
+  const int4* br_pos = nullptr;
+  const double* br_y = nullptr;
   // LU pattern
   const int* cp = nullptr;      // n+1
   const int* ulen = nullptr;    // n
@@ -165,6 +170,7 @@ struct DevWork {
   double* vm0 = nullptr;                 // starting point kept for the FP64 re-solve (mixed mode)
   double* va0 = nullptr;
   int* fallback = nullptr;               // [1] scenarios handed to the FP64 re-solve
+  const int* outage = nullptr;           // [n_scen] branch out of service per scenario, -1 none (or null)
 };
 
 // Launchers (gx_kernels.hip).  All run on `stream`; none synchronizes.  `fp32` selects the

@@ -536,6 +536,98 @@ void triplets_to_csc(int n, const std::vector<int>& rows, const std::vector<int>
   for (int j = 0; j < n; ++j) a.ap[j + 1] += a.ap[j];
 }
 
+// The four Y-bus contributions of one branch (build_ybus, src/network.rs:225-247; transformers
+// by branch_calc_param (1,1), src/network.rs:339-360): y[0..3] = ff, ft, tf, tt.  A line from a
+// bus to itself is a pure shunt (ff only).
+struct BranchStamp {
+  int f = 0, t = 0;
+  bool transformer = false;
+  cx y[4];
+};
+
+int branch_stamps(int n_bus, int n_line, const int* lf, const int* lt, const double* r, const double* x,
+                  const double* bsh, const double* gsh, int n_tr, const int* tf, const int* tt, const double* ys,
+                  const double* ysh, const double* tap, std::vector<BranchStamp>& out) {
+  out.clear();
+  out.reserve((size_t)n_line + n_tr);
+  for (int e = 0; e < n_line; ++e) {
+    if (lf[e] < 0 || lf[e] >= n_bus || lt[e] < 0 || lt[e] >= n_bus) return fail(GX_ERR_ARG, "line bus out of range");
+    BranchStamp b;
+    b.f = lf[e];
+    b.t = lt[e];
+    if (lf[e] == lt[e]) {
+      b.y[0] = {gsh ? gsh[e] : 0.0, bsh[e]};
+      b.y[1] = b.y[2] = b.y[3] = {0.0, 0.0};
+    } else {
+      cx yl = cdiv({1.0, 0.0}, {r[e], x[e]});
+      cx half{(gsh ? gsh[e] : 0.0) / 2.0, bsh[e] / 2.0};
+      b.y[0] = cadd(yl, half);
+      b.y[3] = cadd(yl, half);
+      b.y[1] = cneg(yl);
+      b.y[2] = cneg(yl);
+    }
+    out.push_back(b);
+  }
+  for (int e = 0; e < n_tr; ++e) {
+    if (tf[e] < 0 || tf[e] >= n_bus || tt[e] < 0 || tt[e] >= n_bus) return fail(GX_ERR_ARG, "transformer bus out of range");
+    cx s{ys[2 * e], ys[2 * e + 1]}, sh{ysh[2 * e], ysh[2 * e + 1]}, a{tap[2 * e], tap[2 * e + 1]};
+    double k = std::hypot(a.re, a.im);
+    cx yd = cadd(s, {sh.re * 0.5, sh.im * 0.5});
+    BranchStamp b;
+    b.f = tf[e];
+    b.t = tt[e];
+    b.transformer = true;
+    b.y[0] = cdiv(yd, {k * k, 0.0});
+    b.y[1] = cdiv(cneg(s), cconj(a));
+    b.y[2] = cdiv(cneg(s), a);
+    b.y[3] = yd;
+    out.push_back(b);
+  }
+  return GX_OK;
+}
+
+// Bridges of the branch multigraph (self-loops never are): removing one splits an island.
+std::vector<char> find_bridges(int n_bus, const std::vector<BranchStamp>& br) {
+  std::vector<std::vector<std::pair<int, int>>> adj(n_bus);
+  for (int e = 0; e < (int)br.size(); ++e)
+    if (br[e].f != br[e].t) {
+      adj[br[e].f].push_back({br[e].t, e});
+      adj[br[e].t].push_back({br[e].f, e});
+    }
+  std::vector<char> bridge(br.size(), 0);
+  std::vector<int> disc(n_bus, -1), low(n_bus, 0);
+  int timer = 0;
+  // iterative DFS: (vertex, edge used to enter, next adjacency index)
+  std::vector<std::tuple<int, int, size_t>> st;
+  for (int r0 = 0; r0 < n_bus; ++r0) {
+    if (disc[r0] >= 0) continue;
+    disc[r0] = low[r0] = timer++;
+    st.emplace_back(r0, -1, 0);
+    while (!st.empty()) {
+      auto& [v, pe, idx] = st.back();
+      if (idx < adj[v].size()) {
+        auto [u, e] = adj[v][idx++];
+        if (e == pe) continue;
+        if (disc[u] < 0) {
+          disc[u] = low[u] = timer++;
+          st.emplace_back(u, e, 0);
+        } else {
+          low[v] = std::min(low[v], disc[u]);
+        }
+      } else {
+        const int vv = v, ee = pe;
+        st.pop_back();
+        if (!st.empty()) {
+          const int parent = std::get<0>(st.back());
+          low[parent] = std::min(low[parent], low[vv]);
+          if (low[vv] > disc[parent]) bridge[ee] = 1;
+        }
+      }
+    }
+  }
+  return bridge;
+}
+
 }  // namespace
 
 struct gx_topo {
@@ -591,6 +683,14 @@ struct gx_topo {
   gx::DevLg dlg;
   gx::DevTopo dlgt;
   bool ybus_uploaded = false;
+  // branches for per-scenario outages (gx_topo_set_branches)
+  int n_br = 0;
+  std::vector<int> br_pos;     // 4 per branch: Y CSR positions of ff, ft, tf, tt (-1 if none)
+  std::vector<double> br_y;    // 8 per branch
+  std::vector<char> br_bridge;
+  std::vector<int> h_yp;       // per L+U position: the Y CSR position its recipe reads (-1 for fill)
+  DevMem d_br_pos, d_br_y, d_yp, w_outage;
+  bool br_uploaded = false;
   // the last gx_batch_solve (for island reports)
   gx::DevWork last;
   bool has_last = false;
@@ -607,6 +707,13 @@ struct gx_topo {
 };
 
 namespace {
+
+// Position of Y(i, j) in the CSR Y-bus (rows sorted by column), or -1.
+int ybus_position(const gx_topo& t, int i, int j) {
+  auto b = t.ycol.begin() + t.yrow[i], e = t.ycol.begin() + t.yrow[i + 1];
+  auto it = std::lower_bound(b, e, j);
+  return (it != e && *it == j) ? (int)(it - t.ycol.begin()) : -1;
+}
 
 // Host FP64 injections for one scenario (used only to fix the pivot order).
 void host_injections(const gx_topo& t, const double* vm, const double* va, std::vector<double>& p,
@@ -660,13 +767,14 @@ int finalize_factor(gx_topo& t) {
   const int nnz = lu.nnz();
   std::vector<int> qinv(t.n);
   for (int k = 0; k < t.n; ++k) qinv[lu.q[k]] = k;
-  std::vector<int> kind(nnz, gx::kFill), bi(nnz, 0), bk(nnz, 0);
+  std::vector<int> kind(nnz, gx::kFill), bi(nnz, 0), bk(nnz, 0), yp(nnz, -1);
   std::vector<double> g(nnz, 0.0), b(nnz, 0.0);
   for (size_t e = 0; e < t.jkind.size(); ++e) {
     int pos = lu_position(lu, lu.pinv[t.jrow[e]], qinv[t.jcol[e]]);
     if (pos < 0) return fail(GX_ERR_ARG, "Jacobian entry missing from the L+U pattern");
     if (kind[pos] != gx::kFill) return fail(GX_ERR_UNSUPPORTED, "duplicate Jacobian entries (unmerged Y-bus)");
     kind[pos] = t.jkind[e];
+    yp[pos] = ybus_position(t, t.jbi[e], t.jbk[e]);
     bi[pos] = t.jbi[e];
     bk[pos] = t.jbk[e];
     g[pos] = t.jg[e];
@@ -684,6 +792,7 @@ int finalize_factor(gx_topo& t) {
     }
   }
   t.h_kind.swap(kind);
+  t.h_yp.swap(yp);
   t.h_bi.swap(bi);
   t.h_bk.swap(bk);
   t.h_g.swap(g);
@@ -709,9 +818,10 @@ int upload_topo(gx_topo& t, int window, int tile) {
   if ((rc = upload_ybus(t)) || (rc = t.d_rpos_p.upload(t.h_rp)) ||
       (rc = t.d_rpos_q.upload(t.h_rq)) || (rc = t.d_xpos_a.upload(t.h_xa)) || (rc = t.d_xpos_m.upload(t.h_xm)) ||
       (rc = t.d_kind.upload(t.h_kind)) || (rc = t.d_bi.upload(t.h_bi)) || (rc = t.d_bk.upload(t.h_bk)) ||
-      (rc = t.d_g.upload(t.h_g)) || (rc = t.d_b.upload(t.h_b)))
+      (rc = t.d_g.upload(t.h_g)) || (rc = t.d_b.upload(t.h_b)) || (rc = t.d_yp.upload(t.h_yp)))
     return rc;
   gx::DevTopo& d = t.dt;
+  d.lu_yp = t.d_yp.as<int>();
   d.rpos_p = t.d_rpos_p.as<int>();
   d.rpos_q = t.d_rpos_q.as<int>();
   d.xpos_a = t.d_xpos_a.as<int>();
@@ -752,6 +862,16 @@ int ensure_workspace(gx_topo& t, size_t lanes, int max_iter, bool fp32) {
   t.ws_lanes = lanes;
   t.ws_iter = max_iter;
   t.ws_fp32 = (int)fp32;
+  return GX_OK;
+}
+
+int upload_branches(gx_topo& t) {
+  if (t.br_uploaded || t.n_br == 0) return GX_OK;
+  int rc;
+  if ((rc = t.d_br_pos.upload(t.br_pos)) || (rc = t.d_br_y.upload(t.br_y))) return rc;
+  t.dt.br_pos = t.d_br_pos.as<int4>();
+  t.dt.br_y = t.d_br_y.as<double>();
+  t.br_uploaded = true;
   return GX_OK;
 }
 
@@ -926,6 +1046,9 @@ int gx_ybus_build(int n_bus, int n_line, const int* lf, const int* lt, const dou
                   const double* bsh, const double* gsh, int n_tr, const int* tf, const int* tt, const double* ys,
                   const double* ysh, const double* tap, int cap, int* rowptr, int* col, double* y_re, double* y_im) {
   if (n_bus <= 0 || n_line < 0 || n_tr < 0 || !rowptr) return fail(GX_ERR_ARG, "bad sizes");
+  std::vector<BranchStamp> br;
+  int rc = branch_stamps(n_bus, n_line, lf, lt, r, x, bsh, gsh, n_tr, tf, tt, ys, ysh, tap, br);
+  if (rc) return rc;
   // insertion-ordered accumulation per (row, col)
   std::map<std::pair<int, int>, cx> acc;
   auto add = [&](int i, int j, cx v) {
@@ -933,28 +1056,18 @@ int gx_ybus_build(int n_bus, int n_line, const int* lf, const int* lt, const dou
     if (it == acc.end()) acc.emplace(std::make_pair(i, j), cadd({0.0, 0.0}, v));
     else it->second = cadd(it->second, v);
   };
-  for (int e = 0; e < n_line; ++e) {
-    if (lf[e] < 0 || lf[e] >= n_bus || lt[e] < 0 || lt[e] >= n_bus) return fail(GX_ERR_ARG, "line bus out of range");
-    if (lf[e] == lt[e]) {
-      add(lf[e], lf[e], {gsh ? gsh[e] : 0.0, bsh[e]});
-      continue;
+  for (const BranchStamp& b : br) {
+    add(b.f, b.f, b.y[0]);
+    if (b.f == b.t) continue;
+    if (b.transformer) {  // stamp order of stamp_transformers: ff, ft, tf, tt
+      add(b.f, b.t, b.y[1]);
+      add(b.t, b.f, b.y[2]);
+      add(b.t, b.t, b.y[3]);
+    } else {  // line order: ff, tt, ft, tf
+      add(b.t, b.t, b.y[3]);
+      add(b.f, b.t, b.y[1]);
+      add(b.t, b.f, b.y[2]);
     }
-    cx yl = cdiv({1.0, 0.0}, {r[e], x[e]});
-    cx half{(gsh ? gsh[e] : 0.0) / 2.0, bsh[e] / 2.0};
-    add(lf[e], lf[e], cadd(yl, half));
-    add(lt[e], lt[e], cadd(yl, half));
-    add(lf[e], lt[e], cneg(yl));
-    add(lt[e], lf[e], cneg(yl));
-  }
-  for (int e = 0; e < n_tr; ++e) {
-    if (tf[e] < 0 || tf[e] >= n_bus || tt[e] < 0 || tt[e] >= n_bus) return fail(GX_ERR_ARG, "transformer bus out of range");
-    cx s{ys[2 * e], ys[2 * e + 1]}, sh{ysh[2 * e], ysh[2 * e + 1]}, a{tap[2 * e], tap[2 * e + 1]};
-    double k = std::hypot(a.re, a.im);
-    cx yd = cadd(s, {sh.re * 0.5, sh.im * 0.5});
-    add(tf[e], tf[e], cdiv(yd, {k * k, 0.0}));
-    add(tf[e], tt[e], cdiv(cneg(s), cconj(a)));
-    add(tt[e], tf[e], cdiv(cneg(s), a));
-    add(tt[e], tt[e], yd);
   }
   int nnz = (int)acc.size();
   if (nnz > cap || !col || !y_re || !y_im) return -nnz;
@@ -1197,9 +1310,49 @@ int gx_topo_lu_pattern(const gx_topo* t, int* cp, int* ulen, int* row) {
   return GX_OK;
 }
 
+int gx_topo_set_branches(gx_topo* t, int n_line, const int* lf, const int* lt, const double* r, const double* x,
+                         const double* bsh, const double* gsh, int n_tr, const int* tf, const int* tt,
+                         const double* ys, const double* ysh, const double* tap) {
+  if (!t || n_line < 0 || n_tr < 0) return fail(GX_ERR_ARG, "bad branch list");
+  std::vector<BranchStamp> br;
+  int rc = branch_stamps(t->n_bus, n_line, lf, lt, r, x, bsh, gsh, n_tr, tf, tt, ys, ysh, tap, br);
+  if (rc) return rc;
+  std::vector<int> pos(4 * br.size(), -1);
+  std::vector<double> y(8 * br.size(), 0.0);
+  for (size_t e = 0; e < br.size(); ++e) {
+    const BranchStamp& b = br[e];
+    const int rc4[4][2] = {{b.f, b.f}, {b.f, b.t}, {b.t, b.f}, {b.t, b.t}};
+    for (int q = 0; q < (b.f == b.t ? 1 : 4); ++q) {
+      pos[4 * e + q] = ybus_position(*t, rc4[q][0], rc4[q][1]);
+      if (pos[4 * e + q] < 0) return fail(GX_ERR_ARG, "branch " + std::to_string(e) + " is not in the Y-bus pattern");
+      y[8 * e + 2 * q] = b.y[q].re;
+      y[8 * e + 2 * q + 1] = b.y[q].im;
+    }
+  }
+  t->n_br = (int)br.size();
+  t->br_pos.swap(pos);
+  t->br_y.swap(y);
+  t->br_bridge = find_bridges(t->n_bus, br);
+  t->br_uploaded = false;
+  return GX_OK;
+}
+
+int gx_topo_branch_is_bridge(const gx_topo* t, int branch) {
+  if (!t) return fail(GX_ERR_ARG, "null");
+  if (branch < 0 || branch >= t->n_br) return fail(GX_ERR_ARG, "branch out of range");
+  return t->br_bridge[branch] ? 1 : 0;
+}
+
 int gx_batch_solve(gx_topo* t, int n_scen, const double* p_spec, const double* q_spec, const double* vm0,
                    const double* va0, double tol, int max_iter, int mode, int flags, double* vm, double* va,
                    int* status, int* iters, double* history, void* stream_ptr) {
+  return gx_batch_solve_outages(t, n_scen, p_spec, q_spec, vm0, va0, nullptr, tol, max_iter, mode, flags, vm, va,
+                                status, iters, history, stream_ptr);
+}
+
+int gx_batch_solve_outages(gx_topo* t, int n_scen, const double* p_spec, const double* q_spec, const double* vm0,
+                           const double* va0, const int* outage, double tol, int max_iter, int mode, int flags,
+                           double* vm, double* va, int* status, int* iters, double* history, void* stream_ptr) {
   if (!t) return fail(GX_ERR_ARG, "null topology");
   if (n_scen < 0 || max_iter < 0 || !(tol >= 0.0)) return fail(GX_ERR_ARG, "bad n_scen/max_iter/tol");
   if (mode != GX_MODE_FP64 && mode != GX_MODE_MIXED) return fail(GX_ERR_ARG, "unknown mode");
@@ -1211,6 +1364,24 @@ int gx_batch_solve(gx_topo* t, int n_scen, const double* p_spec, const double* q
   const size_t plane = (size_t)n_scen * nb * sizeof(double);
   int rc;
   const bool fp32 = mode == GX_MODE_MIXED;
+  // one branch out of service per scenario (-1: none); bridges would split an island
+  std::vector<int> h_outage;
+  bool any_outage = false;
+  if (outage) {
+    h_outage.resize(n_scen);
+    if (dev) GX_HIP(hipMemcpy(h_outage.data(), outage, sizeof(int) * n_scen, hipMemcpyDeviceToHost));
+    else std::memcpy(h_outage.data(), outage, sizeof(int) * n_scen);
+    for (int s = 0; s < n_scen; ++s) {
+      const int e = h_outage[s];
+      if (e < -1 || e >= t->n_br)
+        return fail(t->n_br == 0 ? GX_ERR_STATE : GX_ERR_ARG,
+                    "scenario " + std::to_string(s) + ": outage branch out of range (gx_topo_set_branches first)");
+      if (e >= 0 && t->br_bridge[e])
+        return fail(GX_ERR_ARG, "scenario " + std::to_string(s) + ": branch " + std::to_string(e) +
+                                    " is a bridge; its outage splits an island");
+      any_outage = any_outage || e >= 0;
+    }
+  }
   const int tile = t->tile_req ? t->tile_req : auto_tile(n_scen, fp32);
   const int n_tiles = (n_scen + tile - 1) / tile;
   const size_t lanes = (size_t)n_tiles * tile;
@@ -1237,6 +1408,10 @@ int gx_batch_solve(gx_topo* t, int n_scen, const double* p_spec, const double* q
   w.maxmis = t->w_max.as<unsigned long long>();
   w.history = t->w_hist.as<double>();
   w.running = t->w_running.as<int>();
+  if (any_outage) {
+    if ((rc = upload_branches(*t)) || (rc = t->w_outage.upload(h_outage))) return rc;
+    w.outage = t->w_outage.as<int>();
+  }
 
   // inputs -> interleaved tiles
   const double* src[4] = {p_spec, q_spec, vm0, va0};

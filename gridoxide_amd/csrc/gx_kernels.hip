@@ -110,6 +110,46 @@ __device__ __forceinline__ bool all_groups(bool v) {
   return x != 0;
 }
 
+// The branch a scenario has out of service: its four Y-bus positions and stamps, subtracted from
+// that scenario's Y values wherever a kernel reads Y (the pattern is unchanged).
+struct Outage {
+  int4 pos = make_int4(-1, -1, -1, -1);
+  double re[4] = {0.0, 0.0, 0.0, 0.0}, im[4] = {0.0, 0.0, 0.0, 0.0};
+};
+__device__ __forceinline__ Outage load_outage(const DevTopo& t, const DevWork& w, int sc) {
+  Outage o;
+  if (w.outage) {
+    const int e = w.outage[sc < w.n_scen ? sc : w.n_scen - 1];
+    if (e >= 0) {
+      o.pos = t.br_pos[e];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        o.re[q] = t.br_y[8 * e + 2 * q];
+        o.im[q] = t.br_y[8 * e + 2 * q + 1];
+      }
+    }
+  }
+  return o;
+}
+// Y value at position p for this scenario
+__device__ __forceinline__ void y_at(const DevTopo& t, const Outage& o, int p, double& g, double& b) {
+  g = t.y_g[p];
+  b = t.y_b[p];
+  if (p == o.pos.x) {
+    g -= o.re[0];
+    b -= o.im[0];
+  } else if (p == o.pos.y) {
+    g -= o.re[1];
+    b -= o.im[1];
+  } else if (p == o.pos.z) {
+    g -= o.re[2];
+    b -= o.im[2];
+  } else if (p == o.pos.w) {
+    g -= o.re[3];
+    b -= o.im[3];
+  }
+}
+
 // ---------------------------------------------------------------- layout conversion
 // One block converts 64 elements of one tile: rows are read and written contiguously through LDS.
 template <int TW>
@@ -198,13 +238,15 @@ __global__ __launch_bounds__(256) void k_mismatch(DevTopo t, DevWork w) {
   const size_t bb = at<TW>(tile, t.n_bus, 0) + s;
   const size_t ub = at<TW>(tile, t.n, 0) + s;
   T* __restrict__ y = reinterpret_cast<T*>(w.y);
+  const Outage og = load_outage(t, w, sc);
   double mx = 0.0;
   for (int i = elem_first<TW>(); i < t.n_bus; i += elem_stride<TW>()) {
     double ir = 0.0, ii = 0.0;
     const int p2 = t.y_rowptr[i + 1];
     for (int p = t.y_rowptr[i]; p < p2; ++p) {
       const size_t oj = bb + (size_t)t.y_col[p] * TW;
-      const double g = t.y_g[p], b = t.y_b[p];
+      double g, b;
+      y_at(t, og, p, g, b);
       const double vr = w.vr[oj], vi = w.vi[oj];
       ir += g * vr - b * vi;
       ii += g * vi + b * vr;
@@ -302,12 +344,14 @@ __global__ __launch_bounds__(256) void k_fill(DevTopo t, DevWork w) {
   if (!tile_running<TW>(w.status, tile)) return;
   const size_t bb = at<TW>(tile, t.n_bus, 0) + s;
   T* __restrict__ lu = reinterpret_cast<T*>(w.lu) + at<TW>(tile, t.nnz_lu, 0) + s;
+  const Outage og = load_outage(t, w, tile * TW + s);
   for (int p = elem_first<TW>(); p < t.nnz_lu; p += elem_stride<TW>()) {
     const int kind = t.lu_kind[p];
     double v = 0.0;
     if (kind != kFill) {
       const int i = t.lu_bi[p], k = t.lu_bk[p];
-      const double g = t.lu_g[p], b = t.lu_b[p];
+      double g = t.lu_g[p], b = t.lu_b[p];
+      if (w.outage) y_at(t, og, t.lu_yp[p], g, b);
       const size_t oi = bb + (size_t)i * TW, ok = bb + (size_t)k * TW;
       const double vri = w.vr[oi], vii = w.vi[oi], vrk = w.vr[ok], vik = w.vi[ok];
       const double cc = vri * vrk + vii * vik;
@@ -1035,6 +1079,7 @@ __global__ __launch_bounds__(256) void k_resid(DevTopo t, DevWork w, const doubl
   const size_t bb = at<TW>(tile, t.n_bus, 0) + s;
   const size_t ub = at<TW>(tile, t.n, 0) + s;
   T* __restrict__ y = reinterpret_cast<T*>(w.y);
+  const Outage og = load_outage(t, w, tile * TW + s);
   for (int i = elem_first<TW>(); i < t.n_bus; i += elem_stride<TW>()) {
     const int rp = t.rpos_p[i], rq = t.rpos_q[i];
     if (rp < 0 && rq < 0) continue;
@@ -1042,7 +1087,8 @@ __global__ __launch_bounds__(256) void k_resid(DevTopo t, DevWork w, const doubl
     const int p2 = t.y_rowptr[i + 1];
     for (int p = t.y_rowptr[i]; p < p2; ++p) {
       const size_t oj = bb + (size_t)t.y_col[p] * TW;
-      const double g = t.y_g[p], b = t.y_b[p];
+      double g, b;
+      y_at(t, og, p, g, b);
       const double dr = dvr[oj], di = dvi[oj];
       ar += g * dr - b * di;
       ai += g * di + b * dr;
@@ -1101,6 +1147,7 @@ __global__ __launch_bounds__(256) void k_island_max(DevTopo t, DevWork w, const 
   const int tile = blockIdx.y, s = sub<TW>();
   const int sc = tile * TW + s;
   const size_t bb = at<TW>(tile, t.n_bus, 0) + s;
+  const Outage og = load_outage(t, w, sc);
   for (int i = elem_first<TW>(); i < t.n_bus; i += elem_stride<TW>()) {
     const int ty = t.bus_type[i];
     if (ty != 1 && ty != 2) continue;
@@ -1108,7 +1155,8 @@ __global__ __launch_bounds__(256) void k_island_max(DevTopo t, DevWork w, const 
     const int p2 = t.y_rowptr[i + 1];
     for (int p = t.y_rowptr[i]; p < p2; ++p) {
       const size_t oj = bb + (size_t)t.y_col[p] * TW;
-      const double g = t.y_g[p], b = t.y_b[p];
+      double g, b;
+      y_at(t, og, p, g, b);
       const double vr = w.vr[oj], vi = w.vi[oj];
       ir += g * vr - b * vi;
       ii += g * vi + b * vr;
@@ -1136,10 +1184,11 @@ __global__ __launch_bounds__(256) void k_lg_fill(DevTopo t, DevLg lg, DevWork w,
   }
   const size_t bb = at<TW>(tile, t.n_bus, 0) + s;
   zd* out = lu + at<TW>(tile, lg.nnz_lu, 0) + s;
+  const Outage og = load_outage(t, w, tile * TW + s);
   for (int p = elem_first<TW>(); p < lg.nnz_lu; p += elem_stride<TW>()) {
     zd v{0.0, 0.0};
     const int e = lg.src[p];
-    if (e >= 0) v = zd{t.y_g[e], t.y_b[e]};
+    if (e >= 0) y_at(t, og, e, v.re, v.im);
     const int d = lg.dbus[p];
     if (d >= 0) {
       const size_t o = bb + (size_t)d * TW;
@@ -1155,6 +1204,7 @@ __global__ __launch_bounds__(256) void k_lg_rhs(DevTopo t, DevLg lg, DevWork w, 
   const int tile = blockIdx.y, s = sub<TW>();
   const size_t bb = at<TW>(tile, t.n_bus, 0) + s;
   zd* out = y + at<TW>(tile, lg.m, 0) + s;
+  const Outage og = load_outage(t, w, tile * TW + s);
   for (int r = elem_first<TW>(); r < lg.m; r += elem_stride<TW>()) {
     zd b{0.0, 0.0};
     for (int q = lg.rhs_ptr[r]; q < lg.rhs_ptr[r + 1]; ++q) {
@@ -1163,7 +1213,9 @@ __global__ __launch_bounds__(256) void k_lg_rhs(DevTopo t, DevLg lg, DevWork w, 
       double sn, cs;
       sincos(w.va[oj], &sn, &cs);
       const zd u{w.vm[oj] * cs, w.vm[oj] * sn};
-      b = b - zd{t.y_g[e], t.y_b[e]} * u;
+      zd ye;
+      y_at(t, og, e, ye.re, ye.im);
+      b = b - ye * u;
     }
     out[(size_t)lg.row_piv[r] * TW] = b;
   }

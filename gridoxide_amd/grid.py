@@ -244,6 +244,21 @@ def scenario_loads(net: Network, n_scen: int, seed: int):
     return p, q
 
 
+def without_branch(net: Network, branch: int) -> Network:
+    """The network with branch ``branch`` (lines first, then transformers) removed: the reference
+    view of an N-1 scenario, whose Y-bus equals the base one minus that branch's stamps."""
+    import dataclasses
+    nl = net.line_from.size
+    if branch < nl:
+        keep = np.arange(nl) != branch
+        return dataclasses.replace(net, line_from=net.line_from[keep], line_to=net.line_to[keep],
+                                   line_r=net.line_r[keep], line_x=net.line_x[keep], line_b=net.line_b[keep],
+                                   line_g=net.line_g[keep])
+    keep = np.arange(net.tr_from.size) != branch - nl
+    return dataclasses.replace(net, tr_from=net.tr_from[keep], tr_to=net.tr_to[keep], tr_ys=net.tr_ys[keep],
+                               tr_ysh=net.tr_ysh[keep], tr_tap=net.tr_tap[keep])
+
+
 def config_grid(config: int, seed: int = 20261203) -> Network:
     """The grid of workload ``config`` (1: 200 buses, 2: 2,000 buses, 3: 10,000 buses)."""
     sizes = {1: (200, 100), 2: (2000, 1000), 3: (10000, 5000)}

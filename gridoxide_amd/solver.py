@@ -140,6 +140,19 @@ class YBus:
     y: np.ndarray  # complex128
 
 
+def _branch_args(net: Network):
+    """The line/transformer argument block of gx_ybus_build and gx_topo_set_branches."""
+    cplx = lambda a: f64(np.stack([a.real, a.imag], 1).ravel()) if a.size else np.zeros(2)
+    lf, lt = i32(net.line_from), i32(net.line_to)
+    lr, lx, lb, lg = f64(net.line_r), f64(net.line_x), f64(net.line_b), f64(net.line_g)
+    tf, tt = i32(net.tr_from), i32(net.tr_to)
+    ys, ysh, tap = cplx(net.tr_ys), cplx(net.tr_ysh), cplx(net.tr_tap)
+    keep = (lf, lt, lr, lx, lb, lg, tf, tt, ys, ysh, tap)  # the arrays must outlive the call
+    args = (lf.size, ptr(lf, _lib._i32p), ptr(lt, _lib._i32p), ptr(lr), ptr(lx), ptr(lb), ptr(lg), tf.size,
+            ptr(tf, _lib._i32p), ptr(tt, _lib._i32p), ptr(ys), ptr(ysh), ptr(tap))
+    return keep, args
+
+
 def build_ybus(net: Network) -> YBus:
     """Merged Y-bus of a network (gx_ybus_build; build_ybus + stamp_transformers + finish)."""
     L = _lib.lib()
@@ -148,15 +161,10 @@ def build_ybus(net: Network) -> YBus:
     col = np.zeros(cap, np.int32)
     yre = np.zeros(cap)
     yim = np.zeros(cap)
-    cplx = lambda a: f64(np.stack([a.real, a.imag], 1).ravel()) if a.size else np.zeros(2)
-    lf, lt = i32(net.line_from), i32(net.line_to)
-    lr, lx, lb, lg = f64(net.line_r), f64(net.line_x), f64(net.line_b), f64(net.line_g)
-    tf, tt = i32(net.tr_from), i32(net.tr_to)
-    ys, ysh, tap = cplx(net.tr_ys), cplx(net.tr_ysh), cplx(net.tr_tap)
-    nnz = check(L.gx_ybus_build(net.n_bus, lf.size, ptr(lf, _lib._i32p), ptr(lt, _lib._i32p), ptr(lr), ptr(lx),
-                                ptr(lb), ptr(lg), tf.size, ptr(tf, _lib._i32p), ptr(tt, _lib._i32p), ptr(ys),
-                                ptr(ysh), ptr(tap), cap, ptr(rowptr, _lib._i32p), ptr(col, _lib._i32p), ptr(yre),
+    keep, args = _branch_args(net)
+    nnz = check(L.gx_ybus_build(net.n_bus, *args, cap, ptr(rowptr, _lib._i32p), ptr(col, _lib._i32p), ptr(yre),
                                 ptr(yim)))
+    del keep
     return YBus(net.n_bus, rowptr, col[:nnz].copy(), yre[:nnz] + 1j * yim[:nnz])
 
 
@@ -179,7 +187,21 @@ class Topology:
 
     @classmethod
     def from_network(cls, net: Network) -> "Topology":
-        return cls(net.bus_type, build_ybus(net))
+        topo = cls(net.bus_type, build_ybus(net))
+        topo.set_branches(net)
+        return topo
+
+    def set_branches(self, net: Network):
+        """Registers the network's branches (lines, then transformers) for per-scenario outages."""
+        keep, args = _branch_args(net)
+        check(_lib.lib().gx_topo_set_branches(self._h, *args))
+        del keep
+        self.n_branches = int(net.line_from.size + net.tr_from.size)
+
+    def branch_is_bridge(self, branch: int) -> bool:
+        """True when taking ``branch`` out of service would split its island (such outages are
+        rejected)."""
+        return bool(check(_lib.lib().gx_topo_branch_is_bridge(self._h, int(branch))))
 
     def close(self):
         if getattr(self, "_h", None):
@@ -267,9 +289,11 @@ class Topology:
         0 = default) for later solves."""
         check(_lib.lib().gx_topo_set_launch(self._h, int(tile), int(ring_rows)))
 
-    def batch_solve(self, p_spec, q_spec, vm0, va0, tol=1e-8, max_iter=20, mode=Mode.FP64, init="given"):
+    def batch_solve(self, p_spec, q_spec, vm0, va0, tol=1e-8, max_iter=20, mode=Mode.FP64, init="given",
+                    outage=None):
         """Host arrays [n_scen, n_bus] -> (vm, va, status, iters, history).  init: "given" starts
-        from vm0/va0; "linear" first applies linear_initial_guess on the device."""
+        from vm0/va0; "linear" first applies linear_initial_guess on the device.  outage: optional
+        [n_scen] branch index out of service per scenario (-1: none)."""
         p_spec = f64(np.atleast_2d(p_spec))
         S = p_spec.shape[0]
         q_spec = f64(np.broadcast_to(q_spec, (S, self.n_bus)))
@@ -280,18 +304,21 @@ class Topology:
         st = np.zeros(S, np.int32)
         it = np.zeros(S, np.int32)
         hist = np.zeros((S, max(max_iter, 1)))
-        check(_lib.lib().gx_batch_solve(self._h, S, p_spec.ctypes.data, q_spec.ctypes.data, vm0.ctypes.data,
-                                        va0.ctypes.data, float(tol), int(max_iter), int(mode), _init_flag(init),
-                                        vm.ctypes.data,
-                                        va.ctypes.data, st.ctypes.data, it.ctypes.data, hist.ctypes.data, None))
+        out = None if outage is None else i32(np.broadcast_to(outage, (S,)))
+        check(_lib.lib().gx_batch_solve_outages(
+            self._h, S, p_spec.ctypes.data, q_spec.ctypes.data, vm0.ctypes.data, va0.ctypes.data,
+            None if out is None else out.ctypes.data, float(tol), int(max_iter), int(mode), _init_flag(init),
+            vm.ctypes.data, va.ctypes.data, st.ctypes.data, it.ctypes.data, hist.ctypes.data, None))
         return vm, va, st, it, hist[:, :max_iter]
 
     def batch_solve_device(self, p_spec, q_spec, vm0, va0, vm, va, status, iters, history, tol=1e-8, max_iter=20,
-                           mode=Mode.FP64, stream=0, init="given"):
-        """Device-resident variant: every argument is a CUDA/HIP tensor (torch) already in HBM."""
+                           mode=Mode.FP64, stream=0, init="given", outage=None):
+        """Device-resident variant: every argument is a CUDA/HIP tensor (torch) already in HBM
+        (outage: optional int32 tensor [n_scen])."""
         S = int(p_spec.shape[0])
-        check(_lib.lib().gx_batch_solve(self._h, S, p_spec.data_ptr(), q_spec.data_ptr(), vm0.data_ptr(),
-                                        va0.data_ptr(), float(tol), int(max_iter), int(mode),
+        check(_lib.lib().gx_batch_solve_outages(self._h, S, p_spec.data_ptr(), q_spec.data_ptr(), vm0.data_ptr(),
+                                        va0.data_ptr(), None if outage is None else outage.data_ptr(),
+                                        float(tol), int(max_iter), int(mode),
                                         _lib.GX_DEVICE_PTRS | _init_flag(init),
                                         vm.data_ptr(), va.data_ptr(), status.data_ptr(), iters.data_ptr(),
                                         history.data_ptr() if history is not None else None, stream or None))
@@ -344,22 +371,33 @@ class BatchSolver:
     """Scenario batch over one shared topology (src/batch.rs:152-289), all scenarios on the GPU.
 
     Like the reference, every scenario starts from linear_initial_guess (src/batch.rs:268) unless
-    ``init="given"`` asks for the template/override voltages as they are (flat start)."""
+    ``init="given"`` asks for the template/override voltages as they are (flat start).
 
-    def __init__(self, backend: JacobianBackend = JacobianBackend.HIP, mode: Mode = Mode.FP64, init: str = "linear"):
+    ``outages=True`` accepts N-1 scenarios: ``Scenario.branch_outages`` with at most one branch
+    (lines first, then transformers), solved on the shared pattern with that branch's stamps taken
+    out of the scenario's Y-bus.  The default keeps the reference's behaviour of rejecting them."""
+
+    def __init__(self, backend: JacobianBackend = JacobianBackend.HIP, mode: Mode = Mode.FP64, init: str = "linear",
+                 outages: bool = False):
         if backend is not JacobianBackend.HIP:
             raise ValueError("only the HIP backend is available")
         _init_flag(init)
         self.mode = mode
         self.init = init
+        self.outages = outages
 
     @staticmethod
-    def scenario_arrays(template: Network, scenarios: Sequence[Scenario]):
-        """Validates like the reference and expands overrides into dense [S, n_bus] arrays."""
+    def scenario_arrays(template: Network, scenarios: Sequence[Scenario], outages: bool = False):
+        """Validates like the reference and expands overrides into dense [S, n_bus] arrays (with
+        ``outages``, also returns the [S] outaged branch per scenario, -1 for none)."""
         n = template.n_bus
+        n_br = template.line_from.size + template.tr_from.size
         for i, sc in enumerate(scenarios):
-            if sc.branch_outages:
+            if sc.branch_outages and not (outages and len(sc.branch_outages) == 1):
                 raise OutagesUnsupported(i)
+            for e in sc.branch_outages:
+                if e < 0 or e >= n_br:
+                    raise ValueError(f"scenario {i}: branch {e} out of range ({n_br} branches)")
             for ov in sc.bus_overrides:
                 if ov.bus >= n or ov.bus < 0:
                     raise BusOutOfRange(i, ov.bus, n)
@@ -376,16 +414,22 @@ class BatchSolver:
                 if ov.voltage_mag is not None:
                     vm[s, ov.bus] = ov.voltage_mag
         va = np.repeat(template.va[None], S, 0)
+        if outages:
+            out = np.array([sc.branch_outages[0] if sc.branch_outages else -1 for sc in scenarios], np.int32)
+            return p, q, vm, va, out
         return p, q, vm, va
 
     def solve(self, template: Network, scenarios: Sequence[Scenario], tol: float, max_iter: int,
               topology: Optional[Topology] = None) -> List[PowerFlowReport]:
         """Starting point: the template voltages with overrides applied, then (by default) the
         linear initial guess."""
-        p, q, vm0, va0 = self.scenario_arrays(template, scenarios)
+        arrays = self.scenario_arrays(template, scenarios, self.outages)
+        p, q, vm0, va0 = arrays[:4]
+        out = arrays[4] if self.outages and (arrays[4] >= 0).any() else None
         if len(scenarios) == 0:
             return []
         topo = topology or Topology.from_network(template)
-        vm, va, st, it, hist = topo.batch_solve(p, q, vm0, va0, tol, max_iter, self.mode, init=self.init)
+        vm, va, st, it, hist = topo.batch_solve(p, q, vm0, va0, tol, max_iter, self.mode, init=self.init,
+                                                outage=out)
         isl = topo.island_reports(len(scenarios))
         return [PowerFlowReport(vm[s], va[s], _stats(st[s], it[s], hist[s]), isl[s]) for s in range(len(scenarios))]

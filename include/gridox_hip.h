@@ -157,6 +157,30 @@ int gx_topo_profile(gx_topo* t, int enable, double* ms, int* launches);
  * triangular-solve sums.  GX_ERR_ARG for other values. */
 int gx_topo_set_launch(gx_topo* t, int tile, int ring_rows);
 
+/* Branches of t for per-scenario outages, in the argument layout of gx_ybus_build (lines, then
+ * transformers; branch e < n_line is line e, else transformer e - n_line).  The reference's
+ * Scenario::branch_outages (src/batch.rs:70-83) names branches but BatchSolver rejects them
+ * (OutagesUnsupported, src/batch.rs:207-209); this is the N-1 extension that keeps the Y-bus
+ * pattern: an outaged branch's four stamps (build_ybus, src/network.rs:225-247;
+ * branch_calc_param, src/network.rs:339-360) are subtracted from that scenario's Y values, so the
+ * symbolic factorization stays shared.  GX_ERR_ARG if a branch is not in t's Y-bus pattern. */
+int gx_topo_set_branches(gx_topo* t, int n_line, const int* line_from, const int* line_to, const double* r,
+                         const double* x, const double* b_shunt, const double* g_shunt, int n_tr,
+                         const int* tr_from, const int* tr_to, const double* y_series, const double* y_shunt,
+                         const double* tap);
+
+/* 1 if removing branch `branch` would split its island (a bridge; such outages are rejected), else
+ * 0; negative error code. */
+int gx_topo_branch_is_bridge(const gx_topo* t, int branch);
+
+/* gx_batch_solve with one branch out of service per scenario: outage[s] is a branch index of
+ * gx_topo_set_branches or -1 (length n_scen; a device pointer with GX_DEVICE_PTRS).  GX_ERR_ARG for
+ * an index out of range or a bridge, GX_ERR_STATE if branches were never set.  outage == NULL is
+ * gx_batch_solve. */
+int gx_batch_solve_outages(gx_topo* t, int n_scen, const double* p_spec, const double* q_spec, const double* vm0,
+                           const double* va0, const int* outage, double tol, int max_iter, int mode, int flags,
+                           double* vm, double* va, int* status, int* iters, double* history, void* stream);
+
 /* Number of scenarios of the last GX_MODE_MIXED gx_batch_solve on t that were solved again from
  * their starting point with the FP64 path: those whose mismatch stopped falling once below
  * 1e-4 pu (the FP32 factors limit the step) and those with a singular FP32 factor.  Their

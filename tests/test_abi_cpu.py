@@ -163,3 +163,55 @@ def test_set_launch_validates():
         with pytest.raises(_lib.GxError) as e:
             topo.set_launch(*bad)
         assert e.value.code == _lib.GX_ERR_ARG
+
+
+def _connected_without(net, e):
+    """Brute force: is the branch graph still one component with branch e removed?"""
+    import scipy.sparse as sp
+    from scipy.sparse.csgraph import connected_components
+    f = np.r_[net.line_from, net.tr_from]
+    t = np.r_[net.line_to, net.tr_to]
+    keep = np.arange(f.size) != e
+    a = sp.coo_matrix((np.ones(keep.sum()), (f[keep], t[keep])), shape=(net.n_bus, net.n_bus))
+    return connected_components(a, directed=False)[0] == 1
+
+
+def test_branch_bridges_match_brute_force():
+    net = config_grid(1)
+    topo = Topology.from_network(net)
+    n_br = net.line_from.size + net.tr_from.size
+    bridges = [topo.branch_is_bridge(e) for e in range(n_br)]
+    assert any(bridges) and not all(bridges)
+    for e in range(n_br):
+        assert bridges[e] == (not _connected_without(net, e)), e
+
+
+def test_outage_validation_errors():
+    """Bridges and out-of-range branches are rejected before any device work."""
+    net = config_grid(1)
+    topo = Topology.from_network(net)
+    n_br = net.line_from.size + net.tr_from.size
+    bridge = next(e for e in range(n_br) if topo.branch_is_bridge(e))
+    p, q = net.p_spec[None], net.q_spec[None]
+    for bad, code in ((bridge, _lib.GX_ERR_ARG), (n_br, _lib.GX_ERR_ARG), (-2, _lib.GX_ERR_ARG)):
+        with pytest.raises(_lib.GxError) as e:
+            topo.batch_solve(p, q, net.vm[None], net.va[None], outage=[bad])
+        assert e.value.code == code
+    with pytest.raises(OutagesUnsupported):
+        BatchSolver().solve(net, [Scenario([], [3])], 1e-8, 20)
+    with pytest.raises(OutagesUnsupported):  # N-2 is not supported even with outages enabled
+        BatchSolver(outages=True).solve(net, [Scenario([], [3, 4])], 1e-8, 20)
+
+
+def test_without_branch_ybus_is_base_minus_stamps(oracle_mod):
+    """The oracle's N-1 network (branch removed) has the base Y-bus minus that branch's stamps."""
+    from gridoxide_amd import without_branch
+    net = config_grid(1)
+    base = build_ybus(net)
+    for e in (0, 5, net.line_from.size + 1):
+        n1 = build_ybus(without_branch(net, e))
+        dense = lambda yb: __import__("scipy.sparse", fromlist=["csr_matrix"]).csr_matrix(
+            (yb.y, yb.col, yb.rowptr), shape=(net.n_bus, net.n_bus)).toarray()
+        d = dense(base) - dense(n1)
+        nz = np.argwhere(np.abs(d) > 1e-12)
+        assert 1 <= len(nz) <= 4

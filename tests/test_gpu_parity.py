@@ -344,3 +344,39 @@ def test_mixed_fallback_reproduces_fp64_for_failing_scenarios():
     ok = [s for s in range(16) if s != 3]
     assert np.all(mixed[2][ok] == 0)
     assert np.abs(mixed[0][ok] - fp64[0][ok]).max() <= V_TOL
+
+
+@pytest.mark.parametrize("shape", [(0, 0), (64, 0)])
+@pytest.mark.parametrize("mode", [Mode.FP64, Mode.MIXED])
+def test_n1_outages_vs_oracle_on_reduced_networks(oracle_mod, mode, shape):
+    """Half the scenarios take one random non-bridge branch out of service on the shared pattern;
+    each is compared with the oracle solving the network without that branch."""
+    from gridoxide_amd import without_branch
+    _say(f"case: config-1 grid, 48 scenarios, half N-1, mode={mode.name}, tile/ring={shape}")
+    net = config_grid(1)
+    S = 48
+    p, q = scenario_specs(net, S, seed=21)
+    vm0 = np.repeat(net.vm[None], S, 0)
+    va0 = np.repeat(net.va[None], S, 0)
+    topo = Topology.from_network(net)
+    topo.set_launch(*shape)
+    n_br = net.line_from.size + net.tr_from.size
+    rng = np.random.default_rng(5)
+    candidates = [e for e in range(n_br) if not topo.branch_is_bridge(e)]
+    outage = np.full(S, -1, np.int32)
+    outage[1::2] = rng.choice(candidates, S // 2)
+    gpu = topo.batch_solve(p, q, vm0, va0, tol=MIS_TOL, max_iter=20, mode=mode, outage=outage)
+    ref = [np.zeros_like(gpu[0]), np.zeros_like(gpu[1]), np.zeros(S, np.int32), np.zeros(S, np.int32),
+           np.zeros_like(gpu[4])]
+    for e in np.unique(outage):
+        idx = np.nonzero(outage == e)[0]
+        rnet = net if e < 0 else without_branch(net, int(e))
+        r = _oracle_batch(oracle_mod, rnet, p[idx], q[idx], vm0[idx], va0[idx])
+        for a in range(5):
+            ref[a][idx] = r[a]
+    assert np.all(ref[2] == 0)
+    _compare(gpu, tuple(ref), mixed=(mode == Mode.MIXED))
+    # an outage changes the answer: the N-1 scenarios differ from their intact counterparts
+    base = topo.batch_solve(p, q, vm0, va0, tol=MIS_TOL, max_iter=20, mode=mode)
+    assert np.abs(gpu[0][1::2] - base[0][1::2]).max() > 1e-6
+    assert np.array_equal(gpu[0][0::2], base[0][0::2])
