@@ -10,6 +10,7 @@ from .solver import (  # noqa: F401
     BusOutOfRange,
     BusOverride,
     BusType,
+    Engine,
     IslandReport,
     IslandStatus,
     JacobianBackend,

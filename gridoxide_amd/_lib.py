@@ -19,6 +19,7 @@ GX_ERR_DEVICE = -3
 GX_ERR_STATE = -4
 GX_ERR_CAPACITY = -5
 GX_ERR_UNSUPPORTED = -6
+GX_ERR_BUDGET = -7
 GX_CONVERGED, GX_MAX_ITER, GX_SINGULAR = 0, 1, 2
 GX_MODE_FP64, GX_MODE_MIXED = 0, 1
 GX_DEVICE_PTRS = 1
@@ -55,6 +56,17 @@ SIGNATURES = {
     "gx_topo_set_branches": (ctypes.c_int, [_vp, ctypes.c_int, _i32p, _i32p, _f64p, _f64p, _f64p, _f64p,
                                             ctypes.c_int, _i32p, _i32p, _f64p, _f64p, _f64p]),
     "gx_topo_branch_is_bridge": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "gx_engine_create": (_vp, [ctypes.c_int, ctypes.c_int, ctypes.c_longlong, _vp, _i32p]),
+    "gx_engine_destroy": (None, [_vp]),
+    "gx_engine_request_bytes": (ctypes.c_longlong, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                                    ctypes.c_int]),
+    "gx_engine_bytes": (ctypes.c_longlong, [_vp]),
+    "gx_engine_submit": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _vp, _vp, _vp, _vp, _vp, ctypes.c_double,
+                                        ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+    "gx_engine_step": (ctypes.c_int, [_vp]),
+    "gx_engine_run": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "gx_engine_collect": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, _vp, _vp, _vp]),
+    "gx_engine_fallbacks": (ctypes.c_int, [_vp, ctypes.c_int]),
     "gx_batch_solve_outages": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, _vp, _vp, _vp, ctypes.c_double,
                                               ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp, _vp, _vp, _vp, _vp,
                                               _vp]),

@@ -171,6 +171,7 @@ struct DevWork {
   double* va0 = nullptr;
   int* fallback = nullptr;               // [1] scenarios handed to the FP64 re-solve
   const int* outage = nullptr;           // [n_scen] branch out of service per scenario, -1 none (or null)
+  int* iter_dev = nullptr;               // [1] the Newton iteration, when kept on the device (else an argument)
 };
 
 // Launchers (gx_kernels.hip).  All run on `stream`; none synchronizes.  `fp32` selects the
@@ -197,6 +198,13 @@ void launch_linear_guess(const DevTopo& t, const DevLg& lg, const DevTopo& lgt, 
 // Dynamic LDS bytes of the factoring LU kernel for a topology (its window) and tile width:
 // complex (linear guess), FP32 or FP64 values.
 size_t lg_shmem(const DevTopo& lgt, int tile);
+// One persistent LU launch over (request, tile) items of several requests (topos/works/items are
+// device arrays; counter a device int the launch zeroes first); shmem = the largest request's.
+void launch_lu_queue(int tile, const DevTopo* topos, const DevWork* works, const int2* items, int n_items,
+                     int* counter, size_t shmem, int grid, bool fp32, bool factor, hipStream_t stream);
+size_t lu_shmem_tile(const DevTopo& t, bool fp32, bool factor, int tile);
+void launch_iter_advance(int* iter_dev, hipStream_t stream);
+void launch_zero_ints(int* p, int n, hipStream_t stream);
 size_t lu_shmem(const DevTopo& t, bool fp32, int tile);
 
 }  // namespace gx

@@ -628,6 +628,32 @@ std::vector<char> find_bridges(int n_bus, const std::vector<BranchStamp>& br) {
   return bridge;
 }
 
+// Device buffers of one batch: the buffers of gx_batch_solve (one per topology, reused across
+// calls) or of an engine request.
+struct Workspace {
+  size_t lanes = 0;
+  int iter_cap = 0;
+  int fp32 = -1;
+  DevMem vm, va, vr, vi, ps, qs, pc, qc, f, dx, y, lu, status, iters, maxmis, hist, running, dvr, dvi, stage,
+      fallback, vm0, va0, outage, iter_dev, lg_lu, lg_y, lg_status, lg_iters;
+  int* h_running = nullptr;  // pinned host mirror of the running counters
+  int h_running_len = 0;
+  Workspace() = default;
+  Workspace(const Workspace&) = delete;
+  Workspace& operator=(const Workspace&) = delete;
+  ~Workspace() {
+    if (h_running) (void)hipHostFree(h_running);
+  }
+  size_t bytes() const {
+    const DevMem* all[] = {&vm, &va, &vr, &vi, &ps, &qs, &pc, &qc, &f, &dx, &y, &lu, &status, &iters, &maxmis,
+                           &hist, &running, &dvr, &dvi, &stage, &fallback, &vm0, &va0, &outage, &iter_dev,
+                           &lg_lu, &lg_y, &lg_status, &lg_iters};
+    size_t b = 0;
+    for (const DevMem* m : all) b += m->bytes;
+    return b;
+  }
+};
+
 }  // namespace
 
 struct gx_topo {
@@ -663,15 +689,9 @@ struct gx_topo {
   gx::DevTopo dt;
   // launch shape: scenarios per tile and solve ring rows (0 = automatic)
   int tile_req = 0, ring_req = 0;
-  // workspace
-  size_t ws_lanes = 0;
-  int ws_iter = 0;
-  int ws_fp32 = -1;
-  DevMem w_vm, w_va, w_vr, w_vi, w_ps, w_qs, w_pc, w_qc, w_f, w_dx, w_y, w_lu, w_status, w_iters, w_max, w_hist;
-  DevMem w_dvr, w_dvi, w_stage, w_running, w_fallback, w_vm0, w_va0;
+  // workspace of gx_batch_solve
+  Workspace ws;
   int last_fallback = 0;
-  int* h_running = nullptr;  // pinned host mirror of the running counters
-  int h_running_len = 0;
   // linear initial guess system (built on first use)
   bool lg_built = false;
   gx::LuPattern lg_lu;
@@ -679,7 +699,6 @@ struct gx_topo {
   std::vector<int> lg_src, lg_dbus, lg_row_bus, lg_row_piv, lg_row_x, lg_rhs_ptr, lg_rhs_e, lg_rhs_j;
   DevMem d_lg_src, d_lg_dbus, d_lg_row_bus, d_lg_row_piv, d_lg_row_x, d_lg_rhs_ptr, d_lg_rhs_e, d_lg_rhs_j;
   DevSched lg_ds;
-  DevMem w_lg_lu, w_lg_y, w_lg_status, w_lg_iters;
   gx::DevLg dlg;
   gx::DevTopo dlgt;
   bool ybus_uploaded = false;
@@ -689,7 +708,7 @@ struct gx_topo {
   std::vector<double> br_y;    // 8 per branch
   std::vector<char> br_bridge;
   std::vector<int> h_yp;       // per L+U position: the Y CSR position its recipe reads (-1 for fill)
-  DevMem d_br_pos, d_br_y, d_yp, w_outage;
+  DevMem d_br_pos, d_br_y, d_yp;
   bool br_uploaded = false;
   // the last gx_batch_solve (for island reports)
   gx::DevWork last;
@@ -702,7 +721,6 @@ struct gx_topo {
       (void)hipEventDestroy(std::get<1>(e));
       (void)hipEventDestroy(std::get<2>(e));
     }
-    if (h_running) (void)hipHostFree(h_running);
   }
 };
 
@@ -836,32 +854,31 @@ int upload_topo(gx_topo& t, int window, int tile) {
 }
 
 // Per-batch buffers that do not depend on the factorization (the L+U buffer is reserved after it).
-int ensure_workspace(gx_topo& t, size_t lanes, int max_iter, bool fp32) {
-  if (lanes <= t.ws_lanes && max_iter <= t.ws_iter && (int)fp32 == t.ws_fp32) return GX_OK;
+int ensure_workspace(const gx_topo& t, Workspace& ws, size_t lanes, int max_iter, bool fp32) {
+  if (lanes <= ws.lanes && max_iter <= ws.iter_cap && (int)fp32 == ws.fp32) return GX_OK;
   const size_t nb = lanes * t.n_bus * sizeof(double);
   const size_t nu = lanes * t.n * sizeof(double);
-  const size_t es = fp32 ? sizeof(float) : sizeof(double);
   int rc;
-  if ((rc = t.w_vm.reserve(nb)) || (rc = t.w_va.reserve(nb)) || (rc = t.w_vr.reserve(nb)) ||
-      (rc = t.w_vi.reserve(nb)) || (rc = t.w_ps.reserve(nb)) || (rc = t.w_qs.reserve(nb)) ||
-      (rc = t.w_pc.reserve(nb)) || (rc = t.w_qc.reserve(nb)) || (rc = t.w_f.reserve(nu)) ||
-      (rc = t.w_dx.reserve(nu)) || (rc = t.w_y.reserve(lanes * t.n * sizeof(double))) ||
-      (rc = t.w_status.reserve(lanes * sizeof(int))) ||
-      (rc = t.w_iters.reserve(lanes * sizeof(int))) || (rc = t.w_max.reserve(lanes * sizeof(unsigned long long))) ||
-      (rc = t.w_hist.reserve(lanes * (size_t)max_iter * sizeof(double))) ||
-      (rc = t.w_running.reserve((size_t)max_iter * sizeof(int))))
+  if ((rc = ws.vm.reserve(nb)) || (rc = ws.va.reserve(nb)) || (rc = ws.vr.reserve(nb)) || (rc = ws.vi.reserve(nb)) ||
+      (rc = ws.ps.reserve(nb)) || (rc = ws.qs.reserve(nb)) || (rc = ws.pc.reserve(nb)) || (rc = ws.qc.reserve(nb)) ||
+      (rc = ws.f.reserve(nu)) || (rc = ws.dx.reserve(nu)) || (rc = ws.y.reserve(nu)) ||
+      (rc = ws.status.reserve(lanes * sizeof(int))) || (rc = ws.iters.reserve(lanes * sizeof(int))) ||
+      (rc = ws.maxmis.reserve(lanes * sizeof(unsigned long long))) ||
+      (rc = ws.hist.reserve(lanes * (size_t)std::max(max_iter, 1) * sizeof(double))) ||
+      (rc = ws.running.reserve((size_t)std::max(max_iter, 1) * sizeof(int))) ||
+      (rc = ws.fallback.reserve(sizeof(int))))
     return rc;
-  if (t.h_running_len < std::max(max_iter, 1)) {
-    if (t.h_running) (void)hipHostFree(t.h_running);
-    t.h_running = nullptr;
-    t.h_running_len = 0;
-    GX_HIP(hipHostMalloc((void**)&t.h_running, sizeof(int) * std::max(max_iter, 1), hipHostMallocDefault));
-    t.h_running_len = std::max(max_iter, 1);
+  if (ws.h_running_len < std::max(max_iter, 1)) {
+    if (ws.h_running) (void)hipHostFree(ws.h_running);
+    ws.h_running = nullptr;
+    ws.h_running_len = 0;
+    GX_HIP(hipHostMalloc((void**)&ws.h_running, sizeof(int) * std::max(max_iter, 1), hipHostMallocDefault));
+    ws.h_running_len = std::max(max_iter, 1);
   }
-  if (fp32 && ((rc = t.w_dvr.reserve(nb)) || (rc = t.w_dvi.reserve(nb)))) return rc;
-  t.ws_lanes = lanes;
-  t.ws_iter = max_iter;
-  t.ws_fp32 = (int)fp32;
+  if (fp32 && ((rc = ws.dvr.reserve(nb)) || (rc = ws.dvi.reserve(nb)))) return rc;
+  ws.lanes = lanes;
+  ws.iter_cap = max_iter;
+  ws.fp32 = (int)fp32;
   return GX_OK;
 }
 
@@ -895,7 +912,7 @@ int upload_ybus(gx_topo& t) {
 
 // The linear initial guess system (src/network.rs:627-678) for the bus types as given: the
 // reference runs the guess before islands are marked (src/batch.rs:268, src/solver.rs:452-454).
-int build_lg(gx_topo& t, size_t lanes, int tile) {
+int build_lg(gx_topo& t, Workspace& ws, size_t lanes, int tile) {
   int rc;
   if (!t.lg_built) {
     std::vector<int> pq, pos(t.n_bus, -1);
@@ -982,8 +999,8 @@ int build_lg(gx_topo& t, size_t lanes, int tile) {
     const int rows = pick_ring_rows(t.lg_sched.max_col, tile, 16, 0);
     if (rows < 0) return fail(GX_ERR_UNSUPPORTED, "linear guess L+U column exceeds the LDS staging limit");
     if ((rc = upload_sched(t.lg_lu, t.lg_sched, gx::ring_window_of(rows, tile), tile, t.lg_ds, t.dlgt))) return rc;
-    if ((rc = t.w_lg_lu.reserve(lanes * (size_t)t.dlg.nnz_lu * 16)) || (rc = t.w_lg_y.reserve(lanes * t.dlg.m * 16)) ||
-        (rc = t.w_lg_status.reserve(lanes * sizeof(int))) || (rc = t.w_lg_iters.reserve(lanes * sizeof(int))))
+    if ((rc = ws.lg_lu.reserve(lanes * (size_t)t.dlg.nnz_lu * 16)) || (rc = ws.lg_y.reserve(lanes * t.dlg.m * 16)) ||
+        (rc = ws.lg_status.reserve(lanes * sizeof(int))) || (rc = ws.lg_iters.reserve(lanes * sizeof(int))))
       return rc;
   }
   return GX_OK;
@@ -1000,6 +1017,171 @@ int factor_at(gx_topo& t, const double* vm, const double* va, const double* ps, 
   if (!gx::factor_pivot(t.csc, ax.data(), t.ord, 1e-3, t.lu))
     return fail(GX_ERR_SINGULAR, "first factorization found a zero or missing pivot");
   return finalize_factor(t);
+}
+
+// One branch out of service per scenario (-1: none), validated; bridges would split an island.
+int read_outages(const gx_topo& t, int n_scen, const int* outage, bool dev, std::vector<int>& h, bool& any) {
+  any = false;
+  if (!outage) return GX_OK;
+  h.resize(n_scen);
+  if (dev) GX_HIP(hipMemcpy(h.data(), outage, sizeof(int) * n_scen, hipMemcpyDeviceToHost));
+  else std::memcpy(h.data(), outage, sizeof(int) * n_scen);
+  for (int s = 0; s < n_scen; ++s) {
+    const int e = h[s];
+    if (e < -1 || e >= t.n_br)
+      return fail(t.n_br == 0 ? GX_ERR_STATE : GX_ERR_ARG,
+                  "scenario " + std::to_string(s) + ": outage branch out of range (gx_topo_set_branches first)");
+    if (e >= 0 && t.br_bridge[e])
+      return fail(GX_ERR_ARG, "scenario " + std::to_string(s) + ": branch " + std::to_string(e) +
+                                  " is a bridge; its outage splits an island");
+    any = any || e >= 0;
+  }
+  return GX_OK;
+}
+
+struct BatchIn {
+  int n_scen = 0;
+  const double *p = nullptr, *q = nullptr, *vm0 = nullptr, *va0 = nullptr;
+  double tol = 0.0;
+  int max_iter = 0;
+  bool fp32 = false;
+  int flags = 0;
+  bool dev = false;
+  hipStream_t st = nullptr;
+  const std::vector<int>* outage = nullptr;  // validated host copy, or null when none is out
+};
+
+// Stages a batch on the device: inputs in tiles, optional linear initial guess, the topology's first
+// factorization (if not yet factored), the topology upload for this tile width, the mixed-precision
+// starting-point copy and the init kernel.  Fills w.
+int stage_batch(gx_topo& t, Workspace& ws, const BatchIn& in, int tile, gx::DevWork& w) {
+  const int n_scen = in.n_scen, nb = t.n_bus, max_iter = in.max_iter;
+  const size_t plane = (size_t)n_scen * nb * sizeof(double);
+  hipStream_t st = in.st;
+  const int n_tiles = (n_scen + tile - 1) / tile;
+  const size_t lanes = (size_t)n_tiles * tile;
+  int rc;
+  if ((rc = upload_ybus(t)) || (rc = ensure_workspace(t, ws, lanes, max_iter, in.fp32))) return rc;
+  w = gx::DevWork();
+  w.tile = tile;
+  w.n_scen = n_scen;
+  w.n_tiles = n_tiles;
+  w.max_iter = max_iter;
+  w.tol = in.tol;
+  w.vm = ws.vm.as<double>();
+  w.va = ws.va.as<double>();
+  w.vr = ws.vr.as<double>();
+  w.vi = ws.vi.as<double>();
+  w.ps = ws.ps.as<double>();
+  w.qs = ws.qs.as<double>();
+  w.pc = ws.pc.as<double>();
+  w.qc = ws.qc.as<double>();
+  w.f = ws.f.as<double>();
+  w.dx = ws.dx.as<double>();
+  w.y = ws.y.p;
+  w.status = ws.status.as<int>();
+  w.iters = ws.iters.as<int>();
+  w.maxmis = ws.maxmis.as<unsigned long long>();
+  w.history = ws.hist.as<double>();
+  w.running = ws.running.as<int>();
+  w.fallback = ws.fallback.as<int>();
+  if (in.outage) {
+    if ((rc = upload_branches(t)) || (rc = ws.outage.upload(*in.outage))) return rc;
+    w.outage = ws.outage.as<int>();
+  }
+  // inputs -> interleaved tiles
+  const double* src[4] = {in.p, in.q, in.vm0, in.va0};
+  double* dst[4] = {w.ps, w.qs, w.vm, w.va};
+  if ((rc = ws.stage.reserve(plane))) return rc;
+  for (int a = 0; a < 4; ++a) {
+    const double* s = src[a];
+    if (!in.dev) {
+      GX_HIP(hipMemcpyAsync(ws.stage.p, src[a], plane, hipMemcpyHostToDevice, st));
+      s = ws.stage.as<double>();
+    }
+    gx::launch_to_tiles(s, dst[a], n_scen, n_tiles, nb, tile, st);
+  }
+  if (in.flags & GX_INIT_LINEAR) {
+    // BatchSolver runs linear_initial_guess on every scenario first (src/batch.rs:268)
+    if ((rc = build_lg(t, ws, lanes, tile))) return rc;
+    gx::launch_linear_guess(t.dt, t.dlg, t.dlgt, w, ws.lg_lu.p, ws.lg_y.p, ws.lg_status.as<int>(),
+                            ws.lg_iters.as<int>(), st);
+    GX_HIP(hipGetLastError());
+  }
+  if (!t.factored) {
+    // PersistentSolver's first factorization uses the first Jacobian it is asked to solve: the
+    // first scenario whose starting point is not already converged (src/solver.rs:699-713).
+    gx::launch_from_tiles(w.vm, ws.stage.as<double>(), n_scen, nb, tile, st);
+    std::vector<double> hvm((size_t)n_scen * nb);
+    GX_HIP(hipMemcpyAsync(hvm.data(), ws.stage.p, plane, hipMemcpyDeviceToHost, st));
+    GX_HIP(hipStreamSynchronize(st));
+    gx::launch_from_tiles(w.va, ws.stage.as<double>(), n_scen, nb, tile, st);
+    std::vector<double> hva((size_t)n_scen * nb);
+    GX_HIP(hipMemcpyAsync(hva.data(), ws.stage.p, plane, hipMemcpyDeviceToHost, st));
+    GX_HIP(hipStreamSynchronize(st));
+    std::vector<double> hp(nb), hq(nb), pc, qc;
+    int chosen = -1;
+    for (int s = 0; s < n_scen && chosen < 0; ++s) {
+      const size_t off = (size_t)s * nb;
+      hipMemcpyKind k = in.dev ? hipMemcpyDeviceToHost : hipMemcpyHostToHost;
+      GX_HIP(hipMemcpy(hp.data(), in.p + off, nb * sizeof(double), k));
+      GX_HIP(hipMemcpy(hq.data(), in.q + off, nb * sizeof(double), k));
+      for (int i = 0; i < nb; ++i)
+        if (t.dev_type[i] == gx::kDeadBus) hvm[off + i] = hva[off + i] = hp[i] = hq[i] = 0.0;
+      host_injections(t, hvm.data() + off, hva.data() + off, pc, qc);
+      double mx = 0.0;
+      for (int i = 0; i < nb; ++i) {
+        if (t.ns_pos[i] >= 0) mx = std::fmax(mx, std::fabs(hp[i] - pc[i]));
+        if (t.pq_pos[i] >= 0) mx = std::fmax(mx, std::fabs(hq[i] - qc[i]));
+      }
+      if (!(mx < in.tol) || s == n_scen - 1) chosen = s;
+    }
+    const size_t off = (size_t)chosen * nb;
+    if ((rc = factor_at(t, hvm.data() + off, hva.data() + off, hp.data(), hq.data()))) return rc;
+  }
+  // the solve ring is sized for FP64 values: the mixed mode's FP64 re-solve uses the same lists
+  const int rows = pick_ring_rows(t.sched.max_col, tile, sizeof(double), t.ring_req);
+  if (rows < 0)
+    return fail(GX_ERR_UNSUPPORTED, "longest L+U column exceeds the LDS staging limit of the device refactor");
+  if ((rc = upload_topo(t, gx::ring_window_of(rows, tile), tile))) return rc;
+  // FP64-sized in both modes: the mixed mode's FP64 re-solve reuses the same buffer
+  if ((rc = ws.lu.reserve(lanes * (size_t)t.dt.nnz_lu * 8))) return rc;
+  w.lu = ws.lu.p;
+  if (in.fp32) {
+    const size_t nbytes = lanes * nb * sizeof(double);
+    if ((rc = ws.vm0.reserve(nbytes)) || (rc = ws.va0.reserve(nbytes))) return rc;
+    w.vm0 = ws.vm0.as<double>();
+    w.va0 = ws.va0.as<double>();
+    GX_HIP(hipMemcpyAsync(w.vm0, w.vm, nbytes, hipMemcpyDeviceToDevice, st));
+    GX_HIP(hipMemcpyAsync(w.va0, w.va, nbytes, hipMemcpyDeviceToDevice, st));
+  }
+  gx::launch_init(t.dt, w, st);
+  GX_HIP(hipGetLastError());
+  return GX_OK;
+}
+
+// Copies a finished batch's results out (host or device pointers) and waits for them.
+int collect_batch(const gx_topo& t, Workspace& ws, const gx::DevWork& w, bool dev, double* vm, double* va,
+                  int* status, int* iters, double* history, hipStream_t st) {
+  const int n_scen = w.n_scen, nb = t.n_bus;
+  const size_t plane = (size_t)n_scen * nb * sizeof(double);
+  double* outs[2] = {vm, va};
+  const double* tiles[2] = {w.vm, w.va};
+  for (int a = 0; a < 2; ++a) {
+    if (dev) {
+      gx::launch_from_tiles(tiles[a], outs[a], n_scen, nb, w.tile, st);
+    } else {
+      gx::launch_from_tiles(tiles[a], ws.stage.as<double>(), n_scen, nb, w.tile, st);
+      GX_HIP(hipMemcpyAsync(outs[a], ws.stage.p, plane, hipMemcpyDeviceToHost, st));
+    }
+  }
+  hipMemcpyKind k = dev ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+  GX_HIP(hipMemcpyAsync(status, w.status, sizeof(int) * n_scen, k, st));
+  GX_HIP(hipMemcpyAsync(iters, w.iters, sizeof(int) * n_scen, k, st));
+  if (history && w.max_iter > 0)
+    GX_HIP(hipMemcpyAsync(history, w.history, sizeof(double) * n_scen * (size_t)w.max_iter, k, st));
+  GX_HIP(hipStreamSynchronize(st));
+  return GX_OK;
 }
 
 }  // namespace
@@ -1358,130 +1540,28 @@ int gx_batch_solve_outages(gx_topo* t, int n_scen, const double* p_spec, const d
   if (mode != GX_MODE_FP64 && mode != GX_MODE_MIXED) return fail(GX_ERR_ARG, "unknown mode");
   if (n_scen == 0) return GX_OK;
   if (!p_spec || !q_spec || !vm0 || !va0 || !vm || !va || !status || !iters) return fail(GX_ERR_ARG, "null array");
-  hipStream_t st = (hipStream_t)stream_ptr;
-  const bool dev = (flags & GX_DEVICE_PTRS) != 0;
-  const int nb = t->n_bus;
-  const size_t plane = (size_t)n_scen * nb * sizeof(double);
-  int rc;
-  const bool fp32 = mode == GX_MODE_MIXED;
-  // one branch out of service per scenario (-1: none); bridges would split an island
+  BatchIn in;
+  in.n_scen = n_scen;
+  in.p = p_spec;
+  in.q = q_spec;
+  in.vm0 = vm0;
+  in.va0 = va0;
+  in.tol = tol;
+  in.max_iter = max_iter;
+  in.fp32 = mode == GX_MODE_MIXED;
+  in.flags = flags;
+  in.dev = (flags & GX_DEVICE_PTRS) != 0;
+  in.st = (hipStream_t)stream_ptr;
   std::vector<int> h_outage;
   bool any_outage = false;
-  if (outage) {
-    h_outage.resize(n_scen);
-    if (dev) GX_HIP(hipMemcpy(h_outage.data(), outage, sizeof(int) * n_scen, hipMemcpyDeviceToHost));
-    else std::memcpy(h_outage.data(), outage, sizeof(int) * n_scen);
-    for (int s = 0; s < n_scen; ++s) {
-      const int e = h_outage[s];
-      if (e < -1 || e >= t->n_br)
-        return fail(t->n_br == 0 ? GX_ERR_STATE : GX_ERR_ARG,
-                    "scenario " + std::to_string(s) + ": outage branch out of range (gx_topo_set_branches first)");
-      if (e >= 0 && t->br_bridge[e])
-        return fail(GX_ERR_ARG, "scenario " + std::to_string(s) + ": branch " + std::to_string(e) +
-                                    " is a bridge; its outage splits an island");
-      any_outage = any_outage || e >= 0;
-    }
-  }
-  const int tile = t->tile_req ? t->tile_req : auto_tile(n_scen, fp32);
-  const int n_tiles = (n_scen + tile - 1) / tile;
-  const size_t lanes = (size_t)n_tiles * tile;
-  if ((rc = upload_ybus(*t)) || (rc = ensure_workspace(*t, lanes, max_iter, fp32))) return rc;
+  int rc;
+  if ((rc = read_outages(*t, n_scen, outage, in.dev, h_outage, any_outage))) return rc;
+  in.outage = any_outage ? &h_outage : nullptr;
+  hipStream_t st = in.st;
+  const int tile = t->tile_req ? t->tile_req : auto_tile(n_scen, in.fp32);
   gx::DevWork w;
-  w.tile = tile;
-  w.n_scen = n_scen;
-  w.n_tiles = n_tiles;
-  w.max_iter = max_iter;
-  w.tol = tol;
-  w.vm = t->w_vm.as<double>();
-  w.va = t->w_va.as<double>();
-  w.vr = t->w_vr.as<double>();
-  w.vi = t->w_vi.as<double>();
-  w.ps = t->w_ps.as<double>();
-  w.qs = t->w_qs.as<double>();
-  w.pc = t->w_pc.as<double>();
-  w.qc = t->w_qc.as<double>();
-  w.f = t->w_f.as<double>();
-  w.dx = t->w_dx.as<double>();
-  w.y = t->w_y.p;
-  w.status = t->w_status.as<int>();
-  w.iters = t->w_iters.as<int>();
-  w.maxmis = t->w_max.as<unsigned long long>();
-  w.history = t->w_hist.as<double>();
-  w.running = t->w_running.as<int>();
-  if (any_outage) {
-    if ((rc = upload_branches(*t)) || (rc = t->w_outage.upload(h_outage))) return rc;
-    w.outage = t->w_outage.as<int>();
-  }
-
-  // inputs -> interleaved tiles
-  const double* src[4] = {p_spec, q_spec, vm0, va0};
-  double* dst[4] = {w.ps, w.qs, w.vm, w.va};
-  if ((rc = t->w_stage.reserve(plane))) return rc;
-  for (int a = 0; a < 4; ++a) {
-    const double* s = src[a];
-    if (!dev) {
-      GX_HIP(hipMemcpyAsync(t->w_stage.p, src[a], plane, hipMemcpyHostToDevice, st));
-      s = t->w_stage.as<double>();
-    }
-    gx::launch_to_tiles(s, dst[a], n_scen, n_tiles, nb, tile, st);
-  }
-  if (flags & GX_INIT_LINEAR) {
-    // BatchSolver runs linear_initial_guess on every scenario first (src/batch.rs:268)
-    if ((rc = build_lg(*t, lanes, tile))) return rc;
-    gx::launch_linear_guess(t->dt, t->dlg, t->dlgt, w, t->w_lg_lu.p, t->w_lg_y.p, t->w_lg_status.as<int>(),
-                            t->w_lg_iters.as<int>(), st);
-    GX_HIP(hipGetLastError());
-  }
-  if (!t->factored) {
-    // PersistentSolver's first factorization uses the first Jacobian it is asked to solve: the
-    // first scenario whose starting point is not already converged (src/solver.rs:699-713).
-    gx::launch_from_tiles(w.vm, t->w_stage.as<double>(), n_scen, nb, tile, st);
-    std::vector<double> hvm((size_t)n_scen * nb);
-    GX_HIP(hipMemcpyAsync(hvm.data(), t->w_stage.p, plane, hipMemcpyDeviceToHost, st));
-    GX_HIP(hipStreamSynchronize(st));
-    gx::launch_from_tiles(w.va, t->w_stage.as<double>(), n_scen, nb, tile, st);
-    std::vector<double> hva((size_t)n_scen * nb);
-    GX_HIP(hipMemcpyAsync(hva.data(), t->w_stage.p, plane, hipMemcpyDeviceToHost, st));
-    GX_HIP(hipStreamSynchronize(st));
-    std::vector<double> hp(nb), hq(nb), pc, qc;
-    int chosen = -1;
-    for (int s = 0; s < n_scen && chosen < 0; ++s) {
-      const size_t off = (size_t)s * nb;
-      hipMemcpyKind k = dev ? hipMemcpyDeviceToHost : hipMemcpyHostToHost;
-      GX_HIP(hipMemcpy(hp.data(), p_spec + off, nb * sizeof(double), k));
-      GX_HIP(hipMemcpy(hq.data(), q_spec + off, nb * sizeof(double), k));
-      for (int i = 0; i < nb; ++i)
-        if (t->dev_type[i] == gx::kDeadBus) hvm[off + i] = hva[off + i] = hp[i] = hq[i] = 0.0;
-      host_injections(*t, hvm.data() + off, hva.data() + off, pc, qc);
-      double mx = 0.0;
-      for (int i = 0; i < nb; ++i) {
-        if (t->ns_pos[i] >= 0) mx = std::fmax(mx, std::fabs(hp[i] - pc[i]));
-        if (t->pq_pos[i] >= 0) mx = std::fmax(mx, std::fabs(hq[i] - qc[i]));
-      }
-      if (!(mx < tol) || s == n_scen - 1) chosen = s;
-    }
-    const size_t off = (size_t)chosen * nb;
-    if ((rc = factor_at(*t, hvm.data() + off, hva.data() + off, hp.data(), hq.data()))) return rc;
-  }
-  // the solve ring is sized for FP64 values: the mixed mode's FP64 re-solve uses the same lists
-  const int rows = pick_ring_rows(t->sched.max_col, tile, sizeof(double), t->ring_req);
-  if (rows < 0)
-    return fail(GX_ERR_UNSUPPORTED, "longest L+U column exceeds the LDS staging limit of the device refactor");
-  if ((rc = upload_topo(*t, gx::ring_window_of(rows, tile), tile))) return rc;
-  // FP64-sized in both modes: the mixed mode's FP64 re-solve reuses the same buffer
-  if ((rc = t->w_lu.reserve(lanes * (size_t)t->dt.nnz_lu * 8))) return rc;
-  w.lu = t->w_lu.p;
-  if ((rc = t->w_fallback.reserve(sizeof(int)))) return rc;
-  w.fallback = t->w_fallback.as<int>();
-  if (fp32) {
-    const size_t nbytes = lanes * nb * sizeof(double);
-    if ((rc = t->w_vm0.reserve(nbytes)) || (rc = t->w_va0.reserve(nbytes))) return rc;
-    w.vm0 = t->w_vm0.as<double>();
-    w.va0 = t->w_va0.as<double>();
-    GX_HIP(hipMemcpyAsync(w.vm0, w.vm, nbytes, hipMemcpyDeviceToDevice, st));
-    GX_HIP(hipMemcpyAsync(w.va0, w.va, nbytes, hipMemcpyDeviceToDevice, st));
-  }
-  gx::launch_init(t->dt, w, st);
+  Workspace& ws = t->ws;
+  if ((rc = stage_batch(*t, ws, in, tile, w))) return rc;
   // optional per-launch timing of each kernel class on this stream
   auto timed = [&](int cls, auto&& launch) -> int {
     if (!t->profiling) {
@@ -1504,14 +1584,14 @@ int gx_batch_solve_outages(gx_topo* t, int n_scen, const double* p_spec, const d
       if ((rc2 = timed(GX_K_MISMATCH, [&] { gx::launch_mismatch(t->dt, w, lu32, st); }))) return rc2;
       gx::launch_decide(t->dt, w, it, st);
       // stop as soon as no scenario is left running (no empty launches after convergence)
-      GX_HIP(hipMemcpyAsync(t->h_running + it, w.running + it, sizeof(int), hipMemcpyDeviceToHost, st));
+      GX_HIP(hipMemcpyAsync(ws.h_running + it, w.running + it, sizeof(int), hipMemcpyDeviceToHost, st));
       GX_HIP(hipStreamSynchronize(st));
-      if (t->h_running[it] == 0) break;
+      if (ws.h_running[it] == 0) break;
       if ((rc2 = timed(GX_K_FILL, [&] { gx::launch_fill(t->dt, w, lu32, st); }))) return rc2;
       if ((rc2 = timed(GX_K_LU, [&] { gx::launch_lu(t->dt, w, lu32, true, it, st); }))) return rc2;
       if (lu32) {
         if ((rc2 = timed(GX_K_REFINE, [&] {
-               gx::launch_refine(t->dt, w, t->w_dvr.as<double>(), t->w_dvi.as<double>(), st);
+               gx::launch_refine(t->dt, w, ws.dvr.as<double>(), ws.dvi.as<double>(), st);
              })))
           return rc2;
         if ((rc2 = timed(GX_K_SOLVE, [&] { gx::launch_lu(t->dt, w, lu32, false, it, st); }))) return rc2;
@@ -1523,41 +1603,23 @@ int gx_batch_solve_outages(gx_topo* t, int n_scen, const double* p_spec, const d
     return GX_OK;
   };
   t->last_fallback = 0;
-  if (fp32) {
+  if (in.fp32) {
     w.stall_check = 1;
     if ((rc = newton(true))) return rc;
-    // per-instance FP64 fallback: every scenario that stalled, hit max_iter or a singular FP32
-    // factor is solved again from its starting point with the FP64 path
+    // per-instance FP64 fallback: every scenario that stalled or met a singular FP32 factor is
+    // solved again from its starting point with the FP64 path
     w.stall_check = 0;
     gx::launch_fallback(t->dt, w, st);
-    GX_HIP(hipMemcpyAsync(&t->h_running[0], w.fallback, sizeof(int), hipMemcpyDeviceToHost, st));
+    GX_HIP(hipMemcpyAsync(&ws.h_running[0], w.fallback, sizeof(int), hipMemcpyDeviceToHost, st));
     GX_HIP(hipStreamSynchronize(st));
-    t->last_fallback = t->h_running[0];
+    t->last_fallback = ws.h_running[0];
     if (t->last_fallback > 0 && (rc = newton(false))) return rc;
   } else {
     if ((rc = newton(false))) return rc;
   }
   t->last = w;
   t->has_last = true;
-
-  // outputs
-  double* outs[2] = {vm, va};
-  const double* tiles[2] = {w.vm, w.va};
-  for (int a = 0; a < 2; ++a) {
-    if (dev) {
-      gx::launch_from_tiles(tiles[a], outs[a], n_scen, nb, tile, st);
-    } else {
-      gx::launch_from_tiles(tiles[a], t->w_stage.as<double>(), n_scen, nb, tile, st);
-      GX_HIP(hipMemcpyAsync(outs[a], t->w_stage.p, plane, hipMemcpyDeviceToHost, st));
-    }
-  }
-  hipMemcpyKind k = dev ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
-  GX_HIP(hipMemcpyAsync(status, w.status, sizeof(int) * n_scen, k, st));
-  GX_HIP(hipMemcpyAsync(iters, w.iters, sizeof(int) * n_scen, k, st));
-  if (history && max_iter > 0)
-    GX_HIP(hipMemcpyAsync(history, w.history, sizeof(double) * n_scen * (size_t)max_iter, k, st));
-  GX_HIP(hipStreamSynchronize(st));
-  return GX_OK;
+  return collect_batch(*t, ws, w, in.dev, vm, va, status, iters, history, st);
 }
 
 }  // extern "C"
@@ -1684,5 +1746,345 @@ int gx_sys_factor_and_solve_values(gx_sys* s, const double* vals, const double* 
 }
 
 void gx_sys_destroy(gx_sys* s) { delete s; }
+
+}  // extern "C"
+
+// ---------------------------------------------------------------- request engine
+// Many batches ("requests": one topology, its scenarios, tolerance, iteration limit, mode) in
+// flight at once.  Each step runs one Newton iteration of every active request: its element
+// kernels, then ONE persistent LU launch over the tiles of all requests (heaviest tiles first,
+// pulled from a device work queue; tiles whose scenarios all stopped return at once), then its
+// update.  Requests join between steps (admitted while their workspace fits the device-memory
+// budget) and leave when finished, so new requests overlap on the device with those already
+// running.  gx_engine_run captures one step of the current request set in a hipGraph and replays it.
+struct gx_request {
+  int id = 0;
+  gx_topo* t = nullptr;
+  Workspace ws;
+  gx::DevWork w;
+  bool dev = false;  // results go to device pointers
+  bool mixed = false, lu32 = false;
+  int iter = 0;   // Newton iteration of the current phase (host mirror of w.iter_dev)
+  int phase = 0;  // 0 first solve, 1 FP64 re-solve of the mixed mode's fallbacks, 2 finished
+  int fallbacks = 0;
+  double cost = 0.0;  // relative LU time of one tile (queue order)
+  long long reserved = 0;  // bytes admitted against the budget (the estimate at submission)
+};
+
+struct gx_engine {
+  int tile = 4;
+  int flags = 0;
+  long long budget = 0;
+  hipStream_t st = nullptr;
+  bool own_stream = false;
+  int n_cu = 256;
+  int next_id = 0;
+  std::map<int, std::unique_ptr<gx_request>> reqs;
+  std::vector<gx_request*> active;
+  bool dirty = true;  // LU tables / captured step out of date
+  // LU launch kinds: 0 FP64 factor+solve, 1 FP32 factor+solve, 2 FP32 correction solve
+  struct Launch {
+    DevMem topos, works, items, counter;
+    int n_items = 0;
+    size_t shmem = 0;
+    int grid = 0;
+  } lk[3];
+  int* h_run = nullptr;  // pinned: running count of each active request after its last decision
+  size_t h_run_cap = 0;
+  hipGraphExec_t exec = nullptr;
+  ~gx_engine() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (h_run) (void)hipHostFree(h_run);
+    reqs.clear();
+    if (own_stream && st) (void)hipStreamDestroy(st);
+  }
+};
+
+namespace {
+
+// Rebuilds the device tables of the three LU launch kinds for the active set (synchronous: only
+// when the set or a request's phase changed).
+int engine_tables(gx_engine& e) {
+  for (int k = 0; k < 3; ++k) {
+    gx_engine::Launch& L = e.lk[k];
+    std::vector<gx::DevTopo> topos;
+    std::vector<gx::DevWork> works;
+    std::vector<std::pair<double, int2>> items;
+    L.shmem = 0;
+    for (gx_request* r : e.active) {
+      const bool fp32 = r->lu32;
+      if ((k == 0 && fp32) || (k > 0 && !fp32)) continue;
+      const int idx = (int)topos.size();
+      topos.push_back(r->t->dt);
+      works.push_back(r->w);
+      for (int tl = 0; tl < r->w.n_tiles; ++tl) items.push_back({r->cost, make_int2(idx, tl)});
+      L.shmem = std::max(L.shmem, gx::lu_shmem_tile(r->t->dt, fp32, k < 2, e.tile));
+    }
+    std::stable_sort(items.begin(), items.end(), [](const auto& a, const auto& b) { return a.first > b.first; });
+    std::vector<int2> it2(items.size());
+    for (size_t i = 0; i < items.size(); ++i) it2[i] = items[i].second;
+    L.n_items = (int)it2.size();
+    if (L.n_items == 0) continue;
+    int rc;
+    if ((rc = L.topos.reserve(sizeof(gx::DevTopo) * topos.size())) ||
+        (rc = L.works.reserve(sizeof(gx::DevWork) * works.size())) || (rc = L.items.reserve(sizeof(int2) * it2.size())) ||
+        (rc = L.counter.reserve(sizeof(int))))
+      return rc;
+    GX_HIP(hipMemcpyAsync(L.topos.p, topos.data(), sizeof(gx::DevTopo) * topos.size(), hipMemcpyHostToDevice, e.st));
+    GX_HIP(hipMemcpyAsync(L.works.p, works.data(), sizeof(gx::DevWork) * works.size(), hipMemcpyHostToDevice, e.st));
+    GX_HIP(hipMemcpyAsync(L.items.p, it2.data(), sizeof(int2) * it2.size(), hipMemcpyHostToDevice, e.st));
+    GX_HIP(hipStreamSynchronize(e.st));
+    // resident wavefronts: LDS and (at most) eight per CU
+    const int per_cu = std::max(1, std::min(8, (int)(gx::kMaxLds / std::max<size_t>(L.shmem, 1))));
+    L.grid = std::min(L.n_items, e.n_cu * per_cu);
+  }
+  if (e.active.size() > e.h_run_cap) {
+    if (e.h_run) (void)hipHostFree(e.h_run);
+    e.h_run = nullptr;
+    GX_HIP(hipHostMalloc((void**)&e.h_run, sizeof(int) * e.active.size(), hipHostMallocDefault));
+    e.h_run_cap = e.active.size();
+  }
+  e.dirty = false;
+  if (e.exec) {
+    (void)hipGraphExecDestroy(e.exec);
+    e.exec = nullptr;
+  }
+  return GX_OK;
+}
+
+// Launches of one Newton step of every active request (no host synchronization inside).
+void engine_step_launches(gx_engine& e) {
+  for (gx_request* r : e.active) {
+    gx::launch_mismatch(r->t->dt, r->w, r->lu32, e.st);
+    gx::launch_decide(r->t->dt, r->w, 0, e.st);
+  }
+  for (gx_request* r : e.active) gx::launch_fill(r->t->dt, r->w, r->lu32, e.st);
+  for (int k = 0; k < 2; ++k) {
+    gx_engine::Launch& L = e.lk[k];
+    gx::launch_lu_queue(e.tile, L.topos.as<gx::DevTopo>(), L.works.as<gx::DevWork>(), L.items.as<int2>(), L.n_items,
+                        L.counter.as<int>(), L.shmem, L.grid, k == 1, true, e.st);
+  }
+  for (gx_request* r : e.active)
+    if (r->lu32) gx::launch_refine(r->t->dt, r->w, r->ws.dvr.as<double>(), r->ws.dvi.as<double>(), e.st);
+  {
+    gx_engine::Launch& L = e.lk[2];
+    gx::launch_lu_queue(e.tile, L.topos.as<gx::DevTopo>(), L.works.as<gx::DevWork>(), L.items.as<int2>(), L.n_items,
+                        L.counter.as<int>(), L.shmem, L.grid, true, false, e.st);
+  }
+  for (gx_request* r : e.active) {
+    gx::launch_update(r->t->dt, r->w, r->lu32, r->lu32, e.st);
+    gx::launch_iter_advance(r->w.iter_dev, e.st);
+  }
+}
+
+// A request whose current Newton phase ended (no scenario running, or its iteration limit):
+// MaxIterationsReached for those still running; in the mixed mode, the stalled or FP32-singular
+// scenarios start the FP64 re-solve; otherwise the request is finished.
+int engine_end_phase(gx_engine& e, gx_request& r) {
+  gx::launch_finish(r.w, e.st);
+  if (r.mixed && r.phase == 0) {
+    r.w.stall_check = 0;
+    gx::launch_fallback(r.t->dt, r.w, e.st);
+    GX_HIP(hipMemcpyAsync(&r.ws.h_running[0], r.w.fallback, sizeof(int), hipMemcpyDeviceToHost, e.st));
+    GX_HIP(hipStreamSynchronize(e.st));
+    r.fallbacks = r.ws.h_running[0];
+    if (r.fallbacks > 0) {
+      r.phase = 1;
+      r.lu32 = false;
+      r.iter = 0;
+      e.dirty = true;
+      return GX_OK;
+    }
+  }
+  r.phase = 2;
+  e.dirty = true;
+  return GX_OK;
+}
+
+// After steps: each request's running count after its last decision; phase ends.
+int engine_retire(gx_engine& e) {
+  for (size_t i = 0; i < e.active.size(); ++i) {
+    gx_request* r = e.active[i];
+    GX_HIP(hipMemcpyAsync(e.h_run + i, r->w.running + (r->iter - 1), sizeof(int), hipMemcpyDeviceToHost, e.st));
+  }
+  GX_HIP(hipStreamSynchronize(e.st));
+  std::vector<gx_request*> keep;
+  for (size_t i = 0; i < e.active.size(); ++i) {
+    gx_request* r = e.active[i];
+    int rc;
+    if (e.h_run[i] == 0 || r->iter >= r->w.max_iter) {
+      if ((rc = engine_end_phase(e, *r))) return rc;
+    }
+    if (r->phase < 2) keep.push_back(r);
+  }
+  if (keep.size() != e.active.size()) e.dirty = true;
+  e.active.swap(keep);
+  return GX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+gx_engine* gx_engine_create(int tile, int flags, long long hbm_budget, void* stream, int* err) {
+  auto set = [&](int c) {
+    if (err) *err = c;
+  };
+  if (tile != 0 && !valid_tile(tile)) {
+    set(fail(GX_ERR_ARG, "tile must be 0, 4, 8, 16 or 64"));
+    return nullptr;
+  }
+  std::unique_ptr<gx_engine> e(new gx_engine());
+  e->tile = tile ? tile : 4;
+  e->flags = flags;
+  e->budget = hbm_budget;
+  e->st = (hipStream_t)stream;
+  if (!e->st) {
+    if (hipStreamCreateWithFlags(&e->st, hipStreamNonBlocking) != hipSuccess) {
+      set(fail(GX_ERR_DEVICE, "hipStreamCreate"));
+      return nullptr;
+    }
+    e->own_stream = true;
+  }
+  int dev = 0, cus = 0;
+  if (hipGetDevice(&dev) == hipSuccess && hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
+      cus > 0)
+    e->n_cu = cus;
+  set(GX_OK);
+  return e.release();
+}
+
+void gx_engine_destroy(gx_engine* e) {
+  if (e && e->st) (void)hipStreamSynchronize(e->st);
+  delete e;
+}
+
+long long gx_engine_bytes(const gx_engine* e) {
+  if (!e) return fail(GX_ERR_ARG, "null");
+  long long b = 0;
+  for (auto& kv : e->reqs) b += std::max((long long)kv.second->ws.bytes(), kv.second->reserved);
+  return b;
+}
+
+long long gx_engine_request_bytes(const gx_topo* t, int n_scen, int max_iter, int mode, int flags, int tile) {
+  if (!t || n_scen <= 0 || max_iter < 0) return fail(GX_ERR_ARG, "bad request");
+  tile = tile ? tile : 4;
+  const long long lanes = (long long)((n_scen + tile - 1) / tile) * tile;
+  const long long nb = t->n_bus, n = t->n;
+  const long long nnz_lu = t->factored ? t->lu.nnz() : 4 * (long long)t->jkind.size();
+  long long b = lanes * (8 * nb * 8 + 3 * n * 8 + nnz_lu * 8 + (long long)std::max(max_iter, 1) * 8 + 16) +
+                (long long)n_scen * nb * 8 + 4LL * std::max(max_iter, 1) + 64;
+  if (mode == GX_MODE_MIXED) b += lanes * 4 * nb * 8;
+  if (flags & GX_INIT_LINEAR) b += lanes * (4 * nnz_lu / 2 * 16 + nb * 16 + 8);
+  return b;
+}
+
+int gx_engine_submit(gx_engine* e, gx_topo* t, int n_scen, const double* p_spec, const double* q_spec,
+                     const double* vm0, const double* va0, const int* outage, double tol, int max_iter, int mode,
+                     int flags) {
+  if (!e || !t) return fail(GX_ERR_ARG, "null engine or topology");
+  if (n_scen <= 0 || max_iter <= 0 || !(tol >= 0.0)) return fail(GX_ERR_ARG, "bad n_scen/max_iter/tol");
+  if (mode != GX_MODE_FP64 && mode != GX_MODE_MIXED) return fail(GX_ERR_ARG, "unknown mode");
+  if (!p_spec || !q_spec || !vm0 || !va0) return fail(GX_ERR_ARG, "null array");
+  const long long need = gx_engine_request_bytes(t, n_scen, max_iter, mode, flags, e->tile);
+  if (e->budget > 0 && gx_engine_bytes(e) + need > e->budget)
+    return fail(GX_ERR_BUDGET, "request does not fit the device-memory budget now");
+  std::unique_ptr<gx_request> r(new gx_request());
+  r->reserved = need;
+  BatchIn in;
+  in.n_scen = n_scen;
+  in.p = p_spec;
+  in.q = q_spec;
+  in.vm0 = vm0;
+  in.va0 = va0;
+  in.tol = tol;
+  in.max_iter = max_iter;
+  in.fp32 = mode == GX_MODE_MIXED;
+  in.flags = flags;
+  in.dev = (flags & GX_DEVICE_PTRS) != 0;
+  in.st = e->st;
+  std::vector<int> h_outage;
+  bool any = false;
+  int rc;
+  if ((rc = read_outages(*t, n_scen, outage, in.dev, h_outage, any))) return rc;
+  in.outage = any ? &h_outage : nullptr;
+  if ((rc = stage_batch(*t, r->ws, in, e->tile, r->w))) return rc;
+  if ((rc = r->ws.iter_dev.reserve(sizeof(int)))) return rc;
+  r->w.iter_dev = r->ws.iter_dev.as<int>();
+  gx::launch_zero_ints(r->w.iter_dev, 1, e->st);
+  r->id = e->next_id++;
+  r->t = t;
+  r->dev = in.dev;
+  r->mixed = in.fp32;
+  r->lu32 = in.fp32;
+  r->w.stall_check = in.fp32 ? 1 : 0;
+  r->cost = (double)t->dt.n * (1.0 + (double)t->sched.n_upd / std::max(1, t->dt.n) / 16.0);
+  GX_HIP(hipGetLastError());
+  const int id = r->id;
+  e->active.push_back(r.get());
+  e->reqs[id] = std::move(r);
+  e->dirty = true;
+  return id;
+}
+
+int gx_engine_step(gx_engine* e) {
+  if (!e) return fail(GX_ERR_ARG, "null");
+  if (e->active.empty()) return 0;
+  int rc;
+  if (e->dirty && (rc = engine_tables(*e))) return rc;
+  engine_step_launches(*e);
+  for (gx_request* r : e->active) ++r->iter;
+  GX_HIP(hipGetLastError());
+  if ((rc = engine_retire(*e))) return rc;
+  return (int)e->active.size();
+}
+
+int gx_engine_run(gx_engine* e, int max_steps) {
+  if (!e || max_steps < 0) return fail(GX_ERR_ARG, "bad arguments");
+  int rc;
+  while (!e->active.empty() && max_steps > 0) {
+    if (e->dirty && (rc = engine_tables(*e))) return rc;
+    // replay as many steps as no request can finish its phase by its iteration limit
+    int n = max_steps;
+    for (gx_request* r : e->active) n = std::min(n, r->w.max_iter - r->iter);
+    n = std::max(n, 1);
+    if (!e->exec) {
+      hipGraph_t g = nullptr;
+      GX_HIP(hipStreamBeginCapture(e->st, hipStreamCaptureModeThreadLocal));
+      engine_step_launches(*e);
+      GX_HIP(hipStreamEndCapture(e->st, &g));
+      const hipError_t ie = hipGraphInstantiate(&e->exec, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      if (ie != hipSuccess) return fail(GX_ERR_DEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(ie));
+    }
+    // check for converged requests every few steps
+    const int chunk = std::min(n, 4);
+    for (int k = 0; k < chunk; ++k) GX_HIP(hipGraphLaunch(e->exec, e->st));
+    for (gx_request* r : e->active) r->iter += chunk;
+    max_steps -= chunk;
+    if ((rc = engine_retire(*e))) return rc;
+  }
+  return (int)e->active.size();
+}
+
+int gx_engine_collect(gx_engine* e, int id, double* vm, double* va, int* status, int* iters, double* history) {
+  if (!e) return fail(GX_ERR_ARG, "null");
+  auto it = e->reqs.find(id);
+  if (it == e->reqs.end()) return fail(GX_ERR_ARG, "unknown request id");
+  gx_request& r = *it->second;
+  if (r.phase < 2) return 0;
+  if (!vm || !va || !status || !iters) return fail(GX_ERR_ARG, "null output");
+  int rc;
+  if ((rc = collect_batch(*r.t, r.ws, r.w, r.dev, vm, va, status, iters, history, e->st))) return rc;
+  e->reqs.erase(it);
+  return 1;
+}
+
+int gx_engine_fallbacks(const gx_engine* e, int id) {
+  if (!e) return fail(GX_ERR_ARG, "null");
+  auto it = e->reqs.find(id);
+  if (it == e->reqs.end()) return fail(GX_ERR_ARG, "unknown request id");
+  return it->second->fallbacks;
+}
 
 }  // extern "C"

@@ -209,6 +209,8 @@ __global__ __launch_bounds__(256) void k_init(DevTopo t, DevWork w) {
     w.iters[sc] = 0;
     w.maxmis[sc] = 0ull;
   }
+  if (blockIdx.x == 0)  // the mismatch history is defined (zero) past a scenario's last iteration
+    for (int q = threadIdx.x; q < TW * w.max_iter; q += blockDim.x) w.history[(size_t)tile * TW * w.max_iter + q] = 0.0;
   if (blockIdx.x == 0 && tile == 0) {
     for (int it = threadIdx.x; it < w.max_iter; it += blockDim.x) w.running[it] = 0;
     if (threadIdx.x == 0 && w.fallback) *w.fallback = 0;
@@ -279,6 +281,7 @@ __global__ __launch_bounds__(256) void k_mismatch(DevTopo t, DevWork w) {
 }
 
 __global__ void k_decide(DevWork w, int iter) {
+  if (w.iter_dev) iter = *w.iter_dev;
   const int sc = blockIdx.x * blockDim.x + threadIdx.x;
   if (sc >= w.n_tiles * w.tile) return;
   if (w.status[sc] != kRunning) return;
@@ -322,14 +325,17 @@ __global__ __launch_bounds__(256) void k_fallback_restore(DevTopo t, DevWork w) 
 // FP64 re-solve, step 2 (after step 1 completed): those scenarios run again.
 __global__ void k_fallback_reset(DevWork w) {
   const int sc = blockIdx.x * blockDim.x + threadIdx.x;
-  if (sc == 0)
+  if (sc == 0) {
     for (int it = 0; it < w.max_iter; ++it) w.running[it] = 0;
+    if (w.iter_dev) *w.iter_dev = 0;
+  }
   if (sc >= w.n_tiles * w.tile) return;
   const int st = w.status[sc];
   if (st != kStalled && st != kSingular) return;
   w.status[sc] = kRunning;
   w.iters[sc] = 0;
   w.maxmis[sc] = 0ull;
+  for (int it = 0; it < w.max_iter; ++it) w.history[(size_t)sc * w.max_iter + it] = 0.0;
   atomicAdd(w.fallback, 1);
 }
 
@@ -401,7 +407,7 @@ __device__ __forceinline__ int rl(int v, int lane) { return __builtin_amdgcn_rea
 // unused results discarded.  L values of column k-1 (not yet written back when column k's loads
 // are issued) are read from LDS instead.
 template <typename T, int TW, bool kFactor>
-__global__ __launch_bounds__(64) void k_lu(DevTopo t, DevWork w, int cap, int iter) {
+__device__ __forceinline__ void lu_tile(const DevTopo& t, const DevWork& w, int tile, int cap, int iter) {
   constexpr int G = kLanes / TW;
   constexpr int S = TW + 1;
   constexpr int B = kRingGap;
@@ -413,7 +419,7 @@ __global__ __launch_bounds__(64) void k_lu(DevTopo t, DevWork w, int cap, int it
   T* xcb = reinterpret_cast<T*>(smem_raw);
   const int xstride = (cap + 1) * S;
   T* ring = xcb + (kFactor ? 2 * (size_t)xstride : 0);
-  const int tile = blockIdx.x, g = grp<TW>(), s = sub<TW>();
+  const int g = grp<TW>(), s = sub<TW>();
   if (!tile_running<TW>(w.status, tile)) return;
   const int sc = tile * TW + s;
   T* __restrict__ lu = reinterpret_cast<T*>(w.lu) + at<TW>(tile, t.nnz_lu, 0) + s;
@@ -740,14 +746,14 @@ __global__ __launch_bounds__(64) void k_lu(DevTopo t, DevWork w, int cap, int it
 // the host keeps L(:,k-1) out of column k's first round, so those loads never read a column
 // before it is written).
 template <typename T, bool kFactor>
-__global__ __launch_bounds__(64) void k_lu64(DevTopo t, DevWork w, int cap, int iter) {
+__device__ __forceinline__ void lu64_tile(const DevTopo& t, const DevWork& w, int tile, int cap, int iter) {
   constexpr int TW = kWideTile;
   constexpr int IPR = kRoundItems;
   // complex values (the linear initial guess, run once per batch) take the shallow pipeline
   constexpr bool kDeep = sizeof(T) <= 8;
   constexpr int MS = kDeep ? kWideStage : 4;
   constexpr int MF = kWideFar;
-  const int tile = blockIdx.x, s = threadIdx.x;
+  const int s = threadIdx.x;
   if (!tile_running<TW>(w.status, tile)) return;
   const int sc = tile * TW + s;
   T* __restrict__ lu = reinterpret_cast<T*>(w.lu) + at<TW>(tile, t.nnz_lu, 0) + s;
@@ -1040,6 +1046,46 @@ __global__ __launch_bounds__(64) void k_lu64(DevTopo t, DevWork w, int cap, int 
   }
 }
 
+// One launch per batch: wavefront b factors/solves tile b.
+template <typename T, int TW, bool kFactor>
+__global__ __launch_bounds__(64) void k_lu(DevTopo t, DevWork w, int cap, int iter) {
+  if constexpr (TW == kWideTile) lu64_tile<T, kFactor>(t, w, blockIdx.x, cap, iter);
+  else lu_tile<T, TW, kFactor>(t, w, blockIdx.x, cap, iter);
+}
+
+// One persistent launch over the tiles of several requests (topologies of any size): each
+// wavefront takes (request, tile) items, heaviest first, from a device counter until the list is
+// exhausted, so every wavefront reaches the end of the list.  Each request's iteration number is
+// read from its own counter.
+template <typename T, int TW, bool kFactor>
+__global__ __launch_bounds__(64) void k_lu_queue(const DevTopo* __restrict__ topos, const DevWork* __restrict__ works,
+                                                 const int2* __restrict__ items, int n_items, int* __restrict__ counter) {
+  for (;;) {
+    int it = 0;
+    if (threadIdx.x == 0) it = atomicAdd(counter, 1);
+    it = __shfl(it, 0);
+    if (it >= n_items) break;
+    const int2 item = items[it];
+    const DevTopo& t = topos[item.x];
+    const DevWork& w = works[item.x];
+    const int cap = t.max_col < 1 ? 1 : t.max_col;
+    const int iter = w.iter_dev ? *w.iter_dev : 0;
+    if constexpr (TW == kWideTile) lu64_tile<T, kFactor>(t, w, item.y, cap, iter);
+    else lu_tile<T, TW, kFactor>(t, w, item.y, cap, iter);
+    __syncthreads();  // the next item reuses the LDS
+  }
+}
+
+__global__ void k_zero_ints(int* __restrict__ p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+// End of a Newton step of a request whose iteration number lives on the device.
+__global__ void k_iter_advance(int* __restrict__ iter_dev) {
+  if (threadIdx.x == 0) *iter_dev += 1;
+}
+
 // Mixed precision, refinement step 1: dx1 (FP64 copy of the FP32 solution) and
 // dV_i = V_i * (d|V|_i / |V|_i + j dtheta_i) per bus.
 template <typename T, int TW>
@@ -1289,21 +1335,28 @@ void run_lu(const DevTopo& t, const DevWork& w, bool factor, int iter, hipStream
   static size_t granted_f = 65536, granted_s = 65536;
   const int cap = t.max_col < 1 ? 1 : t.max_col;
   const size_t shmem = lu_bytes<T>(t, TW, factor);
-  if constexpr (TW == kWideTile) {
-    if (factor) {
-      allow_lds(k_lu64<T, true>, shmem, granted_f);
-      hipLaunchKernelGGL((k_lu64<T, true>), dim3(w.n_tiles), dim3(64), shmem, stream, t, w, cap, iter);
-    } else {
-      allow_lds(k_lu64<T, false>, shmem, granted_s);
-      hipLaunchKernelGGL((k_lu64<T, false>), dim3(w.n_tiles), dim3(64), shmem, stream, t, w, cap, iter);
-    }
-    return;
-  } else if (factor) {
+  if (factor) {
     allow_lds(k_lu<T, TW, true>, shmem, granted_f);
     hipLaunchKernelGGL((k_lu<T, TW, true>), dim3(w.n_tiles), dim3(64), shmem, stream, t, w, cap, iter);
   } else {
     allow_lds(k_lu<T, TW, false>, shmem, granted_s);
     hipLaunchKernelGGL((k_lu<T, TW, false>), dim3(w.n_tiles), dim3(64), shmem, stream, t, w, cap, iter);
+  }
+}
+
+template <typename T, int TW>
+void run_lu_queue(const DevTopo* topos, const DevWork* works, const int2* items, int n_items, int* counter,
+                  size_t shmem, int grid, bool factor, hipStream_t stream) {
+  static size_t granted_f = 65536, granted_s = 65536;
+  hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, stream, counter, 1);
+  if (factor) {
+    allow_lds(k_lu_queue<T, TW, true>, shmem, granted_f);
+    hipLaunchKernelGGL((k_lu_queue<T, TW, true>), dim3(grid), dim3(64), shmem, stream, topos, works, items, n_items,
+                       counter);
+  } else {
+    allow_lds(k_lu_queue<T, TW, false>, shmem, granted_s);
+    hipLaunchKernelGGL((k_lu_queue<T, TW, false>), dim3(grid), dim3(64), shmem, stream, topos, works, items, n_items,
+                       counter);
   }
 }
 
@@ -1368,6 +1421,28 @@ void launch_lu(const DevTopo& t, const DevWork& w, bool fp32, bool factor, int i
     if (fp32) run_lu<float, TW>(t, w, factor, iter, stream);
     else run_lu<double, TW>(t, w, factor, iter, stream);
   });
+}
+
+void launch_lu_queue(int tile, const DevTopo* topos, const DevWork* works, const int2* items, int n_items,
+                     int* counter, size_t shmem, int grid, bool fp32, bool factor, hipStream_t stream) {
+  if (n_items <= 0) return;
+  with_tile(tile, [&](auto tw) {
+    constexpr int TW = decltype(tw)::value;
+    if (fp32) run_lu_queue<float, TW>(topos, works, items, n_items, counter, shmem, grid, factor, stream);
+    else run_lu_queue<double, TW>(topos, works, items, n_items, counter, shmem, grid, factor, stream);
+  });
+}
+
+void launch_iter_advance(int* iter_dev, hipStream_t stream) {
+  hipLaunchKernelGGL(k_iter_advance, dim3(1), dim3(64), 0, stream, iter_dev);
+}
+
+void launch_zero_ints(int* p, int n, hipStream_t stream) {
+  if (n > 0) hipLaunchKernelGGL(k_zero_ints, dim3((n + 255) / 256), dim3(256), 0, stream, p, n);
+}
+
+size_t lu_shmem_tile(const DevTopo& t, bool fp32, bool factor, int tile) {
+  return fp32 ? lu_bytes<float>(t, tile, factor) : lu_bytes<double>(t, tile, factor);
 }
 
 void launch_refine(const DevTopo& t, const DevWork& w, double* dvr, double* dvi, hipStream_t stream) {

@@ -433,3 +433,92 @@ class BatchSolver:
                                                 outage=out)
         isl = topo.island_reports(len(scenarios))
         return [PowerFlowReport(vm[s], va[s], _stats(st[s], it[s], hist[s]), isl[s]) for s in range(len(scenarios))]
+
+
+class Engine:
+    """Many (topology, scenarios) requests in flight on one GPU (gx_engine): each step is one
+    Newton iteration of every active request with one persistent LU launch over all their tiles.
+    Requests join between steps, subject to the device-memory budget, and leave when finished."""
+
+    def __init__(self, tile: int = 4, budget_bytes: int = 0, graphs: bool = False):
+        err = np.zeros(1, np.int32)
+        self._h = _lib.lib().gx_engine_create(int(tile), 0, int(budget_bytes), None, ptr(err, _lib._i32p))
+        if not self._h:
+            check(int(err[0]))
+        self.tile = int(tile)
+        self.graphs = graphs
+        self._meta = {}
+        self._keep = {}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            try:
+                _lib.lib().gx_engine_destroy(self._h)
+            except Exception:  # interpreter shutdown
+                pass
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    @staticmethod
+    def request_bytes(topo: "Topology", n_scen: int, max_iter: int = 20, mode: Mode = Mode.MIXED,
+                      init: str = "given", tile: int = 4) -> int:
+        return int(check(_lib.lib().gx_engine_request_bytes(topo._h, int(n_scen), int(max_iter), int(mode),
+                                                            _init_flag(init), int(tile))))
+
+    def bytes_in_use(self) -> int:
+        return int(check(_lib.lib().gx_engine_bytes(self._h)))
+
+    def submit(self, topo: "Topology", p_spec, q_spec, vm0, va0, tol=1e-8, max_iter=20, mode=Mode.MIXED,
+               init="given", outage=None):
+        """Host arrays [n_scen, n_bus].  Returns the request id, or None when the request does not
+        fit the budget now (retry after collecting others)."""
+        p_spec = f64(np.atleast_2d(p_spec))
+        S = p_spec.shape[0]
+        q_spec = f64(np.broadcast_to(q_spec, (S, topo.n_bus)))
+        vm0 = f64(np.broadcast_to(vm0, (S, topo.n_bus)))
+        va0 = f64(np.broadcast_to(va0, (S, topo.n_bus)))
+        out = None if outage is None else i32(np.broadcast_to(outage, (S,)))
+        rc = _lib.lib().gx_engine_submit(self._h, topo._h, S, p_spec.ctypes.data, q_spec.ctypes.data,
+                                         vm0.ctypes.data, va0.ctypes.data, None if out is None else out.ctypes.data,
+                                         float(tol), int(max_iter), int(mode), _init_flag(init))
+        if rc == _lib.GX_ERR_BUDGET:
+            return None
+        rid = check(rc)
+        self._meta[rid] = (S, topo.n_bus, int(max_iter))
+        return rid
+
+    def step(self) -> int:
+        """One Newton step of every active request; returns how many remain active."""
+        return int(check(_lib.lib().gx_engine_step(self._h)))
+
+    def run(self, max_steps: int = 1 << 30) -> int:
+        """Steps until every request finished (or max_steps), each step a replayed hipGraph when the
+        engine was created with graphs=True."""
+        if self.graphs:
+            return int(check(_lib.lib().gx_engine_run(self._h, int(max_steps))))
+        active = 1
+        for _ in range(max_steps):
+            active = self.step()
+            if active == 0:
+                break
+        return active
+
+    def collect(self, rid: int):
+        """(vm, va, status, iters, history) of a finished request (it is then released), else None."""
+        S, nb, mi = self._meta[rid]
+        vm = np.zeros((S, nb))
+        va = np.zeros((S, nb))
+        st = np.zeros(S, np.int32)
+        it = np.zeros(S, np.int32)
+        hist = np.zeros((S, max(mi, 1)))
+        done = check(_lib.lib().gx_engine_collect(self._h, int(rid), vm.ctypes.data, va.ctypes.data, st.ctypes.data,
+                                                  it.ctypes.data, hist.ctypes.data))
+        if not done:
+            return None
+        del self._meta[rid]
+        return vm, va, st, it, hist[:, :mi]
+
+    def fallbacks(self, rid: int) -> int:
+        return int(check(_lib.lib().gx_engine_fallbacks(self._h, int(rid))))

@@ -26,6 +26,7 @@ extern "C" {
 #define GX_ERR_STATE (-4)        /* call out of order (e.g. pattern query before a factorization) */
 #define GX_ERR_CAPACITY (-5)     /* caller buffer too small; required size returned where stated */
 #define GX_ERR_UNSUPPORTED (-6)  /* topology exceeds a documented limit of the device path */
+#define GX_ERR_BUDGET (-7)       /* a request does not fit the engine's device-memory budget now */
 
 /* ---- solve status (SolveStatus, src/solver.rs:161-171) ---- */
 #define GX_CONVERGED 0
@@ -204,6 +205,40 @@ void gx_sys_destroy(gx_sys* s);
  * *nblocks receives the BTF block count.  Returns nnz(L+U) (strictly-triangular parts plus the
  * diagonal, the whole matrix factored as one block in the given order) or an error code. */
 int gx_sys_ordering(const gx_sys* s, int* p, int* q, int* r, int* pnum, int* nblocks);
+
+/* ---- request engine (uneven batches and serving) ----
+ * No counterpart in the reference, whose BatchSolver runs one batch over one topology on rayon
+ * workers (src/batch.rs:198-289).  Many requests -- each one topology + its scenarios, as in
+ * gx_batch_solve (a topology used by an engine keeps the engine's tile width) -- are in flight
+ * at once: every step runs one Newton iteration of every active request, with ONE persistent LU
+ * launch over the tiles of all of them (a device work queue, heaviest first).  Requests are
+ * admitted between steps and leave when finished; results per request equal gx_batch_solve's
+ * for the same inputs, tile width and pivot order. */
+typedef struct gx_engine gx_engine;
+/* tile: 4, 8, 16 or 64 (0: 4).  hbm_budget: bytes of request workspaces the engine may hold (0:
+ * unlimited).  stream: HIP stream of all its work (NULL: its own).  NULL on error (*err set). */
+gx_engine* gx_engine_create(int tile, int flags, long long hbm_budget, void* stream, int* err);
+void gx_engine_destroy(gx_engine* e);
+/* Device bytes a request would hold (an estimate before its topology's first factorization). */
+long long gx_engine_request_bytes(const gx_topo* t, int n_scen, int max_iter, int mode, int flags, int tile);
+/* Device bytes held by the engine's requests (active and finished-but-not-collected). */
+long long gx_engine_bytes(const gx_engine* e);
+/* Adds a request (arguments as gx_batch_solve_outages; flags GX_DEVICE_PTRS, GX_INIT_LINEAR).  Returns
+ * its id (>= 0), GX_ERR_BUDGET if it does not fit the budget now (retry after collecting), or an
+ * error code. */
+int gx_engine_submit(gx_engine* e, gx_topo* t, int n_scen, const double* p_spec, const double* q_spec,
+                     const double* vm0, const double* va0, const int* outage, double tol, int max_iter, int mode,
+                     int flags);
+/* One Newton step of every active request; returns how many remain active. */
+int gx_engine_step(gx_engine* e);
+/* Up to max_steps steps with each step captured once in a hipGraph and replayed (re-captured when
+ * the set of requests changes); returns how many remain active. */
+int gx_engine_run(gx_engine* e, int max_steps);
+/* 1 when request id has finished: its results (as gx_batch_solve's outputs, host or device
+ * pointers as submitted) are written and the request is released; 0 while it runs. */
+int gx_engine_collect(gx_engine* e, int id, double* vm, double* va, int* status, int* iters, double* history);
+/* Scenarios of a finished mixed-precision request that were re-solved with the FP64 path. */
+int gx_engine_fallbacks(const gx_engine* e, int id);
 
 #ifdef __cplusplus
 }
