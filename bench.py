@@ -8,6 +8,12 @@ Workloads (the synthetic meshed grids of gridoxide_amd.grid):
   config3       10,000 buses (5,000 chords), 8,192 scenarios drawn from the generator's load
                 distribution, mixed precision (FP32 LU + FP64 refinement)
   config3-fp64  the same with the FP64 path (the reference for the mixed-precision speed-up)
+  config4       64 topologies of log-uniform size (100-20,000 buses), 16-1,024 scenarios each (half
+                N-1), mixed precision, all in flight in one request engine: one persistent LU launch
+                per Newton step over every tile, steps captured in hipGraphs, launch shape autotuned
+  config5       serving: Poisson request stream over a pool of 32 grids (200-50,000 buses), 1-256
+                scenarios per request, admitted by device-memory budget with continuous batching;
+                reports p50/p99 request latency beside solves/s
 
 Besides the converged count, the line carries the rate of all scenarios solved to a final status and
 of Newton steps: at 10,000 buses the generator's loads are infeasible for every scenario (the FP64
@@ -32,6 +38,8 @@ WORKLOADS = {
     "config2": dict(config=2, n_scen=4096, mode="FP64", loads="scaled"),
     "config3": dict(config=3, n_scen=8192, mode="MIXED", loads="drawn"),
     "config3-fp64": dict(config=3, n_scen=8192, mode="FP64", loads="drawn"),
+    "config4": dict(multi="uneven"),
+    "config5": dict(multi="serving"),
 }
 
 
@@ -78,6 +86,10 @@ def main():
     from gridoxide_amd import Mode, Topology, config_grid, scenario_loads, scenario_specs
 
     wl = WORKLOADS[args.workload]
+    if wl.get("multi") == "uneven":
+        return run_uneven(args)
+    if wl.get("multi") == "serving":
+        return run_serving(args)
     mode = Mode[wl["mode"]]
     tol, max_iter = 1e-8, 20
     net = config_grid(wl["config"])
@@ -170,6 +182,152 @@ def main():
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(net, p, q, vm0, va0, tol, max_iter)
+    print(json.dumps(out), flush=True)
+
+
+def _counts(results):
+    conv = sum(int((r[2] == 0).sum()) for r in results)
+    total = sum(int(r[2].size) for r in results)
+    return conv, total
+
+
+def run_uneven(args):
+    """config4: one pass = every topology's scenarios solved in one request engine."""
+    import torch  # noqa: F401  (one HIP runtime)
+    from gridoxide_amd import Engine, Mode, Topology
+    from gridoxide_amd.tune import tune_launch
+    from gridoxide_amd.workloads import uneven_batch
+    mode = Mode.MIXED
+    jobs = uneven_batch()
+    topos = [Topology.from_network(j.net) for j in jobs]
+    # symbolic + first factorization per topology: resident before the timed passes
+    for topo, j in zip(topos, jobs):
+        topo.factor_at(j.vm0[0], j.va0[0], j.p[0], j.q[0])
+    reqs = [(topo, j.p, j.q, j.vm0, j.va0, j.outage) for topo, j in zip(topos, jobs)]
+    tile, ring, tried = tune_launch(reqs, mode, log=log)
+    log(f"autotuned launch shape: tile {tile}, ring rows {ring or 'default'}")
+    for topo in topos:
+        topo.set_launch(tile, ring)
+    n_scen = sum(j.p.shape[0] for j in jobs)
+
+    def one_pass():
+        eng = Engine(tile=tile, graphs=True)
+        ids = [eng.submit(topo, p, q, vm0, va0, max_iter=20, mode=mode, outage=o) for topo, p, q, vm0, va0, o in reqs]
+        eng.run()
+        out = [eng.collect(i) for i in ids]
+        eng.close()
+        return out
+
+    for _ in range(args.warmup):
+        one_pass()
+    t0 = time.perf_counter()
+    conv = total = 0
+    for _ in range(args.steps):
+        res = one_pass()
+        c, tt = _counts(res)
+        conv += c
+        total += tt
+    dt = time.perf_counter() - t0
+    sizes = sorted(j.net.n_bus for j in jobs)
+    out = {
+        "metric": "converged NR solves/sec at 1e-8 pu mismatch",
+        "value": conv / dt,
+        "unit": "converged solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * dt / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64+f32",
+        "data": "synthetic",
+        "config": {"workload": f"config4: {len(jobs)} topologies ({sizes[0]}-{sizes[-1]} buses), {n_scen} scenarios "
+                               f"(half N-1), mixed precision, one request engine, hipGraph steps",
+                   "n_topologies": len(jobs), "n_scenarios": n_scen, "tile": tile, "ring_rows": ring},
+        "converged_per_step": conv / args.steps,
+        "solves_per_s_all": total / dt,
+        "autotune": [{"tile": a, "ring_rows": b, "ms": 1e3 * c} for a, b, c in tried],
+    }
+    print(json.dumps(out), flush=True)
+
+
+def run_serving(args, rate_hz=2.0, n_req=48, budget_gb=96.0):
+    """config5: a Poisson request stream served with continuous batching; latency = completion time
+    minus arrival time (wall clock)."""
+    import torch  # noqa: F401
+    from gridoxide_amd import Engine, Mode, Topology
+    from gridoxide_amd.workloads import poisson_requests, serving_pool
+    mode = Mode.MIXED
+    pool = serving_pool()
+    topos = [Topology.from_network(g) for g in pool]
+    for topo, g in zip(topos, pool):  # resident symbolic + first factorization
+        topo.factor_at(g.vm, g.va, g.p_spec, g.q_spec)
+    # warm-up: one small request per grid builds each topology's device schedule
+    eng = Engine(tile=4, budget_bytes=int(budget_gb * 2**30))
+    for topo, g in zip(topos, pool):
+        rid = eng.submit(topo, g.p_spec[None], g.q_spec[None], g.vm[None], g.va[None], max_iter=2, mode=mode)
+        eng.run()
+        eng.collect(rid)
+    reqs = poisson_requests(pool, rate_hz, n_req)
+    warm = {}  # grid -> (vm, va) of its last converged solution, kept as the next start
+    pending = list(range(len(reqs)))
+    inflight = {}
+    lat = []
+    conv = total = 0
+    t0 = time.perf_counter()
+    while pending or inflight:
+        now = time.perf_counter() - t0
+        while pending and reqs[pending[0]].arrival <= now:
+            k = pending[0]
+            r = reqs[k]
+            g = pool[r.grid]
+            vm0, va0 = warm.get(r.grid, (g.vm, g.va))
+            rid = eng.submit(topos[r.grid], r.p, r.q, vm0, va0, max_iter=20, mode=mode)
+            if rid is None:
+                break  # over the memory budget: admitted when running requests finish
+            inflight[rid] = k
+            pending.pop(0)
+        if not inflight:
+            time.sleep(max(0.0, reqs[pending[0]].arrival - (time.perf_counter() - t0)))
+            continue
+        eng.step()
+        for rid in list(inflight):
+            res = eng.collect(rid)
+            if res is None:
+                continue
+            k = inflight.pop(rid)
+            lat.append(time.perf_counter() - t0 - reqs[k].arrival)
+            st = res[2]
+            conv += int((st == 0).sum())
+            total += int(st.size)
+            ok = np.nonzero(st == 0)[0]
+            if ok.size:
+                warm[reqs[k].grid] = (res[0][ok[0]], res[1][ok[0]])
+    dt = time.perf_counter() - t0
+    lat = np.array(lat)
+    out = {
+        "metric": "converged NR solves/sec at 1e-8 pu mismatch; p50/p99 request latency",
+        "value": conv / dt,
+        "unit": "converged solves/s",
+        "n_gpus": 1,
+        "steps": 1,
+        "warmup": 1,
+        "ms_per_step": 1e3 * dt,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64+f32",
+        "data": "synthetic",
+        "config": {"workload": f"config5: {n_req} Poisson requests at {rate_hz}/s over {len(pool)} grids "
+                               f"({min(g.n_bus for g in pool)}-{max(g.n_bus for g in pool)} buses), 1-256 scenarios "
+                               f"each, mixed precision, {budget_gb:g} GiB admission budget, continuous batching",
+                   "n_requests": n_req, "rate_hz": rate_hz},
+        "latency_p50_s": float(np.percentile(lat, 50)),
+        "latency_p99_s": float(np.percentile(lat, 99)),
+        "solves_per_s_all": total / dt,
+        "requests_per_s": n_req / dt,
+    }
     print(json.dumps(out), flush=True)
 
 
