@@ -34,6 +34,22 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 
+# HBM bytes per k_lu launch measured with the FETCH_SIZE / WRITE_SIZE counters (copy-calibrated) for
+# a workload's default launch shape; the summary that holds the number also gives the commands.
+TRAFFIC_SUMMARIES = {"config2": "profiles/k_lu_fp64_config2_2000bus_4096scen.md"}
+
+
+def measured_traffic(workload: str, tile: int):
+    import re
+    path = TRAFFIC_SUMMARIES.get(workload)
+    if not path or tile != 0:
+        return None, None
+    full = os.path.join(ROOT, path)
+    if not os.path.exists(full):
+        return None, None
+    m = re.search(r"^traffic_bytes_per_launch:\s*(\d+)", open(full).read(), re.M)
+    return (float(m.group(1)), path) if m else (None, None)
+
 WORKLOADS = {
     "config2": dict(config=2, n_scen=4096, mode="FP64", loads="scaled"),
     "config3": dict(config=3, n_scen=8192, mode="MIXED", loads="drawn"),
@@ -76,6 +92,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--tile", type=int, default=0, help="scenarios per tile (4, 8, 16; 0 = automatic)")
     ap.add_argument("--ring", type=int, default=0, help="LDS solve-window rows (power of two; 0 = default)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="first run a plain 256 MiB device copy (a known byte count for scaling counter reads)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (scenario sharding across "
@@ -101,6 +119,13 @@ def main():
     topo = Topology.from_network(net)
     topo.set_launch(args.tile, args.ring)
     dev = torch.device("cuda:0")
+    if args.copy_calibration:
+        a = torch.ones(32 * 2**20, dtype=torch.float64, device=dev)  # 256 MiB read + 256 MiB written
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
     tp, tq, tvm0, tva0 = (torch.from_numpy(np.ascontiguousarray(a)).to(dev) for a in (p, q, vm0, va0))
     vm = torch.empty_like(tvm0)
     va = torch.empty_like(tva0)
@@ -152,6 +177,7 @@ def main():
     b = bytes_lu if kname == "lu" else bytes_solve
     kms, klaunch = prof[kname]
     achieved = participations[kname] * b / (kms / 1e3) / 1e9 if kms > 0 else 0.0
+    traffic, traffic_src = measured_traffic(args.workload, args.tile) if kname == "lu" else (None, None)
     out = {
         "metric": "converged NR solves/sec at 1e-8 pu mismatch",
         "value": conv_total / dt,
@@ -173,7 +199,7 @@ def main():
         "solves_per_s_all": S * args.steps / dt,
         "newton_steps_per_s": newton_total / dt,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_src,
                      "kernel": f"k_lu ({'refactor + solve' if kname == 'lu' else 'correction solve'}, "
                                f"{'FP64' if w == 8 else 'FP32'})",
                      "bytes_per_scenario_step": b, "avg_launch_ms": kms / max(klaunch, 1),
