@@ -389,6 +389,19 @@ constexpr int stage_regs() {
 
 __device__ __forceinline__ int rl(int v, int lane) { return __builtin_amdgcn_readlane(v, lane); }
 
+// Uniform reads through the scalar data cache (the topology arrays are read-only while kernels run).
+__device__ __forceinline__ int sload(const int* p, size_t i) {
+  return ((const __attribute__((address_space(4))) int*)p)[i];
+}
+__device__ __forceinline__ int4 sload(const int4* p, size_t i) {
+  const __attribute__((address_space(4))) int* q = (const __attribute__((address_space(4))) int*)(p + i);
+  return make_int4(q[0], q[1], q[2], q[3]);
+}
+
+// Waits for every outstanding memory operation: closes the rare variable-length paths (long
+// columns, overflow gathers) so that the common path's waits stay exact.
+__device__ __forceinline__ void drain() { __builtin_amdgcn_s_waitcnt(0); }
+
 // Refactor (optional) + forward/back substitution; one wavefront per tile.
 //   LDS: two staged columns xc[2][cap+1][TW+1] (column k in buffer k & 1; the previous column stays
 //   readable for the updates that use it) and ring[R][TW+1] (the right-hand side rows inside the
@@ -399,13 +412,14 @@ __device__ __forceinline__ int rl(int v, int lane) { return __builtin_amdgcn_rea
 // pushes into the ring rows i <= k + W; row k + W + 1 then enters the ring (taking the slot of
 // row k - B) as f minus the contributions of its L entries farther than W, gathered at that
 // moment.  Back substitution mirrors this with U: the ring holds rows [k - W, k + B].
-// Memory: every global load a column needs is issued at the top of an earlier column -- column
-// metadata a batch of G columns ahead (lane group g holds column batch start + g, read with
-// readlane), the staged values and rows, the first kChunkRounds update rounds' L values and the
-// far entries one column ahead, the slots those loads index two columns ahead -- so each column
-// waits once, for loads a column old.  Loads are issued unconditionally at clamped addresses and
-// unused results discarded.  L values of column k-1 (not yet written back when column k's loads
-// are issued) are read from LDS instead.
+// Memory: column metadata comes through the scalar cache two columns ahead; every vector load a
+// column needs (staged values and rows, the first kChunkRounds update rounds' L values, the far
+// entries and the entering row's f) is issued at the top of an earlier column, the slots those
+// loads index two columns ahead.  On the common path every column issues the same loads and
+// stores, unconditionally at clamped addresses (stores of unused slots rewrite the diagonal with
+// its own value), so the compiler's waits are exact: a column waits only for loads a column old.
+// Rare variable paths end with a full wait.  L values of column k-1 (not yet written back when
+// column k's loads are issued) are read from LDS instead.
 template <typename T, int TW, bool kFactor>
 __device__ __forceinline__ void lu_tile(const DevTopo& t, const DevWork& w, int tile, int cap, int iter) {
   constexpr int G = kLanes / TW;
@@ -431,28 +445,31 @@ __device__ __forceinline__ void lu_tile(const DevTopo& t, const DevWork& w, int 
   auto slot = [&](int r) { return (r & (R - 1)) * S + s; };
   auto cl = [](int v, int hi) { return v < 0 ? 0 : (v > hi ? hi : v); };
   const T zero = zero_value<T>();
+  // Lane-group strided loop body(i) over i = base + off + G*j < end, with the loop control on
+  // the uniform base (0 <= off < G) and the lane part predicated: every lane leaves the loop
+  // together, so the loop exit never runs with an empty exec mask (register copies that the
+  // compiler places in such an exit block would be skipped).
+  auto strided = [&](int base, int off, int end, auto&& body) {
+    for (int b = base; b < end; b += G) {
+      const int i = b + off;
+      if (i < end) body(i);
+    }
+  };
 
   // ------------------------------------------------ forward pass (+ refactor)
-  for (int r = g; r <= W && r < n; r += G) ring[slot(r)] = y[(size_t)r * TW];
-  // f of the rows entering the window: within a batch of G columns, group g holds the row that
-  // enters after column (batch start + g); the next batch is loaded one batch ahead
-  auto f_of = [&](int kk) -> T { return y[(size_t)cl(kk + W + 1, n - 1) * TW]; };
-  T fcur = f_of(g), fnext = f_of(G + g);
-  int4 ca = t.col_a[cl(g, n - 1)], cf = t.col_f[cl(g, n - 1)];
-  int4 na = t.col_a[cl(G + g, n - 1)], nf = t.col_f[cl(G + g, n - 1)];
-  int kb = 0;  // first column of the batch in ca/cf
-  // metadata of column kk in [kb, kb + 2G)
-  auto m_c0 = [&](int kk) { return rl(kk - kb >= G ? na.x : ca.x, ((kk - kb) % G) * TW); };
-  auto m_len = [&](int kk) { return rl(kk - kb >= G ? na.y : ca.y, ((kk - kb) % G) * TW); };
-  auto m_ul = [&](int kk) { return rl(kk - kb >= G ? na.z : ca.z, ((kk - kb) % G) * TW); };
-  auto m_ch0 = [&](int kk) { return rl(kk - kb >= G ? na.w : ca.w, ((kk - kb) % G) * TW); };
+  strided(0, g, W + 1 < n ? W + 1 : n, [&](int r) { ring[slot(r)] = y[(size_t)r * TW]; });
+  // column metadata a = {c0, len, ul, chunk0}, f = {rounds, overflow range}: m*1 = next column,
+  // m*2 = the one after; fr1/fr2 = f of the rows entering after those columns
+  int4 ma1 = sload(t.col_a, 0), ma2 = sload(t.col_a, cl(1, n - 1));
+  int4 mf1 = sload(t.col_f, 0), mf2 = sload(t.col_f, cl(1, n - 1));
+  T fr1 = y[(size_t)cl(W + 1, n - 1) * TW], fr2 = y[(size_t)cl(W + 2, n - 1) * TW];
 
   // staged entries: factor -> entries o = first_of_group(c0, g) + G*i of the whole column (owned
   // positions); solve only -> L entries o = ul + 1 + g + G*i
   T sv[MS], cv[MS];
   int sr[MS], cr[MS];
-  auto stage_issue = [&](int kk) {
-    const int c0 = m_c0(kk), len = m_len(kk), ul = m_ul(kk);
+  auto stage_issue = [&](const int4& m) {
+    const int c0 = m.x, len = m.y, ul = m.z;
     const int f0 = kFactor ? first_of_group<G>(c0, g) : ul + 1 + g;
 #pragma unroll
     for (int i = 0; i < MS; ++i) {
@@ -466,9 +483,9 @@ __device__ __forceinline__ void lu_tile(const DevTopo& t, const DevWork& w, int 
   int2 s0[NB], s1[NB], s2[NB];
   T v0[NB], v1[NB];
   const int2* __restrict__ slots = t.slots + g;
-  const int n_slot_rows = kFactor ? t.chunk0[n] * NB : 0;
-  auto slots_issue = [&](int kk, int2 (&d)[NB]) {
-    const int base = m_ch0(kk) * NB;
+  const int n_slot_rows = kFactor ? sload(t.chunk0, n) * NB : 0;
+  auto slots_issue = [&](int chunk, int2 (&d)[NB]) {
+    const int base = chunk * NB;
 #pragma unroll
     for (int r = 0; r < NB; ++r) d[r] = slots[(size_t)cl(base + r, n_slot_rows - 1) * G];
   };
@@ -490,31 +507,27 @@ __device__ __forceinline__ void lu_tile(const DevTopo& t, const DevWork& w, int 
     sv[i] = zero;
     sr[i] = 0;
   }
-  stage_issue(0);
+  stage_issue(ma1);
   if (kFactor) {
-    slots_issue(0, s1);
-    slots_issue(n > 1 ? 1 : 0, s2);
+    slots_issue(ma1.w, s1);
+    slots_issue(ma2.w, s2);
     vals_issue(s1, v1);
   }
   f1s = fslot_issue(0);
   f2s = fslot_issue(1);
   fval_issue(f1s, f1l, f1y);
+  int ch_next = kFactor ? sload(t.chunk0, cl(2, n - 1)) : 0;  // first round chunk of column k+2
   int prev_c0 = 0;  // first packed position of column k-1
   for (int k = 0; k < n; ++k) {
-    const int kg = k % G;
-    if (k > 0 && kg == 0) {
-      kb = k;
-      fcur = fnext;
-      fnext = f_of(k + G + g);
-      ca = na;
-      cf = nf;
-      na = t.col_a[cl(k + G + g, n - 1)];
-      nf = t.col_f[cl(k + G + g, n - 1)];
-    }
-    const int lk = kg * TW;
-    const int c0 = rl(ca.x, lk), len = rl(ca.y, lk), ul = rl(ca.z, lk), ch0 = rl(ca.w, lk);
-    const int nr = rl(cf.x, lk), oq0 = rl(cf.y, lk), oq1 = rl(cf.z, lk);
-    // take this column's data, issue the next columns' loads
+    // this column's data; the next columns' loads
+    const int4 mk = ma1, fk = mf1;
+    ma1 = ma2;
+    mf1 = mf2;
+    ma2 = sload(t.col_a, cl(k + 2, n - 1));
+    mf2 = sload(t.col_f, cl(k + 2, n - 1));
+    const T cfr = fr1;
+    fr1 = fr2;
+    fr2 = y[(size_t)cl(k + 2 + W + 1, n - 1) * TW];
 #pragma unroll
     for (int i = 0; i < MS; ++i) {
       cv[i] = sv[i];
@@ -532,17 +545,19 @@ __device__ __forceinline__ void lu_tile(const DevTopo& t, const DevWork& w, int 
     f0l = f1l;
     f0y = f1y;
     f1s = f2s;
-    {
-      const int k1 = k + 1 < n ? k + 1 : k, k2 = k + 2 < n ? k + 2 : k1;
-      stage_issue(k1);
-      if (kFactor) {
-        vals_issue(s1, v1);
-        slots_issue(k2, s2);
-      }
-      fval_issue(f1s, f1l, f1y);
-      f2s = fslot_issue(k2);
+    stage_issue(ma1);
+    if (kFactor) {
+      vals_issue(s1, v1);
+      slots_issue(ch_next, s2);
+      ch_next = sload(t.chunk0, cl(k + 3, n - 1));
     }
-    const int f0 = kFactor ? first_of_group<G>(c0, g) : ul + 1 + g;
+    fval_issue(f1s, f1l, f1y);
+    f2s = fslot_issue(k + 2);
+    const int c0 = mk.x, len = mk.y, ul = mk.z, ch0 = mk.w;
+    const int nr = fk.x, oq0 = fk.y, oq1 = fk.z;
+    // staged entries o = fb + fo + G*i (fb uniform, 0 <= fo < G); the rest from fb + G*MS on
+    const int fb = kFactor ? 0 : ul + 1, fo = kFactor ? first_of_group<G>(c0, g) : g;
+    const int f0 = fb + fo;
     T* xc = xcb + (k & 1) * xstride;
     const T* xprev = xcb + ((k & 1) ^ 1) * xstride;
     if (kFactor) {
@@ -555,7 +570,10 @@ __device__ __forceinline__ void lu_tile(const DevTopo& t, const DevWork& w, int 
         const int o = f0 + G * i;
         if (o < len) xc[o * S + s] = cv[i];
       }
-      for (int o = f0 + G * MS; o < len; o += G) xc[o * S + s] = lu[(size_t)(c0 + o) * TW];
+      if (fb + G * MS < len) {  // long column: the rest is loaded here
+        strided(fb + G * MS, fo, len, [&](int o) { xc[o * S + s] = lu[(size_t)(c0 + o) * TW]; });
+        drain();
+      }
       // first chunk of rounds from registers, later chunks loaded here
       const int nch = nr > NB ? (nr + NB - 1) / NB : 1;
       for (int c = 0; c < nch; ++c) {
@@ -571,11 +589,15 @@ __device__ __forceinline__ void lu_tile(const DevTopo& t, const DevWork& w, int 
           if (r < rleft) {
             const int wd = s0[r].y;
             const int tg = wd & 0xffff, su = (wd >> 16) & 0x7fff;
-            const T lv = wd < 0 ? xprev[s0[r].x * S + s] : (s0[r].x >= 0 ? v0[r] : zero);
+            // (conditional statements, not conditional lvalues: a select between addresses would
+            // move the register arrays into private memory)
+            T lv = s0[r].x >= 0 ? T(v0[r]) : zero_value<T>();
+            if (wd < 0) lv = xprev[s0[r].x * S + s];
             const T xt = xc[tg * S + s], xu = xc[su * S + s];
             xc[tg * S + s] = xt - lv * xu;
           }
         }
+        if (c > 0 && c + 1 == nch) drain();
       }
       const T ukk = xc[ul * S + s];
       if (is_zero(ukk)) ok = false;
@@ -588,22 +610,31 @@ __device__ __forceinline__ void lu_tile(const DevTopo& t, const DevWork& w, int 
             v = v / ukk;
             xc[o * S + s] = v;
           }
-          lu[(size_t)(c0 + o) * TW] = v;
           cv[i] = v;
         }
       }
-      for (int o = f0 + G * MS; o < len; o += G) {
-        T v = xc[o * S + s];
-        if (o > ul) {
-          v = v / ukk;
-          xc[o * S + s] = v;
-        }
-        lu[(size_t)(c0 + o) * TW] = v;
+      // write back (unused slots rewrite the diagonal: the same value its owner stores)
+#pragma unroll
+      for (int i = 0; i < MS; ++i) {
+        const int o = f0 + G * i;
+        const int oo = o < len ? o : ul;
+        lu[(size_t)(c0 + oo) * TW] = o < len ? cv[i] : ukk;
+      }
+      if (fb + G * MS < len) {
+        strided(fb + G * MS, fo, len, [&](int o) {
+          T v = xc[o * S + s];
+          if (o > ul) {
+            v = v / ukk;
+            xc[o * S + s] = v;
+          }
+          lu[(size_t)(c0 + o) * TW] = v;
+        });
+        drain();
       }
     }
-    // forward substitution: y_k is final
+    // forward substitution: y_k is final (every lane group stores the same value)
     const T yk = ring[slot(k)];
-    if (g == 0) y[(size_t)k * TW] = yk;
+    y[(size_t)k * TW] = yk;
     // the row entering the window after column k: f minus its far L entries
     const int rin = k + W + 1;
     if (rin < n) {
@@ -613,19 +644,26 @@ __device__ __forceinline__ void lu_tile(const DevTopo& t, const DevWork& w, int 
         T lv = f0l;
         if (kFactor && j == k) lv = xc[(f0s.x - c0) * S + s];
         if (kFactor && j == k - 1) lv = xprev[(f0s.x - prev_c0) * S + s];
-        const T yj = j >= k - B ? ring[slot(j)] : f0y;
+        T yj = f0y;
+        if (j >= k - B) yj = ring[slot(j)];
         acc = lv * yj;
       }
-      for (int q = oq0; q < oq1; ++q) {  // entries beyond one per lane group
-        const int p = t.fo_pos[q];
-        if (p % G != g) continue;  // the lane group that wrote L at p reads it
-        const int j = t.fo_col[q];
-        const T yj = j > k - B ? ring[slot(j)] : y[(size_t)j * TW];
-        const T lv = kFactor && j == k ? xc[(p - c0) * S + s] : lu[(size_t)p * TW];
-        acc = acc + lv * yj;
+      if (oq1 > oq0) {  // entries beyond one per lane group
+        for (int q = oq0; q < oq1; ++q) {
+          const int p = sload(t.fo_pos, q);
+          if (p % G != g) continue;  // the lane group that wrote L at p reads it
+          const int j = sload(t.fo_col, q);
+          T yj, lv;
+          if (j > k - B) yj = ring[slot(j)];
+          else yj = y[(size_t)j * TW];
+          if (kFactor && j == k) lv = xc[(p - c0) * S + s];
+          else lv = lu[(size_t)p * TW];
+          acc = acc + lv * yj;
+        }
+        drain();
       }
       acc = group_sum<TW>(acc);
-      if (g == kg) ring[slot(rin)] = fcur - acc;
+      if (g == k % G) ring[slot(rin)] = cfr - acc;
     }
     // push L(:,k) into the window rows (rows farther than W are gathered when they enter)
 #pragma unroll
@@ -633,12 +671,15 @@ __device__ __forceinline__ void lu_tile(const DevTopo& t, const DevWork& w, int 
       const int o = f0 + G * i;
       if (o > ul && o < len && cr[i] - k <= W) ring[slot(cr[i])] = ring[slot(cr[i])] - cv[i] * yk;
     }
-    for (int o = f0 + G * MS; o < len; o += G) {
-      if (o <= ul) continue;
-      const int i = t.lu_row[c0 + o];
-      if (i - k > W) continue;
-      const T lv = kFactor ? xc[o * S + s] : lu[(size_t)(c0 + o) * TW];
-      ring[slot(i)] = ring[slot(i)] - lv * yk;
+    if (fb + G * MS < len) {
+      strided(fb + G * MS, fo, len, [&](int o) {
+        if (o <= ul) return;
+        const int i = t.lu_row[c0 + o];
+        if (i - k > W) return;
+        const T lv = kFactor ? xc[o * S + s] : lu[(size_t)(c0 + o) * TW];
+        ring[slot(i)] = ring[slot(i)] - lv * yk;
+      });
+      drain();
     }
     prev_c0 = c0;
     if ((k + 1) % B == 0) workgroup_fence();
@@ -648,20 +689,18 @@ __device__ __forceinline__ void lu_tile(const DevTopo& t, const DevWork& w, int 
   // ------------------------------------------------ back substitution
   if (!overflow) {
     // ring rows [max(0, n-1-W), n-1] <- y
-    for (int r = n - 1 - g; r >= 0 && r >= n - 1 - W; r -= G) ring[slot(r)] = y[(size_t)r * TW];
-    // column k = n-1-m: metadata in batches of G (lane group g: m = batch start + g), the y of
-    // the row entering after it likewise, its U entries and pivot loaded one column ahead
-    auto y_of = [&](int m) -> T { return y[(size_t)cl(n - 1 - m - W - 1, n - 1) * TW]; };
-    T ycur = y_of(g), ynext = y_of(G + g);
-    int4 ba = t.col_a[n - 1 - cl(g, n - 1)], bb = t.col_b[n - 1 - cl(g, n - 1)];
-    int4 nba = t.col_a[n - 1 - cl(G + g, n - 1)], nbb = t.col_b[n - 1 - cl(G + g, n - 1)];
-    int mb = 0;
-    T uv[MS], cu[MS], pv, cpv;
-    int ur[MS], cur_[MS];
-    auto u_issue = [&](int mm) {
-      const bool second = mm - mb >= G;
-      const int lane = ((mm - mb) % G) * TW;
-      const int c0 = rl(second ? nba.x : ba.x, lane), ul = rl(second ? nba.z : ba.z, lane);
+    strided(n - 1 - W > 0 ? n - 1 - W : 0, g, n, [&](int r) { ring[slot(r)] = y[(size_t)r * TW]; });
+    // column metadata three columns ahead (scalar cache); the U entries, the pivot and the
+    // entering row's y two columns ahead, in two register sets that alternate between columns
+    // (the loop is unrolled by two, so a set is refilled only after it has been consumed and no
+    // in-flight load is copied between registers)
+    int4 ba1 = sload(t.col_a, n - 1), ba2 = sload(t.col_a, cl(n - 2, n - 1));
+    int4 ba3 = sload(t.col_a, cl(n - 3, n - 1));
+    int4 bb1 = sload(t.col_b, n - 1), bb2 = sload(t.col_b, cl(n - 2, n - 1));
+    T uA[MS], uB[MS], pA, pB, yA, yB;
+    int rA[MS], rB[MS];
+    auto u_issue = [&](int kk, const int4& m, T (&uv)[MS], int (&ur)[MS], T& pv, T& yr) {
+      const int c0 = m.x, ul = m.z;
 #pragma unroll
       for (int i = 0; i < MS; ++i) {
         const int u = g + G * i;
@@ -670,56 +709,60 @@ __device__ __forceinline__ void lu_tile(const DevTopo& t, const DevWork& w, int 
         ur[i] = t.lu_row[p];
       }
       pv = lu[(size_t)(c0 + ul) * TW];
+      yr = y[(size_t)cl(kk - W - 1, n - 1) * TW];
     };
-    u_issue(0);
-    for (int m = 0; m < n; ++m) {
-      const int k = n - 1 - m, mg = m % G;
-      if (m > 0 && mg == 0) {
-        mb = m;
-        ycur = ynext;
-        ynext = y_of(m + G + g);
-        ba = nba;
-        bb = nbb;
-        nba = t.col_a[n - 1 - cl(m + G + g, n - 1)];
-        nbb = t.col_b[n - 1 - cl(m + G + g, n - 1)];
-      }
-      const int lk = mg * TW;
-      const int c0 = rl(ba.x, lk), ul = rl(ba.z, lk), bq0 = rl(bb.x, lk), bq1 = rl(bb.y, lk);
-#pragma unroll
-      for (int i = 0; i < MS; ++i) {
-        cu[i] = uv[i];
-        cur_[i] = ur[i];
-      }
-      cpv = pv;
-      u_issue(m + 1 < n ? m + 1 : m);
-      const T xk = ring[slot(k)] / cpv;
+    auto bstep = [&](int k, T (&uv)[MS], int (&ur)[MS], T& pv, T& yr) __attribute__((always_inline)) {
+      const int m = n - 1 - k;
+      const int4 mk = ba1, bk = bb1;
+      ba1 = ba2;
+      ba2 = ba3;
+      ba3 = sload(t.col_a, cl(k - 3, n - 1));
+      bb1 = bb2;
+      bb2 = sload(t.col_b, cl(k - 2, n - 1));
+      const int c0 = mk.x, ul = mk.z, bq0 = bk.x, bq1 = bk.y;
+      const T xk = ring[slot(k)] / pv;
       if (!finite_value(xk)) ok = false;
-      if (g == 0) {
-        ring[slot(k)] = xk;
-        y[(size_t)k * TW] = xk;
-      }
+      if (g == 0) ring[slot(k)] = xk;
+      y[(size_t)k * TW] = xk;
       const int r = k - W - 1;
       if (r >= 0) {
         T acc = zero;
-        for (int q = bq0 + g; q < bq1; q += G) {
-          const int j = t.bwd_col[q];
-          const T xj = j < k + B ? ring[slot(j)] : y[(size_t)j * TW];
-          acc = acc + lu[(size_t)t.bwd_pos[q] * TW] * xj;
+        if (bq1 > bq0) {
+          strided(bq0, g, bq1, [&](int q) {
+            const int j = t.bwd_col[q];
+            T xj;
+            if (j < k + B) xj = ring[slot(j)];
+            else xj = y[(size_t)j * TW];
+            acc = acc + lu[(size_t)t.bwd_pos[q] * TW] * xj;
+          });
+          drain();
+          acc = group_sum<TW>(acc);
         }
-        if (bq1 > bq0) acc = group_sum<TW>(acc);
-        if (g == mg) ring[slot(r)] = ycur - acc;
+        if (g == m % G) ring[slot(r)] = yr - acc;
       }
 #pragma unroll
       for (int i = 0; i < MS; ++i) {
         const int u = g + G * i;
-        if (u < ul && k - cur_[i] <= W) ring[slot(cur_[i])] = ring[slot(cur_[i])] - cu[i] * xk;
+        if (u < ul && k - ur[i] <= W) ring[slot(ur[i])] = ring[slot(ur[i])] - uv[i] * xk;
       }
-      for (int u = g + G * MS; u < ul; u += G) {
-        const int j = t.lu_row[c0 + u];
-        if (k - j > W) continue;  // gathered when row j enters the window
-        ring[slot(j)] = ring[slot(j)] - lu[(size_t)(c0 + u) * TW] * xk;
+      if (G * MS < ul) {
+        strided(G * MS, g, ul, [&](int u) {
+          const int j = t.lu_row[c0 + u];
+          if (k - j > W) return;  // gathered when row j enters the window
+          ring[slot(j)] = ring[slot(j)] - lu[(size_t)(c0 + u) * TW] * xk;
+        });
+        drain();
       }
+      // refill the set (after its last use, so the loads land in the same registers)
+      u_issue(cl(k - 2, n - 1), ba2, uv, ur, pv, yr);
       if ((m + 1) % B == 0) workgroup_fence();
+    };
+    u_issue(n - 1, ba1, uA, rA, pA, yA);
+    u_issue(cl(n - 2, n - 1), ba2, uB, rB, pB, yB);
+    for (int k = n - 1; k >= 0; k -= 2) {
+      bstep(k, uA, rA, pA, yA);
+      if (k == 0) break;
+      bstep(k - 1, uB, rB, pB, yB);
     }
   }
   // a zero pivot seen by any group of the scenario, or a non-finite solution, makes it Singular
@@ -919,7 +962,9 @@ __device__ __forceinline__ void lu64_tile(const DevTopo& t, const DevWork& w, in
       }
       for (int q = qk.y; q < qk.z; ++q) {
         const int p = t.fwd_pos[q], j = t.fwd_col[q];
-        const T lj = (kFactor && j == k) ? xc[xs(p - c0)] : lu[(size_t)p * TW];
+        T lj;
+        if (kFactor && j == k) lj = xc[xs(p - c0)];
+        else lj = lu[(size_t)p * TW];
         const T yj = j == k ? yk : y[(size_t)j * TW];
         v = v - lj * yj;
       }
@@ -1063,7 +1108,9 @@ __global__ __launch_bounds__(64) void k_lu_queue(const DevTopo* __restrict__ top
   for (;;) {
     int it = 0;
     if (threadIdx.x == 0) it = atomicAdd(counter, 1);
-    it = __shfl(it, 0);
+    // lane 0 is the first active lane: the item index becomes a scalar, and with it every
+    // topology field (scalar loads), as in the one-batch kernel
+    it = __builtin_amdgcn_readfirstlane(it);
     if (it >= n_items) break;
     const int2 item = items[it];
     const DevTopo& t = topos[item.x];
