@@ -84,17 +84,20 @@ struct DevTopo {
   // (x_i -= L(i,j) * U(j,k)) in rounds[k] rounds of G slots, one per lane group, stored in chunks of
   // kChunkRounds rounds; every column owns at least one chunk.  Slot: {x, word}, word = target
   // offset (bits 0-15) | source U offset (bits 16-30) | bit 31 set when L(i,j) is read from the
-  // previous column's LDS copy at offset x (else x is its packed position; -1 for an empty slot)
+  // previous column's LDS copy at offset x (else x is its packed position); an empty slot is
+  // {0, word aimed at the unused row just past the column}
   const int* chunk0 = nullptr;  // n + 1: first chunk of column k
   const int* rounds = nullptr;  // n
-  const int2* slots = nullptr;  // chunk0[n] * kChunkRounds * G
+  const int2* slots = nullptr;  // (chunk0[n] + 2) * kChunkRounds * G (two empty chunks at the end)
   int groups = 0;               // G the rounds were built for
   // per column k, for the window and G above:
   //   col_a = {cp[k], column length, ulen[k], chunk0[k]}
   //   col_f = {rounds[k], range [q0, q1) of fo_pos/fo_col: the far L entries of the row
-  //            r = k + W + 1 entering the forward window after column k that fslot does not hold, 0}
-  //   col_b = {far U range [bwd_ptr[r], bwd_ptr[r+1]) of the row r = k - W - 1 entering the
-  //            backward window after column k (empty if r < 0), 0, 0}
+  //            r = k + W + 1 entering the forward window after column k that fslot does not hold,
+  //            1 if that row has far L entries at all}
+  //   col_b = {range [q0, q1) of bwd_pos/bwd_col: the far U entries of the row r = k - W - 1
+  //            entering the backward window after column k that bslot does not hold, 1 if that
+  //            row has far U entries at all, 0}
   const int4* col_a = nullptr;
   const int4* col_f = nullptr;
   const int4* col_b = nullptr;
@@ -110,8 +113,9 @@ struct DevTopo {
   // of column k reads no L(:,k-1)
   const int* uround = nullptr;
   const int2* uitems = nullptr;
-  // wide tile: first kWideFar far entries {position, column} of the rows entering the backward
-  // window after column k (forward: fslot); col_f / col_b then hold the ranges of the others
+  // first far entries {position, column} of the row entering the backward window after column k
+  // ({-1, -1} if none): narrow tiles G per column (entry i for lane group i), the wide tile
+  // kWideFar (and as many forward ones in fslot); col_f / col_b hold the ranges of the others
   const int2* bslot = nullptr;
   // triangular solves with an LDS window of `window` rows: the entries of L (by row) and
   // of U (by row) that reach farther are gathered when their row enters the window

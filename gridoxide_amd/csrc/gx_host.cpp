@@ -190,12 +190,14 @@ int build_rounds(const gx::LuPattern& lu, const LuSchedule& s, int G, RoundPlan&
     out.chunk0[k + 1] = out.chunk0[k] + nch;
     for (int r = 0; r < nch * NB; ++r)
       for (int gg = 0; gg < G; ++gg) {
-        const bool real = r < R;
-        out.slots.push_back(real ? sx[(size_t)r * G + gg] : -1);
+        const bool real = r < R && sx[(size_t)r * G + gg] >= 0;
+        out.slots.push_back(real ? sx[(size_t)r * G + gg] : 0);
         out.slots.push_back(real ? sw[(size_t)r * G + gg] : noop_w);
       }
     if (out.slots.size() >= (size_t)0x7fffffff) return fail(GX_ERR_UNSUPPORTED, "refactor round schedule too large");
   }
+  // two chunks past the last one, read by the kernel's prefetch and never executed
+  out.slots.insert(out.slots.end(), 2 * (size_t)2 * NB * G, 0);
   return GX_OK;
 }
 
@@ -414,7 +416,13 @@ int upload_sched(const gx::LuPattern& lu, const LuSchedule& s, int window, int t
       fcol.swap(f.fcol);
     }
     std::vector<int> a(4 * (size_t)n), fw(4 * (size_t)n, 0), bw(4 * (size_t)n, 0);
-    std::vector<int> fslot(2 * (size_t)n * G, -1), opos, ocol;
+    std::vector<int> fslot(2 * (size_t)n * G, -1), bslot(2 * (size_t)n * G, -1), opos, ocol, bpos, bcol;
+    {
+      FarLists f;
+      build_far(lu, window, f);
+      bpos.swap(f.bpos);
+      bcol.swap(f.bcol);
+    }
     for (int k = 0; k < n; ++k) {
       a[4 * k] = lu.cp[k];
       a[4 * k + 1] = lu.cp[k + 1] - lu.cp[k];
@@ -435,14 +443,22 @@ int upload_sched(const gx::LuPattern& lu, const LuSchedule& s, int window, int t
           }
         }
         fw[4 * k + 2] = (int)opos.size();
+        fw[4 * k + 3] = m.h_fptr[rf + 1] > m.h_fptr[rf];
       }
       if (rb >= 0) {
-        bw[4 * k] = m.h_bptr[rb];
-        bw[4 * k + 1] = m.h_bptr[rb + 1];
+        const int q0 = m.h_bptr[rb], q1 = m.h_bptr[rb + 1];
+        for (int i = 0; i < G && q0 + i < q1; ++i) {
+          bslot[2 * ((size_t)k * G + i)] = bpos[q0 + i];
+          bslot[2 * ((size_t)k * G + i) + 1] = bcol[q0 + i];
+        }
+        bw[4 * k] = std::min(q0 + G, q1);
+        bw[4 * k + 1] = q1;
+        bw[4 * k + 2] = q1 > q0;
       }
     }
     if ((rc = m.col_a.upload(a)) || (rc = m.col_f.upload(fw)) || (rc = m.col_b.upload(bw)) ||
-        (rc = m.fslot.upload(fslot)) || (rc = m.opos.upload(opos)) || (rc = m.ocol.upload(ocol)))
+        (rc = m.fslot.upload(fslot)) || (rc = m.bslot.upload(bslot)) || (rc = m.opos.upload(opos)) ||
+        (rc = m.ocol.upload(ocol)))
       return rc;
   }
   d.n = lu.n;

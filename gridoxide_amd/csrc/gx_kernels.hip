@@ -412,14 +412,15 @@ __device__ __forceinline__ void drain() { __builtin_amdgcn_s_waitcnt(0); }
 // pushes into the ring rows i <= k + W; row k + W + 1 then enters the ring (taking the slot of
 // row k - B) as f minus the contributions of its L entries farther than W, gathered at that
 // moment.  Back substitution mirrors this with U: the ring holds rows [k - W, k + B].
-// Memory: column metadata comes through the scalar cache two columns ahead; every vector load a
-// column needs (staged values and rows, the first kChunkRounds update rounds' L values, the far
-// entries and the entering row's f) is issued at the top of an earlier column, the slots those
-// loads index two columns ahead.  On the common path every column issues the same loads and
-// stores, unconditionally at clamped addresses (stores of unused slots rewrite the diagonal with
-// its own value), so the compiler's waits are exact: a column waits only for loads a column old.
-// Rare variable paths end with a full wait.  L values of column k-1 (not yet written back when
-// column k's loads are issued) are read from LDS instead.
+// Memory: column metadata comes through the scalar cache two columns ahead.  The update rounds
+// form one stream of chunks over all columns: each chunk step issues the slots of the chunk two
+// ahead and the L values of the next chunk (whose slots arrived during the previous step), so no
+// chunk waits for its own loads, in long columns as across columns.  A column's staged values and
+// rows, its far entries and its entering row's f are issued one column ahead.  Addresses are a
+// uniform base plus a 32-bit lane offset.  Every load on the common path is unconditional (clamped
+// or padded addresses), so the compiler's waits are exact; the rare long-column tails load in
+// fixed batches.  L values of column k-1 (not yet written back when the loads of column k's first
+// chunk are issued) are read from LDS instead.
 template <typename T, int TW, bool kFactor>
 __device__ __forceinline__ void lu_tile(const DevTopo& t, const DevWork& w, int tile, int cap, int iter) {
   constexpr int G = kLanes / TW;
@@ -427,6 +428,7 @@ __device__ __forceinline__ void lu_tile(const DevTopo& t, const DevWork& w, int 
   constexpr int B = kRingGap;
   constexpr int NB = kChunkRounds;
   constexpr int MS = stage_regs<G>();
+  constexpr int MT = 4;  // long-column tail entries per lane group and batch
   const int W = t.window;
   const int R = W + B + 1;
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -436,10 +438,14 @@ __device__ __forceinline__ void lu_tile(const DevTopo& t, const DevWork& w, int 
   const int g = grp<TW>(), s = sub<TW>();
   if (!tile_running<TW>(w.status, tile)) return;
   const int sc = tile * TW + s;
-  T* __restrict__ lu = reinterpret_cast<T*>(w.lu) + at<TW>(tile, t.nnz_lu, 0) + s;
-  T* __restrict__ y = reinterpret_cast<T*>(w.y) + at<TW>(tile, t.n, 0) + s;
   const int n = t.n;
   const int nl = t.nnz_lu;
+  // this tile's arrays (uniform bases); entry e of lane s at e * TW + s
+  T* __restrict__ lub = reinterpret_cast<T*>(w.lu) + (size_t)tile * nl * TW;
+  T* __restrict__ yb = reinterpret_cast<T*>(w.y) + (size_t)tile * n * TW;
+  auto LU = [&](int p) -> T& { return lub[(unsigned)(p * TW + s)]; };
+  auto Y = [&](int r) -> T& { return yb[(unsigned)(r * TW + s)]; };
+  const int* __restrict__ rowp = t.lu_row;
   bool ok = true;        // per lane: no zero pivot, finite solution
   bool overflow = false; // a column longer than the staging area (the same for every lane)
   auto slot = [&](int r) { return (r & (R - 1)) * S + s; };
@@ -457,12 +463,12 @@ __device__ __forceinline__ void lu_tile(const DevTopo& t, const DevWork& w, int 
   };
 
   // ------------------------------------------------ forward pass (+ refactor)
-  strided(0, g, W + 1 < n ? W + 1 : n, [&](int r) { ring[slot(r)] = y[(size_t)r * TW]; });
-  // column metadata a = {c0, len, ul, chunk0}, f = {rounds, overflow range}: m*1 = next column,
-  // m*2 = the one after; fr1/fr2 = f of the rows entering after those columns
+  strided(0, g, W + 1 < n ? W + 1 : n, [&](int r) { ring[slot(r)] = Y(r); });
+  // column metadata a = {c0, len, ul, chunk0}, f = {rounds, overflow range, far flag}: m*1 = next
+  // column, m*2 = the one after; fr1/fr2 = f of the rows entering after those columns
   int4 ma1 = sload(t.col_a, 0), ma2 = sload(t.col_a, cl(1, n - 1));
   int4 mf1 = sload(t.col_f, 0), mf2 = sload(t.col_f, cl(1, n - 1));
-  T fr1 = y[(size_t)cl(W + 1, n - 1) * TW], fr2 = y[(size_t)cl(W + 2, n - 1) * TW];
+  T fr1 = Y(cl(W + 1, n - 1)), fr2 = Y(cl(W + 2, n - 1));
 
   // staged entries: factor -> entries o = first_of_group(c0, g) + G*i of the whole column (owned
   // positions); solve only -> L entries o = ul + 1 + g + G*i
@@ -474,92 +480,62 @@ __device__ __forceinline__ void lu_tile(const DevTopo& t, const DevWork& w, int 
 #pragma unroll
     for (int i = 0; i < MS; ++i) {
       const int o = f0 + G * i;
-      const int p = o < len ? c0 + o : c0;
-      sv[i] = lu[(size_t)p * TW];
-      sr[i] = t.lu_row[p];
+      const int p = c0 + (o < len ? o : 0);
+      sv[i] = LU(p);
+      sr[i] = rowp[p];
     }
   };
-  // update rounds: s0/v0 = column k (slots, L values), s1/v1 = column k+1, s2 = slots of k+2
-  int2 s0[NB], s1[NB], s2[NB];
-  T v0[NB], v1[NB];
-  const int2* __restrict__ slots = t.slots + g;
-  const int n_slot_rows = kFactor ? sload(t.chunk0, n) * NB : 0;
-  auto slots_issue = [&](int chunk, int2 (&d)[NB]) {
-    const int base = chunk * NB;
+  // update-round chunk stream: sA/vA = the chunk being run (slots, L values), sB = the next one
+  // (its values vB are issued while sA runs), sC = the one after
+  int2 sA[NB], sB[NB], sC[NB];
+  T vA[NB], vB[NB];
+  const int2* __restrict__ slb = t.slots + g;
+  auto slots_issue = [&](int q, int2 (&d)[NB]) {
 #pragma unroll
-    for (int r = 0; r < NB; ++r) d[r] = slots[(size_t)cl(base + r, n_slot_rows - 1) * G];
+    for (int r = 0; r < NB; ++r) d[r] = slb[(unsigned)((q * NB + r) * G)];
   };
   auto vals_issue = [&](const int2 (&sm)[NB], T (&d)[NB]) {
 #pragma unroll
-    for (int r = 0; r < NB; ++r) d[r] = lu[(size_t)cl(sm[r].x, nl - 1) * TW];
+    for (int r = 0; r < NB; ++r) d[r] = LU(sm[r].x);
   };
   // far entries of the row entering after a column: f0s/f0l/f0y = column k (slot, L value, y value),
   // f1s/f1l/f1y = column k+1, f2s = slot of column k+2
   int2 f0s, f1s, f2s;
   T f0l, f0y, f1l, f1y;
-  auto fslot_issue = [&](int kk) { return t.fslot[(size_t)cl(kk, n - 1) * G + g]; };
+  auto fslot_issue = [&](int kk) { return t.fslot[(unsigned)(cl(kk, n - 1) * G + g)]; };
   auto fval_issue = [&](int2 fs, T& lv, T& yv) {
-    lv = lu[(size_t)cl(fs.x, nl - 1) * TW];
-    yv = y[(size_t)cl(fs.y, n - 1) * TW];
+    lv = LU(fs.x < 0 ? 0 : fs.x);
+    yv = Y(fs.y < 0 ? 0 : fs.y);
   };
 #pragma unroll
-  for (int i = 0; i < MS; ++i) {
-    sv[i] = zero;
-    sr[i] = 0;
-  }
+  for (int i = 0; i < MS; ++i) cv[i] = zero;
   stage_issue(ma1);
   if (kFactor) {
-    slots_issue(ma1.w, s1);
-    slots_issue(ma2.w, s2);
-    vals_issue(s1, v1);
+    slots_issue(0, sA);
+    slots_issue(1, sB);
+    vals_issue(sA, vA);
   }
   f1s = fslot_issue(0);
   f2s = fslot_issue(1);
   fval_issue(f1s, f1l, f1y);
-  int ch_next = kFactor ? sload(t.chunk0, cl(2, n - 1)) : 0;  // first round chunk of column k+2
   int prev_c0 = 0;  // first packed position of column k-1
   for (int k = 0; k < n; ++k) {
-    // this column's data; the next columns' loads
     const int4 mk = ma1, fk = mf1;
     ma1 = ma2;
     mf1 = mf2;
     ma2 = sload(t.col_a, cl(k + 2, n - 1));
     mf2 = sload(t.col_f, cl(k + 2, n - 1));
-    const T cfr = fr1;
-    fr1 = fr2;
-    fr2 = y[(size_t)cl(k + 2 + W + 1, n - 1) * TW];
-#pragma unroll
-    for (int i = 0; i < MS; ++i) {
-      cv[i] = sv[i];
-      cr[i] = sr[i];
-    }
-    if (kFactor) {
-#pragma unroll
-      for (int r = 0; r < NB; ++r) {
-        s0[r] = s1[r];
-        v0[r] = v1[r];
-        s1[r] = s2[r];
-      }
-    }
-    f0s = f1s;
-    f0l = f1l;
-    f0y = f1y;
-    f1s = f2s;
-    stage_issue(ma1);
-    if (kFactor) {
-      vals_issue(s1, v1);
-      slots_issue(ch_next, s2);
-      ch_next = sload(t.chunk0, cl(k + 3, n - 1));
-    }
-    fval_issue(f1s, f1l, f1y);
-    f2s = fslot_issue(k + 2);
     const int c0 = mk.x, len = mk.y, ul = mk.z, ch0 = mk.w;
-    const int nr = fk.x, oq0 = fk.y, oq1 = fk.z;
+    const int nr = fk.x, oq0 = fk.y, oq1 = fk.z, far = fk.w;
     // staged entries o = fb + fo + G*i (fb uniform, 0 <= fo < G); the rest from fb + G*MS on
     const int fb = kFactor ? 0 : ul + 1, fo = kFactor ? first_of_group<G>(c0, g) : g;
     const int f0 = fb + fo;
+    const int tail = fb + G * MS;  // first uniform base past the staged entries
     T* xc = xcb + (k & 1) * xstride;
     const T* xprev = xcb + ((k & 1) ^ 1) * xstride;
+    // this column's staged data (the registers are refilled for the next column below)
+#pragma unroll
+    for (int i = 0; i < MS; ++i) cr[i] = sr[i];
     if (kFactor) {
       if (len > cap) {  // the host sizes cap to the longest column; kept as a guard
         overflow = true;
@@ -568,36 +544,63 @@ __device__ __forceinline__ void lu_tile(const DevTopo& t, const DevWork& w, int 
 #pragma unroll
       for (int i = 0; i < MS; ++i) {
         const int o = f0 + G * i;
-        if (o < len) xc[o * S + s] = cv[i];
+        if (o < len) xc[o * S + s] = sv[i];
       }
-      if (fb + G * MS < len) {  // long column: the rest is loaded here
-        strided(fb + G * MS, fo, len, [&](int o) { xc[o * S + s] = lu[(size_t)(c0 + o) * TW]; });
-        drain();
+      for (int b = tail; b < len; b += G * MT) {  // long column: the rest, in batches
+        T tv[MT];
+#pragma unroll
+        for (int u = 0; u < MT; ++u) {
+          const int o = b + G * u + fo;
+          tv[u] = LU(c0 + (o < len ? o : 0));
+        }
+#pragma unroll
+        for (int u = 0; u < MT; ++u) {
+          const int o = b + G * u + fo;
+          if (o < len) xc[o * S + s] = tv[u];
+        }
       }
-      // first chunk of rounds from registers, later chunks loaded here
+    } else {
+#pragma unroll
+      for (int i = 0; i < MS; ++i) cv[i] = sv[i];
+    }
+    // loads for the next columns
+    f0s = f1s;
+    f0l = f1l;
+    f0y = f1y;
+    f1s = f2s;
+    const T cfr = fr1;
+    fr1 = fr2;
+    fr2 = Y(cl(k + W + 3, n - 1));
+    stage_issue(ma1);
+    fval_issue(f1s, f1l, f1y);
+    f2s = fslot_issue(k + 2);
+    if (kFactor) {
       const int nch = nr > NB ? (nr + NB - 1) / NB : 1;
       for (int c = 0; c < nch; ++c) {
-        if (c > 0) {
-#pragma unroll
-          for (int r = 0; r < NB; ++r) s0[r] = slots[((size_t)(ch0 + c) * NB + r) * G];
-#pragma unroll
-          for (int r = 0; r < NB; ++r) v0[r] = lu[(size_t)cl(s0[r].x, nl - 1) * TW];
-        }
+        slots_issue(ch0 + c + 2, sC);
+        vals_issue(sB, vB);
         const int rleft = nr - c * NB;
 #pragma unroll
         for (int r = 0; r < NB; ++r) {
           if (r < rleft) {
-            const int wd = s0[r].y;
+            const int wd = sA[r].y;
             const int tg = wd & 0xffff, su = (wd >> 16) & 0x7fff;
             // (conditional statements, not conditional lvalues: a select between addresses would
             // move the register arrays into private memory)
-            T lv = s0[r].x >= 0 ? T(v0[r]) : zero_value<T>();
-            if (wd < 0) lv = xprev[s0[r].x * S + s];
+            T lv = vA[r];
+            if (wd < 0) lv = xprev[sA[r].x * S + s];
             const T xt = xc[tg * S + s], xu = xc[su * S + s];
             xc[tg * S + s] = xt - lv * xu;
           }
         }
-        if (c > 0 && c + 1 == nch) drain();
+        if (c + 1 < nch) {  // next chunk of this column (the last one rotates after the column)
+#pragma unroll
+          for (int r = 0; r < NB; ++r) {
+            sA[r] = sB[r];
+            sB[r] = sC[r];
+            vA[r] = vB[r];
+          }
+        }
       }
       const T ukk = xc[ul * S + s];
       if (is_zero(ukk)) ok = false;
@@ -618,68 +621,99 @@ __device__ __forceinline__ void lu_tile(const DevTopo& t, const DevWork& w, int 
       for (int i = 0; i < MS; ++i) {
         const int o = f0 + G * i;
         const int oo = o < len ? o : ul;
-        lu[(size_t)(c0 + oo) * TW] = o < len ? cv[i] : ukk;
+        LU(c0 + oo) = o < len ? cv[i] : ukk;
       }
-      if (fb + G * MS < len) {
-        strided(fb + G * MS, fo, len, [&](int o) {
-          T v = xc[o * S + s];
-          if (o > ul) {
-            v = v / ukk;
-            xc[o * S + s] = v;
+      for (int b = tail; b < len; b += G * MT) {
+        T tv[MT];
+#pragma unroll
+        for (int u = 0; u < MT; ++u) {
+          const int o = b + G * u + fo;
+          tv[u] = xc[(o < len ? o : ul) * S + s];
+        }
+#pragma unroll
+        for (int u = 0; u < MT; ++u) {
+          const int o = b + G * u + fo;
+          if (o < len) {
+            T v = tv[u];
+            if (o > ul) {
+              v = v / ukk;
+              xc[o * S + s] = v;
+            }
+            LU(c0 + o) = v;
           }
-          lu[(size_t)(c0 + o) * TW] = v;
-        });
-        drain();
+        }
       }
     }
     // forward substitution: y_k is final (every lane group stores the same value)
     const T yk = ring[slot(k)];
-    y[(size_t)k * TW] = yk;
+    Y(k) = yk;
     // the row entering the window after column k: f minus its far L entries
     const int rin = k + W + 1;
     if (rin < n) {
-      T acc = zero;
-      if (f0s.x >= 0) {
-        const int j = f0s.y;
-        T lv = f0l;
-        if (kFactor && j == k) lv = xc[(f0s.x - c0) * S + s];
-        if (kFactor && j == k - 1) lv = xprev[(f0s.x - prev_c0) * S + s];
-        T yj = f0y;
-        if (j >= k - B) yj = ring[slot(j)];
-        acc = lv * yj;
-      }
-      if (oq1 > oq0) {  // entries beyond one per lane group
-        for (int q = oq0; q < oq1; ++q) {
+      T enter = cfr;
+      if (far) {
+        T acc = zero;
+        if (f0s.x >= 0) {
+          const int j = f0s.y;
+          T lv = f0l;
+          if (kFactor && j == k) lv = xc[(f0s.x - c0) * S + s];
+          if (kFactor && j == k - 1) lv = xprev[(f0s.x - prev_c0) * S + s];
+          T yj = f0y;
+          if (j >= k - B) yj = ring[slot(j)];
+          acc = lv * yj;
+        }
+        for (int q = oq0; q < oq1; ++q) {  // entries beyond one per lane group
           const int p = sload(t.fo_pos, q);
           if (p % G != g) continue;  // the lane group that wrote L at p reads it
           const int j = sload(t.fo_col, q);
           T yj, lv;
           if (j > k - B) yj = ring[slot(j)];
-          else yj = y[(size_t)j * TW];
+          else yj = Y(j);
           if (kFactor && j == k) lv = xc[(p - c0) * S + s];
-          else lv = lu[(size_t)p * TW];
+          else lv = LU(p);
           acc = acc + lv * yj;
         }
-        drain();
+        enter = cfr - group_sum<TW>(acc);
       }
-      acc = group_sum<TW>(acc);
-      if (g == k % G) ring[slot(rin)] = cfr - acc;
+      if (g == k % G) ring[slot(rin)] = enter;
     }
-    // push L(:,k) into the window rows (rows farther than W are gathered when they enter)
+    // push L(:,k) into the window rows (rows farther than W are gathered when they enter); the
+    // rows of one column are distinct, so all reads go first
+    {
+      T rv[MS];
 #pragma unroll
-    for (int i = 0; i < MS; ++i) {
-      const int o = f0 + G * i;
-      if (o > ul && o < len && cr[i] - k <= W) ring[slot(cr[i])] = ring[slot(cr[i])] - cv[i] * yk;
+      for (int i = 0; i < MS; ++i) rv[i] = ring[slot(cr[i])];
+#pragma unroll
+      for (int i = 0; i < MS; ++i) {
+        const int o = f0 + G * i;
+        if (o > ul && o < len && cr[i] - k <= W) ring[slot(cr[i])] = rv[i] - cv[i] * yk;
+      }
     }
-    if (fb + G * MS < len) {
-      strided(fb + G * MS, fo, len, [&](int o) {
-        if (o <= ul) return;
-        const int i = t.lu_row[c0 + o];
-        if (i - k > W) return;
-        const T lv = kFactor ? xc[o * S + s] : lu[(size_t)(c0 + o) * TW];
-        ring[slot(i)] = ring[slot(i)] - lv * yk;
-      });
-      drain();
+    for (int b = tail; b < len; b += G * MT) {
+      int ri[MT];
+      T lv[MT], rv[MT];
+#pragma unroll
+      for (int u = 0; u < MT; ++u) {
+        const int o = b + G * u + fo;
+        const int oc = o < len ? o : ul;
+        ri[u] = rowp[c0 + oc];
+        lv[u] = kFactor ? xc[oc * S + s] : LU(c0 + oc);
+      }
+#pragma unroll
+      for (int u = 0; u < MT; ++u) rv[u] = ring[slot(ri[u])];
+#pragma unroll
+      for (int u = 0; u < MT; ++u) {
+        const int o = b + G * u + fo;
+        if (o > ul && o < len && ri[u] - k <= W) ring[slot(ri[u])] = rv[u] - lv[u] * yk;
+      }
+    }
+    if (kFactor) {  // the chunk stream moves on to the next column's first chunk
+#pragma unroll
+      for (int r = 0; r < NB; ++r) {
+        sA[r] = sB[r];
+        sB[r] = sC[r];
+        vA[r] = vB[r];
+      }
     }
     prev_c0 = c0;
     if ((k + 1) % B == 0) workgroup_fence();
@@ -688,30 +722,41 @@ __device__ __forceinline__ void lu_tile(const DevTopo& t, const DevWork& w, int 
 
   // ------------------------------------------------ back substitution
   if (!overflow) {
-    // ring rows [max(0, n-1-W), n-1] <- y
-    strided(n - 1 - W > 0 ? n - 1 - W : 0, g, n, [&](int r) { ring[slot(r)] = y[(size_t)r * TW]; });
-    // column metadata three columns ahead (scalar cache); the U entries, the pivot and the
-    // entering row's y two columns ahead, in two register sets that alternate between columns
-    // (the loop is unrolled by two, so a set is refilled only after it has been consumed and no
-    // in-flight load is copied between registers)
+    strided(n - 1 - W > 0 ? n - 1 - W : 0, g, n, [&](int r) { ring[slot(r)] = Y(r); });
+    // column metadata three columns ahead (scalar cache); the U entries, the pivot, the entering
+    // row's y and its first far entries two columns ahead, in two register sets that alternate
+    // between columns (the loop is unrolled by two, so a set is refilled only after it has been
+    // consumed and no in-flight load is copied between registers)
     int4 ba1 = sload(t.col_a, n - 1), ba2 = sload(t.col_a, cl(n - 2, n - 1));
     int4 ba3 = sload(t.col_a, cl(n - 3, n - 1));
     int4 bb1 = sload(t.col_b, n - 1), bb2 = sload(t.col_b, cl(n - 2, n - 1));
-    T uA[MS], uB[MS], pA, pB, yA, yB;
-    int rA[MS], rB[MS];
-    auto u_issue = [&](int kk, const int4& m, T (&uv)[MS], int (&ur)[MS], T& pv, T& yr) {
+    struct USet {
+      T uv[MS];
+      int ur[MS];
+      T pv, yr, fl, fx;  // pivot, entering row's y, first far U value and its x (or y) value
+      int2 fs;           // first far entry of this lane group {position, column}
+    };
+    USet A, Bs;
+    // (the far-entry slot of a set is loaded one step before the set, so that its value loads do
+    // not wait for it)
+    auto bslot_of = [&](int kk) { return t.bslot[(unsigned)(cl(kk, n - 1) * G + g)]; };
+    auto u_issue = [&](int kk, const int4& m, USet& d, int2 fs) {
       const int c0 = m.x, ul = m.z;
 #pragma unroll
       for (int i = 0; i < MS; ++i) {
         const int u = g + G * i;
-        const int p = u < ul ? c0 + u : c0;
-        uv[i] = lu[(size_t)p * TW];
-        ur[i] = t.lu_row[p];
+        const int p = c0 + (u < ul ? u : 0);
+        d.uv[i] = LU(p);
+        d.ur[i] = rowp[p];
       }
-      pv = lu[(size_t)(c0 + ul) * TW];
-      yr = y[(size_t)cl(kk - W - 1, n - 1) * TW];
+      d.pv = LU(c0 + ul);
+      d.yr = Y(cl(kk - W - 1, n - 1));
+      d.fs = fs;
+      d.fl = LU(fs.x < 0 ? 0 : fs.x);
+      d.fx = Y(fs.y < 0 ? 0 : fs.y);
     };
-    auto bstep = [&](int k, T (&uv)[MS], int (&ur)[MS], T& pv, T& yr) __attribute__((always_inline)) {
+    int2 fs_pre;  // slot of the set refilled at the next step
+    auto bstep = [&](int k, USet& d) __attribute__((always_inline)) {
       const int m = n - 1 - k;
       const int4 mk = ba1, bk = bb1;
       ba1 = ba2;
@@ -719,50 +764,74 @@ __device__ __forceinline__ void lu_tile(const DevTopo& t, const DevWork& w, int 
       ba3 = sload(t.col_a, cl(k - 3, n - 1));
       bb1 = bb2;
       bb2 = sload(t.col_b, cl(k - 2, n - 1));
-      const int c0 = mk.x, ul = mk.z, bq0 = bk.x, bq1 = bk.y;
-      const T xk = ring[slot(k)] / pv;
+      const int c0 = mk.x, ul = mk.z, bq0 = bk.x, bq1 = bk.y, bfar = bk.z;
+      const T xk = ring[slot(k)] / d.pv;
       if (!finite_value(xk)) ok = false;
       if (g == 0) ring[slot(k)] = xk;
-      y[(size_t)k * TW] = xk;
+      Y(k) = xk;
       const int r = k - W - 1;
       if (r >= 0) {
-        T acc = zero;
-        if (bq1 > bq0) {
-          strided(bq0, g, bq1, [&](int q) {
+        T enter = d.yr;
+        if (bfar) {
+          T acc = zero;
+          if (d.fs.x >= 0) {
+            const int j = d.fs.y;
+            T xj = d.fx;  // x_j for j >= k + B was stored before this set was loaded
+            if (j < k + B) xj = ring[slot(j)];
+            acc = d.fl * xj;
+          }
+          strided(bq0, g, bq1, [&](int q) {  // entries beyond one per lane group
             const int j = t.bwd_col[q];
             T xj;
             if (j < k + B) xj = ring[slot(j)];
-            else xj = y[(size_t)j * TW];
-            acc = acc + lu[(size_t)t.bwd_pos[q] * TW] * xj;
+            else xj = Y(j);
+            acc = acc + LU(t.bwd_pos[q]) * xj;
           });
-          drain();
-          acc = group_sum<TW>(acc);
+          enter = d.yr - group_sum<TW>(acc);
         }
-        if (g == m % G) ring[slot(r)] = yr - acc;
+        if (g == m % G) ring[slot(r)] = enter;
       }
+      {
+        T rv[MS];
 #pragma unroll
-      for (int i = 0; i < MS; ++i) {
-        const int u = g + G * i;
-        if (u < ul && k - ur[i] <= W) ring[slot(ur[i])] = ring[slot(ur[i])] - uv[i] * xk;
+        for (int i = 0; i < MS; ++i) rv[i] = ring[slot(d.ur[i])];
+#pragma unroll
+        for (int i = 0; i < MS; ++i) {
+          const int u = g + G * i;
+          if (u < ul && k - d.ur[i] <= W) ring[slot(d.ur[i])] = rv[i] - d.uv[i] * xk;
+        }
       }
-      if (G * MS < ul) {
-        strided(G * MS, g, ul, [&](int u) {
-          const int j = t.lu_row[c0 + u];
-          if (k - j > W) return;  // gathered when row j enters the window
-          ring[slot(j)] = ring[slot(j)] - lu[(size_t)(c0 + u) * TW] * xk;
-        });
-        drain();
+      for (int b = G * MS; b < ul; b += G * MT) {  // long column: the rest, in batches
+        int rj[MT];
+        T uvt[MT], rv[MT];
+#pragma unroll
+        for (int u = 0; u < MT; ++u) {
+          const int o = b + G * u + g;
+          const int p = c0 + (o < ul ? o : 0);
+          rj[u] = rowp[p];
+          uvt[u] = LU(p);
+        }
+#pragma unroll
+        for (int u = 0; u < MT; ++u) rv[u] = ring[slot(rj[u])];
+#pragma unroll
+        for (int u = 0; u < MT; ++u) {
+          const int o = b + G * u + g;
+          if (o < ul && k - rj[u] <= W) ring[slot(rj[u])] = rv[u] - uvt[u] * xk;
+        }
       }
       // refill the set (after its last use, so the loads land in the same registers)
-      u_issue(cl(k - 2, n - 1), ba2, uv, ur, pv, yr);
+      const int2 fs = fs_pre;
+      fs_pre = bslot_of(k - 3);
+      u_issue(cl(k - 2, n - 1), ba2, d, fs);
       if ((m + 1) % B == 0) workgroup_fence();
     };
-    u_issue(n - 1, ba1, uA, rA, pA, yA);
-    u_issue(cl(n - 2, n - 1), ba2, uB, rB, pB, yB);
+    u_issue(n - 1, ba1, A, bslot_of(n - 1));
+    u_issue(cl(n - 2, n - 1), ba2, Bs, bslot_of(n - 2));
+    fs_pre = bslot_of(n - 3);
     for (int k = n - 1; k >= 0; k -= 2) {
-      bstep(k, uA, rA, pA, yA);
+      bstep(k, A);
       if (k == 0) break;
-      bstep(k - 1, uB, rB, pB, yB);
+      bstep(k - 1, Bs);
     }
   }
   // a zero pivot seen by any group of the scenario, or a non-finite solution, makes it Singular
