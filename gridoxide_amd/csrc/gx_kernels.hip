@@ -398,6 +398,21 @@ __device__ __forceinline__ int4 sload(const int4* p, size_t i) {
   return make_int4(q[0], q[1], q[2], q[3]);
 }
 
+// Column metadata through the vector memory path (every lane reads the same 16 bytes, a few
+// columns ahead): waits on it are counted with the other vector loads.  A scalar-cache load would
+// share its wait counter with the LDS operations, so every LDS wait of the column would also wait
+// for it.  The empty asm only moves the index into a vector register, which keeps the load a
+// vector load; uni4 takes the (uniform) value back into scalar registers where it is used.
+__device__ __forceinline__ int4 vload4(const int4* p, int i) {
+  int j = i;
+  asm("" : "+v"(j));
+  return p[j];
+}
+__device__ __forceinline__ int4 uni4(int4 v) {
+  return make_int4(__builtin_amdgcn_readfirstlane(v.x), __builtin_amdgcn_readfirstlane(v.y),
+                   __builtin_amdgcn_readfirstlane(v.z), __builtin_amdgcn_readfirstlane(v.w));
+}
+
 // Waits for every outstanding memory operation: closes the rare variable-length paths (long
 // columns, overflow gathers) so that the common path's waits stay exact.
 __device__ __forceinline__ void drain() { __builtin_amdgcn_s_waitcnt(0); }
@@ -412,7 +427,7 @@ __device__ __forceinline__ void drain() { __builtin_amdgcn_s_waitcnt(0); }
 // pushes into the ring rows i <= k + W; row k + W + 1 then enters the ring (taking the slot of
 // row k - B) as f minus the contributions of its L entries farther than W, gathered at that
 // moment.  Back substitution mirrors this with U: the ring holds rows [k - W, k + B].
-// Memory: column metadata comes through the scalar cache two columns ahead.  The update rounds
+// Memory: column metadata is loaded two columns ahead.  The update rounds
 // form one stream of chunks over all columns: each chunk step issues the slots of the chunk two
 // ahead and the L values of the next chunk (whose slots arrived during the previous step), so no
 // chunk waits for its own loads, in long columns as across columns.  A column's staged values and
@@ -466,8 +481,8 @@ __device__ __forceinline__ void lu_tile(const DevTopo& t, const DevWork& w, int 
   strided(0, g, W + 1 < n ? W + 1 : n, [&](int r) { ring[slot(r)] = Y(r); });
   // column metadata a = {c0, len, ul, chunk0}, f = {rounds, overflow range, far flag}: m*1 = next
   // column, m*2 = the one after; fr1/fr2 = f of the rows entering after those columns
-  int4 ma1 = sload(t.col_a, 0), ma2 = sload(t.col_a, cl(1, n - 1));
-  int4 mf1 = sload(t.col_f, 0), mf2 = sload(t.col_f, cl(1, n - 1));
+  int4 ma1 = vload4(t.col_a, 0), ma2 = vload4(t.col_a, cl(1, n - 1));
+  int4 mf1 = vload4(t.col_f, 0), mf2 = vload4(t.col_f, cl(1, n - 1));
   T fr1 = Y(cl(W + 1, n - 1)), fr2 = Y(cl(W + 2, n - 1));
 
   // staged entries: factor -> entries o = first_of_group(c0, g) + G*i of the whole column (owned
@@ -509,7 +524,7 @@ __device__ __forceinline__ void lu_tile(const DevTopo& t, const DevWork& w, int 
   };
 #pragma unroll
   for (int i = 0; i < MS; ++i) cv[i] = zero;
-  stage_issue(ma1);
+  stage_issue(uni4(ma1));
   if (kFactor) {
     slots_issue(0, sA);
     slots_issue(1, sB);
@@ -518,13 +533,16 @@ __device__ __forceinline__ void lu_tile(const DevTopo& t, const DevWork& w, int 
   f1s = fslot_issue(0);
   f2s = fslot_issue(1);
   fval_issue(f1s, f1l, f1y);
+  // (the loop's waits are counted from what the previous column issued; without this wait the
+  // loads above, which the first column reads, would make every column's waits stricter)
+  drain();
   int prev_c0 = 0;  // first packed position of column k-1
   for (int k = 0; k < n; ++k) {
-    const int4 mk = ma1, fk = mf1;
+    const int4 mk = uni4(ma1), fk = uni4(mf1);
     ma1 = ma2;
     mf1 = mf2;
-    ma2 = sload(t.col_a, cl(k + 2, n - 1));
-    mf2 = sload(t.col_f, cl(k + 2, n - 1));
+    ma2 = vload4(t.col_a, cl(k + 2, n - 1));
+    mf2 = vload4(t.col_f, cl(k + 2, n - 1));
     const int c0 = mk.x, len = mk.y, ul = mk.z, ch0 = mk.w;
     const int nr = fk.x, oq0 = fk.y, oq1 = fk.z, far = fk.w;
     // staged entries o = fb + fo + G*i (fb uniform, 0 <= fo < G); the rest from fb + G*MS on
@@ -571,7 +589,7 @@ __device__ __forceinline__ void lu_tile(const DevTopo& t, const DevWork& w, int 
     const T cfr = fr1;
     fr1 = fr2;
     fr2 = Y(cl(k + W + 3, n - 1));
-    stage_issue(ma1);
+    stage_issue(uni4(ma1));
     fval_issue(f1s, f1l, f1y);
     f2s = fslot_issue(k + 2);
     if (kFactor) {
@@ -723,13 +741,13 @@ __device__ __forceinline__ void lu_tile(const DevTopo& t, const DevWork& w, int 
   // ------------------------------------------------ back substitution
   if (!overflow) {
     strided(n - 1 - W > 0 ? n - 1 - W : 0, g, n, [&](int r) { ring[slot(r)] = Y(r); });
-    // column metadata three columns ahead (scalar cache); the U entries, the pivot, the entering
+    // column metadata three columns ahead; the U entries, the pivot, the entering
     // row's y and its first far entries two columns ahead, in two register sets that alternate
     // between columns (the loop is unrolled by two, so a set is refilled only after it has been
     // consumed and no in-flight load is copied between registers)
-    int4 ba1 = sload(t.col_a, n - 1), ba2 = sload(t.col_a, cl(n - 2, n - 1));
-    int4 ba3 = sload(t.col_a, cl(n - 3, n - 1));
-    int4 bb1 = sload(t.col_b, n - 1), bb2 = sload(t.col_b, cl(n - 2, n - 1));
+    int4 ba1 = vload4(t.col_a, n - 1), ba2 = vload4(t.col_a, cl(n - 2, n - 1));
+    int4 ba3 = vload4(t.col_a, cl(n - 3, n - 1));
+    int4 bb1 = vload4(t.col_b, n - 1), bb2 = vload4(t.col_b, cl(n - 2, n - 1));
     struct USet {
       T uv[MS];
       int ur[MS];
@@ -758,12 +776,12 @@ __device__ __forceinline__ void lu_tile(const DevTopo& t, const DevWork& w, int 
     int2 fs_pre;  // slot of the set refilled at the next step
     auto bstep = [&](int k, USet& d) __attribute__((always_inline)) {
       const int m = n - 1 - k;
-      const int4 mk = ba1, bk = bb1;
+      const int4 mk = uni4(ba1), bk = uni4(bb1);
       ba1 = ba2;
       ba2 = ba3;
-      ba3 = sload(t.col_a, cl(k - 3, n - 1));
+      ba3 = vload4(t.col_a, cl(k - 3, n - 1));
       bb1 = bb2;
-      bb2 = sload(t.col_b, cl(k - 2, n - 1));
+      bb2 = vload4(t.col_b, cl(k - 2, n - 1));
       const int c0 = mk.x, ul = mk.z, bq0 = bk.x, bq1 = bk.y, bfar = bk.z;
       const T xk = ring[slot(k)] / d.pv;
       if (!finite_value(xk)) ok = false;
@@ -822,12 +840,13 @@ __device__ __forceinline__ void lu_tile(const DevTopo& t, const DevWork& w, int 
       // refill the set (after its last use, so the loads land in the same registers)
       const int2 fs = fs_pre;
       fs_pre = bslot_of(k - 3);
-      u_issue(cl(k - 2, n - 1), ba2, d, fs);
+      u_issue(cl(k - 2, n - 1), uni4(ba2), d, fs);
       if ((m + 1) % B == 0) workgroup_fence();
     };
-    u_issue(n - 1, ba1, A, bslot_of(n - 1));
-    u_issue(cl(n - 2, n - 1), ba2, Bs, bslot_of(n - 2));
+    u_issue(n - 1, uni4(ba1), A, bslot_of(n - 1));
+    u_issue(cl(n - 2, n - 1), uni4(ba2), Bs, bslot_of(n - 2));
     fs_pre = bslot_of(n - 3);
+    drain();  // (as before the forward loop)
     for (int k = n - 1; k >= 0; k -= 2) {
       bstep(k, A);
       if (k == 0) break;
