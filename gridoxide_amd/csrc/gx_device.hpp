@@ -101,8 +101,8 @@ struct DevTopo {
   const int4* col_a = nullptr;
   const int4* col_f = nullptr;
   const int4* col_b = nullptr;
-  // far L entries of the row entering the forward window after column k: fslot[k * G + g] = the
-  // first entry {packed position, column} whose position is g mod G ({-1, -1} if none)
+  // far L entries of the row entering the forward window after column k: fslot[k * G + i] = the
+  // i-th entry {packed position, column} for i < G ({-1, -1} if none); fo_pos/fo_col the others
   const int2* fslot = nullptr;
   const int* fo_pos = nullptr;
   const int* fo_col = nullptr;

@@ -123,9 +123,9 @@ int build_schedule(const gx::LuPattern& lu, LuSchedule& s) {
 // Column k's update items (target row offset t, source U offset u, L(i,j) at position p) are taken
 // in the order of the sequential left-looking refactor (j ascending, then i ascending); each goes to
 // the earliest round after every earlier item that targets t or u (so every entry receives its
-// updates in that same order, and U(j,k) is final before it is used), into slot p mod G -- the
-// lane group that wrote L at p is the one that reads it -- or, when L(i,j) comes from the previous
-// column (read from LDS), into any free slot; no other item of the round may target t or read t.
+// updates in that same order, and U(j,k) is final before it is used), into any free slot (the
+// lanes of one wavefront see each other's global stores in program order); no other item of the
+// round may target t or read t.
 struct RoundPlan {
   std::vector<int> chunk0, rounds;
   std::vector<int> slots;  // pairs {x, word}
@@ -167,12 +167,8 @@ int build_rounds(const gx::LuPattern& lu, const LuSchedule& s, int G, RoundPlan&
         for (int r = std::max(last[tg], last[u]) + 1;; ++r) {
           while (r >= R) add_round();
           int slot = -1;
-          if (prev) {
-            for (int gg = 0; gg < G && slot < 0; ++gg)
-              if (sx[(size_t)r * G + gg] == -1) slot = gg;
-          } else if (sx[(size_t)r * G + p % G] == -1) {
-            slot = p % G;
-          }
+          for (int gg = 0; gg < G && slot < 0; ++gg)
+            if (sx[(size_t)r * G + gg] == -1) slot = gg;
           if (slot < 0) continue;
           if (std::find(rt[r].begin(), rt[r].end(), tg) != rt[r].end()) continue;
           if (std::find(rs[r].begin(), rs[r].end(), tg) != rs[r].end()) continue;
@@ -405,8 +401,8 @@ int upload_sched(const gx::LuPattern& lu, const LuSchedule& s, int window, int t
         (rc = m.fslot.upload(fslot)) || (rc = m.bslot.upload(bslot)))
       return rc;
   } else if (regroup || rewindow) {
-    // far L entries of the row entering the forward window after column k: the first one of each
-    // lane group (position mod G) in fslot[k][g], the others in an overflow list
+    // far L entries of the row entering the forward window after column k: the first G in
+    // fslot[k][i] (entry i for lane group i), the others in an overflow list
     const int n = lu.n;
     std::vector<int> fpos, fcol;
     {
@@ -433,10 +429,10 @@ int upload_sched(const gx::LuPattern& lu, const LuSchedule& s, int window, int t
       fw[4 * k + 1] = fw[4 * k + 2] = (int)opos.size();
       if (rf < n) {
         for (int q = m.h_fptr[rf]; q < m.h_fptr[rf + 1]; ++q) {
-          int* sl = &fslot[2 * ((size_t)k * G + fpos[q] % G)];
-          if (sl[0] < 0) {
-            sl[0] = fpos[q];
-            sl[1] = fcol[q];
+          const int i = q - m.h_fptr[rf];
+          if (i < G) {
+            fslot[2 * ((size_t)k * G + i)] = fpos[q];
+            fslot[2 * ((size_t)k * G + i) + 1] = fcol[q];
           } else {
             opos.push_back(fpos[q]);
             ocol.push_back(fcol[q]);

@@ -389,15 +389,6 @@ constexpr int stage_regs() {
 
 __device__ __forceinline__ int rl(int v, int lane) { return __builtin_amdgcn_readlane(v, lane); }
 
-// Uniform reads through the scalar data cache (the topology arrays are read-only while kernels run).
-__device__ __forceinline__ int sload(const int* p, size_t i) {
-  return ((const __attribute__((address_space(4))) int*)p)[i];
-}
-__device__ __forceinline__ int4 sload(const int4* p, size_t i) {
-  const __attribute__((address_space(4))) int* q = (const __attribute__((address_space(4))) int*)(p + i);
-  return make_int4(q[0], q[1], q[2], q[3]);
-}
-
 // Column metadata through the vector memory path (every lane reads the same 16 bytes, a few
 // columns ahead): waits on it are counted with the other vector loads.  A scalar-cache load would
 // share its wait counter with the LDS operations, so every LDS wait of the column would also wait
@@ -680,17 +671,15 @@ __device__ __forceinline__ void lu_tile(const DevTopo& t, const DevWork& w, int 
           if (j >= k - B) yj = ring[slot(j)];
           acc = lv * yj;
         }
-        for (int q = oq0; q < oq1; ++q) {  // entries beyond one per lane group
-          const int p = sload(t.fo_pos, q);
-          if (p % G != g) continue;  // the lane group that wrote L at p reads it
-          const int j = sload(t.fo_col, q);
+        strided(oq0, g, oq1, [&](int q) {  // entries beyond the first G
+          const int p = t.fo_pos[q], j = t.fo_col[q];
           T yj, lv;
           if (j > k - B) yj = ring[slot(j)];
           else yj = Y(j);
           if (kFactor && j == k) lv = xc[(p - c0) * S + s];
           else lv = LU(p);
           acc = acc + lv * yj;
-        }
+        });
         enter = cfr - group_sum<TW>(acc);
       }
       if (g == k % G) ring[slot(rin)] = enter;
