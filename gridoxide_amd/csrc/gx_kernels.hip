@@ -56,6 +56,20 @@ __device__ __forceinline__ bool tile_running(const int* status, int tile) {
   return __any(status[tile * TW + sub<TW>()] == kRunning);
 }
 
+// L+U values are stored position-major across tiles: entry p of lane s of tile t at
+// (p * n_tiles + t) * TW + s.  The wavefronts of all tiles walk the same columns at about the same
+// pace, so neighbouring tiles (one cache line holds 128 bytes of a position) re-read L columns from
+// shared lines; xcd_tile keeps those tiles on one XCD and its L2.
+__device__ __forceinline__ size_t lu_at(int p, int n_tiles, int tile, int TW) {
+  return ((size_t)p * n_tiles + tile) * TW;
+}
+// Workgroup b of a one-dimensional grid of n runs on XCD b mod 8: number the tiles so that each XCD
+// runs a contiguous range of them.
+__device__ __forceinline__ int xcd_tile(int b, int n) {
+  const int x = b & 7, i = b >> 3, q = n >> 3, r = n & 7;
+  return x * q + (x < r ? x : r) + i;
+}
+
 // first index i >= 0 with (base + i) % G == g
 template <int G>
 __device__ __forceinline__ int first_of_group(int base, int g) {
@@ -346,39 +360,43 @@ __global__ void k_fallback_reset(DevWork w) {
 // L += Q_i / V_i (src/jacobian.rs:220-243).
 template <typename T, int TW>
 __global__ __launch_bounds__(256) void k_fill(DevTopo t, DevWork w) {
-  const int tile = blockIdx.y, s = sub<TW>();
-  if (!tile_running<TW>(w.status, tile)) return;
-  const size_t bb = at<TW>(tile, t.n_bus, 0) + s;
-  T* __restrict__ lu = reinterpret_cast<T*>(w.lu) + at<TW>(tile, t.nnz_lu, 0) + s;
-  const Outage og = load_outage(t, w, tile * TW + s);
-  for (int p = elem_first<TW>(); p < t.nnz_lu; p += elem_stride<TW>()) {
-    const int kind = t.lu_kind[p];
-    double v = 0.0;
-    if (kind != kFill) {
-      const int i = t.lu_bi[p], k = t.lu_bk[p];
-      double g = t.lu_g[p], b = t.lu_b[p];
-      if (w.outage) y_at(t, og, t.lu_yp[p], g, b);
-      const size_t oi = bb + (size_t)i * TW, ok = bb + (size_t)k * TW;
-      const double vri = w.vr[oi], vii = w.vi[oi], vrk = w.vr[ok], vik = w.vi[ok];
-      const double cc = vri * vrk + vii * vik;
-      const double ss = vii * vrk - vri * vik;
-      const double hl = g * ss - b * cc;
-      const double nm = g * cc + b * ss;
-      const bool is_h = kind == kHii || kind == kHik;
-      const bool is_n = kind == kNii || kind == kNik;
-      const bool is_m = kind == kMii || kind == kMik;
-      const bool diag = kind == kHii || kind == kNii || kind == kMii || kind == kLii;
-      double base = is_h ? hl : (is_n ? nm : (is_m ? -nm : hl));
-      const double vmk = w.vm[ok];
-      if (is_n || !(is_h || is_m)) base = base / vmk;
-      if (diag) {
-        const double pc = w.pc[oi], qc = w.qc[oi];
-        base += is_h ? -qc : (is_n ? pc / vmk : (is_m ? pc : qc / vmk));
-      }
-      v = base;
+  // one L+U position per workgroup, one tile per lane group: the position's kind, buses and
+  // admittance are uniform and the stores of the workgroup's tiles are contiguous
+  const int p = blockIdx.x, tile = blockIdx.y * (256 / TW) + (int)threadIdx.x / TW, s = sub<TW>();
+  if (tile >= w.n_tiles) return;
+  const int sc = tile * TW + s;
+  bool run = false;  // any scenario of the tile still iterating (the LU kernel skips the others)
+  for (int q = 0; q < TW; ++q) run = run || w.status[tile * TW + q] == kRunning;
+  if (!run) return;
+  T* __restrict__ lu = reinterpret_cast<T*>(w.lu) + lu_at(p, w.n_tiles, tile, TW) + s;
+  const int kind = t.lu_kind[p];
+  double v = 0.0;
+  if (kind != kFill) {
+    const size_t bb = at<TW>(tile, t.n_bus, 0) + s;
+    const Outage og = load_outage(t, w, sc);
+    const int i = t.lu_bi[p], k = t.lu_bk[p];
+    double g = t.lu_g[p], b = t.lu_b[p];
+    if (w.outage) y_at(t, og, t.lu_yp[p], g, b);
+    const size_t oi = bb + (size_t)i * TW, ok = bb + (size_t)k * TW;
+    const double vri = w.vr[oi], vii = w.vi[oi], vrk = w.vr[ok], vik = w.vi[ok];
+    const double cc = vri * vrk + vii * vik;
+    const double ss = vii * vrk - vri * vik;
+    const double hl = g * ss - b * cc;
+    const double nm = g * cc + b * ss;
+    const bool is_h = kind == kHii || kind == kHik;
+    const bool is_n = kind == kNii || kind == kNik;
+    const bool is_m = kind == kMii || kind == kMik;
+    const bool diag = kind == kHii || kind == kNii || kind == kMii || kind == kLii;
+    double base = is_h ? hl : (is_n ? nm : (is_m ? -nm : hl));
+    const double vmk = w.vm[ok];
+    if (is_n || !(is_h || is_m)) base = base / vmk;
+    if (diag) {
+      const double pc = w.pc[oi], qc = w.qc[oi];
+      base += is_h ? -qc : (is_n ? pc / vmk : (is_m ? pc : qc / vmk));
     }
-    lu[(size_t)p * TW] = (T)v;
+    v = base;
   }
+  *lu = (T)v;
 }
 
 // Entries of a column a lane holds in registers (longer columns take a slower loop).
@@ -447,9 +465,10 @@ __device__ __forceinline__ void lu_tile(const DevTopo& t, const DevWork& w, int 
   const int n = t.n;
   const int nl = t.nnz_lu;
   // this tile's arrays (uniform bases); entry e of lane s at e * TW + s
-  T* __restrict__ lub = reinterpret_cast<T*>(w.lu) + (size_t)tile * nl * TW;
+  T* __restrict__ lub = reinterpret_cast<T*>(w.lu) + lu_at(0, w.n_tiles, tile, TW) + s;
   T* __restrict__ yb = reinterpret_cast<T*>(w.y) + (size_t)tile * n * TW;
-  auto LU = [&](int p) -> T& { return lub[(unsigned)(p * TW + s)]; };
+  const unsigned pst = (unsigned)(w.n_tiles * TW);  // elements between consecutive positions
+  auto LU = [&](int p) -> T& { return lub[(size_t)(unsigned)p * pst]; };
   auto Y = [&](int r) -> T& { return yb[(unsigned)(r * TW + s)]; };
   const int* __restrict__ rowp = t.lu_row;
   bool ok = true;        // per lane: no zero pivot, finite solution
@@ -876,7 +895,8 @@ __device__ __forceinline__ void lu64_tile(const DevTopo& t, const DevWork& w, in
   const int s = threadIdx.x;
   if (!tile_running<TW>(w.status, tile)) return;
   const int sc = tile * TW + s;
-  T* __restrict__ lu = reinterpret_cast<T*>(w.lu) + at<TW>(tile, t.nnz_lu, 0) + s;
+  T* __restrict__ lu = reinterpret_cast<T*>(w.lu) + lu_at(0, w.n_tiles, tile, TW) + s;
+  const size_t pst = (size_t)w.n_tiles * TW;  // elements between consecutive positions
   T* __restrict__ y = reinterpret_cast<T*>(w.y) + at<TW>(tile, t.n, 0) + s;
   const int* __restrict__ lrow = t.lu_row;
   const int2* __restrict__ uitems = t.uitems;
@@ -912,7 +932,7 @@ __device__ __forceinline__ void lu64_tile(const DevTopo& t, const DevWork& w, in
 #pragma unroll
     for (int i = 0; i < MS; ++i) {
       const int o = f0 + i;
-      nsv[i] = lu[(size_t)(c0 + (o < len ? o : 0)) * TW];
+      nsv[i] = lu[(size_t)(c0 + (o < len ? o : 0)) * pst];
     }
 #pragma unroll
     for (int i = 0; i < MS; ++i) {
@@ -921,7 +941,7 @@ __device__ __forceinline__ void lu64_tile(const DevTopo& t, const DevWork& w, in
     }
 #pragma unroll
     for (int i = 0; i < MF; ++i) {
-      nfl[i] = lu[(size_t)cl(fs1[i].x, nl - 1) * TW];
+      nfl[i] = lu[(size_t)cl(fs1[i].x, nl - 1) * pst];
       nfy[i] = y[(size_t)cl(fs1[i].y, n - 1) * TW];
     }
     nfr = y[(size_t)cl(kk + W + 1, n - 1) * TW];
@@ -938,7 +958,7 @@ __device__ __forceinline__ void lu64_tile(const DevTopo& t, const DevWork& w, in
       i2[i] = uitems[(size_t)cl(1, n_rounds - 1) * IPR + i];
     }
 #pragma unroll
-    for (int i = 0; i < IPR; ++i) v1[i] = lu[(size_t)(i1[i].y < 0 ? 0 : i1[i].y) * TW];
+    for (int i = 0; i < IPR; ++i) v1[i] = lu[(size_t)(i1[i].y < 0 ? 0 : i1[i].y) * pst];
   }
   for (int k = 0; k < n; ++k) {
     // this column's data; the next columns' loads
@@ -978,7 +998,7 @@ __device__ __forceinline__ void lu64_tile(const DevTopo& t, const DevWork& w, in
 #pragma unroll
       for (int i = 0; i < MS; ++i)
         if (i < len) xc[xs(i)] = cvs[i];
-      for (int o = MS; o < len; ++o) xc[xs(o)] = lu[(size_t)(c0 + o) * TW];
+      for (int o = MS; o < len; ++o) xc[xs(o)] = lu[(size_t)(c0 + o) * pst];
       const int r0 = mk.w, r1 = k + 1 < n ? m1.w : n_rounds;
       for (int rho = r0; rho < r1; ++rho) {
         int2 it[IPR];
@@ -991,7 +1011,7 @@ __device__ __forceinline__ void lu64_tile(const DevTopo& t, const DevWork& w, in
             i1[i] = i2[i];
           }
 #pragma unroll
-          for (int i = 0; i < IPR; ++i) v1[i] = lu[(size_t)(i1[i].y < 0 ? 0 : i1[i].y) * TW];
+          for (int i = 0; i < IPR; ++i) v1[i] = lu[(size_t)(i1[i].y < 0 ? 0 : i1[i].y) * pst];
           const int rn = cl(rho + 2, n_rounds - 1);
 #pragma unroll
           for (int i = 0; i < IPR; ++i) i2[i] = uitems[(size_t)rn * IPR + i];
@@ -999,7 +1019,7 @@ __device__ __forceinline__ void lu64_tile(const DevTopo& t, const DevWork& w, in
 #pragma unroll
           for (int i = 0; i < IPR; ++i) {
             it[i] = uitems[(size_t)rho * IPR + i];
-            lv[i] = it[i].y < 0 ? zero : lu[(size_t)(it[i].y < 0 ? 0 : it[i].y) * TW];
+            lv[i] = it[i].y < 0 ? zero : lu[(size_t)(it[i].y < 0 ? 0 : it[i].y) * pst];
           }
         }
         if (it[0].y >= 0) {  // empty rounds (columns without updates) do nothing
@@ -1016,7 +1036,7 @@ __device__ __forceinline__ void lu64_tile(const DevTopo& t, const DevWork& w, in
       const T ukk = xc[xs(ul)];
       if (is_zero(ukk)) ok = false;
       for (int o = ul + 1; o < len; ++o) xc[xs(o)] = xc[xs(o)] / ukk;
-      for (int o = 0; o < len; ++o) lu[(size_t)(c0 + o) * TW] = xc[xs(o)];
+      for (int o = 0; o < len; ++o) lu[(size_t)(c0 + o) * pst] = xc[xs(o)];
     }
     // forward substitution: y_k is final
     const T yk = ring[rs(k)];
@@ -1031,7 +1051,7 @@ __device__ __forceinline__ void lu64_tile(const DevTopo& t, const DevWork& w, in
           const int p = cfs[i].x, j = cfs[i].y;
           T lj = cfl[i], yj = cfy[i];
           if (kFactor && j == k) lj = xc[xs(p - c0)];
-          if (kFactor && j == k - 1) lj = lu[(size_t)p * TW];
+          if (kFactor && j == k - 1) lj = lu[(size_t)p * pst];
           if (j == k) yj = yk;
           if (j == k - 1) yj = y[(size_t)j * TW];
           v = v - lj * yj;
@@ -1041,7 +1061,7 @@ __device__ __forceinline__ void lu64_tile(const DevTopo& t, const DevWork& w, in
         const int p = t.fwd_pos[q], j = t.fwd_col[q];
         T lj;
         if (kFactor && j == k) lj = xc[xs(p - c0)];
-        else lj = lu[(size_t)p * TW];
+        else lj = lu[(size_t)p * pst];
         const T yj = j == k ? yk : y[(size_t)j * TW];
         v = v - lj * yj;
       }
@@ -1059,7 +1079,7 @@ __device__ __forceinline__ void lu64_tile(const DevTopo& t, const DevWork& w, in
     }
     for (int o = ul + 1 + MS; o < len; ++o) {
       const int r = lrow[c0 + o];
-      const T lv = kFactor ? xc[xs(o)] : lu[(size_t)(c0 + o) * TW];
+      const T lv = kFactor ? xc[xs(o)] : lu[(size_t)(c0 + o) * pst];
       if (r - k <= W) ring[rs(r)] = ring[rs(r)] - lv * yk;
     }
   }
@@ -1085,13 +1105,13 @@ __device__ __forceinline__ void lu64_tile(const DevTopo& t, const DevWork& w, in
 #pragma unroll
       for (int i = 0; i < MS; ++i) {
         const int p = c0 + (i < ul ? i : 0);
-        nuv[i] = lu[(size_t)p * TW];
+        nuv[i] = lu[(size_t)p * pst];
         nur[i] = lrow[p];
       }
-      npv = lu[(size_t)(c0 + ul) * TW];
+      npv = lu[(size_t)(c0 + ul) * pst];
 #pragma unroll
       for (int i = 0; i < MF; ++i) {
-        nbl[i] = lu[(size_t)cl(bs1[i].x, nl - 1) * TW];
+        nbl[i] = lu[(size_t)cl(bs1[i].x, nl - 1) * pst];
         nbx[i] = y[(size_t)cl(bs1[i].y, n - 1) * TW];
       }
       nyr = y[(size_t)cl(kk - W - 1, n - 1) * TW];
@@ -1145,7 +1165,7 @@ __device__ __forceinline__ void lu64_tile(const DevTopo& t, const DevWork& w, in
         for (int q = ek.x; q < ek.y; ++q) {
           const int j = t.bwd_col[q];
           const T xj = j == k ? xk : y[(size_t)j * TW];
-          v = v - lu[(size_t)t.bwd_pos[q] * TW] * xj;
+          v = v - lu[(size_t)t.bwd_pos[q] * pst] * xj;
         }
         ring[rs(rin)] = v;
       }
@@ -1158,7 +1178,7 @@ __device__ __forceinline__ void lu64_tile(const DevTopo& t, const DevWork& w, in
       }
       for (int u = MS; u < ul; ++u) {
         const int r = lrow[c0 + u];
-        if (k - r <= W) ring[rs(r)] = ring[rs(r)] - lu[(size_t)(c0 + u) * TW] * xk;
+        if (k - r <= W) ring[rs(r)] = ring[rs(r)] - lu[(size_t)(c0 + u) * pst] * xk;
       }
     }
   }
@@ -1168,11 +1188,12 @@ __device__ __forceinline__ void lu64_tile(const DevTopo& t, const DevWork& w, in
   }
 }
 
-// One launch per batch: wavefront b factors/solves tile b.
+// One launch per batch: one wavefront per tile (XCD-aware numbering).
 template <typename T, int TW, bool kFactor>
 __global__ __launch_bounds__(64) void k_lu(DevTopo t, DevWork w, int cap, int iter) {
-  if constexpr (TW == kWideTile) lu64_tile<T, kFactor>(t, w, blockIdx.x, cap, iter);
-  else lu_tile<T, TW, kFactor>(t, w, blockIdx.x, cap, iter);
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  if constexpr (TW == kWideTile) lu64_tile<T, kFactor>(t, w, tile, cap, iter);
+  else lu_tile<T, TW, kFactor>(t, w, tile, cap, iter);
 }
 
 // One persistent launch over the tiles of several requests (topologies of any size): each
@@ -1353,7 +1374,8 @@ __global__ __launch_bounds__(256) void k_lg_fill(DevTopo t, DevLg lg, DevWork w,
     lg_status[sc] = sc < w.n_scen ? kRunning : kPadLane;
   }
   const size_t bb = at<TW>(tile, t.n_bus, 0) + s;
-  zd* out = lu + at<TW>(tile, lg.nnz_lu, 0) + s;
+  zd* out = lu + lu_at(0, w.n_tiles, tile, TW) + s;
+  const size_t pst = (size_t)w.n_tiles * TW;
   const Outage og = load_outage(t, w, tile * TW + s);
   for (int p = elem_first<TW>(); p < lg.nnz_lu; p += elem_stride<TW>()) {
     zd v{0.0, 0.0};
@@ -1365,7 +1387,7 @@ __global__ __launch_bounds__(256) void k_lg_fill(DevTopo t, DevLg lg, DevWork w,
       v.re += -w.ps[o];  // -conj(P + jQ) = -P + jQ
       v.im += w.qs[o];
     }
-    out[(size_t)p * TW] = v;
+    out[(size_t)p * pst] = v;
   }
 }
 
@@ -1529,7 +1551,7 @@ void launch_decide(const DevTopo& t, const DevWork& w, int iter, hipStream_t str
 void launch_fill(const DevTopo& t, const DevWork& w, bool fp32, hipStream_t stream) {
   with_tile(w.tile, [&](auto tw) {
     constexpr int TW = decltype(tw)::value;
-    dim3 grid(elem_blocks(t.nnz_lu, TW), w.n_tiles);
+    dim3 grid(t.nnz_lu, (w.n_tiles + 256 / TW - 1) / (256 / TW));
     if (fp32) hipLaunchKernelGGL((k_fill<float, TW>), grid, dim3(256), 0, stream, t, w);
     else hipLaunchKernelGGL((k_fill<double, TW>), grid, dim3(256), 0, stream, t, w);
   });
