@@ -465,10 +465,11 @@ __device__ __forceinline__ void lu_tile(const DevTopo& t, const DevWork& w, int 
   const int n = t.n;
   const int nl = t.nnz_lu;
   // this tile's arrays (uniform bases); entry e of lane s at e * TW + s
-  T* __restrict__ lub = reinterpret_cast<T*>(w.lu) + lu_at(0, w.n_tiles, tile, TW) + s;
+  char* __restrict__ lub = reinterpret_cast<char*>(reinterpret_cast<T*>(w.lu) + lu_at(0, w.n_tiles, tile, TW) + s);
   T* __restrict__ yb = reinterpret_cast<T*>(w.y) + (size_t)tile * n * TW;
-  const unsigned pst = (unsigned)(w.n_tiles * TW);  // elements between consecutive positions
-  auto LU = [&](int p) -> T& { return lub[(size_t)(unsigned)p * pst]; };
+  const unsigned pstb = (unsigned)(w.n_tiles * TW * sizeof(T));  // bytes between consecutive positions
+  // (one multiply-add per address: position x stride + this lane's base)
+  auto LU = [&](int p) -> T& { return *reinterpret_cast<T*>(lub + (size_t)(unsigned)p * pstb); };
   auto Y = [&](int r) -> T& { return yb[(unsigned)(r * TW + s)]; };
   const int* __restrict__ rowp = t.lu_row;
   bool ok = true;        // per lane: no zero pivot, finite solution
