@@ -65,6 +65,9 @@ struct DevTopo {
   const int* xpos_a = nullptr;  // angle unknown -> pivot column
   const int* xpos_m = nullptr;  // magnitude unknown -> pivot column
   const int* bus_type = nullptr;
+  // per bus, 6 doubles: the (P, Q) of the constant-power, constant-current and constant-impedance
+  // ZIP terms at |V| = 1; null when no bus has any
+  const double* zip = nullptr;
   // per packed L+U position: Jacobian recipe (kind, bus i, bus k, g, b)
   const int* lu_kind = nullptr;
   const int* lu_bi = nullptr;

@@ -714,6 +714,10 @@ struct gx_topo {
   gx::DevLg dlg;
   gx::DevTopo dlgt;
   bool ybus_uploaded = false;
+  // ZIP terms (gx_topo_set_zip): 6 per bus, empty when no bus has any
+  std::vector<double> zip;
+  DevMem d_zip;
+  bool zip_uploaded = true;
   // branches for per-scenario outages (gx_topo_set_branches)
   int n_br = 0;
   std::vector<int> br_pos;     // 4 per branch: Y CSR positions of ff, ft, tf, tt (-1 if none)
@@ -905,8 +909,17 @@ int upload_branches(gx_topo& t) {
 }
 
 int upload_ybus(gx_topo& t) {
-  if (t.ybus_uploaded) return GX_OK;
   int rc;
+  if (!t.zip_uploaded) {
+    if (t.zip.empty()) {
+      t.dt.zip = nullptr;
+    } else {
+      if ((rc = t.d_zip.upload(t.zip))) return rc;
+      t.dt.zip = t.d_zip.as<double>();
+    }
+    t.zip_uploaded = true;
+  }
+  if (t.ybus_uploaded) return GX_OK;
   if ((rc = t.d_yrow.upload(t.yrow)) || (rc = t.d_ycol.upload(t.ycol)) || (rc = t.d_yg.upload(t.yg)) ||
       (rc = t.d_yb.upload(t.yb)) || (rc = t.d_type.upload(t.dev_type)) || (rc = t.d_comp.upload(t.comp)))
     return rc;
@@ -1141,6 +1154,12 @@ int stage_batch(gx_topo& t, Workspace& ws, const BatchIn& in, int tile, gx::DevW
       for (int i = 0; i < nb; ++i)
         if (t.dev_type[i] == gx::kDeadBus) hvm[off + i] = hva[off + i] = hp[i] = hq[i] = 0.0;
       host_injections(t, hvm.data() + off, hva.data() + off, pc, qc);
+      if (!t.zip.empty())
+        for (int i = 0; i < nb; ++i) {
+          const double v = hvm[off + i], *z = &t.zip[6 * (size_t)i];
+          hp[i] = hp[i] + z[0] + z[2] * v + z[4] * v * v;
+          hq[i] = hq[i] + z[1] + z[3] * v + z[5] * v * v;
+        }
       double mx = 0.0;
       for (int i = 0; i < nb; ++i) {
         if (t.ns_pos[i] >= 0) mx = std::fmax(mx, std::fabs(hp[i] - pc[i]));
@@ -1501,6 +1520,19 @@ int gx_topo_lu_pattern(const gx_topo* t, int* cp, int* ulen, int* row) {
   if (cp) std::memcpy(cp, t->lu.cp.data(), sizeof(int) * t->lu.cp.size());
   if (ulen) std::memcpy(ulen, t->lu.ulen.data(), sizeof(int) * t->lu.ulen.size());
   if (row) std::memcpy(row, t->lu.row.data(), sizeof(int) * t->lu.row.size());
+  return GX_OK;
+}
+
+int gx_topo_set_zip(gx_topo* t, const double* zip) {
+  if (!t) return fail(GX_ERR_ARG, "null topology");
+  if (zip) {
+    for (size_t k = 0; k < 6 * (size_t)t->n_bus; ++k)
+      if (!std::isfinite(zip[k])) return fail(GX_ERR_ARG, "non-finite ZIP coefficient");
+    t->zip.assign(zip, zip + 6 * (size_t)t->n_bus);
+  } else {
+    t->zip.clear();
+  }
+  t->zip_uploaded = false;
   return GX_OK;
 }
 

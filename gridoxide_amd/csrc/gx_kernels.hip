@@ -145,6 +145,15 @@ __device__ __forceinline__ Outage load_outage(const DevTopo& t, const DevWork& w
   }
   return o;
 }
+// effective_injection (src/network.rs:600-611): the constant-power (P, Q) plus the bus's ZIP terms
+// at its current voltage magnitude vm (constant power, S * vm, S * vm^2)
+__device__ __forceinline__ void effective_injection(const DevTopo& t, int i, double vm, double& p, double& q) {
+  if (t.zip) {
+    const double* z = t.zip + 6 * (size_t)i;
+    p = p + z[0] + z[2] * vm + z[4] * vm * vm;
+    q = q + z[1] + z[3] * vm + z[5] * vm * vm;
+  }
+}
 // Y value at position p for this scenario
 __device__ __forceinline__ void y_at(const DevTopo& t, const Outage& o, int p, double& g, double& b) {
   g = t.y_g[p];
@@ -273,16 +282,18 @@ __global__ __launch_bounds__(256) void k_mismatch(DevTopo t, DevWork w) {
     const double qcal = vi * ir - vr * ii;
     w.pc[o] = pcal;
     w.qc[o] = qcal;
+    double pe = w.ps[o], qe = w.qs[o];
+    effective_injection(t, i, w.vm[o], pe, qe);
     const int rp = t.rpos_p[i];
     if (rp >= 0) {
-      const double fp = w.ps[o] - pcal;
+      const double fp = pe - pcal;
       w.f[ub + (size_t)rp * TW] = fp;
       y[ub + (size_t)rp * TW] = (T)fp;
       mx = fmax(mx, fabs(fp));
     }
     const int rq = t.rpos_q[i];
     if (rq >= 0) {
-      const double fq = w.qs[o] - qcal;
+      const double fq = qe - qcal;
       w.f[ub + (size_t)rq * TW] = fq;
       y[ub + (size_t)rq * TW] = (T)fq;
       mx = fmax(mx, fabs(fq));
@@ -1355,8 +1366,10 @@ __global__ __launch_bounds__(256) void k_island_max(DevTopo t, DevWork w, const 
     }
     const size_t o = bb + (size_t)i * TW;
     const double vr = w.vr[o], vi = w.vi[o];
-    double mx = fabs(w.ps[o] - (vr * ir + vi * ii));
-    if (ty == 2) mx = fmax(mx, fabs(w.qs[o] - (vi * ir - vr * ii)));
+    double pe = w.ps[o], qe = w.qs[o];
+    effective_injection(t, i, w.vm[o], pe, qe);
+    double mx = fabs(pe - (vr * ir + vi * ii));
+    if (ty == 2) mx = fmax(mx, fabs(qe - (vi * ir - vr * ii)));
     if (sc < w.n_scen && mx > 0.0)
       atomicMax(&out[(size_t)sc * n_comp + comp[i]], (unsigned long long)__double_as_longlong(mx));
   }
@@ -1385,8 +1398,10 @@ __global__ __launch_bounds__(256) void k_lg_fill(DevTopo t, DevLg lg, DevWork w,
     const int d = lg.dbus[p];
     if (d >= 0) {
       const size_t o = bb + (size_t)d * TW;
-      v.re += -w.ps[o];  // -conj(P + jQ) = -P + jQ
-      v.im += w.qs[o];
+      double pe = w.ps[o], qe = w.qs[o];
+      effective_injection(t, d, w.vm[o], pe, qe);
+      v.re += -pe;  // -conj(P + jQ) = -P + jQ
+      v.im += qe;
     }
     out[(size_t)p * pst] = v;
   }

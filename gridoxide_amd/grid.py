@@ -38,6 +38,7 @@ import numpy as np
 SLACK, PV, PQ = 0, 1, 2
 _TYPE_NAMES = {SLACK: "Slack", PV: "PV", PQ: "PQ"}
 _TYPE_CODES = {v: k for k, v in _TYPE_NAMES.items()}
+_ZIP_KINDS = ("ConstPower", "ConstCurrent", "ConstImpedance")  # ZipKind (src/types.rs:14-18)
 
 
 @dataclasses.dataclass
@@ -60,6 +61,9 @@ class Network:
     tr_ys: np.ndarray             # complex128 series admittance
     tr_ysh: np.ndarray            # complex128 shunt admittance (split half to each end)
     tr_tap: np.ndarray            # complex128 tap ratio k*exp(j*shift)
+    # ZIP terms (Bus::zip_terms, src/types.rs:14-24, 40-41) summed per bus and kind: [n_bus, 6] =
+    # (P, Q) of the constant-power, constant-current and constant-impedance terms; None when no bus has any
+    zip_coef: Optional[np.ndarray] = None
     # generator bookkeeping used by the scenario model (not part of the reference's types)
     load_p: Optional[np.ndarray] = None   # float64 [n_bus] load P at the template (PQ buses)
     load_q: Optional[np.ndarray] = None
@@ -81,10 +85,15 @@ class Network:
     def to_json(self) -> str:
         buses = []
         for i in range(self.n_bus):
-            buses.append({"idx": i, "bus_type": _TYPE_NAMES[int(self.bus_type[i])],
-                          "voltage_mag": float(self.vm[i]), "voltage_ang": float(self.va[i]),
-                          "p_spec": float(self.p_spec[i]), "q_spec": float(self.q_spec[i]),
-                          "q_min": -999.0, "q_max": 999.0})
+            bus = {"idx": i, "bus_type": _TYPE_NAMES[int(self.bus_type[i])],
+                   "voltage_mag": float(self.vm[i]), "voltage_ang": float(self.va[i]),
+                   "p_spec": float(self.p_spec[i]), "q_spec": float(self.q_spec[i]),
+                   "q_min": -999.0, "q_max": 999.0}
+            if self.zip_coef is not None:
+                z = self.zip_coef[i]
+                bus["zip_terms"] = [{"s_const": [float(z[2 * k]), float(z[2 * k + 1])], "kind": _ZIP_KINDS[k]}
+                                    for k in range(3) if z[2 * k] != 0.0 or z[2 * k + 1] != 0.0]
+            buses.append(bus)
         lines = [{"from": int(f), "to": int(t), "r": float(r), "x": float(x), "b_shunt": float(b), "g_shunt": float(g)}
                  for f, t, r, x, b, g in zip(self.line_from, self.line_to, self.line_r, self.line_x, self.line_b, self.line_g)]
         trs = [{"from": int(f), "to": int(t), "from_status": 1, "to_status": 1,
@@ -105,6 +114,16 @@ class Network:
             if t.get("from_status", 1) != 1 or t.get("to_status", 1) != 1:
                 raise ValueError("only closed (1,1) transformer branches are supported")
         c = lambda v: complex(v[0], v[1])
+        zip_coef = None
+        if any(b.get("zip_terms") for b in buses):
+            zip_coef = np.zeros((len(buses), 6))
+            for i, b in enumerate(buses):
+                for term in b.get("zip_terms", []):
+                    if term["kind"] not in _ZIP_KINDS:
+                        raise ValueError(f"bus {i}: unknown ZIP kind {term['kind']!r}")
+                    k = _ZIP_KINDS.index(term["kind"])
+                    zip_coef[i, 2 * k] += float(term["s_const"][0])
+                    zip_coef[i, 2 * k + 1] += float(term["s_const"][1])
         return Network(
             bus_type=np.array([_TYPE_CODES[b["bus_type"]] for b in buses], dtype=np.int32),
             vm=np.array([b["voltage_mag"] for b in buses], dtype=np.float64),
@@ -122,6 +141,7 @@ class Network:
             tr_ys=np.array([c(t["y_series"]) for t in trs], dtype=np.complex128),
             tr_ysh=np.array([c(t["y_shunt"]) for t in trs], dtype=np.complex128),
             tr_tap=np.array([c(t["tap"]) for t in trs], dtype=np.complex128),
+            zip_coef=zip_coef,
         )
 
 

@@ -71,6 +71,10 @@ class SolveStats:
     def iterations(self) -> int:
         return len(self.mismatch_history)
 
+    def final_mismatch(self) -> float:
+        """Max |mismatch| at the last iteration, 0.0 if none ran (src/solver.rs:153-156)."""
+        return self.mismatch_history[-1] if self.mismatch_history else 0.0
+
 
 @dataclasses.dataclass
 class PowerFlowReport:
@@ -189,7 +193,22 @@ class Topology:
     def from_network(cls, net: Network) -> "Topology":
         topo = cls(net.bus_type, build_ybus(net))
         topo.set_branches(net)
+        if net.zip_coef is not None:
+            topo.set_zip(net.zip_coef)
         return topo
+
+    def set_zip(self, zip_coef: Optional[np.ndarray]):
+        """Per-bus ZIP terms ([n_bus, 6]: (P, Q) of the constant-power, -current and -impedance
+        parts) used by every later solve's mismatch; None removes them."""
+        if zip_coef is None:
+            check(_lib.lib().gx_topo_set_zip(self._h, None))
+            self._zip = None
+            return
+        z = f64(zip_coef).reshape(-1)
+        if z.size != 6 * self.n_bus:
+            raise ValueError(f"zip_coef must have shape [{self.n_bus}, 6]")
+        check(_lib.lib().gx_topo_set_zip(self._h, ptr(z)))
+        self._zip = z
 
     def set_branches(self, net: Network):
         """Registers the network's branches (lines, then transformers) for per-scenario outages."""
@@ -352,6 +371,8 @@ class PersistentSolver:
         ``&mut [Bus]`` and returns ``SolveStats``."""
         if self._topo is None:
             self._topo = Topology(net.bus_type, ybus if ybus is not None else build_ybus(net))
+            if net.zip_coef is not None:
+                self._topo.set_zip(net.zip_coef)
         vm, va, st, it, hist = self._topo.batch_solve(net.p_spec[None], net.q_spec[None], net.vm[None], net.va[None],
                                                       tol, max_iter, self.mode)
         net.vm[:] = vm[0]

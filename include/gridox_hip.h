@@ -158,6 +158,15 @@ int gx_topo_profile(gx_topo* t, int enable, double* ms, int* launches);
  * triangular-solve sums.  GX_ERR_ARG for other values. */
 int gx_topo_set_launch(gx_topo* t, int tile, int ring_rows);
 
+/* Voltage-dependent ZIP injection terms of t's buses (Bus::zip_terms, src/types.rs:14-24, 40-41),
+ * summed per bus and kind: zip[6*i .. 6*i+5] = (P, Q) of bus i's constant-power, constant-current
+ * and constant-impedance terms in per unit at |V| = 1.  Every later solve evaluates
+ * effective_injection (src/network.rs:600-611) with them: P_eff = p_spec + P_p + P_i*vm + P_z*vm^2
+ * (Q alike), in the mismatch, the island check and the linear initial guess; the Jacobian keeps the
+ * reference's constant-power form.  zip == NULL removes them.  Takes effect at the next solve
+ * or engine submission; not to be called while an engine request on t is in flight. */
+int gx_topo_set_zip(gx_topo* t, const double* zip);
+
 /* Branches of t for per-scenario outages, in the argument layout of gx_ybus_build (lines, then
  * transformers; branch e < n_line is line e, else transformer e - n_line).  The reference's
  * Scenario::branch_outages (src/batch.rs:70-83) names branches but BatchSolver rejects them

@@ -89,13 +89,18 @@ def ybus(net):
 class Oracle:
     """One topology on the CPU oracle."""
 
-    def __init__(self, net):
+    def __init__(self, net, zip_coef=None):
+        """zip_coef: optional [n_bus, 6] per-bus (P, Q) of constant-power, -current and -impedance
+        ZIP terms (defaults to ``net.zip_coef``)."""
         self.net = net
         self.n = net.n_bus
         self.rowptr, self.col, self.y = ybus(net)
         self.yre = c64(self.y.real)
         self.yim = c64(self.y.imag)
         self.type = c32(net.bus_type)
+        if zip_coef is None:
+            zip_coef = getattr(net, "zip_coef", None)
+        self.zip = None if zip_coef is None else c64(zip_coef).reshape(self.n, 6)
 
     def _ya(self):
         return (_p(self.rowptr, _i32p), _p(self.col, _i32p), _p(self.yre, _f64p), _p(self.yim, _f64p))
@@ -121,8 +126,9 @@ class Oracle:
     def linear_initial_guess(self, p_spec, q_spec, vm, va):
         vm = c64(vm).copy()
         va = c64(va).copy()
-        rc = lib().orc_linear_initial_guess(self.n, *self._ya(), _p(self.type, _i32p), _p(c64(p_spec), _f64p),
-                                            _p(c64(q_spec), _f64p), _p(vm, _f64p), _p(va, _f64p))
+        rc = lib().orc_linear_initial_guess_zip(self.n, *self._ya(), _p(self.type, _i32p), _p(self.zip, _f64p),
+                                                _p(c64(p_spec), _f64p), _p(c64(q_spec), _f64p), _p(vm, _f64p),
+                                                _p(va, _f64p))
         return vm, va, rc
 
     def batch_solve(self, p_spec, q_spec, vm0, va0, tol=1e-8, max_iter=20, init=0, threads=1):
@@ -136,10 +142,41 @@ class Oracle:
         status = np.zeros(S, np.int32)
         iters = np.zeros(S, np.int32)
         hist = np.zeros((S, max_iter))
-        lib().orc_batch_solve(self.n, *self._ya(), _p(self.type, _i32p), S, _p(p_spec, _f64p), _p(q_spec, _f64p),
-                              _p(vm0, _f64p), _p(va0, _f64p), init, ctypes.c_double(tol), max_iter, threads,
-                              _p(vm, _f64p), _p(va, _f64p), _p(status, _i32p), _p(iters, _i32p), _p(hist, _f64p))
+        lib().orc_batch_solve_zip(self.n, *self._ya(), _p(self.type, _i32p), _p(self.zip, _f64p), S,
+                                  _p(p_spec, _f64p), _p(q_spec, _f64p), _p(vm0, _f64p), _p(va0, _f64p), init,
+                                  ctypes.c_double(tol), max_iter, threads, _p(vm, _f64p), _p(va, _f64p),
+                                  _p(status, _i32p), _p(iters, _i32p), _p(hist, _f64p))
         return vm, va, status, iters, hist
+
+    def components(self):
+        """Connected components of the Y-bus graph (src/network.rs:121-145)."""
+        comp = np.zeros(self.n, np.int32)
+        nc = lib().orc_components(self.n, _p(self.rowptr, _i32p), _p(self.col, _i32p), _p(comp, _i32p))
+        return comp, nc
+
+    def persistent_solve(self, p_spec, q_spec, vm0, va0, tol=1e-8, max_iter=20, init=1):
+        """One scenario the way PowerFlowModel.solve runs it (src/python.rs:256-274): the linear
+        initial guess on the buses as given, then every component without a Slack bus pinned to V = 0
+        with no injection (classify / mark_unreferenced_islands, src/network.rs:176-223), then the
+        Newton loop.  Returns (vm, va, status, iters, history)."""
+        vm = c64(vm0).copy()
+        va = c64(va0).copy()
+        p = c64(p_spec).copy()
+        q = c64(q_spec).copy()
+        if init:
+            vm, va, _ = self.linear_initial_guess(p, q, vm, va)
+        comp, nc = self.components()
+        types = self.type.copy()
+        for c in range(nc):
+            members = np.nonzero(comp == c)[0]
+            if not (types[members] == 0).any():
+                types[members] = 0
+                vm[members] = va[members] = p[members] = q[members] = 0.0
+        marked = Oracle.__new__(Oracle)
+        marked.__dict__.update(self.__dict__)
+        marked.type = c32(types)
+        out = marked.batch_solve(p, q, vm, va, tol, max_iter, init=0)
+        return tuple(a[0] for a in out)
 
 
 def sparse_solve(ap, ai, ax, b):

@@ -11,7 +11,7 @@
 //                        stamp_transformers     src/network.rs:364-373
 //                        YBus::finish           src/network.rs:52-71 (duplicates summed in push order)
 //   power_injections     src/network.rs:680-701 (I = Y*V, S_k = V_k*conj(I_k))
-//   effective_injection  src/network.rs:600-611 (no ZIP terms in these workloads: (p_spec, q_spec))
+//   effective_injection  src/network.rs:600-611 (ZIP terms summed per kind, evaluated at the bus's vm)
 //   Jacobian triplets    build_jacobian_triplets src/solver.rs:818-888 (H/N then M/L emission order)
 //   NR loop              newton_raphson_cached  src/solver.rs:639-736 (history, status, update)
 //   sparse LU            KLU factor with diagonal-preference partial pivoting at tol 1e-3
@@ -306,7 +306,21 @@ struct Net {
   const double* yre;
   const double* yim;
   const int* type;  // 0 slack, 1 PV, 2 PQ
+  const double* zip = nullptr;  // per bus (P, Q) of constant-power, -current, -impedance ZIP terms, or null
 };
+
+// effective_injection (src/network.rs:600-611) with the bus's ZIP terms summed per kind: the
+// constant-power part, then S_p, S_i * vm and S_z * vm * vm added in that order.
+void effective(const Net& net, int i, double vm, double& p, double& q) {
+  if (!net.zip) return;
+  const double* z = net.zip + 6 * (size_t)i;
+  p = p + z[0];
+  q = q + z[1];
+  p = p + z[2] * vm;
+  q = q + z[3] * vm;
+  p = p + z[4] * vm * vm;
+  q = q + z[5] * vm * vm;
+}
 
 void injections(const Net& net, const double* vm, const double* va, double* p, double* q) {
   std::vector<cx> v(net.n);
@@ -412,8 +426,16 @@ int nr_solve(const Net& net, const Unknowns& u, const double* p_spec, const doub
   for (; it < max_iter; ++it) {
     injections(net, vm, va, p.data(), q.data());
     int m = 0;
-    for (int i : u.non_slack) f[m++] = p_spec[i] - p[i];
-    for (int i : u.pq) f[m++] = q_spec[i] - q[i];
+    for (int i : u.non_slack) {
+      double pe = p_spec[i], qe = q_spec[i];
+      effective(net, i, vm[i], pe, qe);
+      f[m++] = pe - p[i];
+    }
+    for (int i : u.pq) {
+      double pe = p_spec[i], qe = q_spec[i];
+      effective(net, i, vm[i], pe, qe);
+      f[m++] = qe - q[i];
+    }
     double mx = 0.0;
     for (int t = 0; t < u.n; ++t) mx = std::fmax(mx, std::fabs(f[t]));  // NaN-ignoring, as f64::max
     if (history) history[it] = mx;
@@ -459,7 +481,9 @@ int linear_guess(const Net& net, const double* p_spec, const double* q_spec, dou
   std::vector<cx> rhs(m, cx{0.0, 0.0});
   for (int r = 0; r < m; ++r) {
     int i = unk[r];
-    cx yl = cneg(cconj({p_spec[i], q_spec[i]}));
+    double pe = p_spec[i], qe = q_spec[i];
+    effective(net, i, vm[i], pe, qe);
+    cx yl = cneg(cconj({pe, qe}));
     bool diag_seen = false;
     for (int t = net.rowptr[i]; t < net.rowptr[i + 1]; ++t) {
       int j = net.col[t];
@@ -566,6 +590,14 @@ int orc_linear_initial_guess(int n, const int* rowptr, const int* col, const dou
   return linear_guess(net, p_spec, q_spec, vm, va);
 }
 
+// orc_linear_initial_guess with ZIP terms (zip: 6 per bus as in Net, or null).
+int orc_linear_initial_guess_zip(int n, const int* rowptr, const int* col, const double* yre, const double* yim,
+                                 const int* type, const double* zip, const double* p_spec, const double* q_spec,
+                                 double* vm, double* va) {
+  Net net{n, rowptr, col, yre, yim, type, zip};
+  return linear_guess(net, p_spec, q_spec, vm, va);
+}
+
 // Real sparse solve A x = b through the oracle LU (CSC input, bus-agnostic natural column order).
 // Returns 0 on success, 1 when singular / non-finite.
 int orc_sparse_solve(int n, const int* ap, const int* ai, const double* ax, const double* b, double* x) {
@@ -582,11 +614,25 @@ int orc_sparse_solve(int n, const int* ap, const int* ai, const double* ax, cons
 // worker performs; with n_threads == 1 that is scenario 0's first Jacobian.
 // Returns 0; writes status[s] (0 converged, 1 max iterations, 2 singular), iters[s] and
 // history[s*max_iter + it].
+int orc_batch_solve_zip(int n, const int* rowptr, const int* col, const double* yre, const double* yim,
+                        const int* type, const double* zip, int n_scen, const double* p_spec, const double* q_spec,
+                        const double* vm0, const double* va0, int init, double tol, int max_iter, int n_threads,
+                        double* vm, double* va, int* status, int* iters, double* history);
+
 int orc_batch_solve(int n, const int* rowptr, const int* col, const double* yre, const double* yim, const int* type,
                     int n_scen, const double* p_spec, const double* q_spec, const double* vm0, const double* va0,
                     int init, double tol, int max_iter, int n_threads, double* vm, double* va, int* status,
                     int* iters, double* history) {
-  Net net{n, rowptr, col, yre, yim, type};
+  return orc_batch_solve_zip(n, rowptr, col, yre, yim, type, nullptr, n_scen, p_spec, q_spec, vm0, va0, init, tol,
+                             max_iter, n_threads, vm, va, status, iters, history);
+}
+
+// orc_batch_solve with ZIP terms (zip: 6 per bus as in Net, or null).
+int orc_batch_solve_zip(int n, const int* rowptr, const int* col, const double* yre, const double* yim,
+                        const int* type, const double* zip, int n_scen, const double* p_spec, const double* q_spec,
+                        const double* vm0, const double* va0, int init, double tol, int max_iter, int n_threads,
+                        double* vm, double* va, int* status, int* iters, double* history) {
+  Net net{n, rowptr, col, yre, yim, type, zip};
   Unknowns u = unknowns(net);
   std::atomic<int> next{0};
   auto worker = [&]() {
