@@ -129,7 +129,24 @@ struct DevTopo {
   const int* bwd_pos = nullptr;  // packed position of U(r, j)
   const int* bwd_col = nullptr;  // j
   int window = 0;                // window W the far lists were built for (ring rows W + kRingGap + 1)
+  // narrow tiles: back substitution by levels (null: column by column).  Steps of kBackItems
+  // items per lane group, item i of step q for group g at back_items[(q * kBackItems + i) * G + g];
+  // back_steps is a multiple of 2 kBackAhead; see kBack* for the encoding
+  const int2* back_items = nullptr;
+  int back_steps = 0;
 };
+
+// Back-substitution items {a, b}: a = kind << 29 | packed position; kinds
+//   kBackEntry: acc -= U(k, c) * x_c with U(k, c) at the position; b = c (x_c from y), or
+//               LDS slot | 1 << 31 (x_c from the ring row of that number)
+//   kBackStart: acc = y_k; b = k
+//   kBackEnd:   x_k = acc / U(k, k) with the pivot at the position; b = k | LDS slot << kBackSlotShift
+//   kBackPad:   nothing (position 0)
+// The kernel loads a step's values at the end of the step kBackAhead before it.
+constexpr int kBackItems = 4;
+constexpr int kBackAhead = 4;
+constexpr int kBackEntry = 0, kBackStart = 1, kBackEnd = 2, kBackPad = 3;
+constexpr int kBackSlotShift = 19;  // rows below 2^19, LDS slots below 2^12
 
 // The linear initial guess system of a topology (complex, m = number of PQ buses): recipes
 // for its packed L+U values and right-hand side.  Its LU pattern lives in a second DevTopo.

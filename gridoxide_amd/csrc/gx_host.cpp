@@ -309,9 +309,133 @@ int build_uniform_rounds(const gx::LuPattern& lu, const LuSchedule& s, std::vect
 }
 
 // Device copy of a pattern's numeric schedule, pointed to by the schedule fields of a DevTopo.
+// Back substitution by levels for narrow tiles (DevTopo::back_items).  x_k = (y_k - sum_c U(k,c) x_c)
+// / U(k,k) with the subtractions in descending c, the order in which the column-oriented back
+// substitution applies them (src/klu_native/solve.rs:35-43).  The level of row k is one more than the
+// highest level among the c of its U row (0 if none), so rows of a level depend only on earlier
+// levels.  A level's rows go to the G lane groups, longest first, each to the least loaded group, as
+// START, one ENTRY per U entry (descending c) and END; a level fills whole steps of kBackItems items
+// per group (PAD after a group's last row), and the stream is padded to a multiple of 2 kBackAhead steps.
+// Rows are numbered in the order they end (step, then group); an END stores x_k into y and into LDS
+// slot (number mod ring_rows).  An ENTRY of step s reads x_c from its LDS slot while fewer than
+// ring_rows rows ended after c before step s, else from y, which the kernel loads at the end of step
+// s - kBackAhead: c must have ended by then.  Returns false (no schedule) when the encoding's limits
+// are exceeded or a read would come too early.
+bool build_back_levels(const gx::LuPattern& lu, int G, int ring_rows, std::vector<int>& items, int& steps) {
+  const int n = lu.n, P = gx::kBackItems, D = gx::kBackAhead;
+  items.clear();
+  steps = 0;
+  if (n <= 0 || n >= (1 << gx::kBackSlotShift) || ring_rows < 1 || ring_rows > (1 << 12) || lu.nnz() >= (1 << 29))
+    return false;
+  // U by rows: row k -> (c, packed position of U(k, c)), descending c
+  std::vector<int> rptr(n + 1, 0);
+  for (int c = 0; c < n; ++c)
+    for (int q = lu.cp[c]; q < lu.cp[c] + lu.ulen[c]; ++q) rptr[lu.row[q] + 1]++;
+  for (int k = 0; k < n; ++k) rptr[k + 1] += rptr[k];
+  std::vector<int> rc(rptr[n]), rq(rptr[n]), cur(rptr.begin(), rptr.end() - 1);
+  for (int c = 0; c < n; ++c)
+    for (int q = lu.cp[c]; q < lu.cp[c] + lu.ulen[c]; ++q) {
+      const int k = lu.row[q];
+      rc[cur[k]] = c;
+      rq[cur[k]] = q;
+      ++cur[k];
+    }
+  for (int k = 0; k < n; ++k) {  // filled in ascending c: reverse each row
+    std::reverse(rc.begin() + rptr[k], rc.begin() + rptr[k + 1]);
+    std::reverse(rq.begin() + rptr[k], rq.begin() + rptr[k + 1]);
+  }
+  std::vector<int> lev(n, 0);
+  int nlev = 0;
+  for (int k = n - 1; k >= 0; --k) {
+    int l = 0;
+    for (int e = rptr[k]; e < rptr[k + 1]; ++e) l = std::max(l, lev[rc[e]] + 1);
+    lev[k] = l;
+    nlev = std::max(nlev, l + 1);
+  }
+  std::vector<std::vector<int>> bucket(nlev);
+  for (int k = n - 1; k >= 0; --k) bucket[lev[k]].push_back(k);
+  // layout: per step, item slot, group: a row id (START / ENTRY e / END) or none
+  struct Cell {
+    int kind = gx::kBackPad, k = 0, e = 0;
+  };
+  std::vector<Cell> cells;  // (step * P + i) * G + g
+  std::vector<int> end_step(n, -1), end_group(n, -1);
+  std::vector<std::vector<Cell>> gl(G);
+  std::vector<long long> load(G);
+  int nst = 0;
+  for (int l = 0; l < nlev; ++l) {
+    std::vector<int> rows = bucket[l];
+    std::stable_sort(rows.begin(), rows.end(),
+                     [&](int a, int b) { return rptr[a + 1] - rptr[a] > rptr[b + 1] - rptr[b]; });
+    for (int g = 0; g < G; ++g) {
+      gl[g].clear();
+      load[g] = 0;
+    }
+    for (int k : rows) {
+      const int g = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+      gl[g].push_back({gx::kBackStart, k, 0});
+      for (int e = rptr[k]; e < rptr[k + 1]; ++e) gl[g].push_back({gx::kBackEntry, k, e});
+      gl[g].push_back({gx::kBackEnd, k, 0});
+      load[g] += rptr[k + 1] - rptr[k] + 2;
+    }
+    const long long M = *std::max_element(load.begin(), load.end());
+    const int ls = (int)((M + P - 1) / P);
+    cells.resize((size_t)(nst + ls) * P * G);
+    for (int g = 0; g < G; ++g)
+      for (size_t idx = 0; idx < gl[g].size(); ++idx) {
+        const int q = nst + (int)(idx / P), i = (int)(idx % P);
+        cells[((size_t)q * P + i) * G + g] = gl[g][idx];
+        if (gl[g][idx].kind == gx::kBackEnd) {
+          end_step[gl[g][idx].k] = q;
+          end_group[gl[g][idx].k] = g;
+        }
+      }
+    nst += ls;
+  }
+  const int total = (nst + 2 * D - 1) / (2 * D) * (2 * D);
+  cells.resize((size_t)total * P * G);
+  // numbering in the order rows end; ended_before[s] = rows that ended in steps before s
+  std::vector<int> order(n), fin(n);
+  for (int k = 0; k < n; ++k) order[k] = k;
+  std::sort(order.begin(), order.end(), [&](int a, int b) {
+    return end_step[a] != end_step[b] ? end_step[a] < end_step[b] : end_group[a] < end_group[b];
+  });
+  for (int i = 0; i < n; ++i) fin[order[i]] = i;
+  std::vector<int> ended_before(total + 1, 0);
+  for (int k = 0; k < n; ++k) ended_before[end_step[k] + 1]++;
+  for (int q = 0; q < total; ++q) ended_before[q + 1] += ended_before[q];
+  items.resize(2 * cells.size());
+  for (size_t x = 0; x < cells.size(); ++x) {
+    const Cell& c = cells[x];
+    const int s = (int)(x / ((size_t)P * G));
+    int a = gx::kBackPad << 29, b = 0;
+    if (c.kind == gx::kBackStart) {
+      a = gx::kBackStart << 29;
+      b = c.k;
+    } else if (c.kind == gx::kBackEnd) {
+      a = (gx::kBackEnd << 29) | (lu.cp[c.k] + lu.ulen[c.k]);
+      b = c.k | ((fin[c.k] % ring_rows) << gx::kBackSlotShift);
+    } else if (c.kind == gx::kBackEntry) {
+      const int src = rc[c.e];
+      a = (gx::kBackEntry << 29) | rq[c.e];
+      if (ended_before[s] <= fin[src] + ring_rows - 1) {
+        b = (int)(0x80000000u | (unsigned)(fin[src] % ring_rows));
+      } else {
+        if (end_step[src] > s - D) return false;  // the kernel loads it at the end of step s - D
+        b = src;
+      }
+    }
+    items[2 * x] = a;
+    items[2 * x + 1] = b;
+  }
+  steps = total;
+  return true;
+}
+
 struct DevSched {
   DevMem cp, ulen, row, chunk0, rounds, slots, fptr, fpos, fcol, bptr, bpos, bcol, col_a, col_f, col_b, fslot, opos, ocol;
-  DevMem uround, uitems, bslot;
+  DevMem uround, uitems, bslot, bitems;
+  int back_steps = 0;
   std::vector<int> h_chunk0, h_rounds, h_fptr, h_bptr;
   bool uploaded = false;
   int window = -1;
@@ -456,6 +580,12 @@ int upload_sched(const gx::LuPattern& lu, const LuSchedule& s, int window, int t
         (rc = m.fslot.upload(fslot)) || (rc = m.bslot.upload(bslot)) || (rc = m.opos.upload(opos)) ||
         (rc = m.ocol.upload(ocol)))
       return rc;
+    // the back substitution's LDS slots are the solve ring's rows (W + kRingGap + 1)
+    std::vector<int> bitems;
+    m.back_steps = 0;
+    if (build_back_levels(lu, G, window + gx::kRingGap + 1, bitems, m.back_steps)) {
+      if ((rc = m.bitems.upload(bitems))) return rc;
+    }
   }
   d.n = lu.n;
   d.nnz_lu = lu.nnz();
@@ -483,6 +613,8 @@ int upload_sched(const gx::LuPattern& lu, const LuSchedule& s, int window, int t
   d.bwd_pos = m.bpos.as<int>();
   d.bwd_col = m.bcol.as<int>();
   d.window = window;
+  d.back_items = tile != gx::kWideTile && m.back_steps > 0 ? m.bitems.as<int2>() : nullptr;
+  d.back_steps = d.back_items ? m.back_steps : 0;
   return GX_OK;
 }
 
@@ -1521,6 +1653,19 @@ int gx_topo_lu_pattern(const gx_topo* t, int* cp, int* ulen, int* row) {
   if (ulen) std::memcpy(ulen, t->lu.ulen.data(), sizeof(int) * t->lu.ulen.size());
   if (row) std::memcpy(row, t->lu.row.data(), sizeof(int) * t->lu.row.size());
   return GX_OK;
+}
+
+int gx_topo_back_levels(const gx_topo* t, int tile, int ring_rows, int* items, long long cap, int* steps) {
+  if (!t || (tile != 4 && tile != 8 && tile != 16) || ring_rows < 2) return fail(GX_ERR_ARG, "bad arguments");
+  if (!t->factored) return fail(GX_ERR_STATE, "not factored yet");
+  std::vector<int> it;
+  int st = 0;
+  if (!build_back_levels(t->lu, gx::kLanes / tile, ring_rows, it, st))
+    return fail(GX_ERR_UNSUPPORTED, "system beyond the level schedule's encoding");
+  if (steps) *steps = st;
+  if ((long long)it.size() > cap || !items) return -(int)std::min<size_t>(it.size(), 0x7fffffff);
+  std::memcpy(items, it.data(), sizeof(int) * it.size());
+  return (int)it.size();
 }
 
 int gx_topo_set_zip(gx_topo* t, const double* zip) {

@@ -99,6 +99,10 @@ __device__ __forceinline__ bool is_zero(zd v) { return v.re == 0.0 && v.im == 0.
 __device__ __forceinline__ bool finite_value(float v) { return isfinite(v); }
 __device__ __forceinline__ bool finite_value(double v) { return isfinite(v); }
 __device__ __forceinline__ bool finite_value(zd v) { return isfinite(v.re) && isfinite(v.im); }
+// Keeps the compiler from moving memory accesses across this point.  Lanes of a wavefront that pass
+// values through memory rely on the program order of the accesses; without it the compiler may, for
+// example, sink a load into the branch that uses it, past another lane's store.
+__device__ __forceinline__ void memory_order_point() { asm volatile("" ::: "memory"); }
 template <typename T>
 __device__ __forceinline__ T zero_value() { return T(0); }
 template <>
@@ -759,7 +763,89 @@ __device__ __forceinline__ void lu_tile(const DevTopo& t, const DevWork& w, int 
   workgroup_fence();
 
   // ------------------------------------------------ back substitution
-  if (!overflow) {
+  if (!overflow && t.back_items) {
+    // by levels (item encoding at kBackItems): every lane group runs its rows of a level as one
+    // dependent chain in registers, the levels one after another.  Four register sets take turns:
+    // after a set's step has run, it loads the values of the step kBackAhead later (their
+    // descriptors were loaded kBackAhead steps earlier still), so no load is waited for soon after
+    // it was issued and no in-flight load is copied between registers.  x of rows that ended
+    // recently comes from the LDS ring, older x from y
+    constexpr int P = kBackItems, D = kBackAhead;
+    static_assert(D == 4, "the step loop below is unrolled for four register sets");
+    constexpr int kPos = (1 << 29) - 1, kRow = (1 << kBackSlotShift) - 1, kSlot = (1 << 12) - 1;
+    const int2* __restrict__ bib = t.back_items + g;
+    const int nst = t.back_steps;
+    // a set holds the values of its step and the descriptors of that step and of the step D later in
+    // da / db, whose roles alternate from one turn of the set to the next (so that the loop needs no
+    // register copies, which would wait for the loads in flight)
+    struct BSet {
+      int2 da[P], db[P];
+      T u[P], y[P];
+    };
+    auto dsc_issue = [&](int q, int2 (&d)[P]) {
+      const int qq = q < nst ? q : nst - 1;
+#pragma unroll
+      for (int i = 0; i < P; ++i) d[i] = bib[(unsigned)((qq * P + i) * G)];
+    };
+    auto val_issue = [&](const int2 (&d)[P], BSet& b) {
+#pragma unroll
+      for (int i = 0; i < P; ++i) {
+        b.u[i] = LU(d[i].x & kPos);
+        b.y[i] = Y(d[i].y >= 0 ? (d[i].y & kRow) : 0);
+      }
+    };
+    T acc = zero;
+    // step q with the set's values and descriptors cur; then the values of step q + D (descriptors
+    // nxt) and the descriptors of step q + 2D into cur
+    auto run = [&](int q, BSet& b, int2 (&cur)[P], int2 (&nxt)[P]) __attribute__((always_inline)) {
+      T lx[P];
+#pragma unroll
+      for (int i = 0; i < P; ++i) lx[i] = ring[(cur[i].y < 0 ? (cur[i].y & kSlot) : 0) * S + s];
+      memory_order_point();  // every LDS read of the step before any of its writes
+#pragma unroll
+      for (int i = 0; i < P; ++i) {
+        const int a = cur[i].x, bb = cur[i].y;
+        const int kind = (int)((unsigned)a >> 29);
+        if (kind == kBackStart) {
+          acc = b.y[i];
+        } else if (kind == kBackEntry) {
+          acc = acc - b.u[i] * (bb < 0 ? lx[i] : b.y[i]);
+        } else if (kind == kBackEnd) {
+          const T xk = acc / b.u[i];
+          if (!finite_value(xk)) ok = false;
+          Y(bb & kRow) = xk;
+          ring[((bb >> kBackSlotShift) & kSlot) * S + s] = xk;
+        }
+      }
+      memory_order_point();  // the step's stores before the next loads
+      val_issue(nxt, b);
+      dsc_issue(q + 2 * D, cur);
+    };
+    BSet S0, S1, S2, S3;
+    dsc_issue(0, S0.da);
+    dsc_issue(1, S1.da);
+    dsc_issue(2, S2.da);
+    dsc_issue(3, S3.da);
+    dsc_issue(D, S0.db);
+    dsc_issue(D + 1, S1.db);
+    dsc_issue(D + 2, S2.db);
+    dsc_issue(D + 3, S3.db);
+    val_issue(S0.da, S0);
+    val_issue(S1.da, S1);
+    val_issue(S2.da, S2);
+    val_issue(S3.da, S3);
+    drain();  // (as before the forward loop)
+    for (int q = 0; q < nst; q += 2 * D) {
+      run(q, S0, S0.da, S0.db);
+      run(q + 1, S1, S1.da, S1.db);
+      run(q + 2, S2, S2.da, S2.db);
+      run(q + 3, S3, S3.da, S3.db);
+      run(q + 4, S0, S0.db, S0.da);
+      run(q + 5, S1, S1.db, S1.da);
+      run(q + 6, S2, S2.db, S2.da);
+      run(q + 7, S3, S3.db, S3.da);
+    }
+  } else if (!overflow) {
     strided(n - 1 - W > 0 ? n - 1 - W : 0, g, n, [&](int r) { ring[slot(r)] = Y(r); });
     // column metadata three columns ahead; the U entries, the pivot, the entering
     // row's y and its first far entries two columns ahead, in two register sets that alternate

@@ -100,6 +100,13 @@ int gx_topo_ordering(const gx_topo* t, int* p, int* q, int* r, int* pnum);
  * row[nnz(L+U)] row of each entry (column k: U rows ascending, diagonal, L rows ascending). */
 int gx_topo_lu_pattern(const gx_topo* t, int* cp, int* ulen, int* row);
 
+/* The level schedule of the back substitution that narrow tiles of `tile` scenarios use with a
+ * solve ring of ring_rows rows (no counterpart in the reference; exposed so that tests can run it
+ * on the CPU): 2 ints per item, items of step q, slot i, lane group g at index (q * 4 + i) * G + g
+ * (G = 64 / tile); *steps = the number of steps.  Returns the ints written, or -(ints needed) when
+ * cap is too small or items is NULL. */
+int gx_topo_back_levels(const gx_topo* t, int tile, int ring_rows, int* items, long long cap, int* steps);
+
 /* ---- batched Newton-Raphson ---- */
 /* Replaces BatchSolver::solve (src/batch.rs:198-289) driving PersistentSolver::solve_with_stats
  * (src/solver.rs:444-475) / newton_raphson_cached (src/solver.rs:639-736) once per scenario.
