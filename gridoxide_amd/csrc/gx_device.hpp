@@ -136,17 +136,18 @@ struct DevTopo {
   int back_steps = 0;
 };
 
-// Back-substitution items {a, b}: a = kind << 29 | packed position; kinds
-//   kBackEntry: acc -= U(k, c) * x_c with U(k, c) at the position; b = c (x_c from y), or
-//               LDS slot | 1 << 31 (x_c from the ring row of that number)
-//   kBackStart: acc = y_k; b = k
-//   kBackEnd:   x_k = acc / U(k, k) with the pivot at the position; b = k | LDS slot << kBackSlotShift
-//   kBackPad:   nothing (position 0)
+// Back-substitution items {a, b} for a tile of TW scenarios (ring row stride S = TW + 1): a = kind << 29
+// | payload, and
+//   kBackEntry: acc -= U(k, c) * x_c; payload = position of U(k, c); b = c * TW (x_c from y), or
+//               1 << 31 | slot * S (x_c from the LDS ring row `slot`)
+//   kBackStart: acc = y_k; payload = slot * S of the ring row that will hold x_k; b = k * TW
+//   kBackEnd:   x_k = acc / U(k, k) into y and that ring row; payload = position of U(k, k); b = k * TW
+//               (at most one END per lane group and step)
+//   kBackPad:   nothing; payload 0, b = 0
 // The kernel loads a step's values at the end of the step kBackAhead before it.
 constexpr int kBackItems = 4;
 constexpr int kBackAhead = 4;
 constexpr int kBackEntry = 0, kBackStart = 1, kBackEnd = 2, kBackPad = 3;
-constexpr int kBackSlotShift = 19;  // rows below 2^19, LDS slots below 2^12
 
 // The linear initial guess system of a topology (complex, m = number of PQ buses): recipes
 // for its packed L+U values and right-hand side.  Its LU pattern lives in a second DevTopo.

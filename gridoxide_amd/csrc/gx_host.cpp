@@ -314,18 +314,20 @@ int build_uniform_rounds(const gx::LuPattern& lu, const LuSchedule& s, std::vect
 // substitution applies them (src/klu_native/solve.rs:35-43).  The level of row k is one more than the
 // highest level among the c of its U row (0 if none), so rows of a level depend only on earlier
 // levels.  A level's rows go to the G lane groups, longest first, each to the least loaded group, as
-// START, one ENTRY per U entry (descending c) and END; a level fills whole steps of kBackItems items
-// per group (PAD after a group's last row), and the stream is padded to a multiple of 2 kBackAhead steps.
+// START, one ENTRY per U entry (descending c) and END, a row moved to the next step where its END would
+// share a step with the group's previous END; a level fills whole steps of kBackItems items per group
+// (PAD after a group's last row), and the stream is padded to a multiple of 2 kBackAhead steps.
 // Rows are numbered in the order they end (step, then group); an END stores x_k into y and into LDS
-// slot (number mod ring_rows).  An ENTRY of step s reads x_c from its LDS slot while fewer than
+// ring row (number mod ring_rows), which the row's START names.  An ENTRY of step s reads x_c from its LDS slot while fewer than
 // ring_rows rows ended after c before step s, else from y, which the kernel loads at the end of step
 // s - kBackAhead: c must have ended by then.  Returns false (no schedule) when the encoding's limits
 // are exceeded or a read would come too early.
 bool build_back_levels(const gx::LuPattern& lu, int G, int ring_rows, std::vector<int>& items, int& steps) {
-  const int n = lu.n, P = gx::kBackItems, D = gx::kBackAhead;
+  const int n = lu.n, P = gx::kBackItems, D = gx::kBackAhead, TW = gx::kLanes / G, S = TW + 1;
   items.clear();
   steps = 0;
-  if (n <= 0 || n >= (1 << gx::kBackSlotShift) || ring_rows < 1 || ring_rows > (1 << 12) || lu.nnz() >= (1 << 29))
+  if (n <= 0 || (long long)n * TW >= (1LL << 31) || ring_rows < 1 || (long long)ring_rows * S >= (1LL << 29) ||
+      lu.nnz() >= (1 << 29))
     return false;
   // U by rows: row k -> (c, packed position of U(k, c)), descending c
   std::vector<int> rptr(n + 1, 0);
@@ -362,6 +364,7 @@ bool build_back_levels(const gx::LuPattern& lu, int G, int ring_rows, std::vecto
   std::vector<int> end_step(n, -1), end_group(n, -1);
   std::vector<std::vector<Cell>> gl(G);
   std::vector<long long> load(G);
+  std::vector<size_t> last_end(G, 0);
   int nst = 0;
   for (int l = 0; l < nlev; ++l) {
     std::vector<int> rows = bucket[l];
@@ -370,13 +373,19 @@ bool build_back_levels(const gx::LuPattern& lu, int G, int ring_rows, std::vecto
     for (int g = 0; g < G; ++g) {
       gl[g].clear();
       load[g] = 0;
+      last_end[g] = 0;
     }
     for (int k : rows) {
       const int g = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+      // at most one END per group and step: the kernel divides once per step
+      const size_t len = (size_t)(rptr[k + 1] - rptr[k]) + 2;
+      if (!gl[g].empty() && (gl[g].size() + len - 1) / P == (last_end[g]) / P)
+        gl[g].resize((gl[g].size() + P - 1) / P * P);
       gl[g].push_back({gx::kBackStart, k, 0});
       for (int e = rptr[k]; e < rptr[k + 1]; ++e) gl[g].push_back({gx::kBackEntry, k, e});
       gl[g].push_back({gx::kBackEnd, k, 0});
-      load[g] += rptr[k + 1] - rptr[k] + 2;
+      last_end[g] = gl[g].size() - 1;
+      load[g] = (long long)gl[g].size();
     }
     const long long M = *std::max_element(load.begin(), load.end());
     const int ls = (int)((M + P - 1) / P);
@@ -410,19 +419,19 @@ bool build_back_levels(const gx::LuPattern& lu, int G, int ring_rows, std::vecto
     const int s = (int)(x / ((size_t)P * G));
     int a = gx::kBackPad << 29, b = 0;
     if (c.kind == gx::kBackStart) {
-      a = gx::kBackStart << 29;
-      b = c.k;
+      a = (gx::kBackStart << 29) | (fin[c.k] % ring_rows) * S;
+      b = c.k * TW;
     } else if (c.kind == gx::kBackEnd) {
       a = (gx::kBackEnd << 29) | (lu.cp[c.k] + lu.ulen[c.k]);
-      b = c.k | ((fin[c.k] % ring_rows) << gx::kBackSlotShift);
+      b = c.k * TW;
     } else if (c.kind == gx::kBackEntry) {
       const int src = rc[c.e];
       a = (gx::kBackEntry << 29) | rq[c.e];
       if (ended_before[s] <= fin[src] + ring_rows - 1) {
-        b = (int)(0x80000000u | (unsigned)(fin[src] % ring_rows));
+        b = (int)(0x80000000u | (unsigned)((fin[src] % ring_rows) * S));
       } else {
         if (end_step[src] > s - D) return false;  // the kernel loads it at the end of step s - D
-        b = src;
+        b = src * TW;
       }
     }
     items[2 * x] = a;

@@ -772,9 +772,12 @@ __device__ __forceinline__ void lu_tile(const DevTopo& t, const DevWork& w, int 
     // recently comes from the LDS ring, older x from y
     constexpr int P = kBackItems, D = kBackAhead;
     static_assert(D == 4, "the step loop below is unrolled for four register sets");
-    constexpr int kPos = (1 << 29) - 1, kRow = (1 << kBackSlotShift) - 1, kSlot = (1 << 12) - 1;
+    constexpr unsigned kPayload = (1u << 29) - 1;
+    const unsigned pos_max = (unsigned)(nl - 1);
     const int2* __restrict__ bib = t.back_items + g;
     const int nst = t.back_steps;
+    T* __restrict__ yl = yb + s;  // this lane's y; item offsets are row * TW
+    T* rl = ring + s;             // this lane's ring; item offsets are slot * S
     // a set holds the values of its step and the descriptors of that step and of the step D later in
     // da / db, whose roles alternate from one turn of the set to the next (so that the loop needs no
     // register copies, which would wait for the loads in flight)
@@ -790,32 +793,47 @@ __device__ __forceinline__ void lu_tile(const DevTopo& t, const DevWork& w, int 
     auto val_issue = [&](const int2 (&d)[P], BSet& b) {
 #pragma unroll
       for (int i = 0; i < P; ++i) {
-        b.u[i] = LU(d[i].x & kPos);
-        b.y[i] = Y(d[i].y >= 0 ? (d[i].y & kRow) : 0);
+        // (a START's payload is an LDS offset: clamped into the L+U range, the value is unused)
+        const unsigned p = (unsigned)d[i].x & kPayload;
+        b.u[i] = LU((int)(p < pos_max ? p : pos_max));
+        b.y[i] = yl[(unsigned)(d[i].y >= 0 ? d[i].y : 0)];
       }
     };
     T acc = zero;
+    unsigned xslot = 0;  // ring offset of the row being run (from its START)
     // step q with the set's values and descriptors cur; then the values of step q + D (descriptors
     // nxt) and the descriptors of step q + 2D into cur
     auto run = [&](int q, BSet& b, int2 (&cur)[P], int2 (&nxt)[P]) __attribute__((always_inline)) {
       T lx[P];
 #pragma unroll
-      for (int i = 0; i < P; ++i) lx[i] = ring[(cur[i].y < 0 ? (cur[i].y & kSlot) : 0) * S + s];
+      for (int i = 0; i < P; ++i) lx[i] = rl[(unsigned)(cur[i].y < 0 ? cur[i].y & kPayload : 0)];
       memory_order_point();  // every LDS read of the step before any of its writes
+      T pend = zero, pend_u = zero;
+      unsigned pend_y = 0, pend_s = 0;
+      bool ended = false;
 #pragma unroll
       for (int i = 0; i < P; ++i) {
-        const int a = cur[i].x, bb = cur[i].y;
-        const int kind = (int)((unsigned)a >> 29);
-        if (kind == kBackStart) {
-          acc = b.y[i];
-        } else if (kind == kBackEntry) {
-          acc = acc - b.u[i] * (bb < 0 ? lx[i] : b.y[i]);
-        } else if (kind == kBackEnd) {
-          const T xk = acc / b.u[i];
-          if (!finite_value(xk)) ok = false;
-          Y(bb & kRow) = xk;
-          ring[((bb >> kBackSlotShift) & kSlot) * S + s] = xk;
-        }
+        const unsigned a = (unsigned)cur[i].x;
+        const int bb = cur[i].y;
+        const unsigned kind = a >> 29;
+        const bool start = kind == (unsigned)kBackStart, entry = kind == (unsigned)kBackEntry;
+        xslot = start ? (a & kPayload) : xslot;
+        acc = start ? b.y[i] : acc;
+        const T xv = entry ? (bb < 0 ? lx[i] : b.y[i]) : zero;
+        const T uv = entry ? b.u[i] : zero;
+        acc = acc - uv * xv;
+        const bool end = kind == (unsigned)kBackEnd;  // (at most one per lane group and step)
+        pend = end ? acc : pend;
+        pend_u = end ? b.u[i] : pend_u;
+        pend_y = end ? (unsigned)bb : pend_y;
+        pend_s = end ? xslot : pend_s;
+        ended = ended || end;
+      }
+      if (ended) {  // the step's row end: x_k = acc / U(k, k) into y and the ring
+        const T xk = pend / pend_u;
+        if (!finite_value(xk)) ok = false;
+        yl[pend_y] = xk;
+        rl[pend_s] = xk;
       }
       memory_order_point();  // the step's stores before the next loads
       val_issue(nxt, b);

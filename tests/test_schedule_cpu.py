@@ -1,7 +1,7 @@
 """The LU kernel's back-substitution level schedule (gx_topo_back_levels), run on the CPU by an
 emulation of the kernel's item loop with its memory timing: the values of a step are read at the end
 of the step four before it (the first four steps' before the loop), LDS values at the start of the
-step, and every END writes x to y and to its LDS ring slot as the kernel does.  A dependency the schedule got wrong reads a stale value
+step, and every END writes x to y and to the LDS ring row its START named, as the kernel does.  A dependency the schedule got wrong reads a stale value
 and changes the answer.  The answer must equal, bit for bit, the column-oriented back substitution
 (x_k = y_k / U(k,k), then y_r -= U(r,k) x_k for the U rows r of column k, k descending), which
 applies every row's subtractions in the same order."""
@@ -14,9 +14,7 @@ from gridoxide_amd._lib import check, ptr
 
 P = 4                   # items per lane group and step (kBackItems)
 D = 4                   # steps between loading a step's values and running it (kBackAhead)
-POS = (1 << 29) - 1
-ROW = (1 << 19) - 1
-SLOT = (1 << 12) - 1
+POS = (1 << 29) - 1     # payload of the first word
 ENTRY, START, END, PAD = 0, 1, 2, 3
 
 
@@ -50,35 +48,42 @@ def _column_oriented(cp, ulen, row, lu, y):
 
 
 def _emulate(items, G, lu, y, ring_rows):
+    """Runs the items as the kernel does for one scenario (offsets are row * TW and slot * S)."""
     steps = items.shape[0]
-    assert steps % D == 0
+    assert steps % (2 * D) == 0
+    TW = 64 // G
+    S = TW + 1
     mem = y.copy()
-    lds = np.full(ring_rows, np.nan)
+    lds = np.full(ring_rows * S, np.nan)
+    first = items[..., 0].astype(np.int64) & 0xFFFFFFFF
+    kinds = first >> 29
+    payload = first & POS
 
     def values(q):
-        d = items[min(q, steps - 1)]
-        u = lu[d[..., 0] & POS]
-        idx = np.where(d[..., 1] >= 0, d[..., 1] & ROW, 0)
-        return u, mem[idx]
+        q = min(q, steps - 1)
+        u = lu[np.minimum(payload[q], lu.size - 1)]
+        b = items[q][..., 1]
+        return u, mem[np.where(b >= 0, b, 0) // TW]
 
     acc = np.zeros(G)
+    xslot = np.zeros(G, np.int64)
     loaded = {q: values(q) for q in range(min(D, steps))}
     for q in range(steps):
-        d = items[q]
-        lx = lds[np.where(d[..., 1] < 0, d[..., 1] & SLOT, 0)]
+        b_all = items[q][..., 1]
+        lx = lds[np.where(b_all < 0, b_all & POS, 0)]
         u, yv = loaded.pop(q)
         for i in range(P):
             for g in range(G):
-                a, b = int(d[i, g, 0]), int(d[i, g, 1])
-                kind = (a & 0xFFFFFFFF) >> 29
+                kind, b = int(kinds[q, i, g]), int(b_all[i, g])
                 if kind == START:
+                    xslot[g] = payload[q, i, g]
                     acc[g] = yv[i, g]
                 elif kind == ENTRY:
                     acc[g] = acc[g] - u[i, g] * (lx[i, g] if b < 0 else yv[i, g])
                 elif kind == END:
                     xk = acc[g] / u[i, g]
-                    mem[b & ROW] = xk
-                    lds[(b >> 19) & SLOT] = xk
+                    mem[b // TW] = xk
+                    lds[xslot[g]] = xk
         loaded[q + D] = values(q + D)  # the set refilled after this step
     return mem
 
@@ -101,9 +106,11 @@ def test_back_levels_equal_column_oriented(grid, tile, ring_rows):
     assert np.array_equal(got, want)
     # every row ends exactly once, every U entry is used exactly once
     kinds = (items[..., 0].astype(np.int64) & 0xFFFFFFFF) >> 29
-    ends = items[..., 1][kinds == END] & ROW
+    ends = items[..., 1][kinds == END] // (64 // G)
     assert np.array_equal(np.sort(ends), np.arange(n))
     assert np.count_nonzero(kinds == ENTRY) == int(ulen.sum())
+    # at most one END per lane group and step (the kernel divides once per step)
+    assert (np.count_nonzero(kinds == END, axis=1) <= 1).all()
     # on the workload grids, fewer steps than columns: the rows of a level run in parallel lane groups
     if grid != "g":
         assert items.shape[0] < n
