@@ -141,9 +141,10 @@ struct DevTopo {
 //   kBackEntry: acc -= U(k, c) * x_c; payload = position of U(k, c); b = c * TW (x_c from y), or
 //               1 << 31 | slot * S (x_c from the LDS ring row `slot`)
 //   kBackStart: acc = y_k; payload = slot * S of the ring row that will hold x_k; b = k * TW
-//   kBackEnd:   x_k = acc / U(k, k) into y and that ring row; payload = position of U(k, k); b = k * TW
-//               (at most one END per lane group and step)
-//   kBackPad:   nothing; payload 0, b = 0
+//   kBackEnd:   x_k = acc / U(k, k) into y and that ring row; payload = position of U(k, k); b = the
+//               zero row (at most one END per lane group and step)
+//   kBackPad:   nothing; payload 0, b = the zero row
+// The zero row is 1 << 31 | (R - 1) * S, the last of the R ring rows, which the kernel clears.
 // The kernel loads a step's values at the end of the step kBackAhead before it.
 constexpr int kBackItems = 4;
 constexpr int kBackAhead = 4;

@@ -318,17 +318,20 @@ int build_uniform_rounds(const gx::LuPattern& lu, const LuSchedule& s, std::vect
 // share a step with the group's previous END; a level fills whole steps of kBackItems items per group
 // (PAD after a group's last row), and the stream is padded to a multiple of 2 kBackAhead steps.
 // Rows are numbered in the order they end (step, then group); an END stores x_k into y and into LDS
-// ring row (number mod ring_rows), which the row's START names.  An ENTRY of step s reads x_c from its LDS slot while fewer than
-// ring_rows rows ended after c before step s, else from y, which the kernel loads at the end of step
-// s - kBackAhead: c must have ended by then.  Returns false (no schedule) when the encoding's limits
+// ring row (number mod (ring_rows - 1)), which the row's START names; the last ring row holds zeros,
+// the x that END and PAD items multiply by.  An ENTRY of step s reads x_c from its ring row while
+// fewer than ring_rows - 1 rows ended after c before step s, else from y, which the kernel loads at the
+// end of step s - kBackAhead: c must have ended by then.  Returns false (no schedule) when the encoding's limits
 // are exceeded or a read would come too early.
 bool build_back_levels(const gx::LuPattern& lu, int G, int ring_rows, std::vector<int>& items, int& steps) {
   const int n = lu.n, P = gx::kBackItems, D = gx::kBackAhead, TW = gx::kLanes / G, S = TW + 1;
   items.clear();
   steps = 0;
-  if (n <= 0 || (long long)n * TW >= (1LL << 31) || ring_rows < 1 || (long long)ring_rows * S >= (1LL << 29) ||
+  if (n <= 0 || (long long)n * TW >= (1LL << 31) || ring_rows < 2 || (long long)ring_rows * S >= (1LL << 29) ||
       lu.nnz() >= (1 << 29))
     return false;
+  const int xrows = ring_rows - 1;                                   // ring rows for x values
+  const int zero_b = (int)(0x80000000u | (unsigned)(xrows * S));     // the last ring row holds zeros
   // U by rows: row k -> (c, packed position of U(k, c)), descending c
   std::vector<int> rptr(n + 1, 0);
   for (int c = 0; c < n; ++c)
@@ -417,18 +420,17 @@ bool build_back_levels(const gx::LuPattern& lu, int G, int ring_rows, std::vecto
   for (size_t x = 0; x < cells.size(); ++x) {
     const Cell& c = cells[x];
     const int s = (int)(x / ((size_t)P * G));
-    int a = gx::kBackPad << 29, b = 0;
+    int a = gx::kBackPad << 29, b = zero_b;
     if (c.kind == gx::kBackStart) {
-      a = (gx::kBackStart << 29) | (fin[c.k] % ring_rows) * S;
+      a = (gx::kBackStart << 29) | (fin[c.k] % xrows) * S;
       b = c.k * TW;
     } else if (c.kind == gx::kBackEnd) {
       a = (gx::kBackEnd << 29) | (lu.cp[c.k] + lu.ulen[c.k]);
-      b = c.k * TW;
     } else if (c.kind == gx::kBackEntry) {
       const int src = rc[c.e];
       a = (gx::kBackEntry << 29) | rq[c.e];
-      if (ended_before[s] <= fin[src] + ring_rows - 1) {
-        b = (int)(0x80000000u | (unsigned)((fin[src] % ring_rows) * S));
+      if (ended_before[s] <= fin[src] + xrows - 1) {
+        b = (int)(0x80000000u | (unsigned)((fin[src] % xrows) * S));
       } else {
         if (end_step[src] > s - D) return false;  // the kernel loads it at the end of step s - D
         b = src * TW;

@@ -800,7 +800,8 @@ __device__ __forceinline__ void lu_tile(const DevTopo& t, const DevWork& w, int 
       }
     };
     T acc = zero;
-    unsigned xslot = 0;  // ring offset of the row being run (from its START)
+    unsigned xslot = 0, xrow = 0;  // ring and y offsets of the row being run (from its START)
+    rl[(unsigned)((R - 1) * S)] = zero;  // the zero row (END and PAD items multiply by it)
     // step q with the set's values and descriptors cur; then the values of step q + D (descriptors
     // nxt) and the descriptors of step q + 2D into cur
     auto run = [&](int q, BSet& b, int2 (&cur)[P], int2 (&nxt)[P]) __attribute__((always_inline)) {
@@ -816,16 +817,17 @@ __device__ __forceinline__ void lu_tile(const DevTopo& t, const DevWork& w, int 
         const unsigned a = (unsigned)cur[i].x;
         const int bb = cur[i].y;
         const unsigned kind = a >> 29;
-        const bool start = kind == (unsigned)kBackStart, entry = kind == (unsigned)kBackEntry;
+        const bool start = kind == (unsigned)kBackStart;
         xslot = start ? (a & kPayload) : xslot;
-        acc = start ? b.y[i] : acc;
-        const T xv = entry ? (bb < 0 ? lx[i] : b.y[i]) : zero;
-        const T uv = entry ? b.u[i] : zero;
-        acc = acc - uv * xv;
+        xrow = start ? (unsigned)bb : xrow;
+        // ENTRY: acc - U(k, c) x_c; END and PAD read the zero row (acc unchanged); START: y_k
+        const T xv = bb < 0 ? lx[i] : b.y[i];
+        const T v = acc - b.u[i] * xv;
+        acc = start ? b.y[i] : v;
         const bool end = kind == (unsigned)kBackEnd;  // (at most one per lane group and step)
         pend = end ? acc : pend;
         pend_u = end ? b.u[i] : pend_u;
-        pend_y = end ? (unsigned)bb : pend_y;
+        pend_y = end ? xrow : pend_y;
         pend_s = end ? xslot : pend_s;
         ended = ended || end;
       }

@@ -1,7 +1,8 @@
 """The LU kernel's back-substitution level schedule (gx_topo_back_levels), run on the CPU by an
 emulation of the kernel's item loop with its memory timing: the values of a step are read at the end
 of the step four before it (the first four steps' before the loop), LDS values at the start of the
-step, and every END writes x to y and to the LDS ring row its START named, as the kernel does.  A dependency the schedule got wrong reads a stale value
+step, and every row's x goes to y and to the LDS ring row its START named at the end of its END's
+step, as the kernel does.  A dependency the schedule got wrong reads a stale value
 and changes the answer.  The answer must equal, bit for bit, the column-oriented back substitution
 (x_k = y_k / U(k,k), then y_r -= U(r,k) x_k for the U rows r of column k, k descending), which
 applies every row's subtractions in the same order."""
@@ -55,6 +56,7 @@ def _emulate(items, G, lu, y, ring_rows):
     S = TW + 1
     mem = y.copy()
     lds = np.full(ring_rows * S, np.nan)
+    lds[(ring_rows - 1) * S] = 0.0  # the zero row
     first = items[..., 0].astype(np.int64) & 0xFFFFFFFF
     kinds = first >> 29
     payload = first & POS
@@ -67,23 +69,28 @@ def _emulate(items, G, lu, y, ring_rows):
 
     acc = np.zeros(G)
     xslot = np.zeros(G, np.int64)
+    xrow = np.zeros(G, np.int64)
     loaded = {q: values(q) for q in range(min(D, steps))}
     for q in range(steps):
         b_all = items[q][..., 1]
         lx = lds[np.where(b_all < 0, b_all & POS, 0)]
         u, yv = loaded.pop(q)
+        pend = {}
         for i in range(P):
             for g in range(G):
                 kind, b = int(kinds[q, i, g]), int(b_all[i, g])
                 if kind == START:
                     xslot[g] = payload[q, i, g]
-                    acc[g] = yv[i, g]
-                elif kind == ENTRY:
-                    acc[g] = acc[g] - u[i, g] * (lx[i, g] if b < 0 else yv[i, g])
-                elif kind == END:
-                    xk = acc[g] / u[i, g]
-                    mem[b // TW] = xk
-                    lds[xslot[g]] = xk
+                    xrow[g] = b
+                v = acc[g] - u[i, g] * (lx[i, g] if b < 0 else yv[i, g])
+                acc[g] = yv[i, g] if kind == START else v
+                if kind == END:
+                    assert g not in pend
+                    pend[g] = (acc[g], u[i, g], xrow[g], xslot[g])
+        for g, (a, pv, row, sl) in pend.items():  # the step's row ends
+            xk = a / pv
+            mem[row // TW] = xk
+            lds[sl] = xk
         loaded[q + D] = values(q + D)  # the set refilled after this step
     return mem
 
@@ -106,8 +113,9 @@ def test_back_levels_equal_column_oriented(grid, tile, ring_rows):
     assert np.array_equal(got, want)
     # every row ends exactly once, every U entry is used exactly once
     kinds = (items[..., 0].astype(np.int64) & 0xFFFFFFFF) >> 29
-    ends = items[..., 1][kinds == END] // (64 // G)
-    assert np.array_equal(np.sort(ends), np.arange(n))
+    starts = items[..., 1][kinds == START] // (64 // G)
+    assert np.array_equal(np.sort(starts), np.arange(n))
+    assert np.count_nonzero(kinds == END) == n
     assert np.count_nonzero(kinds == ENTRY) == int(ulen.sum())
     # at most one END per lane group and step (the kernel divides once per step)
     assert (np.count_nonzero(kinds == END, axis=1) <= 1).all()
