@@ -75,6 +75,18 @@ struct DevTopo {
   const double* lu_g = nullptr;
   const double* lu_b = nullptr;
   const int* lu_yp = nullptr;   // per packed position: the Y-bus position its recipe reads (-1: fill)
+  // the same recipes grouped by bus pair for the fill kernel: for bus i, entries [fb_ptr[i],
+  // fb_ptr[i+1]) = its Y-bus neighbours k (k = i included) with the positions of H, N, M, L (i, k)
+  // (-1 where the bus types leave one out), the admittance and its Y-bus position; fz_pos lists the
+  // positions that are no Jacobian entry (structural fill, zero before each refactor)
+  const int* fb_ptr = nullptr;
+  const int* fb_k = nullptr;
+  const int4* fb_pos = nullptr;
+  const double* fb_g = nullptr;
+  const double* fb_b = nullptr;
+  const int* fb_yp = nullptr;
+  const int* fz_pos = nullptr;
+  int n_fz = 0;
   // branches (per-scenario outages): Y-bus positions of the ff, ft, tf, tt stamps (-1: none) and
   // their values (re, im per stamp)
   const int4* br_pos = nullptr;

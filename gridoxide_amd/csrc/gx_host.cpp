@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -867,6 +868,10 @@ struct gx_topo {
   std::vector<double> br_y;    // 8 per branch
   std::vector<char> br_bridge;
   std::vector<int> h_yp;       // per L+U position: the Y CSR position its recipe reads (-1 for fill)
+  // recipes by bus pair (DevTopo::fb_*) and the fill positions
+  std::vector<int> h_fb_ptr, h_fb_k, h_fb_pos, h_fb_yp, h_fz;
+  std::vector<double> h_fb_g, h_fb_b;
+  DevMem d_fb_ptr, d_fb_k, d_fb_pos, d_fb_g, d_fb_b, d_fb_yp, d_fz;
   DevMem d_br_pos, d_br_y, d_yp;
   bool br_uploaded = false;
   // the last gx_batch_solve (for island reports)
@@ -968,6 +973,59 @@ int finalize_factor(gx_topo& t) {
       xm[i] = qinv[t.n_ang + t.pq_pos[i]];
     }
   }
+  // the fill kernel's recipes: one entry per (row bus, column bus) pair with its up to four positions
+  {
+    std::map<std::pair<int, int>, int> pair_of;
+    std::vector<std::array<int, 4>> pp;
+    std::vector<int> pk, pi, pyp;
+    std::vector<double> pg, pb;
+    std::vector<int> fz;
+    for (int pos = 0; pos < nnz; ++pos) {
+      if (kind[pos] == gx::kFill) {
+        fz.push_back(pos);
+        continue;
+      }
+      const auto key = std::make_pair(bi[pos], bk[pos]);
+      auto it = pair_of.find(key);
+      int e;
+      if (it == pair_of.end()) {
+        e = (int)pp.size();
+        pair_of.emplace(key, e);
+        pp.push_back({-1, -1, -1, -1});
+        pi.push_back(bi[pos]);
+        pk.push_back(bk[pos]);
+        pg.push_back(g[pos]);
+        pb.push_back(b[pos]);
+        pyp.push_back(yp[pos]);
+      } else {
+        e = it->second;
+      }
+      const int kd = kind[pos];
+      const int slot = (kd == gx::kHii || kd == gx::kHik) ? 0 : (kd == gx::kNii || kd == gx::kNik) ? 1
+                       : (kd == gx::kMii || kd == gx::kMik) ? 2 : 3;
+      pp[e][slot] = pos;
+    }
+    // by row bus (map order: row bus, then column bus)
+    std::vector<int> fptr(t.n_bus + 1, 0), fk, fpos, fyp;
+    std::vector<double> fg, fb;
+    for (auto& kv : pair_of) fptr[kv.first.first + 1]++;
+    for (int i = 0; i < t.n_bus; ++i) fptr[i + 1] += fptr[i];
+    for (auto& kv : pair_of) {
+      const int e = kv.second;
+      fk.push_back(pk[e]);
+      for (int c = 0; c < 4; ++c) fpos.push_back(pp[e][c]);
+      fg.push_back(pg[e]);
+      fb.push_back(pb[e]);
+      fyp.push_back(pyp[e]);
+    }
+    t.h_fb_ptr.swap(fptr);
+    t.h_fb_k.swap(fk);
+    t.h_fb_pos.swap(fpos);
+    t.h_fb_g.swap(fg);
+    t.h_fb_b.swap(fb);
+    t.h_fb_yp.swap(fyp);
+    t.h_fz.swap(fz);
+  }
   t.h_kind.swap(kind);
   t.h_yp.swap(yp);
   t.h_bi.swap(bi);
@@ -995,9 +1053,20 @@ int upload_topo(gx_topo& t, int window, int tile) {
   if ((rc = upload_ybus(t)) || (rc = t.d_rpos_p.upload(t.h_rp)) ||
       (rc = t.d_rpos_q.upload(t.h_rq)) || (rc = t.d_xpos_a.upload(t.h_xa)) || (rc = t.d_xpos_m.upload(t.h_xm)) ||
       (rc = t.d_kind.upload(t.h_kind)) || (rc = t.d_bi.upload(t.h_bi)) || (rc = t.d_bk.upload(t.h_bk)) ||
-      (rc = t.d_g.upload(t.h_g)) || (rc = t.d_b.upload(t.h_b)) || (rc = t.d_yp.upload(t.h_yp)))
+      (rc = t.d_g.upload(t.h_g)) || (rc = t.d_b.upload(t.h_b)) || (rc = t.d_yp.upload(t.h_yp)) ||
+      (rc = t.d_fb_ptr.upload(t.h_fb_ptr)) || (rc = t.d_fb_k.upload(t.h_fb_k)) ||
+      (rc = t.d_fb_pos.upload(t.h_fb_pos)) || (rc = t.d_fb_g.upload(t.h_fb_g)) ||
+      (rc = t.d_fb_b.upload(t.h_fb_b)) || (rc = t.d_fb_yp.upload(t.h_fb_yp)) || (rc = t.d_fz.upload(t.h_fz)))
     return rc;
   gx::DevTopo& d = t.dt;
+  d.fb_ptr = t.d_fb_ptr.as<int>();
+  d.fb_k = t.d_fb_k.as<int>();
+  d.fb_pos = t.d_fb_pos.as<int4>();
+  d.fb_g = t.d_fb_g.as<double>();
+  d.fb_b = t.d_fb_b.as<double>();
+  d.fb_yp = t.d_fb_yp.as<int>();
+  d.fz_pos = t.d_fz.as<int>();
+  d.n_fz = (int)t.h_fz.size();
   d.lu_yp = t.d_yp.as<int>();
   d.rpos_p = t.d_rpos_p.as<int>();
   d.rpos_q = t.d_rpos_q.as<int>();

@@ -414,6 +414,65 @@ __global__ __launch_bounds__(256) void k_fill(DevTopo t, DevWork w) {
   *lu = (T)v;
 }
 
+// Jacobian fill by bus (JacobianPattern::fill_into, src/jacobian.rs:210-250, in rectangular form):
+// one row bus i and a block of tiles per workgroup; V, P and Q of bus i are read once, and each
+// neighbour k's V once for its up to four entries H, N, M, L (i, k).  The positions' stores of the
+// workgroup's tiles are contiguous.
+template <typename T, int TW>
+__global__ __launch_bounds__(256) void k_fill_bus(DevTopo t, DevWork w) {
+  const int i = blockIdx.x, tile = blockIdx.y * (256 / TW) + (int)threadIdx.x / TW, s = sub<TW>();
+  if (tile >= w.n_tiles) return;
+  const int e0 = t.fb_ptr[i], e1 = t.fb_ptr[i + 1];
+  if (e0 == e1) return;
+  bool run = false;  // any scenario of the tile still iterating (the LU kernel skips the others)
+  for (int q = 0; q < TW; ++q) run = run || w.status[tile * TW + q] == kRunning;
+  if (!run) return;
+  const int sc = tile * TW + s;
+  const size_t bb = at<TW>(tile, t.n_bus, 0) + s;
+  const Outage og = load_outage(t, w, sc);
+  const size_t oi = bb + (size_t)i * TW;
+  const double vri = w.vr[oi], vii = w.vi[oi], pci = w.pc[oi], qci = w.qc[oi];
+  T* __restrict__ lu = reinterpret_cast<T*>(w.lu) + lu_at(0, w.n_tiles, tile, TW) + s;
+  const size_t pst = (size_t)w.n_tiles * TW;
+  for (int e = e0; e < e1; ++e) {
+    const int k = t.fb_k[e];
+    const int4 pos = t.fb_pos[e];
+    double g = t.fb_g[e], b = t.fb_b[e];
+    if (w.outage) y_at(t, og, t.fb_yp[e], g, b);
+    const size_t ok = bb + (size_t)k * TW;
+    const double vrk = w.vr[ok], vik = w.vi[ok], vmk = w.vm[ok];
+    const double cc = vri * vrk + vii * vik;
+    const double ss = vii * vrk - vri * vik;
+    const double hl = g * ss - b * cc;
+    const double nm = g * cc + b * ss;
+    const bool diag = k == i;
+    double vh = hl, vn = nm / vmk, vmm = -nm, vl = hl / vmk;
+    if (diag) {
+      vh += -qci;
+      vn += pci / vmk;
+      vmm += pci;
+      vl += qci / vmk;
+    }
+    if (pos.x >= 0) lu[(size_t)pos.x * pst] = (T)vh;
+    if (pos.y >= 0) lu[(size_t)pos.y * pst] = (T)vn;
+    if (pos.z >= 0) lu[(size_t)pos.z * pst] = (T)vmm;
+    if (pos.w >= 0) lu[(size_t)pos.w * pst] = (T)vl;
+  }
+}
+
+// Zero into the positions that are no Jacobian entry (structural fill), ahead of each refactor.
+template <typename T, int TW>
+__global__ __launch_bounds__(256) void k_fill_zero(DevTopo t, DevWork w) {
+  const int tile = blockIdx.y * (256 / TW) + (int)threadIdx.x / TW, s = sub<TW>();
+  if (tile >= w.n_tiles) return;
+  bool run = false;
+  for (int q = 0; q < TW; ++q) run = run || w.status[tile * TW + q] == kRunning;
+  if (!run) return;
+  T* __restrict__ lu = reinterpret_cast<T*>(w.lu) + lu_at(0, w.n_tiles, tile, TW) + s;
+  const size_t pst = (size_t)w.n_tiles * TW;
+  for (int f = blockIdx.x; f < t.n_fz; f += gridDim.x) lu[(size_t)t.fz_pos[f] * pst] = (T)0;
+}
+
 // Entries of a column a lane holds in registers (longer columns take a slower loop).
 template <int G>
 constexpr int stage_regs() {
@@ -1673,9 +1732,21 @@ void launch_decide(const DevTopo& t, const DevWork& w, int iter, hipStream_t str
 void launch_fill(const DevTopo& t, const DevWork& w, bool fp32, hipStream_t stream) {
   with_tile(w.tile, [&](auto tw) {
     constexpr int TW = decltype(tw)::value;
-    dim3 grid(t.nnz_lu, (w.n_tiles + 256 / TW - 1) / (256 / TW));
-    if (fp32) hipLaunchKernelGGL((k_fill<float, TW>), grid, dim3(256), 0, stream, t, w);
-    else hipLaunchKernelGGL((k_fill<double, TW>), grid, dim3(256), 0, stream, t, w);
+    const int tb = (w.n_tiles + 256 / TW - 1) / (256 / TW);
+    if (!t.fb_ptr) {  // (no bus-pair recipes: one position per workgroup)
+      dim3 grid(t.nnz_lu, tb);
+      if (fp32) hipLaunchKernelGGL((k_fill<float, TW>), grid, dim3(256), 0, stream, t, w);
+      else hipLaunchKernelGGL((k_fill<double, TW>), grid, dim3(256), 0, stream, t, w);
+      return;
+    }
+    dim3 grid(t.n_bus, tb);
+    if (fp32) hipLaunchKernelGGL((k_fill_bus<float, TW>), grid, dim3(256), 0, stream, t, w);
+    else hipLaunchKernelGGL((k_fill_bus<double, TW>), grid, dim3(256), 0, stream, t, w);
+    if (t.n_fz > 0) {
+      dim3 zgrid(t.n_fz < 4096 ? t.n_fz : 4096, tb);
+      if (fp32) hipLaunchKernelGGL((k_fill_zero<float, TW>), zgrid, dim3(256), 0, stream, t, w);
+      else hipLaunchKernelGGL((k_fill_zero<double, TW>), zgrid, dim3(256), 0, stream, t, w);
+    }
   });
 }
 
