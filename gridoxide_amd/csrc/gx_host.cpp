@@ -880,13 +880,36 @@ struct gx_topo {
   // opt-in per-launch timing: (kernel class, start, stop)
   bool profiling = false;
   std::vector<std::tuple<int, hipEvent_t, hipEvent_t>> prof;
+  // events of the pending timings come from a pool that each finished solve refills; the timings
+  // are summed per kernel class until gx_topo_profile reads them
+  std::vector<hipEvent_t> ev_pool;
+  double prof_ms[GX_K_COUNT] = {};
+  int prof_n[GX_K_COUNT] = {};
   ~gx_topo() {
     for (auto& e : prof) {
       (void)hipEventDestroy(std::get<1>(e));
       (void)hipEventDestroy(std::get<2>(e));
     }
+    for (hipEvent_t e : ev_pool) (void)hipEventDestroy(e);
   }
 };
+
+namespace {
+// Sums the pending timings (their events must have completed) and returns the events to the pool.
+int fold_profile(gx_topo& t) {
+  for (auto& e : t.prof) {
+    float v = 0.0f;
+    GX_HIP(hipEventSynchronize(std::get<2>(e)));
+    GX_HIP(hipEventElapsedTime(&v, std::get<1>(e), std::get<2>(e)));
+    t.prof_ms[std::get<0>(e)] += v;
+    t.prof_n[std::get<0>(e)] += 1;
+    t.ev_pool.push_back(std::get<1>(e));
+    t.ev_pool.push_back(std::get<2>(e));
+  }
+  t.prof.clear();
+  return GX_OK;
+}
+}  // namespace
 
 namespace {
 
@@ -1435,24 +1458,14 @@ const char* gx_version(void) { return "gridox_hip 0.1.0 (gfx950)"; }
 
 int gx_topo_profile(gx_topo* t, int enable, double* ms, int* launches) {
   if (!t) return fail(GX_ERR_ARG, "null");
-  if (ms || launches) {
-    for (int c = 0; c < GX_K_COUNT; ++c) {
-      if (ms) ms[c] = 0.0;
-      if (launches) launches[c] = 0;
-    }
-    for (auto& e : t->prof) {
-      float v = 0.0f;
-      GX_HIP(hipEventSynchronize(std::get<2>(e)));
-      GX_HIP(hipEventElapsedTime(&v, std::get<1>(e), std::get<2>(e)));
-      if (ms) ms[std::get<0>(e)] += v;
-      if (launches) launches[std::get<0>(e)] += 1;
-    }
+  int rc;
+  if ((rc = fold_profile(*t))) return rc;
+  for (int c = 0; c < GX_K_COUNT; ++c) {
+    if (ms) ms[c] = t->prof_ms[c];
+    if (launches) launches[c] = t->prof_n[c];
+    t->prof_ms[c] = 0.0;
+    t->prof_n[c] = 0;
   }
-  for (auto& e : t->prof) {
-    (void)hipEventDestroy(std::get<1>(e));
-    (void)hipEventDestroy(std::get<2>(e));
-  }
-  t->prof.clear();
   t->profiling = enable != 0;
   return GX_OK;
 }
@@ -1837,13 +1850,19 @@ int gx_batch_solve_outages(gx_topo* t, int n_scen, const double* p_spec, const d
       launch();
       return GX_OK;
     }
-    hipEvent_t a, b;
-    GX_HIP(hipEventCreate(&a));
-    GX_HIP(hipEventCreate(&b));
-    GX_HIP(hipEventRecord(a, st));
+    hipEvent_t ab[2];
+    for (hipEvent_t& e : ab) {
+      if (t->ev_pool.empty()) {
+        GX_HIP(hipEventCreate(&e));
+      } else {
+        e = t->ev_pool.back();
+        t->ev_pool.pop_back();
+      }
+    }
+    GX_HIP(hipEventRecord(ab[0], st));
     launch();
-    GX_HIP(hipEventRecord(b, st));
-    t->prof.emplace_back(cls, a, b);
+    GX_HIP(hipEventRecord(ab[1], st));
+    t->prof.emplace_back(cls, ab[0], ab[1]);
     return GX_OK;
   };
   // the Newton loop (newton_raphson_cached, src/solver.rs:663-736) for every running scenario
@@ -1888,7 +1907,8 @@ int gx_batch_solve_outages(gx_topo* t, int n_scen, const double* p_spec, const d
   }
   t->last = w;
   t->has_last = true;
-  return collect_batch(*t, ws, w, in.dev, vm, va, status, iters, history, st);
+  if ((rc = collect_batch(*t, ws, w, in.dev, vm, va, status, iters, history, st))) return rc;
+  return fold_profile(*t);  // (the collection waited for the stream)
 }
 
 }  // extern "C"
