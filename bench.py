@@ -8,6 +8,10 @@ Workloads (the synthetic meshed grids of gridoxide_amd.grid):
   config3       10,000 buses (5,000 chords), 8,192 scenarios drawn from the generator's load
                 distribution, mixed precision (FP32 LU + FP64 refinement)
   config3-fp64  the same with the FP64 path (the reference for the mixed-precision speed-up)
+  config3-fp64-single
+                the plain FP64 path one launch sequence per scenario: the first 4 scenarios of
+                config3, each solved alone (the baseline of the mixed-precision batched target)
+  config2-mixed config2 with the mixed-precision path
   config4       64 topologies of log-uniform size (100-20,000 buses), 16-1,024 scenarios each (half
                 N-1), mixed precision, all in flight in one request engine: one persistent LU launch
                 per Newton step over every tile, steps captured in hipGraphs, launch shape autotuned
@@ -54,6 +58,8 @@ WORKLOADS = {
     "config2": dict(config=2, n_scen=4096, mode="FP64", loads="scaled"),
     "config3": dict(config=3, n_scen=8192, mode="MIXED", loads="drawn"),
     "config3-fp64": dict(config=3, n_scen=8192, mode="FP64", loads="drawn"),
+    "config3-fp64-single": dict(config=3, n_scen=8192, mode="FP64", loads="drawn", single=4),
+    "config2-mixed": dict(config=2, n_scen=4096, mode="MIXED", loads="scaled"),
     "config4": dict(multi="uneven"),
     "config5": dict(multi="serving"),
 }
@@ -134,9 +140,17 @@ def main():
     hist = torch.empty((S, max_iter), dtype=torch.float64, device=dev)
     stream = torch.cuda.current_stream().cuda_stream
 
+    single = wl.get("single", 0)  # solve this many scenarios one per call instead of the batch
+
     def step():
-        topo.batch_solve_device(tp, tq, tvm0, tva0, vm, va, st, it, hist, tol=tol, max_iter=max_iter, mode=mode,
-                                stream=stream)
+        if not single:
+            topo.batch_solve_device(tp, tq, tvm0, tva0, vm, va, st, it, hist, tol=tol, max_iter=max_iter,
+                                    mode=mode, stream=stream)
+            return
+        for s in range(single):
+            sl = slice(s, s + 1)
+            topo.batch_solve_device(tp[sl], tq[sl], tvm0[sl], tva0[sl], vm[sl], va[sl], st[sl], it[sl], hist[sl],
+                                    tol=tol, max_iter=max_iter, mode=mode, stream=stream)
 
     for _ in range(args.warmup):
         step()
@@ -148,10 +162,11 @@ def main():
     newton_total = 0
     participations = {"lu": 0, "solve": 0}
     t0 = time.perf_counter()
+    n_run = single or S
     for k in range(args.steps):
         step()
-        stat = st.cpu().numpy()
-        iters = it.cpu().numpy()
+        stat = st[:n_run].cpu().numpy()
+        iters = it[:n_run].cpu().numpy()
         conv_total += int((stat == 0).sum())
         newton_total += int(iters.sum())
         # Newton steps that needed a factorization: every iteration except the converged last one
@@ -191,12 +206,13 @@ def main():
         "vs_baseline": None,
         "dtype": "f64" if mode == Mode.FP64 else "f64+f32",
         "data": "synthetic",
-        "config": {"workload": f"{args.workload}: {net.n_bus}-bus meshed grid, {S} scenarios, flat start, "
-                               f"tol 1e-8, max 20 iterations, mode {mode.name}",
-                   "n_bus": net.n_bus, "n_scenarios": S, "mode": mode.name},
+        "config": {"workload": f"{args.workload}: {net.n_bus}-bus meshed grid, "
+                               + (f"{single} scenarios one per launch sequence" if single else f"{S} scenarios")
+                               + f", flat start, tol 1e-8, max 20 iterations, mode {mode.name}",
+                   "n_bus": net.n_bus, "n_scenarios": n_run, "mode": mode.name},
         "iterations_mean": float(iters.mean()),
         "converged_per_step": conv_total / args.steps,
-        "solves_per_s_all": S * args.steps / dt,
+        "solves_per_s_all": n_run * args.steps / dt,
         "newton_steps_per_s": newton_total / dt,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_src,
