@@ -198,6 +198,135 @@ int build_rounds(const gx::LuPattern& lu, const LuSchedule& s, int G, RoundPlan&
   return GX_OK;
 }
 
+// Refactor by parts (narrow tiles): the columns of one elimination level depend only on columns of
+// earlier levels, so a level's columns are factored together.  Column k's level is one more than the
+// highest level among the j of its U column (0 if none).  Each level's columns, in ascending k, are
+// cut into parts of at most `pcap` entries; parts run level by level.  A part's columns lie one after
+// the other in its LDS buffer (column k at base_k), and its update items (x_t -= L(i,j) * x_u for
+// column k's U entries u in ascending order, the sequential order, so the factors are bit-identical
+// to the column-by-column refactor) are scheduled into rounds of G items with the rules of
+// build_rounds: one item per lane group, distinct targets, no target that is also read as a source,
+// a target's updates in order and after its source is final.  L(i,j) of the previous part is read
+// from that part's LDS buffer (bit 31 of the word, x = its offset there), any other from memory.
+struct PartPlan {
+  std::vector<int> part_a;    // 4 per part: first entry (into pent_*), entries, first chunk, rounds
+  std::vector<int> pent_pos;  // per part entry (in buffer order): its packed L+U position
+  std::vector<int> pent_div;  // -1 U entry, -2 pivot, else the buffer offset of its column's pivot
+  std::vector<int> slots;     // pairs {x, word} as in build_rounds, chunks of kChunkRounds rounds
+  int n_parts = 0;
+  int cap = 0;                // entries of the largest part
+};
+
+int build_parts(const gx::LuPattern& lu, const LuSchedule& s, int G, int pcap, PartPlan& out) {
+  const int n = lu.n, NB = gx::kChunkRounds;
+  out = PartPlan();
+  if (s.max_col > pcap || pcap >= 32767) return fail(GX_ERR_UNSUPPORTED, "a column does not fit a refactor part");
+  std::vector<int> lev(n, 0);
+  int nlev = 0;
+  for (int k = 0; k < n; ++k) {
+    int l = 0;
+    for (int q = lu.cp[k]; q < lu.cp[k] + lu.ulen[k]; ++q) l = std::max(l, lev[lu.row[q]] + 1);
+    lev[k] = l;
+    nlev = std::max(nlev, l + 1);
+  }
+  std::vector<std::vector<int>> bucket(nlev);
+  for (int k = 0; k < n; ++k) bucket[lev[k]].push_back(k);
+  std::vector<int> part_of(n, -1), base(n, 0);
+  std::vector<std::vector<int>> parts;
+  for (int l = 0; l < nlev; ++l) {
+    int fill = pcap + 1;
+    for (int k : bucket[l]) {
+      const int len = lu.cp[k + 1] - lu.cp[k];
+      if (fill + len > pcap) {
+        parts.emplace_back();
+        fill = 0;
+      }
+      parts.back().push_back(k);
+      part_of[k] = (int)parts.size() - 1;
+      base[k] = fill;
+      fill += len;
+    }
+  }
+  out.n_parts = (int)parts.size();
+  out.part_a.assign(4 * (size_t)out.n_parts, 0);
+  std::vector<int> last;
+  std::vector<int> sx, sw;
+  std::vector<std::vector<int>> rt, rs;
+  int chunk = 0;
+  for (int pi = 0; pi < out.n_parts; ++pi) {
+    const std::vector<int>& cols = parts[pi];
+    const int pbase = (int)out.pent_pos.size();
+    for (int k : cols) {
+      const int c0 = lu.cp[k], len = lu.cp[k + 1] - c0, ul = lu.ulen[k];
+      for (int o = 0; o < len; ++o) {
+        out.pent_pos.push_back(c0 + o);
+        out.pent_div.push_back(o < ul ? -1 : (o == ul ? -2 : base[k] + ul));
+      }
+    }
+    const int Lp = (int)out.pent_pos.size() - pbase;
+    out.cap = std::max(out.cap, Lp);
+    const int noop_w = Lp;  // empty slot: target the unused row just past the part
+    last.assign(Lp, -1);
+    sx.clear();
+    sw.clear();
+    int R = 0;
+    auto add_round = [&]() {
+      sx.insert(sx.end(), G, -1);
+      sw.insert(sw.end(), G, noop_w);
+      if ((int)rt.size() <= R) {
+        rt.emplace_back();
+        rs.emplace_back();
+      }
+      rt[R].clear();
+      rs[R].clear();
+      ++R;
+    };
+    for (int k : cols) {
+      const int c0 = lu.cp[k], ul = lu.ulen[k];
+      for (int u = 0; u < ul; ++u) {
+        const int j = lu.row[c0 + u], l0 = lu.cp[j] + lu.ulen[j] + 1, ll = lu.cp[j + 1] - l0;
+        const int q0 = s.upd_ptr[c0 + u];
+        const bool prev = part_of[j] == pi - 1;
+        const int su = base[k] + u;
+        for (int e = 0; e < ll; ++e) {
+          const int p = l0 + e, tg = base[k] + s.upd_off[q0 + e];
+          for (int r = std::max(last[tg], last[su]) + 1;; ++r) {
+            while (r >= R) add_round();
+            int slot = -1;
+            for (int gg = 0; gg < G && slot < 0; ++gg)
+              if (sx[(size_t)r * G + gg] == -1) slot = gg;
+            if (slot < 0) continue;
+            if (std::find(rt[r].begin(), rt[r].end(), tg) != rt[r].end()) continue;
+            if (std::find(rs[r].begin(), rs[r].end(), tg) != rs[r].end()) continue;
+            sx[(size_t)r * G + slot] = prev ? base[j] + (p - lu.cp[j]) : p;
+            sw[(size_t)r * G + slot] = tg | (su << 16) | (prev ? (int)0x80000000u : 0);
+            rt[r].push_back(tg);
+            rs[r].push_back(su);
+            last[tg] = r;
+            break;
+          }
+        }
+      }
+    }
+    const int nch = std::max(1, (R + NB - 1) / NB);
+    out.part_a[4 * pi] = pbase;
+    out.part_a[4 * pi + 1] = Lp;
+    out.part_a[4 * pi + 2] = chunk;
+    out.part_a[4 * pi + 3] = R;
+    chunk += nch;
+    for (int r = 0; r < nch * NB; ++r)
+      for (int gg = 0; gg < G; ++gg) {
+        const bool real = r < R && sx[(size_t)r * G + gg] >= 0;
+        out.slots.push_back(real ? sx[(size_t)r * G + gg] : 0);
+        out.slots.push_back(real ? sw[(size_t)r * G + gg] : noop_w);
+      }
+    if (out.slots.size() >= (size_t)0x7fffffff) return fail(GX_ERR_UNSUPPORTED, "refactor round schedule too large");
+  }
+  // two chunks past the last one, read by the kernel's prefetch and never executed
+  out.slots.insert(out.slots.end(), 2 * (size_t)2 * NB * G, 0);
+  return GX_OK;
+}
+
 // The entries the triangular solves gather instead of pushing, for an LDS window of W rows:
 // L(r, j) with r - j > W and U(r, j) with j - r > W, grouped by row r (ascending j).
 struct FarLists {
@@ -324,7 +453,23 @@ int build_uniform_rounds(const gx::LuPattern& lu, const LuSchedule& s, std::vect
 // fewer than ring_rows - 1 rows ended after c before step s, else from y, which the kernel loads at the
 // end of step s - kBackAhead: c must have ended by then.  Returns false (no schedule) when the encoding's limits
 // are exceeded or a read would come too early.
+bool build_solve_levels(const gx::LuPattern& lu, int G, int ring_rows, bool forward, std::vector<int>& items,
+                        int& steps);
+
 bool build_back_levels(const gx::LuPattern& lu, int G, int ring_rows, std::vector<int>& items, int& steps) {
+  return build_solve_levels(lu, G, ring_rows, false, items, steps);
+}
+
+// The forward substitution L y = f (L unit lower) by levels, in the same item format: row k's items
+// are START (acc = f_k), one ENTRY per L(k, j) in ascending j (the order in which the column-oriented
+// forward substitution, src/klu_native/solve.rs:23-30, applies them) and END (y_k = acc, no division;
+// payload 0).  A row's level is one more than the highest level among the j of its L row.
+bool build_forward_levels(const gx::LuPattern& lu, int G, int ring_rows, std::vector<int>& items, int& steps) {
+  return build_solve_levels(lu, G, ring_rows, true, items, steps);
+}
+
+bool build_solve_levels(const gx::LuPattern& lu, int G, int ring_rows, bool forward, std::vector<int>& items,
+                        int& steps) {
   const int n = lu.n, P = gx::kBackItems, D = gx::kBackAhead, TW = gx::kLanes / G, S = TW + 1;
   items.clear();
   steps = 0;
@@ -333,33 +478,41 @@ bool build_back_levels(const gx::LuPattern& lu, int G, int ring_rows, std::vecto
     return false;
   const int xrows = ring_rows - 1;                                   // ring rows for x values
   const int zero_b = (int)(0x80000000u | (unsigned)(xrows * S));     // the last ring row holds zeros
-  // U by rows: row k -> (c, packed position of U(k, c)), descending c
+  // U by rows: row k -> (c, packed position of U(k, c)), descending c; or L by rows: row k ->
+  // (j, position of L(k, j)), ascending j
+  auto first = [&](int c) { return forward ? lu.cp[c] + lu.ulen[c] + 1 : lu.cp[c]; };
+  auto past = [&](int c) { return forward ? lu.cp[c + 1] : lu.cp[c] + lu.ulen[c]; };
   std::vector<int> rptr(n + 1, 0);
   for (int c = 0; c < n; ++c)
-    for (int q = lu.cp[c]; q < lu.cp[c] + lu.ulen[c]; ++q) rptr[lu.row[q] + 1]++;
+    for (int q = first(c); q < past(c); ++q) rptr[lu.row[q] + 1]++;
   for (int k = 0; k < n; ++k) rptr[k + 1] += rptr[k];
   std::vector<int> rc(rptr[n]), rq(rptr[n]), cur(rptr.begin(), rptr.end() - 1);
   for (int c = 0; c < n; ++c)
-    for (int q = lu.cp[c]; q < lu.cp[c] + lu.ulen[c]; ++q) {
+    for (int q = first(c); q < past(c); ++q) {
       const int k = lu.row[q];
       rc[cur[k]] = c;
       rq[cur[k]] = q;
       ++cur[k];
     }
-  for (int k = 0; k < n; ++k) {  // filled in ascending c: reverse each row
-    std::reverse(rc.begin() + rptr[k], rc.begin() + rptr[k + 1]);
-    std::reverse(rq.begin() + rptr[k], rq.begin() + rptr[k + 1]);
-  }
+  if (!forward)
+    for (int k = 0; k < n; ++k) {  // filled in ascending c: reverse each row
+      std::reverse(rc.begin() + rptr[k], rc.begin() + rptr[k + 1]);
+      std::reverse(rq.begin() + rptr[k], rq.begin() + rptr[k + 1]);
+    }
   std::vector<int> lev(n, 0);
   int nlev = 0;
-  for (int k = n - 1; k >= 0; --k) {
+  for (int i = 0; i < n; ++i) {
+    const int k = forward ? i : n - 1 - i;  // every row after the rows it reads
     int l = 0;
     for (int e = rptr[k]; e < rptr[k + 1]; ++e) l = std::max(l, lev[rc[e]] + 1);
     lev[k] = l;
     nlev = std::max(nlev, l + 1);
   }
   std::vector<std::vector<int>> bucket(nlev);
-  for (int k = n - 1; k >= 0; --k) bucket[lev[k]].push_back(k);
+  for (int i = 0; i < n; ++i) {
+    const int k = forward ? i : n - 1 - i;
+    bucket[lev[k]].push_back(k);
+  }
   // layout: per step, item slot, group: a row id (START / ENTRY e / END) or none
   struct Cell {
     int kind = gx::kBackPad, k = 0, e = 0;
@@ -426,7 +579,7 @@ bool build_back_levels(const gx::LuPattern& lu, int G, int ring_rows, std::vecto
       a = (gx::kBackStart << 29) | (fin[c.k] % xrows) * S;
       b = c.k * TW;
     } else if (c.kind == gx::kBackEnd) {
-      a = (gx::kBackEnd << 29) | (lu.cp[c.k] + lu.ulen[c.k]);
+      a = (gx::kBackEnd << 29) | (forward ? 0 : lu.cp[c.k] + lu.ulen[c.k]);
     } else if (c.kind == gx::kBackEntry) {
       const int src = rc[c.e];
       a = (gx::kBackEntry << 29) | rq[c.e];

@@ -500,6 +500,116 @@ __device__ __forceinline__ int4 uni4(int4 v) {
 // columns, overflow gathers) so that the common path's waits stay exact.
 __device__ __forceinline__ void drain() { __builtin_amdgcn_s_waitcnt(0); }
 
+// A triangular solve by levels (item encoding at kBackItems) for one tile: every lane group runs its
+// rows of a level as one dependent chain in registers, the levels one after another.  Four register
+// sets take turns: after a set's step has run, it loads the values of the step kBackAhead later
+// (their descriptors were loaded kBackAhead steps earlier still), so no load is waited for soon after
+// it was issued and no in-flight load is copied between registers.  x of rows that ended recently
+// comes from the LDS ring, older x from y.  kDivide: END divides by the pivot (back substitution);
+// else END keeps the sum (forward substitution with the unit L).
+//   items: this lane group's first item; lub/pstb: the lane's L+U base and position stride (bytes);
+//   yl: the lane's y (offsets row * TW); rl: the lane's ring (offsets slot * S); R ring rows
+template <typename T, int TW, bool kDivide>
+__device__ __forceinline__ void level_solve(const int2* __restrict__ bib, int nst, int R, int nl,
+                                            char* __restrict__ lub, unsigned pstb, T* __restrict__ yl, T* rl,
+                                            bool& ok) {
+  constexpr int G = kLanes / TW;
+  constexpr int S = TW + 1;
+  constexpr int P = kBackItems, D = kBackAhead;
+  static_assert(D == 4, "the step loop below is unrolled for four register sets");
+  constexpr unsigned kPayload = (1u << 29) - 1;
+  const T zero = zero_value<T>();
+  auto LU = [&](int p) -> T& { return *reinterpret_cast<T*>(lub + (size_t)(unsigned)p * pstb); };
+  const unsigned pos_max = (unsigned)(nl - 1);
+  // a set holds the values of its step and the descriptors of that step and of the step D later in
+  // da / db, whose roles alternate from one turn of the set to the next (so that the loop needs no
+  // register copies, which would wait for the loads in flight)
+  struct BSet {
+    int2 da[P], db[P];
+    T u[P], y[P];
+  };
+  auto dsc_issue = [&](int q, int2 (&d)[P]) {
+    const int qq = q < nst ? q : nst - 1;
+#pragma unroll
+    for (int i = 0; i < P; ++i) d[i] = bib[(unsigned)((qq * P + i) * G)];
+  };
+  auto val_issue = [&](const int2 (&d)[P], BSet& b) {
+#pragma unroll
+    for (int i = 0; i < P; ++i) {
+      // (a START's payload is an LDS offset: clamped into the L+U range, the value is unused)
+      const unsigned p = (unsigned)d[i].x & kPayload;
+      b.u[i] = LU((int)(p < pos_max ? p : pos_max));
+      b.y[i] = yl[(unsigned)(d[i].y >= 0 ? d[i].y : 0)];
+    }
+  };
+  T acc = zero;
+  unsigned xslot = 0, xrow = 0;  // ring and y offsets of the row being run (from its START)
+  rl[(unsigned)((R - 1) * S)] = zero;  // the zero row (END and PAD items multiply by it)
+  // step q with the set's values and descriptors cur; then the values of step q + D (descriptors
+  // nxt) and the descriptors of step q + 2D into cur
+  auto run = [&](int q, BSet& b, int2 (&cur)[P], int2 (&nxt)[P]) __attribute__((always_inline)) {
+    T lx[P];
+#pragma unroll
+    for (int i = 0; i < P; ++i) lx[i] = rl[(unsigned)(cur[i].y < 0 ? cur[i].y & kPayload : 0)];
+    memory_order_point();  // every LDS read of the step before any of its writes
+    T pend = zero, pend_u = zero;
+    unsigned pend_y = 0, pend_s = 0;
+    bool ended = false;
+#pragma unroll
+    for (int i = 0; i < P; ++i) {
+      const unsigned a = (unsigned)cur[i].x;
+      const int bb = cur[i].y;
+      const unsigned kind = a >> 29;
+      const bool start = kind == (unsigned)kBackStart;
+      xslot = start ? (a & kPayload) : xslot;
+      xrow = start ? (unsigned)bb : xrow;
+      // ENTRY: acc - U(k, c) x_c; END and PAD read the zero row (acc unchanged); START: y_k
+      const T xv = bb < 0 ? lx[i] : b.y[i];
+      const T v = acc - b.u[i] * xv;
+      acc = start ? b.y[i] : v;
+      const bool end = kind == (unsigned)kBackEnd;  // (at most one per lane group and step)
+      pend = end ? acc : pend;
+      pend_u = end ? b.u[i] : pend_u;
+      pend_y = end ? xrow : pend_y;
+      pend_s = end ? xslot : pend_s;
+      ended = ended || end;
+    }
+    if (ended) {  // the step's row end: x_k into y and the ring
+      const T xk = kDivide ? pend / pend_u : pend;
+      if (!finite_value(xk)) ok = false;
+      yl[pend_y] = xk;
+      rl[pend_s] = xk;
+    }
+    memory_order_point();  // the step's stores before the next loads
+    val_issue(nxt, b);
+    dsc_issue(q + 2 * D, cur);
+  };
+  BSet S0, S1, S2, S3;
+  dsc_issue(0, S0.da);
+  dsc_issue(1, S1.da);
+  dsc_issue(2, S2.da);
+  dsc_issue(3, S3.da);
+  dsc_issue(D, S0.db);
+  dsc_issue(D + 1, S1.db);
+  dsc_issue(D + 2, S2.db);
+  dsc_issue(D + 3, S3.db);
+  val_issue(S0.da, S0);
+  val_issue(S1.da, S1);
+  val_issue(S2.da, S2);
+  val_issue(S3.da, S3);
+  drain();  // (the loop's waits are counted from what the loop itself issues)
+  for (int q = 0; q < nst; q += 2 * D) {
+    run(q, S0, S0.da, S0.db);
+    run(q + 1, S1, S1.da, S1.db);
+    run(q + 2, S2, S2.da, S2.db);
+    run(q + 3, S3, S3.da, S3.db);
+    run(q + 4, S0, S0.db, S0.da);
+    run(q + 5, S1, S1.db, S1.da);
+    run(q + 6, S2, S2.db, S2.da);
+    run(q + 7, S3, S3.db, S3.da);
+  }
+}
+
 // Refactor (optional) + forward/back substitution; one wavefront per tile.
 //   LDS: two staged columns xc[2][cap+1][TW+1] (column k in buffer k & 1; the previous column stays
 //   readable for the updates that use it) and ring[R][TW+1] (the right-hand side rows inside the
@@ -823,107 +933,7 @@ __device__ __forceinline__ void lu_tile(const DevTopo& t, const DevWork& w, int 
 
   // ------------------------------------------------ back substitution
   if (!overflow && t.back_items) {
-    // by levels (item encoding at kBackItems): every lane group runs its rows of a level as one
-    // dependent chain in registers, the levels one after another.  Four register sets take turns:
-    // after a set's step has run, it loads the values of the step kBackAhead later (their
-    // descriptors were loaded kBackAhead steps earlier still), so no load is waited for soon after
-    // it was issued and no in-flight load is copied between registers.  x of rows that ended
-    // recently comes from the LDS ring, older x from y
-    constexpr int P = kBackItems, D = kBackAhead;
-    static_assert(D == 4, "the step loop below is unrolled for four register sets");
-    constexpr unsigned kPayload = (1u << 29) - 1;
-    const unsigned pos_max = (unsigned)(nl - 1);
-    const int2* __restrict__ bib = t.back_items + g;
-    const int nst = t.back_steps;
-    T* __restrict__ yl = yb + s;  // this lane's y; item offsets are row * TW
-    T* rl = ring + s;             // this lane's ring; item offsets are slot * S
-    // a set holds the values of its step and the descriptors of that step and of the step D later in
-    // da / db, whose roles alternate from one turn of the set to the next (so that the loop needs no
-    // register copies, which would wait for the loads in flight)
-    struct BSet {
-      int2 da[P], db[P];
-      T u[P], y[P];
-    };
-    auto dsc_issue = [&](int q, int2 (&d)[P]) {
-      const int qq = q < nst ? q : nst - 1;
-#pragma unroll
-      for (int i = 0; i < P; ++i) d[i] = bib[(unsigned)((qq * P + i) * G)];
-    };
-    auto val_issue = [&](const int2 (&d)[P], BSet& b) {
-#pragma unroll
-      for (int i = 0; i < P; ++i) {
-        // (a START's payload is an LDS offset: clamped into the L+U range, the value is unused)
-        const unsigned p = (unsigned)d[i].x & kPayload;
-        b.u[i] = LU((int)(p < pos_max ? p : pos_max));
-        b.y[i] = yl[(unsigned)(d[i].y >= 0 ? d[i].y : 0)];
-      }
-    };
-    T acc = zero;
-    unsigned xslot = 0, xrow = 0;  // ring and y offsets of the row being run (from its START)
-    rl[(unsigned)((R - 1) * S)] = zero;  // the zero row (END and PAD items multiply by it)
-    // step q with the set's values and descriptors cur; then the values of step q + D (descriptors
-    // nxt) and the descriptors of step q + 2D into cur
-    auto run = [&](int q, BSet& b, int2 (&cur)[P], int2 (&nxt)[P]) __attribute__((always_inline)) {
-      T lx[P];
-#pragma unroll
-      for (int i = 0; i < P; ++i) lx[i] = rl[(unsigned)(cur[i].y < 0 ? cur[i].y & kPayload : 0)];
-      memory_order_point();  // every LDS read of the step before any of its writes
-      T pend = zero, pend_u = zero;
-      unsigned pend_y = 0, pend_s = 0;
-      bool ended = false;
-#pragma unroll
-      for (int i = 0; i < P; ++i) {
-        const unsigned a = (unsigned)cur[i].x;
-        const int bb = cur[i].y;
-        const unsigned kind = a >> 29;
-        const bool start = kind == (unsigned)kBackStart;
-        xslot = start ? (a & kPayload) : xslot;
-        xrow = start ? (unsigned)bb : xrow;
-        // ENTRY: acc - U(k, c) x_c; END and PAD read the zero row (acc unchanged); START: y_k
-        const T xv = bb < 0 ? lx[i] : b.y[i];
-        const T v = acc - b.u[i] * xv;
-        acc = start ? b.y[i] : v;
-        const bool end = kind == (unsigned)kBackEnd;  // (at most one per lane group and step)
-        pend = end ? acc : pend;
-        pend_u = end ? b.u[i] : pend_u;
-        pend_y = end ? xrow : pend_y;
-        pend_s = end ? xslot : pend_s;
-        ended = ended || end;
-      }
-      if (ended) {  // the step's row end: x_k = acc / U(k, k) into y and the ring
-        const T xk = pend / pend_u;
-        if (!finite_value(xk)) ok = false;
-        yl[pend_y] = xk;
-        rl[pend_s] = xk;
-      }
-      memory_order_point();  // the step's stores before the next loads
-      val_issue(nxt, b);
-      dsc_issue(q + 2 * D, cur);
-    };
-    BSet S0, S1, S2, S3;
-    dsc_issue(0, S0.da);
-    dsc_issue(1, S1.da);
-    dsc_issue(2, S2.da);
-    dsc_issue(3, S3.da);
-    dsc_issue(D, S0.db);
-    dsc_issue(D + 1, S1.db);
-    dsc_issue(D + 2, S2.db);
-    dsc_issue(D + 3, S3.db);
-    val_issue(S0.da, S0);
-    val_issue(S1.da, S1);
-    val_issue(S2.da, S2);
-    val_issue(S3.da, S3);
-    drain();  // (as before the forward loop)
-    for (int q = 0; q < nst; q += 2 * D) {
-      run(q, S0, S0.da, S0.db);
-      run(q + 1, S1, S1.da, S1.db);
-      run(q + 2, S2, S2.da, S2.db);
-      run(q + 3, S3, S3.da, S3.db);
-      run(q + 4, S0, S0.db, S0.da);
-      run(q + 5, S1, S1.db, S1.da);
-      run(q + 6, S2, S2.db, S2.da);
-      run(q + 7, S3, S3.db, S3.da);
-    }
+    level_solve<T, TW, true>(t.back_items + g, t.back_steps, R, nl, lub, pstb, yb + s, ring + s, ok);
   } else if (!overflow) {
     strided(n - 1 - W > 0 ? n - 1 - W : 0, g, n, [&](int r) { ring[slot(r)] = Y(r); });
     // column metadata three columns ahead; the U entries, the pivot, the entering
