@@ -55,6 +55,8 @@ SIGNATURES = {
     "gx_topo_set_launch": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
     "gx_topo_set_zip": (ctypes.c_int, [_vp, _f64p]),
     "gx_topo_back_levels": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _i32p, ctypes.c_longlong, _i32p]),
+    "gx_topo_forward_levels": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _i32p, ctypes.c_longlong, _i32p]),
+    "gx_topo_refactor_parts": (ctypes.c_int, [_vp, ctypes.c_int, _i64p, _i32p, _i32p, _i32p, _i32p]),
     "gx_topo_set_branches": (ctypes.c_int, [_vp, ctypes.c_int, _i32p, _i32p, _f64p, _f64p, _f64p, _f64p,
                                             ctypes.c_int, _i32p, _i32p, _f64p, _f64p, _f64p]),
     "gx_topo_branch_is_bridge": (ctypes.c_int, [_vp, ctypes.c_int]),

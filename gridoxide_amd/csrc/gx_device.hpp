@@ -146,6 +146,18 @@ struct DevTopo {
   // back_steps is a multiple of 2 kBackAhead; see kBack* for the encoding
   const int2* back_items = nullptr;
   int back_steps = 0;
+  // narrow tiles, refactor by parts (null: column by column with the fused forward substitution):
+  // part_a[p] = {first entry in pent_pos / pent_div, entries, first chunk of part_slots, rounds};
+  // pent_div = -1 for a U entry, -2 for a pivot, else the part-buffer offset of the entry's pivot;
+  // part_slots as slots, with offsets into the part buffer; the forward substitution by levels
+  // (fwd_items / fwd_steps, encoding as back_items with END keeping the sum)
+  const int4* part_a = nullptr;
+  const int* pent_pos = nullptr;
+  const int* pent_div = nullptr;
+  const int2* part_slots = nullptr;
+  int n_parts = 0;
+  const int2* fwd_items = nullptr;
+  int fwd_steps = 0;
 };
 
 // Back-substitution items {a, b} for a tile of TW scenarios (ring row stride S = TW + 1): a = kind << 29
@@ -160,6 +172,7 @@ struct DevTopo {
 // The kernel loads a step's values at the end of the step kBackAhead before it.
 constexpr int kBackItems = 4;
 constexpr int kBackAhead = 4;
+constexpr int kPartRegs = 4;  // refactor part entries per lane group staged through registers
 constexpr int kBackEntry = 0, kBackStart = 1, kBackEnd = 2, kBackPad = 3;
 
 // The linear initial guess system of a topology (complex, m = number of PQ buses): recipes

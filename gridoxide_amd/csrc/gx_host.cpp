@@ -599,8 +599,8 @@ bool build_solve_levels(const gx::LuPattern& lu, int G, int ring_rows, bool forw
 
 struct DevSched {
   DevMem cp, ulen, row, chunk0, rounds, slots, fptr, fpos, fcol, bptr, bpos, bcol, col_a, col_f, col_b, fslot, opos, ocol;
-  DevMem uround, uitems, bslot, bitems;
-  int back_steps = 0;
+  DevMem uround, uitems, bslot, bitems, fitems, part_a, pent_pos, pent_div, part_slots;
+  int back_steps = 0, fwd_steps = 0, n_parts = 0, part_cap = 0;
   std::vector<int> h_chunk0, h_rounds, h_fptr, h_bptr;
   bool uploaded = false;
   int window = -1;
@@ -611,6 +611,16 @@ struct DevSched {
     groups = -1;
   }
 };
+
+// Narrow tiles refactor by parts (lu_tile_parts) when this is set; else column by column.
+constexpr bool kUseParts = true;
+
+// Entries of the LU kernel's staging buffer: a refactor part holds up to kPartRegs entries per
+// lane group, or one column if that is longer.
+int lu_cap(int max_col, int tile) {
+  if (!kUseParts || tile == gx::kWideTile) return max_col;
+  return std::max(max_col, (gx::kLanes / tile) * gx::kPartRegs);
+}
 
 // Uploads the pattern once, the update rounds whenever the lane-group count changes and the far
 // lists whenever the window changes.
@@ -751,6 +761,24 @@ int upload_sched(const gx::LuPattern& lu, const LuSchedule& s, int window, int t
     if (build_back_levels(lu, G, window + gx::kRingGap + 1, bitems, m.back_steps)) {
       if ((rc = m.bitems.upload(bitems))) return rc;
     }
+    // refactor by parts with the forward substitution by levels
+    m.n_parts = 0;
+    m.fwd_steps = 0;
+    if (kUseParts && m.back_steps > 0) {
+      std::vector<int> fitems;
+      int fsteps = 0;
+      PartPlan plan;
+      if (build_forward_levels(lu, G, window + gx::kRingGap + 1, fitems, fsteps) &&
+          build_parts(lu, s, G, lu_cap(s.max_col, tile), plan) == GX_OK) {
+        if ((rc = m.fitems.upload(fitems)) || (rc = m.part_a.upload(plan.part_a)) ||
+            (rc = m.pent_pos.upload(plan.pent_pos)) || (rc = m.pent_div.upload(plan.pent_div)) ||
+            (rc = m.part_slots.upload(plan.slots)))
+          return rc;
+        m.fwd_steps = fsteps;
+        m.n_parts = plan.n_parts;
+        m.part_cap = lu_cap(s.max_col, tile);
+      }
+    }
   }
   d.n = lu.n;
   d.nnz_lu = lu.nnz();
@@ -780,6 +808,15 @@ int upload_sched(const gx::LuPattern& lu, const LuSchedule& s, int window, int t
   d.window = window;
   d.back_items = tile != gx::kWideTile && m.back_steps > 0 ? m.bitems.as<int2>() : nullptr;
   d.back_steps = d.back_items ? m.back_steps : 0;
+  const bool parts = tile != gx::kWideTile && m.n_parts > 0;
+  d.part_a = parts ? m.part_a.as<int4>() : nullptr;
+  d.pent_pos = parts ? m.pent_pos.as<int>() : nullptr;
+  d.pent_div = parts ? m.pent_div.as<int>() : nullptr;
+  d.part_slots = parts ? m.part_slots.as<int2>() : nullptr;
+  d.n_parts = parts ? m.n_parts : 0;
+  d.fwd_items = parts ? m.fitems.as<int2>() : nullptr;
+  d.fwd_steps = parts ? m.fwd_steps : 0;
+  if (parts) d.max_col = m.part_cap;  // (sizes the staging buffers: lu_cap)
   return GX_OK;
 }
 
@@ -1411,7 +1448,7 @@ int build_lg(gx_topo& t, Workspace& ws, size_t lanes, int tile) {
     t.lg_built = true;
   }
   if (t.dlg.m > 0) {
-    const int rows = pick_ring_rows(t.lg_sched.max_col, tile, 16, 0);
+    const int rows = pick_ring_rows(lu_cap(t.lg_sched.max_col, tile), tile, 16, 0);
     if (rows < 0) return fail(GX_ERR_UNSUPPORTED, "linear guess L+U column exceeds the LDS staging limit");
     if ((rc = upload_sched(t.lg_lu, t.lg_sched, gx::ring_window_of(rows, tile), tile, t.lg_ds, t.dlgt))) return rc;
     if ((rc = ws.lg_lu.reserve(lanes * (size_t)t.dlg.nnz_lu * 16)) || (rc = ws.lg_y.reserve(lanes * t.dlg.m * 16)) ||
@@ -1561,7 +1598,7 @@ int stage_batch(gx_topo& t, Workspace& ws, const BatchIn& in, int tile, gx::DevW
     if ((rc = factor_at(t, hvm.data() + off, hva.data() + off, hp.data(), hq.data()))) return rc;
   }
   // the solve ring is sized for FP64 values: the mixed mode's FP64 re-solve uses the same lists
-  const int rows = pick_ring_rows(t.sched.max_col, tile, sizeof(double), t.ring_req);
+  const int rows = pick_ring_rows(lu_cap(t.sched.max_col, tile), tile, sizeof(double), t.ring_req);
   if (rows < 0)
     return fail(GX_ERR_UNSUPPORTED, "longest L+U column exceeds the LDS staging limit of the device refactor");
   if ((rc = upload_topo(t, gx::ring_window_of(rows, tile), tile))) return rc;
@@ -1916,6 +1953,40 @@ int gx_topo_back_levels(const gx_topo* t, int tile, int ring_rows, int* items, l
   return (int)it.size();
 }
 
+int gx_topo_forward_levels(const gx_topo* t, int tile, int ring_rows, int* items, long long cap, int* steps) {
+  if (!t || (tile != 4 && tile != 8 && tile != 16) || ring_rows < 2) return fail(GX_ERR_ARG, "bad arguments");
+  if (!t->factored) return fail(GX_ERR_STATE, "not factored yet");
+  std::vector<int> it;
+  int st = 0;
+  if (!build_forward_levels(t->lu, gx::kLanes / tile, ring_rows, it, st))
+    return fail(GX_ERR_UNSUPPORTED, "system beyond the level schedule's encoding");
+  if (steps) *steps = st;
+  if ((long long)it.size() > cap || !items) return -(int)std::min<size_t>(it.size(), 0x7fffffff);
+  std::memcpy(items, it.data(), sizeof(int) * it.size());
+  return (int)it.size();
+}
+
+int gx_topo_refactor_parts(const gx_topo* t, int tile, long long* sizes, int* part_a, int* pent_pos, int* pent_div,
+                           int* slots) {
+  if (!t || (tile != 4 && tile != 8 && tile != 16) || !sizes) return fail(GX_ERR_ARG, "bad arguments");
+  if (!t->factored) return fail(GX_ERR_STATE, "not factored yet");
+  PartPlan plan;
+  const int pcap = lu_cap(t->sched.max_col, tile);
+  int rc = build_parts(t->lu, t->sched, gx::kLanes / tile, pcap, plan);
+  if (rc) return rc;
+  sizes[0] = plan.n_parts;
+  sizes[1] = (long long)plan.pent_pos.size();
+  sizes[2] = (long long)plan.slots.size();
+  sizes[3] = pcap;
+  if (part_a && pent_pos && pent_div && slots) {
+    std::memcpy(part_a, plan.part_a.data(), sizeof(int) * plan.part_a.size());
+    std::memcpy(pent_pos, plan.pent_pos.data(), sizeof(int) * plan.pent_pos.size());
+    std::memcpy(pent_div, plan.pent_div.data(), sizeof(int) * plan.pent_div.size());
+    std::memcpy(slots, plan.slots.data(), sizeof(int) * plan.slots.size());
+  }
+  return GX_OK;
+}
+
 int gx_topo_set_zip(gx_topo* t, const double* zip) {
   if (!t) return fail(GX_ERR_ARG, "null topology");
   if (zip) {
@@ -2128,7 +2199,7 @@ constexpr int kSysTile = 4;
 int sys_to_device(gx_sys* s) {
   if (s->on_device) return GX_OK;
   const int n = s->n, nlu = s->lu.nnz();
-  const int rows = pick_ring_rows(s->sched.max_col, kSysTile, sizeof(double), 0);
+  const int rows = pick_ring_rows(lu_cap(s->sched.max_col, kSysTile), kSysTile, sizeof(double), 0);
   if (rows < 0) return fail(GX_ERR_UNSUPPORTED, "longest L+U column exceeds the LDS staging limit");
   int rc;
   if ((rc = upload_sched(s->lu, s->sched, gx::ring_window_of(rows, kSysTile), kSysTile, s->ds, s->dt)) ||

@@ -1376,11 +1376,239 @@ __device__ __forceinline__ void lu64_tile(const DevTopo& t, const DevWork& w, in
 }
 
 // One launch per batch: one wavefront per tile (XCD-aware numbering).
+// Refactor by parts, then the forward and back substitutions by levels; one wavefront per tile.
+//   LDS: two part buffers xc[2][cap+1][TW+1] (part p in buffer p & 1: the previous part stays
+//   readable for the updates that use its L values) and ring[R][TW+1] for the solves.
+// A part is a set of columns of one elimination level (independent of each other), laid one after
+// the other in its buffer: stage A at their entries, run the part's update rounds (the chunk stream
+// of the column-by-column kernel, host-scheduled), divide each L entry by its column's pivot, write
+// back.  The first kPartRegs * G entries of a part are staged through registers loaded a part ahead
+// (their positions two parts ahead); longer parts load the rest where they are used.
+template <typename T, int TW, bool kFactor>
+__device__ __forceinline__ void lu_tile_parts(const DevTopo& t, const DevWork& w, int tile, int cap, int iter) {
+  constexpr int G = kLanes / TW;
+  constexpr int S = TW + 1;
+  constexpr int NB = kChunkRounds;
+  constexpr int MP = kPartRegs;
+  constexpr int MT = 4;  // long-part entries per lane group and batch
+  const int W = t.window;
+  const int R = W + kRingGap + 1;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T* xcb = reinterpret_cast<T*>(smem_raw);
+  const int xstride = (cap + 1) * S;
+  T* ring = xcb + (kFactor ? 2 * (size_t)xstride : 0);
+  const int g = grp<TW>(), s = sub<TW>();
+  if (!tile_running<TW>(w.status, tile)) return;
+  const int sc = tile * TW + s;
+  const int nl = t.nnz_lu;
+  char* __restrict__ lub = reinterpret_cast<char*>(reinterpret_cast<T*>(w.lu) + lu_at(0, w.n_tiles, tile, TW) + s);
+  T* __restrict__ yb = reinterpret_cast<T*>(w.y) + (size_t)tile * t.n * TW;
+  const unsigned pstb = (unsigned)(w.n_tiles * TW * sizeof(T));
+  auto LU = [&](int p) -> T& { return *reinterpret_cast<T*>(lub + (size_t)(unsigned)p * pstb); };
+  bool ok = true;
+  bool overflow = false;
+  auto cl = [](int v, int hi) { return v < 0 ? 0 : (v > hi ? hi : v); };
+  if (kFactor) {
+    const int np = t.n_parts;
+    const int* __restrict__ ppos = t.pent_pos;
+    const int* __restrict__ pdiv = t.pent_div;
+    // staged entries o = g + G * i of a part: c* = the part being factored (values in registers),
+    // x* = the next part (values in flight), y* = the one after (positions in flight)
+    int cpos[MP], cdiv[MP], xpos[MP], xdiv[MP], ypos[MP], ydiv[MP];
+    T cval[MP], xval[MP];
+    auto ent_issue = [&](const int4& m, int (&pp)[MP], int (&dd)[MP]) {
+#pragma unroll
+      for (int i = 0; i < MP; ++i) {
+        const int o = g + G * i;
+        const int e = m.x + (o < m.y ? o : 0);
+        pp[i] = ppos[e];
+        dd[i] = pdiv[e];
+      }
+    };
+    auto val_issue_p = [&](const int (&pp)[MP], T (&vv)[MP]) {
+#pragma unroll
+      for (int i = 0; i < MP; ++i) vv[i] = LU(pp[i]);
+    };
+    // update-round chunk stream over all parts (as in the column-by-column kernel)
+    int2 sA[NB], sB[NB], sC[NB];
+    T vA[NB], vB[NB];
+    const int2* __restrict__ slb = t.part_slots + g;
+    auto slots_issue = [&](int q, int2 (&d)[NB]) {
+#pragma unroll
+      for (int r = 0; r < NB; ++r) d[r] = slb[(unsigned)((q * NB + r) * G)];
+    };
+    auto vals_issue = [&](const int2 (&sm)[NB], T (&d)[NB]) {
+#pragma unroll
+      for (int r = 0; r < NB; ++r) d[r] = LU(sm[r].x);
+    };
+    int4 m1 = vload4(t.part_a, cl(1, np - 1)), m2 = vload4(t.part_a, cl(2, np - 1));
+    {
+      const int4 m0 = uni4(vload4(t.part_a, 0));
+      ent_issue(m0, cpos, cdiv);
+      ent_issue(uni4(m1), xpos, xdiv);
+      val_issue_p(cpos, cval);
+    }
+    slots_issue(0, sA);
+    slots_issue(1, sB);
+    vals_issue(sA, vA);
+    drain();
+    int4 mk = uni4(vload4(t.part_a, 0));
+    for (int p = 0; p < np; ++p) {
+      const int pb = mk.x, len = mk.y, ch0 = mk.z, nr = mk.w;
+      const int4 mx = uni4(m1);
+      m1 = m2;
+      m2 = vload4(t.part_a, cl(p + 3, np - 1));
+      T* xc = xcb + (p & 1) * xstride;
+      const T* xprev = xcb + ((p & 1) ^ 1) * xstride;
+      if (len > cap) {  // the host sizes cap to the largest part; kept as a guard
+        overflow = true;
+        break;
+      }
+      // stage A: registered entries, then (long parts) the rest
+#pragma unroll
+      for (int i = 0; i < MP; ++i) {
+        const int o = g + G * i;
+        if (o < len) xc[o * S + s] = cval[i];
+      }
+      for (int b = G * MP; b < len; b += G * MT) {
+        int pp[MT];
+        T tv[MT];
+#pragma unroll
+        for (int u = 0; u < MT; ++u) {
+          const int o = b + G * u + g;
+          pp[u] = ppos[pb + (o < len ? o : 0)];
+        }
+#pragma unroll
+        for (int u = 0; u < MT; ++u) tv[u] = LU(pp[u]);
+#pragma unroll
+        for (int u = 0; u < MT; ++u) {
+          const int o = b + G * u + g;
+          if (o < len) xc[o * S + s] = tv[u];
+        }
+      }
+      // loads for the next parts
+      val_issue_p(xpos, xval);
+      {
+        const int4 my = uni4(m1);
+        ent_issue(my, ypos, ydiv);
+      }
+      memory_order_point();  // the staged values before the rounds read them
+      const int nch = nr > NB ? (nr + NB - 1) / NB : 1;
+      for (int c = 0; c < nch; ++c) {
+        slots_issue(ch0 + c + 2, sC);
+        vals_issue(sB, vB);
+        const int rleft = nr - c * NB;
+#pragma unroll
+        for (int r = 0; r < NB; ++r) {
+          if (r < rleft) {
+            const int wd = sA[r].y;
+            const int tg = wd & 0xffff, su = (wd >> 16) & 0x7fff;
+            T lv = vA[r];
+            if (wd < 0) lv = xprev[sA[r].x * S + s];
+            const T xt = xc[tg * S + s], xu = xc[su * S + s];
+            xc[tg * S + s] = xt - lv * xu;
+          }
+        }
+        if (c + 1 < nch) {
+#pragma unroll
+          for (int r = 0; r < NB; ++r) {
+            sA[r] = sB[r];
+            sB[r] = sC[r];
+            vA[r] = vB[r];
+          }
+        }
+      }
+      memory_order_point();  // the rounds' writes before the division reads
+      // divide the L entries by their columns' pivots, write back
+#pragma unroll
+      for (int i = 0; i < MP; ++i) {
+        const int o = g + G * i;
+        if (o < len) {
+          T v = xc[o * S + s];
+          const int d = cdiv[i];
+          if (d >= 0) v = v / xc[d * S + s];
+          if (d == -2 && is_zero(v)) ok = false;
+          cval[i] = v;
+        }
+      }
+      memory_order_point();  // every pivot read before the divided values are stored
+#pragma unroll
+      for (int i = 0; i < MP; ++i) {
+        const int o = g + G * i;
+        if (o < len) {
+          if (cdiv[i] >= 0) xc[o * S + s] = cval[i];
+          LU(cpos[i]) = cval[i];
+        }
+      }
+      for (int b = G * MP; b < len; b += G * MT) {
+        int pp[MT], dd[MT];
+        T tv[MT];
+#pragma unroll
+        for (int u = 0; u < MT; ++u) {
+          const int o = b + G * u + g;
+          pp[u] = ppos[pb + (o < len ? o : 0)];
+          dd[u] = pdiv[pb + (o < len ? o : 0)];
+        }
+#pragma unroll
+        for (int u = 0; u < MT; ++u) {
+          const int o = b + G * u + g;
+          tv[u] = xc[(o < len ? o : 0) * S + s];
+          if (dd[u] >= 0) tv[u] = tv[u] / xc[dd[u] * S + s];
+          if (o < len && dd[u] == -2 && is_zero(tv[u])) ok = false;
+        }
+        memory_order_point();
+#pragma unroll
+        for (int u = 0; u < MT; ++u) {
+          const int o = b + G * u + g;
+          if (o < len) {
+            if (dd[u] >= 0) xc[o * S + s] = tv[u];
+            LU(pp[u]) = tv[u];
+          }
+        }
+      }
+      // the chunk stream moves on to the next part's first chunk; the staged registers rotate
+#pragma unroll
+      for (int r = 0; r < NB; ++r) {
+        sA[r] = sB[r];
+        sB[r] = sC[r];
+        vA[r] = vB[r];
+      }
+#pragma unroll
+      for (int i = 0; i < MP; ++i) {
+        cpos[i] = xpos[i];
+        cdiv[i] = xdiv[i];
+        cval[i] = xval[i];
+        xpos[i] = ypos[i];
+        xdiv[i] = ydiv[i];
+      }
+      mk = mx;
+      memory_order_point();  // this part's LDS and memory writes before the next part's reads
+    }
+  }
+  if (!overflow) {
+    memory_order_point();
+    level_solve<T, TW, false>(t.fwd_items + g, t.fwd_steps, R, nl, lub, pstb, yb + s, ring + s, ok);
+    memory_order_point();
+    level_solve<T, TW, true>(t.back_items + g, t.back_steps, R, nl, lub, pstb, yb + s, ring + s, ok);
+  }
+  // a zero pivot seen by any group of the scenario, or a non-finite solution, makes it Singular
+  ok = all_groups<TW>(ok && !overflow);
+  if (g == 0 && !ok && w.status[sc] == kRunning) {
+    w.status[sc] = kSingular;
+    w.iters[sc] = iter + 1;
+  }
+}
+
 template <typename T, int TW, bool kFactor>
 __global__ __launch_bounds__(64) void k_lu(DevTopo t, DevWork w, int cap, int iter) {
   const int tile = xcd_tile(blockIdx.x, gridDim.x);
   if constexpr (TW == kWideTile) lu64_tile<T, kFactor>(t, w, tile, cap, iter);
   else lu_tile<T, TW, kFactor>(t, w, tile, cap, iter);
+}
+
+template <typename T, int TW, bool kFactor>
+__global__ __launch_bounds__(64) void k_lu_parts(DevTopo t, DevWork w, int cap, int iter) {
+  lu_tile_parts<T, TW, kFactor>(t, w, xcd_tile(blockIdx.x, gridDim.x), cap, iter);
 }
 
 // One persistent launch over the tiles of several requests (topologies of any size): each
@@ -1402,8 +1630,12 @@ __global__ __launch_bounds__(64) void k_lu_queue(const DevTopo* __restrict__ top
     const DevWork& w = works[item.x];
     const int cap = t.max_col < 1 ? 1 : t.max_col;
     const int iter = w.iter_dev ? *w.iter_dev : 0;
-    if constexpr (TW == kWideTile) lu64_tile<T, kFactor>(t, w, item.y, cap, iter);
-    else lu_tile<T, TW, kFactor>(t, w, item.y, cap, iter);
+    if constexpr (TW == kWideTile) {
+      lu64_tile<T, kFactor>(t, w, item.y, cap, iter);
+    } else {
+      if (t.part_a) lu_tile_parts<T, TW, kFactor>(t, w, item.y, cap, iter);
+      else lu_tile<T, TW, kFactor>(t, w, item.y, cap, iter);
+    }
     __syncthreads();  // the next item reuses the LDS
   }
 }
@@ -1669,9 +1901,21 @@ void allow_lds(K kernel, size_t bytes, size_t& granted) {
 
 template <typename T, int TW>
 void run_lu(const DevTopo& t, const DevWork& w, bool factor, int iter, hipStream_t stream) {
-  static size_t granted_f = 65536, granted_s = 65536;
+  static size_t granted_f = 65536, granted_s = 65536, granted_pf = 65536, granted_ps = 65536;
   const int cap = t.max_col < 1 ? 1 : t.max_col;
   const size_t shmem = lu_bytes<T>(t, TW, factor);
+  if constexpr (TW != kWideTile) {
+    if (t.part_a) {
+      if (factor) {
+        allow_lds(k_lu_parts<T, TW, true>, shmem, granted_pf);
+        hipLaunchKernelGGL((k_lu_parts<T, TW, true>), dim3(w.n_tiles), dim3(64), shmem, stream, t, w, cap, iter);
+      } else {
+        allow_lds(k_lu_parts<T, TW, false>, shmem, granted_ps);
+        hipLaunchKernelGGL((k_lu_parts<T, TW, false>), dim3(w.n_tiles), dim3(64), shmem, stream, t, w, cap, iter);
+      }
+      return;
+    }
+  }
   if (factor) {
     allow_lds(k_lu<T, TW, true>, shmem, granted_f);
     hipLaunchKernelGGL((k_lu<T, TW, true>), dim3(w.n_tiles), dim3(64), shmem, stream, t, w, cap, iter);

@@ -107,6 +107,19 @@ int gx_topo_lu_pattern(const gx_topo* t, int* cp, int* ulen, int* row);
  * cap is too small or items is NULL. */
 int gx_topo_back_levels(const gx_topo* t, int tile, int ring_rows, int* items, long long cap, int* steps);
 
+/* The forward substitution's level schedule for the same tiles (no counterpart in the reference;
+ * for tests), in the same format: an END stores its sum (no division, payload 0). */
+int gx_topo_forward_levels(const gx_topo* t, int tile, int ring_rows, int* items, long long cap, int* steps);
+
+/* The refactor-by-parts schedule of narrow tiles of `tile` scenarios (no counterpart in the
+ * reference; for tests).  sizes[4] receives the part count, the part entries, the ints of the round
+ * slots and the staging capacity; when all four arrays are given (sized from a first call) they
+ * receive part_a[4 * parts] (first entry, entries, first chunk, rounds of each part), pent_pos and
+ * pent_div per entry (the packed L+U position; -1 U entry, -2 pivot, else the staging offset of the
+ * column's pivot) and the slots ({x, word} per round and lane group, 16 rounds per chunk). */
+int gx_topo_refactor_parts(const gx_topo* t, int tile, long long* sizes, int* part_a, int* pent_pos, int* pent_div,
+                           int* slots);
+
 /* ---- batched Newton-Raphson ---- */
 /* Replaces BatchSolver::solve (src/batch.rs:198-289) driving PersistentSolver::solve_with_stats
  * (src/solver.rs:444-475) / newton_raphson_cached (src/solver.rs:639-736) once per scenario.

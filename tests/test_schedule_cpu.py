@@ -5,7 +5,12 @@ step, and every row's x goes to y and to the LDS ring row its START named at the
 step, as the kernel does.  A dependency the schedule got wrong reads a stale value
 and changes the answer.  The answer must equal, bit for bit, the column-oriented back substitution
 (x_k = y_k / U(k,k), then y_r -= U(r,k) x_k for the U rows r of column k, k descending), which
-applies every row's subtractions in the same order."""
+applies every row's subtractions in the same order.
+
+The forward substitution's levels (gx_topo_forward_levels) run through the same emulation with no
+division, against the column-oriented forward substitution; the refactor-by-parts schedule
+(gx_topo_refactor_parts) runs through an emulation of its part loop (stage, update rounds with the
+value loads a chunk ahead, division, write-back) against the column-by-column left-looking refactor."""
 import numpy as np
 import pytest
 
@@ -25,14 +30,14 @@ def _factored(net):
     return t
 
 
-def _schedule(topo, tile, ring_rows):
+def _schedule(topo, tile, ring_rows, forward=False):
     L = _lib.lib()
+    fn = L.gx_topo_forward_levels if forward else L.gx_topo_back_levels
     steps = np.zeros(1, np.int32)
-    need = L.gx_topo_back_levels(topo._h, tile, ring_rows, None, 0, ptr(steps, _lib._i32p))
+    need = fn(topo._h, tile, ring_rows, None, 0, ptr(steps, _lib._i32p))
     assert need < 0
     items = np.zeros(-need, np.int32)
-    got = check(L.gx_topo_back_levels(topo._h, tile, ring_rows, ptr(items, _lib._i32p), items.size,
-                                      ptr(steps, _lib._i32p)))
+    got = check(fn(topo._h, tile, ring_rows, ptr(items, _lib._i32p), items.size, ptr(steps, _lib._i32p)))
     assert got == items.size
     G = 64 // tile
     return items.reshape(int(steps[0]), P, G, 2), G
@@ -48,7 +53,7 @@ def _column_oriented(cp, ulen, row, lu, y):
     return x
 
 
-def _emulate(items, G, lu, y, ring_rows):
+def _emulate(items, G, lu, y, ring_rows, divide=True):
     """Runs the items as the kernel does for one scenario (offsets are row * TW and slot * S)."""
     steps = items.shape[0]
     assert steps % (2 * D) == 0
@@ -88,7 +93,7 @@ def _emulate(items, G, lu, y, ring_rows):
                     assert g not in pend
                     pend[g] = (acc[g], u[i, g], xrow[g], xslot[g])
         for g, (a, pv, row, sl) in pend.items():  # the step's row ends
-            xk = a / pv
+            xk = a / pv if divide else a
             mem[row // TW] = xk
             lds[sl] = xk
         loaded[q + D] = values(q + D)  # the set refilled after this step
@@ -130,3 +135,130 @@ def test_back_levels_arguments():
     assert L.gx_topo_back_levels(topo._h, 3, 64, None, 0, None) == _lib.GX_ERR_ARG
     fresh = Topology.from_network(config_grid(1))
     assert L.gx_topo_back_levels(fresh._h, 4, 64, None, 0, None) == _lib.GX_ERR_STATE
+
+
+def _forward_column_oriented(cp, ulen, row, lu, f):
+    y = f.copy()
+    for j in range(len(ulen)):
+        for q in range(cp[j] + ulen[j] + 1, cp[j + 1]):
+            y[row[q]] = y[row[q]] - lu[q] * y[j]
+    return y
+
+
+@pytest.mark.parametrize("grid,tile,ring_rows", [("c1", 4, 64), ("c1", 16, 256), ("c2", 4, 512), ("c2", 8, 64),
+                                                 ("g", 4, 64)])
+def test_forward_levels_equal_column_oriented(grid, tile, ring_rows):
+    net = {"c1": lambda: config_grid(1), "c2": lambda: config_grid(2), "g": lambda: generate_grid(700, 900, 5)}[grid]()
+    topo = _factored(net)
+    cp, ulen, row = topo.lu_pattern()
+    items, G = _schedule(topo, tile, ring_rows, forward=True)
+    n = ulen.size
+    rng = np.random.default_rng(12)
+    lu = rng.uniform(-0.3, 0.3, cp[-1])
+    f = rng.uniform(-1.0, 1.0, n)
+    want = _forward_column_oriented(cp, ulen, row, lu, f)
+    got = _emulate(items, G, lu, f, ring_rows, divide=False)
+    assert np.array_equal(got, want)
+    kinds = (items[..., 0].astype(np.int64) & 0xFFFFFFFF) >> 29
+    assert np.count_nonzero(kinds == END) == n
+    assert np.count_nonzero(kinds == ENTRY) == int((np.diff(cp) - ulen - 1).sum())
+    assert (np.count_nonzero(kinds == END, axis=1) <= 1).all()
+
+
+NB = 16  # update rounds per chunk (kChunkRounds)
+
+
+def _parts(topo, tile):
+    L = _lib.lib()
+    sizes = np.zeros(4, np.int64)
+    none = ctypes_null = None
+    check(L.gx_topo_refactor_parts(topo._h, tile, ptr(sizes, _lib._i64p), none, none, none, ctypes_null))
+    n_parts, n_ent, n_slot, cap = (int(v) for v in sizes)
+    pa = np.zeros(4 * n_parts, np.int32)
+    pos = np.zeros(n_ent, np.int32)
+    div = np.zeros(n_ent, np.int32)
+    slots = np.zeros(n_slot, np.int32)
+    check(L.gx_topo_refactor_parts(topo._h, tile, ptr(sizes, _lib._i64p), ptr(pa, _lib._i32p), ptr(pos, _lib._i32p),
+                                   ptr(div, _lib._i32p), ptr(slots, _lib._i32p)))
+    G = 64 // tile
+    return pa.reshape(n_parts, 4), pos, div, slots.reshape(-1, G, 2), cap
+
+
+def _sequential_refactor(cp, ulen, row, a):
+    """Left-looking refactor with the pivots in place: column k gathers A(:, k), subtracts
+    L(:, j) x_j for its U entries j in ascending order, then divides its L entries by the pivot."""
+    n = ulen.size
+    lu = a.copy()
+    x = np.zeros(n)
+    for k in range(n):
+        c0, c1 = cp[k], cp[k + 1]
+        rows = row[c0:c1]
+        x[rows] = lu[c0:c1]
+        for u in range(ulen[k]):
+            j = row[c0 + u]
+            l0, l1 = cp[j] + ulen[j] + 1, cp[j + 1]
+            x[row[l0:l1]] = x[row[l0:l1]] - lu[l0:l1] * x[j]
+        vals = x[rows].copy()
+        piv = c0 + ulen[k]
+        vals[ulen[k] + 1:] = vals[ulen[k] + 1:] / vals[ulen[k]]
+        lu[c0:c1] = vals
+        assert row[piv] == k
+        x[rows] = 0.0
+    return lu
+
+
+def _emulate_parts(pa, pos, div, slots, a, cap):
+    """The part loop of the kernel for one scenario.  A non-flagged L value of chunk c of part p is
+    loaded a chunk ahead: after the write-back of part p - 2 (c = 0) or p - 1 (c > 0); a position not
+    written back by then still holds its Jacobian value.  Staging rows hold NaN until staged."""
+    mem = a.copy()
+    owner = np.full(a.size, np.iinfo(np.int64).max)  # the part that wrote each position back
+    buf = [np.full(cap + 1, np.nan), np.full(cap + 1, np.nan)]
+    for p, (pb, ln, ch0, nr) in enumerate(pa):
+        assert 0 < ln <= cap
+        xc, xprev = buf[p & 1], buf[(p & 1) ^ 1]
+        xc[:] = np.nan
+        ent = np.arange(pb, pb + ln)
+        assert (owner[pos[ent]] == np.iinfo(np.int64).max).all()
+        xc[:ln] = mem[pos[ent]]
+        for r in range(nr):
+            seen = p - 2 if r < NB else p - 1
+            x = slots[ch0 * NB + r, :, 0].astype(np.int64)
+            wd = slots[ch0 * NB + r, :, 1].astype(np.int64) & 0xFFFFFFFF
+            tg, su, fl = wd & 0xFFFF, (wd >> 16) & 0x7FFF, (wd >> 31) & 1
+            real = tg != ln  # empty slots target the row past the part
+            assert (tg <= ln).all() and (su[real] < ln).all()
+            assert np.unique(tg[real]).size == np.count_nonzero(real)
+            assert not np.isin(tg[real], su[real]).any()
+            xm = np.where(fl == 1, 0, x)
+            lv = np.where(owner[xm] <= seen, mem[xm], a[xm])
+            lv = np.where(fl == 1, xprev[np.where(fl == 1, x, 0)], lv)
+            xt, xu = xc[tg], xc[su]
+            xc[tg] = xt - lv * xu
+        d = div[ent]
+        v = xc[:ln].copy()
+        m = d >= 0
+        v[m] = v[m] / xc[d[m]]
+        xc[:ln][m] = v[m]
+        mem[pos[ent]] = v
+        owner[pos[ent]] = p
+    return mem
+
+
+@pytest.mark.parametrize("grid,tile", [("c1", 4), ("c1", 8), ("c1", 16), ("c2", 4), ("g", 8)])
+def test_refactor_parts_equal_column_by_column(grid, tile):
+    net = {"c1": lambda: config_grid(1), "c2": lambda: config_grid(2), "g": lambda: generate_grid(700, 900, 5)}[grid]()
+    topo = _factored(net)
+    cp, ulen, row = topo.lu_pattern()
+    pa, pos, div, slots, cap = _parts(topo, tile)
+    n = ulen.size
+    # every entry of L+U is staged by exactly one part
+    assert np.array_equal(np.sort(pos), np.arange(cp[-1]))
+    rng = np.random.default_rng(13)
+    a = rng.uniform(-0.2, 0.2, cp[-1])
+    a[cp[:-1] + ulen] = rng.uniform(3.0, 4.0, n) * rng.choice([-1.0, 1.0], n)
+    want = _sequential_refactor(cp, ulen, row, a)
+    got = _emulate_parts(pa, pos, div, slots, a, cap)
+    assert np.array_equal(got, want)
+    # fewer parts than columns: the columns of a level are factored together
+    assert pa.shape[0] < n
