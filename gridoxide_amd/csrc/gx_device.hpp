@@ -20,6 +20,7 @@ constexpr int kChunkRounds = 16;  // refactor update rounds fetched into registe
 constexpr int kMaxLds = 160 * 1024;  // LDS bytes per workgroup on gfx950
 
 constexpr int kWideTile = 64;     // one lane per scenario: every index of the LU kernel is uniform
+constexpr int kLuBlock = 2;       // tiles whose L+U values interleave position by position
 constexpr int kRoundItems = 16;   // update items per round of the wide-tile LU kernel
 constexpr int kWideStage = 16;    // entries of a column the wide-tile LU kernel prefetches
 constexpr int kWideFar = 4;       // far entries of an entering row the wide-tile LU kernel prefetches
@@ -241,7 +242,7 @@ void launch_finish(const DevWork& w, hipStream_t stream);
 // Mixed precision: restarts every scenario that did not converge from its starting point, for an
 // FP64 re-solve (status back to running, count in w.fallback[0]).
 void launch_fallback(const DevTopo& t, const DevWork& w, hipStream_t stream);
-void launch_lane0(const double* src, double* dst, int count, bool to_tile, int tile, hipStream_t stream);
+void launch_lane0(const double* src, double* dst, int count, bool to_tile, int stride, hipStream_t stream);
 void launch_island_max(const DevTopo& t, const DevWork& w, const int* comp, int n_comp, unsigned long long* out,
                        hipStream_t stream);
 void launch_linear_guess(const DevTopo& t, const DevLg& lg, const DevTopo& lgt, const DevWork& w, void* lu,

@@ -612,6 +612,13 @@ struct DevSched {
   }
 };
 
+// Lanes of an L+U buffer for `lanes` lanes of tiles of `tile`: the tile count rounded up to whole
+// blocks of gx::kLuBlock tiles (the device layout interleaves a block's tiles).
+size_t lu_lanes(size_t lanes, int tile) {
+  const size_t tiles = (lanes + tile - 1) / tile;
+  return (tiles + gx::kLuBlock - 1) / gx::kLuBlock * gx::kLuBlock * (size_t)tile;
+}
+
 // Narrow tiles refactor by parts (lu_tile_parts) when this is set; else column by column.
 constexpr bool kUseParts = true;
 
@@ -1451,7 +1458,7 @@ int build_lg(gx_topo& t, Workspace& ws, size_t lanes, int tile) {
     const int rows = pick_ring_rows(lu_cap(t.lg_sched.max_col, tile), tile, 16, 0);
     if (rows < 0) return fail(GX_ERR_UNSUPPORTED, "linear guess L+U column exceeds the LDS staging limit");
     if ((rc = upload_sched(t.lg_lu, t.lg_sched, gx::ring_window_of(rows, tile), tile, t.lg_ds, t.dlgt))) return rc;
-    if ((rc = ws.lg_lu.reserve(lanes * (size_t)t.dlg.nnz_lu * 16)) || (rc = ws.lg_y.reserve(lanes * t.dlg.m * 16)) ||
+    if ((rc = ws.lg_lu.reserve(lu_lanes(lanes, tile) * (size_t)t.dlg.nnz_lu * 16)) || (rc = ws.lg_y.reserve(lanes * t.dlg.m * 16)) ||
         (rc = ws.lg_status.reserve(lanes * sizeof(int))) || (rc = ws.lg_iters.reserve(lanes * sizeof(int))))
       return rc;
   }
@@ -1603,7 +1610,7 @@ int stage_batch(gx_topo& t, Workspace& ws, const BatchIn& in, int tile, gx::DevW
     return fail(GX_ERR_UNSUPPORTED, "longest L+U column exceeds the LDS staging limit of the device refactor");
   if ((rc = upload_topo(t, gx::ring_window_of(rows, tile), tile))) return rc;
   // FP64-sized in both modes: the mixed mode's FP64 re-solve reuses the same buffer
-  if ((rc = ws.lu.reserve(lanes * (size_t)t.dt.nnz_lu * 8))) return rc;
+  if ((rc = ws.lu.reserve(lu_lanes(lanes, tile) * (size_t)t.dt.nnz_lu * 8))) return rc;
   w.lu = ws.lu.p;
   if (in.fp32) {
     const size_t nbytes = lanes * nb * sizeof(double);
@@ -2203,7 +2210,7 @@ int sys_to_device(gx_sys* s) {
   if (rows < 0) return fail(GX_ERR_UNSUPPORTED, "longest L+U column exceeds the LDS staging limit");
   int rc;
   if ((rc = upload_sched(s->lu, s->sched, gx::ring_window_of(rows, kSysTile), kSysTile, s->ds, s->dt)) ||
-      (rc = s->d_lu.reserve((size_t)nlu * kSysTile * sizeof(double))) ||
+      (rc = s->d_lu.reserve((size_t)nlu * lu_lanes(kSysTile, kSysTile) * sizeof(double))) ||
       (rc = s->d_y.reserve((size_t)n * kSysTile * sizeof(double))) ||
       (rc = s->d_status.reserve(kSysTile * sizeof(int))) || (rc = s->d_iters.reserve(kSysTile * sizeof(int))) ||
       (rc = s->d_vals.reserve((size_t)nlu * sizeof(double))) || (rc = s->d_rhs.reserve((size_t)n * sizeof(double))))
@@ -2248,7 +2255,8 @@ int gx_sys_factor_and_solve_values(gx_sys* s, const double* vals, const double* 
   GX_HIP(hipMemcpy(s->d_vals.p, packed.data(), sizeof(double) * nlu, hipMemcpyHostToDevice));
   GX_HIP(hipMemcpy(s->d_rhs.p, b.data(), sizeof(double) * n, hipMemcpyHostToDevice));
   GX_HIP(hipMemcpy(s->d_status.p, st.data(), sizeof(int) * kSysTile, hipMemcpyHostToDevice));
-  gx::launch_lane0(s->d_vals.as<double>(), s->d_lu.as<double>(), nlu, true, kSysTile, nullptr);
+  // lane 0 of tile 0: consecutive positions lie kLuBlock tiles of values apart
+  gx::launch_lane0(s->d_vals.as<double>(), s->d_lu.as<double>(), nlu, true, kSysTile * gx::kLuBlock, nullptr);
   gx::launch_lane0(s->d_rhs.as<double>(), s->d_y.as<double>(), n, true, kSysTile, nullptr);
   gx::launch_lu(s->dt, s->w, false, true, 0, nullptr);
   gx::launch_lane0(s->d_y.as<double>(), s->d_rhs.as<double>(), n, false, kSysTile, nullptr);
@@ -2487,7 +2495,8 @@ long long gx_engine_request_bytes(const gx_topo* t, int n_scen, int max_iter, in
   const long long lanes = (long long)((n_scen + tile - 1) / tile) * tile;
   const long long nb = t->n_bus, n = t->n;
   const long long nnz_lu = t->factored ? t->lu.nnz() : 4 * (long long)t->jkind.size();
-  long long b = lanes * (8 * nb * 8 + 3 * n * 8 + nnz_lu * 8 + (long long)std::max(max_iter, 1) * 8 + 16) +
+  long long b = lanes * (8 * nb * 8 + 3 * n * 8 + (long long)std::max(max_iter, 1) * 8 + 16) +
+                (long long)lu_lanes((size_t)lanes, tile) * nnz_lu * 8 +
                 (long long)n_scen * nb * 8 + 4LL * std::max(max_iter, 1) + 64;
   if (mode == GX_MODE_MIXED) b += lanes * 4 * nb * 8;
   if (flags & GX_INIT_LINEAR) b += lanes * (4 * nnz_lu / 2 * 16 + nb * 16 + 8);

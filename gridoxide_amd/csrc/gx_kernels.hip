@@ -56,12 +56,15 @@ __device__ __forceinline__ bool tile_running(const int* status, int tile) {
   return __any(status[tile * TW + sub<TW>()] == kRunning);
 }
 
-// L+U values are stored position-major across tiles: entry p of lane s of tile t at
-// (p * n_tiles + t) * TW + s.  The wavefronts of all tiles walk the same columns at about the same
-// pace, so neighbouring tiles (one cache line holds 128 bytes of a position) re-read L columns from
-// shared lines; xcd_tile keeps those tiles on one XCD and its L2.
-__device__ __forceinline__ size_t lu_at(int p, int n_tiles, int tile, int TW) {
-  return ((size_t)p * n_tiles + tile) * TW;
+// L+U values: the tiles go in blocks of kLuBlock, each block holding its positions one after the
+// other, each position the block's tiles side by side: entry p of lane s of tile t at
+// ((t / B * nnz + p) * B + t % B) * TW + s.  A wavefront's gathers stay inside its block's region
+// (nnz * B * TW values) instead of striding over all tiles' values, which keeps the address
+// translations a wavefront needs few; a block's two tiles still share the cache lines of a position,
+// and the fill kernels' stores of a position stay B * TW values long.  The allocation rounds the
+// tile count up to a multiple of B.
+__device__ __forceinline__ size_t lu_at(int p, int nnz, int tile, int TW) {
+  return (((size_t)(tile / kLuBlock) * nnz + p) * kLuBlock + tile % kLuBlock) * TW;
 }
 // Workgroup b of a one-dimensional grid of n runs on XCD b mod 8: number the tiles so that each XCD
 // runs a contiguous range of them.
@@ -383,7 +386,7 @@ __global__ __launch_bounds__(256) void k_fill(DevTopo t, DevWork w) {
   bool run = false;  // any scenario of the tile still iterating (the LU kernel skips the others)
   for (int q = 0; q < TW; ++q) run = run || w.status[tile * TW + q] == kRunning;
   if (!run) return;
-  T* __restrict__ lu = reinterpret_cast<T*>(w.lu) + lu_at(p, w.n_tiles, tile, TW) + s;
+  T* __restrict__ lu = reinterpret_cast<T*>(w.lu) + lu_at(p, t.nnz_lu, tile, TW) + s;
   const int kind = t.lu_kind[p];
   double v = 0.0;
   if (kind != kFill) {
@@ -432,8 +435,8 @@ __global__ __launch_bounds__(256) void k_fill_bus(DevTopo t, DevWork w) {
   const Outage og = load_outage(t, w, sc);
   const size_t oi = bb + (size_t)i * TW;
   const double vri = w.vr[oi], vii = w.vi[oi], pci = w.pc[oi], qci = w.qc[oi];
-  T* __restrict__ lu = reinterpret_cast<T*>(w.lu) + lu_at(0, w.n_tiles, tile, TW) + s;
-  const size_t pst = (size_t)w.n_tiles * TW;
+  T* __restrict__ lu = reinterpret_cast<T*>(w.lu) + lu_at(0, t.nnz_lu, tile, TW) + s;
+  const size_t pst = (size_t)kLuBlock * TW;
   for (int e = e0; e < e1; ++e) {
     const int k = t.fb_k[e];
     const int4 pos = t.fb_pos[e];
@@ -468,8 +471,8 @@ __global__ __launch_bounds__(256) void k_fill_zero(DevTopo t, DevWork w) {
   bool run = false;
   for (int q = 0; q < TW; ++q) run = run || w.status[tile * TW + q] == kRunning;
   if (!run) return;
-  T* __restrict__ lu = reinterpret_cast<T*>(w.lu) + lu_at(0, w.n_tiles, tile, TW) + s;
-  const size_t pst = (size_t)w.n_tiles * TW;
+  T* __restrict__ lu = reinterpret_cast<T*>(w.lu) + lu_at(0, t.nnz_lu, tile, TW) + s;
+  const size_t pst = (size_t)kLuBlock * TW;
   for (int f = blockIdx.x; f < t.n_fz; f += gridDim.x) lu[(size_t)t.fz_pos[f] * pst] = (T)0;
 }
 
@@ -649,9 +652,9 @@ __device__ __forceinline__ void lu_tile(const DevTopo& t, const DevWork& w, int 
   const int n = t.n;
   const int nl = t.nnz_lu;
   // this tile's arrays (uniform bases); entry e of lane s at e * TW + s
-  char* __restrict__ lub = reinterpret_cast<char*>(reinterpret_cast<T*>(w.lu) + lu_at(0, w.n_tiles, tile, TW) + s);
+  char* __restrict__ lub = reinterpret_cast<char*>(reinterpret_cast<T*>(w.lu) + lu_at(0, t.nnz_lu, tile, TW) + s);
   T* __restrict__ yb = reinterpret_cast<T*>(w.y) + (size_t)tile * n * TW;
-  const unsigned pstb = (unsigned)(w.n_tiles * TW * sizeof(T));  // bytes between consecutive positions
+  const unsigned pstb = (unsigned)(kLuBlock * TW * sizeof(T));  // bytes between consecutive positions
   // (one multiply-add per address: position x stride + this lane's base)
   auto LU = [&](int p) -> T& { return *reinterpret_cast<T*>(lub + (size_t)(unsigned)p * pstb); };
   auto Y = [&](int r) -> T& { return yb[(unsigned)(r * TW + s)]; };
@@ -1082,8 +1085,8 @@ __device__ __forceinline__ void lu64_tile(const DevTopo& t, const DevWork& w, in
   const int s = threadIdx.x;
   if (!tile_running<TW>(w.status, tile)) return;
   const int sc = tile * TW + s;
-  T* __restrict__ lu = reinterpret_cast<T*>(w.lu) + lu_at(0, w.n_tiles, tile, TW) + s;
-  const size_t pst = (size_t)w.n_tiles * TW;  // elements between consecutive positions
+  T* __restrict__ lu = reinterpret_cast<T*>(w.lu) + lu_at(0, t.nnz_lu, tile, TW) + s;
+  const size_t pst = (size_t)kLuBlock * TW;  // elements between consecutive positions
   T* __restrict__ y = reinterpret_cast<T*>(w.y) + at<TW>(tile, t.n, 0) + s;
   const int* __restrict__ lrow = t.lu_row;
   const int2* __restrict__ uitems = t.uitems;
@@ -1401,9 +1404,9 @@ __device__ __forceinline__ void lu_tile_parts(const DevTopo& t, const DevWork& w
   if (!tile_running<TW>(w.status, tile)) return;
   const int sc = tile * TW + s;
   const int nl = t.nnz_lu;
-  char* __restrict__ lub = reinterpret_cast<char*>(reinterpret_cast<T*>(w.lu) + lu_at(0, w.n_tiles, tile, TW) + s);
+  char* __restrict__ lub = reinterpret_cast<char*>(reinterpret_cast<T*>(w.lu) + lu_at(0, t.nnz_lu, tile, TW) + s);
   T* __restrict__ yb = reinterpret_cast<T*>(w.y) + (size_t)tile * t.n * TW;
-  const unsigned pstb = (unsigned)(w.n_tiles * TW * sizeof(T));
+  const unsigned pstb = (unsigned)(kLuBlock * TW * sizeof(T));
   auto LU = [&](int p) -> T& { return *reinterpret_cast<T*>(lub + (size_t)(unsigned)p * pstb); };
   bool ok = true;
   bool overflow = false;
@@ -1795,8 +1798,8 @@ __global__ __launch_bounds__(256) void k_lg_fill(DevTopo t, DevLg lg, DevWork w,
     lg_status[sc] = sc < w.n_scen ? kRunning : kPadLane;
   }
   const size_t bb = at<TW>(tile, t.n_bus, 0) + s;
-  zd* out = lu + lu_at(0, w.n_tiles, tile, TW) + s;
-  const size_t pst = (size_t)w.n_tiles * TW;
+  zd* out = lu + lu_at(0, lg.nnz_lu, tile, TW) + s;
+  const size_t pst = (size_t)kLuBlock * TW;
   const Outage og = load_outage(t, w, tile * TW + s);
   for (int p = elem_first<TW>(); p < lg.nnz_lu; p += elem_stride<TW>()) {
     zd v{0.0, 0.0};
@@ -1861,12 +1864,12 @@ __global__ void k_finish(DevWork w) {
   }
 }
 
-// Copies a compact vector into / out of scenario 0 of a single-tile interleaved array.
-__global__ void k_lane0(const double* __restrict__ src, double* __restrict__ dst, int count, int to_tile, int tw) {
+// Copies a compact vector into / out of scenario 0 of an interleaved array (elements `stride` apart).
+__global__ void k_lane0(const double* __restrict__ src, double* __restrict__ dst, int count, int to_tile, int stride) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= count) return;
-  if (to_tile) dst[(size_t)e * tw] = src[e];
-  else dst[e] = src[(size_t)e * tw];
+  if (to_tile) dst[(size_t)e * stride] = src[e];
+  else dst[e] = src[(size_t)e * stride];
 }
 
 inline int elem_blocks(int count, int tile) {
@@ -2105,10 +2108,10 @@ void launch_linear_guess(const DevTopo& t, const DevLg& lg, const DevTopo& lgt, 
 
 size_t lg_shmem(const DevTopo& lgt, int tile) { return lu_bytes<zd>(lgt, tile, true); }
 
-void launch_lane0(const double* src, double* dst, int count, bool to_tile, int tile, hipStream_t stream) {
+void launch_lane0(const double* src, double* dst, int count, bool to_tile, int stride, hipStream_t stream) {
   if (count <= 0) return;
   hipLaunchKernelGGL(k_lane0, dim3((count + 255) / 256), dim3(256), 0, stream, src, dst, count, to_tile ? 1 : 0,
-                     tile);
+                     stride);
 }
 
 }  // namespace gx
