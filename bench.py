@@ -96,7 +96,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="config2", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--tile", type=int, default=0, help="scenarios per tile (4, 8, 16; 0 = automatic)")
+    ap.add_argument("--tile", type=int, default=0, help="scenarios per tile (2, 4, 8, 16; 0 = automatic)")
     ap.add_argument("--ring", type=int, default=0, help="LDS solve-window rows (power of two; 0 = default)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run a plain 256 MiB device copy (a known byte count for scaling counter reads)")

@@ -14,13 +14,15 @@
 namespace gx {
 
 constexpr int kLanes = 64;        // wavefront width
-constexpr int kMaxTile = 16;      // widest tile (scenarios per tile): 4, 8 or 16 are compiled
+constexpr int kMaxTile = 16;      // widest narrow tile (scenarios per tile): 2, 4, 8 or 16 are compiled
 constexpr int kRingGap = 32;      // solve ring: columns between fences / rows kept behind the front
 constexpr int kChunkRounds = 16;  // refactor update rounds fetched into registers at a time
 constexpr int kMaxLds = 160 * 1024;  // LDS bytes per workgroup on gfx950
 
 constexpr int kWideTile = 64;     // one lane per scenario: every index of the LU kernel is uniform
-constexpr int kLuBlock = 2;       // tiles whose L+U values interleave position by position
+// tiles whose L+U values interleave position by position (a position's block is 64 bytes of FP64
+// values or more)
+constexpr int lu_block(int tile) { return tile == 2 ? 4 : 2; }
 constexpr int kRoundItems = 16;   // update items per round of the wide-tile LU kernel
 constexpr int kWideStage = 16;    // entries of a column the wide-tile LU kernel prefetches
 constexpr int kWideFar = 4;       // far entries of an entering row the wide-tile LU kernel prefetches

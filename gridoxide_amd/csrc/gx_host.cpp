@@ -613,10 +613,10 @@ struct DevSched {
 };
 
 // Lanes of an L+U buffer for `lanes` lanes of tiles of `tile`: the tile count rounded up to whole
-// blocks of gx::kLuBlock tiles (the device layout interleaves a block's tiles).
+// blocks of gx::lu_block(tile) tiles (the device layout interleaves a block's tiles).
 size_t lu_lanes(size_t lanes, int tile) {
-  const size_t tiles = (lanes + tile - 1) / tile;
-  return (tiles + gx::kLuBlock - 1) / gx::kLuBlock * gx::kLuBlock * (size_t)tile;
+  const size_t tiles = (lanes + tile - 1) / tile, b = (size_t)gx::lu_block(tile);
+  return (tiles + b - 1) / b * b * (size_t)tile;
 }
 
 // Narrow tiles refactor by parts (lu_tile_parts) when this is set; else column by column.
@@ -836,15 +836,19 @@ int pick_ring_rows(int max_col, int tile, size_t elem, int requested) {
   return bytes(rows) <= (size_t)gx::kMaxLds ? rows : -1;
 }
 
-bool valid_tile(int tile) { return tile == 4 || tile == 8 || tile == 16 || tile == gx::kWideTile; }
+bool valid_tile(int tile) { return tile == 2 || tile == 4 || tile == 8 || tile == 16 || tile == gx::kWideTile; }
 bool valid_ring(int rows) { return rows == 0 || (rows >= 64 && rows <= 4096 && (rows & (rows - 1)) == 0); }
 
-// Scenarios per tile when the caller leaves it open: the narrowest tile whose wavefronts stay within
-// a budget per launch (2048 at FP64, 1024 at FP32), else the next wider one.  Measured on the
-// 10,000-bus grid with 8,192 scenarios: at FP64, 4-scenario tiles (two wavefronts per SIMD, run one
-// after the other at the kernel's register count) beat 8-scenario tiles (134 vs 157 ms per LU
-// launch); at FP32 the 8-scenario tiles win (88 vs 134 ms).
+// Scenarios per tile when the caller leaves it open: 2-scenario tiles while their wavefronts fit
+// one per SIMD (1,024), else the narrowest of 4 and 8 whose wavefronts stay within a budget per
+// launch (2048 at FP64, 1024 at FP32), else 16.  Measured: on the 10,000-bus grid with 8,192
+// scenarios, at FP64 4-scenario tiles (two wavefronts per SIMD, run one after the other at the
+// kernel's register count) beat 8-scenario tiles (134 vs 157 ms per LU launch), at FP32 the
+// 8-scenario tiles win (88 vs 134 ms); with 1,024 scenarios 2-scenario tiles beat 4-scenario tiles
+// (27.9 vs 36.4 ms at FP64), while on the 2,000-bus grid with 4,096 scenarios the 2,048
+// wavefronts of 2-scenario tiles lose to 4-scenario tiles (16.1 vs 10.3 ms).
 int auto_tile(int n_scen, bool fp32) {
+  if ((n_scen + 1) / 2 <= 1024) return 2;
   const int resident = fp32 ? 1024 : 2048;
   for (int tw : {4, 8}) if ((n_scen + tw - 1) / tw <= resident) return tw;
   return 16;
@@ -1948,7 +1952,7 @@ int gx_topo_lu_pattern(const gx_topo* t, int* cp, int* ulen, int* row) {
 }
 
 int gx_topo_back_levels(const gx_topo* t, int tile, int ring_rows, int* items, long long cap, int* steps) {
-  if (!t || (tile != 4 && tile != 8 && tile != 16) || ring_rows < 2) return fail(GX_ERR_ARG, "bad arguments");
+  if (!t || (tile != 2 && tile != 4 && tile != 8 && tile != 16) || ring_rows < 2) return fail(GX_ERR_ARG, "bad arguments");
   if (!t->factored) return fail(GX_ERR_STATE, "not factored yet");
   std::vector<int> it;
   int st = 0;
@@ -1961,7 +1965,7 @@ int gx_topo_back_levels(const gx_topo* t, int tile, int ring_rows, int* items, l
 }
 
 int gx_topo_forward_levels(const gx_topo* t, int tile, int ring_rows, int* items, long long cap, int* steps) {
-  if (!t || (tile != 4 && tile != 8 && tile != 16) || ring_rows < 2) return fail(GX_ERR_ARG, "bad arguments");
+  if (!t || (tile != 2 && tile != 4 && tile != 8 && tile != 16) || ring_rows < 2) return fail(GX_ERR_ARG, "bad arguments");
   if (!t->factored) return fail(GX_ERR_STATE, "not factored yet");
   std::vector<int> it;
   int st = 0;
@@ -1975,7 +1979,7 @@ int gx_topo_forward_levels(const gx_topo* t, int tile, int ring_rows, int* items
 
 int gx_topo_refactor_parts(const gx_topo* t, int tile, long long* sizes, int* part_a, int* pent_pos, int* pent_div,
                            int* slots) {
-  if (!t || (tile != 4 && tile != 8 && tile != 16) || !sizes) return fail(GX_ERR_ARG, "bad arguments");
+  if (!t || (tile != 2 && tile != 4 && tile != 8 && tile != 16) || !sizes) return fail(GX_ERR_ARG, "bad arguments");
   if (!t->factored) return fail(GX_ERR_STATE, "not factored yet");
   PartPlan plan;
   const int pcap = lu_cap(t->sched.max_col, tile);
@@ -2255,8 +2259,8 @@ int gx_sys_factor_and_solve_values(gx_sys* s, const double* vals, const double* 
   GX_HIP(hipMemcpy(s->d_vals.p, packed.data(), sizeof(double) * nlu, hipMemcpyHostToDevice));
   GX_HIP(hipMemcpy(s->d_rhs.p, b.data(), sizeof(double) * n, hipMemcpyHostToDevice));
   GX_HIP(hipMemcpy(s->d_status.p, st.data(), sizeof(int) * kSysTile, hipMemcpyHostToDevice));
-  // lane 0 of tile 0: consecutive positions lie kLuBlock tiles of values apart
-  gx::launch_lane0(s->d_vals.as<double>(), s->d_lu.as<double>(), nlu, true, kSysTile * gx::kLuBlock, nullptr);
+  // lane 0 of tile 0: consecutive positions lie a block of tiles' values apart
+  gx::launch_lane0(s->d_vals.as<double>(), s->d_lu.as<double>(), nlu, true, kSysTile * gx::lu_block(kSysTile), nullptr);
   gx::launch_lane0(s->d_rhs.as<double>(), s->d_y.as<double>(), n, true, kSysTile, nullptr);
   gx::launch_lu(s->dt, s->w, false, true, 0, nullptr);
   gx::launch_lane0(s->d_y.as<double>(), s->d_rhs.as<double>(), n, false, kSysTile, nullptr);

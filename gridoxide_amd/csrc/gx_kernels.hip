@@ -56,15 +56,16 @@ __device__ __forceinline__ bool tile_running(const int* status, int tile) {
   return __any(status[tile * TW + sub<TW>()] == kRunning);
 }
 
-// L+U values: the tiles go in blocks of kLuBlock, each block holding its positions one after the
-// other, each position the block's tiles side by side: entry p of lane s of tile t at
+// L+U values: the tiles go in blocks of B = lu_block(TW), each block holding its positions one
+// after the other, each position the block's tiles side by side: entry p of lane s of tile t at
 // ((t / B * nnz + p) * B + t % B) * TW + s.  A wavefront's gathers stay inside its block's region
 // (nnz * B * TW values) instead of striding over all tiles' values, which keeps the address
-// translations a wavefront needs few; a block's two tiles still share the cache lines of a position,
+// translations a wavefront needs few; a block's tiles still share the cache lines of a position,
 // and the fill kernels' stores of a position stay B * TW values long.  The allocation rounds the
 // tile count up to a multiple of B.
 __device__ __forceinline__ size_t lu_at(int p, int nnz, int tile, int TW) {
-  return (((size_t)(tile / kLuBlock) * nnz + p) * kLuBlock + tile % kLuBlock) * TW;
+  const int B = lu_block(TW);
+  return (((size_t)(tile / B) * nnz + p) * B + tile % B) * TW;
 }
 // Workgroup b of a one-dimensional grid of n runs on XCD b mod 8: number the tiles so that each XCD
 // runs a contiguous range of them.
@@ -436,7 +437,7 @@ __global__ __launch_bounds__(256) void k_fill_bus(DevTopo t, DevWork w) {
   const size_t oi = bb + (size_t)i * TW;
   const double vri = w.vr[oi], vii = w.vi[oi], pci = w.pc[oi], qci = w.qc[oi];
   T* __restrict__ lu = reinterpret_cast<T*>(w.lu) + lu_at(0, t.nnz_lu, tile, TW) + s;
-  const size_t pst = (size_t)kLuBlock * TW;
+  const size_t pst = (size_t)lu_block(TW) * TW;
   for (int e = e0; e < e1; ++e) {
     const int k = t.fb_k[e];
     const int4 pos = t.fb_pos[e];
@@ -472,7 +473,7 @@ __global__ __launch_bounds__(256) void k_fill_zero(DevTopo t, DevWork w) {
   for (int q = 0; q < TW; ++q) run = run || w.status[tile * TW + q] == kRunning;
   if (!run) return;
   T* __restrict__ lu = reinterpret_cast<T*>(w.lu) + lu_at(0, t.nnz_lu, tile, TW) + s;
-  const size_t pst = (size_t)kLuBlock * TW;
+  const size_t pst = (size_t)lu_block(TW) * TW;
   for (int f = blockIdx.x; f < t.n_fz; f += gridDim.x) lu[(size_t)t.fz_pos[f] * pst] = (T)0;
 }
 
@@ -654,7 +655,7 @@ __device__ __forceinline__ void lu_tile(const DevTopo& t, const DevWork& w, int 
   // this tile's arrays (uniform bases); entry e of lane s at e * TW + s
   char* __restrict__ lub = reinterpret_cast<char*>(reinterpret_cast<T*>(w.lu) + lu_at(0, t.nnz_lu, tile, TW) + s);
   T* __restrict__ yb = reinterpret_cast<T*>(w.y) + (size_t)tile * n * TW;
-  const unsigned pstb = (unsigned)(kLuBlock * TW * sizeof(T));  // bytes between consecutive positions
+  const unsigned pstb = (unsigned)(lu_block(TW) * TW * sizeof(T));  // bytes between consecutive positions
   // (one multiply-add per address: position x stride + this lane's base)
   auto LU = [&](int p) -> T& { return *reinterpret_cast<T*>(lub + (size_t)(unsigned)p * pstb); };
   auto Y = [&](int r) -> T& { return yb[(unsigned)(r * TW + s)]; };
@@ -1086,7 +1087,7 @@ __device__ __forceinline__ void lu64_tile(const DevTopo& t, const DevWork& w, in
   if (!tile_running<TW>(w.status, tile)) return;
   const int sc = tile * TW + s;
   T* __restrict__ lu = reinterpret_cast<T*>(w.lu) + lu_at(0, t.nnz_lu, tile, TW) + s;
-  const size_t pst = (size_t)kLuBlock * TW;  // elements between consecutive positions
+  const size_t pst = (size_t)lu_block(TW) * TW;  // elements between consecutive positions
   T* __restrict__ y = reinterpret_cast<T*>(w.y) + at<TW>(tile, t.n, 0) + s;
   const int* __restrict__ lrow = t.lu_row;
   const int2* __restrict__ uitems = t.uitems;
@@ -1406,7 +1407,7 @@ __device__ __forceinline__ void lu_tile_parts(const DevTopo& t, const DevWork& w
   const int nl = t.nnz_lu;
   char* __restrict__ lub = reinterpret_cast<char*>(reinterpret_cast<T*>(w.lu) + lu_at(0, t.nnz_lu, tile, TW) + s);
   T* __restrict__ yb = reinterpret_cast<T*>(w.y) + (size_t)tile * t.n * TW;
-  const unsigned pstb = (unsigned)(kLuBlock * TW * sizeof(T));
+  const unsigned pstb = (unsigned)(lu_block(TW) * TW * sizeof(T));
   auto LU = [&](int p) -> T& { return *reinterpret_cast<T*>(lub + (size_t)(unsigned)p * pstb); };
   bool ok = true;
   bool overflow = false;
@@ -1799,7 +1800,7 @@ __global__ __launch_bounds__(256) void k_lg_fill(DevTopo t, DevLg lg, DevWork w,
   }
   const size_t bb = at<TW>(tile, t.n_bus, 0) + s;
   zd* out = lu + lu_at(0, lg.nnz_lu, tile, TW) + s;
-  const size_t pst = (size_t)kLuBlock * TW;
+  const size_t pst = (size_t)lu_block(TW) * TW;
   const Outage og = load_outage(t, w, tile * TW + s);
   for (int p = elem_first<TW>(); p < lg.nnz_lu; p += elem_stride<TW>()) {
     zd v{0.0, 0.0};
@@ -1882,6 +1883,7 @@ inline int elem_blocks(int count, int tile) {
 template <class F>
 void with_tile(int tile, F&& f) {
   switch (tile) {
+    case 2: f(std::integral_constant<int, 2>{}); break;
     case 4: f(std::integral_constant<int, 4>{}); break;
     case 8: f(std::integral_constant<int, 8>{}); break;
     case kWideTile: f(std::integral_constant<int, kWideTile>{}); break;

@@ -304,7 +304,7 @@ class Topology:
         return {k: (float(a), int(b)) for k, a, b in zip(_lib.KERNEL_CLASSES, ms, n)}
 
     def set_launch(self, tile: int = 0, ring_rows: int = 0):
-        """Scenarios per tile (4, 8, 16; 0 = automatic) and LDS solve-window rows (power of two,
+        """Scenarios per tile (2, 4, 8, 16; 0 = automatic) and LDS solve-window rows (power of two,
         0 = default) for later solves."""
         check(_lib.lib().gx_topo_set_launch(self._h, int(tile), int(ring_rows)))
 

@@ -95,10 +95,10 @@ def test_three_bus_linear_start_matches_doc_step(three_bus, known, oracle_mod):
     assert abs(dvm[2] - ka["dx_iter1"][2]) < ka["printed_tol"]
 
 
-# launch shapes: (scenarios per tile, solve ring rows); 0 = automatic.  A 64-row ring gathers
-# every triangular-solve entry that reaches farther than 31 rows.
-SHAPES_C1 = [(0, 0), (8, 0), (16, 64), (64, 0), (64, 64)]
-SHAPES_C2 = [(0, 0), (16, 0), (4, 64), (64, 0)]
+# launch shapes: (scenarios per tile, solve ring rows); 0 = automatic (2-scenario tiles at these
+# batch sizes).  A 64-row ring gathers every triangular-solve entry that reaches farther than 31 rows.
+SHAPES_C1 = [(0, 0), (4, 0), (8, 0), (16, 64), (64, 0), (64, 64)]
+SHAPES_C2 = [(0, 0), (2, 64), (16, 0), (4, 64), (64, 0)]
 
 
 @pytest.mark.parametrize("shape", SHAPES_C1)

@@ -145,8 +145,8 @@ def _forward_column_oriented(cp, ulen, row, lu, f):
     return y
 
 
-@pytest.mark.parametrize("grid,tile,ring_rows", [("c1", 4, 64), ("c1", 16, 256), ("c2", 4, 512), ("c2", 8, 64),
-                                                 ("g", 4, 64)])
+@pytest.mark.parametrize("grid,tile,ring_rows", [("c1", 4, 64), ("c1", 16, 256), ("c2", 2, 512), ("c2", 4, 512),
+                                                 ("c2", 8, 64), ("g", 4, 64)])
 def test_forward_levels_equal_column_oriented(grid, tile, ring_rows):
     net = {"c1": lambda: config_grid(1), "c2": lambda: config_grid(2), "g": lambda: generate_grid(700, 900, 5)}[grid]()
     topo = _factored(net)
@@ -245,7 +245,7 @@ def _emulate_parts(pa, pos, div, slots, a, cap):
     return mem
 
 
-@pytest.mark.parametrize("grid,tile", [("c1", 4), ("c1", 8), ("c1", 16), ("c2", 4), ("g", 8)])
+@pytest.mark.parametrize("grid,tile", [("c1", 2), ("c1", 4), ("c1", 8), ("c1", 16), ("c2", 4), ("g", 8)])
 def test_refactor_parts_equal_column_by_column(grid, tile):
     net = {"c1": lambda: config_grid(1), "c2": lambda: config_grid(2), "g": lambda: generate_grid(700, 900, 5)}[grid]()
     topo = _factored(net)
