@@ -7,7 +7,9 @@ from typing import Iterable, List, Sequence, Tuple
 
 from .solver import Engine, Mode, Topology
 
-DEFAULT_CANDIDATES = ((2, 0), (4, 0), (8, 0), (4, 256), (8, 1024))
+# 2-scenario tiles are left out: they pay off only while a launch has fewer tiles than SIMDs, which
+# the multi-request engine's launches exceed
+DEFAULT_CANDIDATES = ((4, 0), (8, 0), (4, 256), (8, 1024))
 
 
 def _cost(topo: Topology, n_scen: int) -> float:
