@@ -238,8 +238,9 @@ void launch_mismatch(const DevTopo& t, const DevWork& w, bool fp32, hipStream_t 
 void launch_decide(const DevTopo& t, const DevWork& w, int iter, hipStream_t stream);
 void launch_fill(const DevTopo& t, const DevWork& w, bool fp32, hipStream_t stream);
 void launch_lu(const DevTopo& t, const DevWork& w, bool fp32, bool factor, int iter, hipStream_t stream);
-void launch_refine(const DevTopo& t, const DevWork& w, double* dvr, double* dvi, hipStream_t stream);
-void launch_update(const DevTopo& t, const DevWork& w, bool fp32, bool mixed, hipStream_t stream);
+void launch_refine(const DevTopo& t, const DevWork& w, double* dvr, double* dvi, int iter, hipStream_t stream);
+// (iter: the Newton step, for launches without a device iteration counter)
+void launch_update(const DevTopo& t, const DevWork& w, bool fp32, bool mixed, int iter, hipStream_t stream);
 void launch_finish(const DevWork& w, hipStream_t stream);
 // Mixed precision: restarts every scenario that did not converge from its starting point, for an
 // FP64 re-solve (status back to running, count in w.fallback[0]).

@@ -2116,12 +2116,12 @@ int gx_batch_solve_outages(gx_topo* t, int n_scen, const double* p_spec, const d
       if ((rc2 = timed(GX_K_LU, [&] { gx::launch_lu(t->dt, w, lu32, true, it, st); }))) return rc2;
       if (lu32) {
         if ((rc2 = timed(GX_K_REFINE, [&] {
-               gx::launch_refine(t->dt, w, ws.dvr.as<double>(), ws.dvi.as<double>(), st);
+               gx::launch_refine(t->dt, w, ws.dvr.as<double>(), ws.dvi.as<double>(), it, st);
              })))
           return rc2;
         if ((rc2 = timed(GX_K_SOLVE, [&] { gx::launch_lu(t->dt, w, lu32, false, it, st); }))) return rc2;
       }
-      if ((rc2 = timed(GX_K_UPDATE, [&] { gx::launch_update(t->dt, w, lu32, lu32, st); }))) return rc2;
+      if ((rc2 = timed(GX_K_UPDATE, [&] { gx::launch_update(t->dt, w, lu32, lu32, it, st); }))) return rc2;
     }
     gx::launch_finish(w, st);
     GX_HIP(hipGetLastError());
@@ -2392,14 +2392,14 @@ void engine_step_launches(gx_engine& e) {
                         L.counter.as<int>(), L.shmem, L.grid, k == 1, true, e.st);
   }
   for (gx_request* r : e.active)
-    if (r->lu32) gx::launch_refine(r->t->dt, r->w, r->ws.dvr.as<double>(), r->ws.dvi.as<double>(), e.st);
+    if (r->lu32) gx::launch_refine(r->t->dt, r->w, r->ws.dvr.as<double>(), r->ws.dvi.as<double>(), 0, e.st);
   {
     gx_engine::Launch& L = e.lk[2];
     gx::launch_lu_queue(e.tile, L.topos.as<gx::DevTopo>(), L.works.as<gx::DevWork>(), L.items.as<int2>(), L.n_items,
                         L.counter.as<int>(), L.shmem, L.grid, true, false, e.st);
   }
   for (gx_request* r : e.active) {
-    gx::launch_update(r->t->dt, r->w, r->lu32, r->lu32, e.st);
+    gx::launch_update(r->t->dt, r->w, r->lu32, r->lu32, 0, e.st);
     gx::launch_iter_advance(r->w.iter_dev, e.st);
   }
 }

@@ -39,6 +39,10 @@ namespace {
 
 // Mixed precision: stagnation below this mismatch (pu) is attributed to the FP32 factors.
 constexpr double kStallFloor = 1e-4;
+// Mixed precision: a tile runs the refinement sweep (FP64 residual, FP32 correction solve) at a step
+// once one of its running scenarios has a mismatch below this (pu) at that step; farther from the
+// solution the FP32 solve's rounding is far below the error of the Newton step itself.
+constexpr double kRefineBelow = 1e-2;
 
 template <int TW>
 __device__ __forceinline__ size_t at(int tile, int count, int e) {
@@ -54,6 +58,19 @@ __device__ __forceinline__ int sub() { return lane_id() % TW; }
 template <int TW>
 __device__ __forceinline__ bool tile_running(const int* status, int tile) {
   return __any(status[tile * TW + sub<TW>()] == kRunning);
+}
+
+// Whether scenario sc refines at this step (kRefineBelow): the step's mismatch is in the history.
+__device__ __forceinline__ bool sc_refines(const DevWork& w, int sc, int iter) {
+  const int it = w.iter_dev ? *w.iter_dev : iter;
+  return w.status[sc] == kRunning && w.history[(size_t)sc * w.max_iter + it] < kRefineBelow;
+}
+// Whether any scenario of the tile refines (the tile then runs the sweep; every lane takes part).
+// A scenario that does not refine ignores the sweep's results, so its own result does not depend
+// on the scenarios it shares a tile with.
+template <int TW>
+__device__ __forceinline__ bool tile_refines(const DevWork& w, int tile, int iter) {
+  return __any(sc_refines(w, tile * TW + sub<TW>(), iter));
 }
 
 // L+U values: the tiles go in blocks of B = lu_block(TW), each block holding its positions one
@@ -649,6 +666,7 @@ __device__ __forceinline__ void lu_tile(const DevTopo& t, const DevWork& w, int 
   T* ring = xcb + (kFactor ? 2 * (size_t)xstride : 0);
   const int g = grp<TW>(), s = sub<TW>();
   if (!tile_running<TW>(w.status, tile)) return;
+  if (!kFactor && !tile_refines<TW>(w, tile, iter)) return;  // (the mixed correction solve)
   const int sc = tile * TW + s;
   const int n = t.n;
   const int nl = t.nnz_lu;
@@ -1054,7 +1072,7 @@ __device__ __forceinline__ void lu_tile(const DevTopo& t, const DevWork& w, int 
   }
   // a zero pivot seen by any group of the scenario, or a non-finite solution, makes it Singular
   ok = all_groups<TW>(ok && !overflow);
-  if (g == 0 && !ok && w.status[sc] == kRunning) {
+  if (g == 0 && !ok && w.status[sc] == kRunning && (kFactor || sc_refines(w, sc, iter))) {
     w.status[sc] = kSingular;
     w.iters[sc] = iter + 1;
   }
@@ -1085,6 +1103,7 @@ __device__ __forceinline__ void lu64_tile(const DevTopo& t, const DevWork& w, in
   constexpr int MF = kWideFar;
   const int s = threadIdx.x;
   if (!tile_running<TW>(w.status, tile)) return;
+  if (!kFactor && !tile_refines<TW>(w, tile, iter)) return;  // (the mixed correction solve)
   const int sc = tile * TW + s;
   T* __restrict__ lu = reinterpret_cast<T*>(w.lu) + lu_at(0, t.nnz_lu, tile, TW) + s;
   const size_t pst = (size_t)lu_block(TW) * TW;  // elements between consecutive positions
@@ -1373,7 +1392,7 @@ __device__ __forceinline__ void lu64_tile(const DevTopo& t, const DevWork& w, in
       }
     }
   }
-  if (!ok && w.status[sc] == kRunning) {
+  if (!ok && w.status[sc] == kRunning && (kFactor || sc_refines(w, sc, iter))) {
     w.status[sc] = kSingular;
     w.iters[sc] = iter + 1;
   }
@@ -1403,6 +1422,7 @@ __device__ __forceinline__ void lu_tile_parts(const DevTopo& t, const DevWork& w
   T* ring = xcb + (kFactor ? 2 * (size_t)xstride : 0);
   const int g = grp<TW>(), s = sub<TW>();
   if (!tile_running<TW>(w.status, tile)) return;
+  if (!kFactor && !tile_refines<TW>(w, tile, iter)) return;  // (the mixed correction solve)
   const int sc = tile * TW + s;
   const int nl = t.nnz_lu;
   char* __restrict__ lub = reinterpret_cast<char*>(reinterpret_cast<T*>(w.lu) + lu_at(0, t.nnz_lu, tile, TW) + s);
@@ -1597,7 +1617,7 @@ __device__ __forceinline__ void lu_tile_parts(const DevTopo& t, const DevWork& w
   }
   // a zero pivot seen by any group of the scenario, or a non-finite solution, makes it Singular
   ok = all_groups<TW>(ok && !overflow);
-  if (g == 0 && !ok && w.status[sc] == kRunning) {
+  if (g == 0 && !ok && w.status[sc] == kRunning && (kFactor || sc_refines(w, sc, iter))) {
     w.status[sc] = kSingular;
     w.iters[sc] = iter + 1;
   }
@@ -1658,9 +1678,9 @@ __global__ void k_iter_advance(int* __restrict__ iter_dev) {
 // dV_i = V_i * (d|V|_i / |V|_i + j dtheta_i) per bus.
 template <typename T, int TW>
 __global__ __launch_bounds__(256) void k_dv(DevTopo t, DevWork w, double* __restrict__ dvr,
-                                            double* __restrict__ dvi) {
+                                            double* __restrict__ dvi, int iter) {
   const int tile = blockIdx.y, s = sub<TW>();
-  if (!tile_running<TW>(w.status, tile)) return;
+  if (!tile_running<TW>(w.status, tile) || !tile_refines<TW>(w, tile, iter)) return;
   const size_t bb = at<TW>(tile, t.n_bus, 0) + s;
   const size_t ub = at<TW>(tile, t.n, 0) + s;
   const T* __restrict__ y = reinterpret_cast<const T*>(w.y);
@@ -1687,9 +1707,9 @@ __global__ __launch_bounds__(256) void k_dv(DevTopo t, DevWork w, double* __rest
 // right-hand side of the correction solve.
 template <typename T, int TW>
 __global__ __launch_bounds__(256) void k_resid(DevTopo t, DevWork w, const double* __restrict__ dvr,
-                                               const double* __restrict__ dvi) {
+                                               const double* __restrict__ dvi, int iter) {
   const int tile = blockIdx.y, s = sub<TW>();
-  if (!tile_running<TW>(w.status, tile)) return;
+  if (!tile_running<TW>(w.status, tile) || !tile_refines<TW>(w, tile, iter)) return;
   const size_t bb = at<TW>(tile, t.n_bus, 0) + s;
   const size_t ub = at<TW>(tile, t.n, 0) + s;
   T* __restrict__ y = reinterpret_cast<T*>(w.y);
@@ -1721,10 +1741,14 @@ __global__ __launch_bounds__(256) void k_resid(DevTopo t, DevWork w, const doubl
 }
 
 template <typename T, int TW, bool kMixed>
-__global__ __launch_bounds__(256) void k_update(DevTopo t, DevWork w) {
+__global__ __launch_bounds__(256) void k_update(DevTopo t, DevWork w, int iter) {
   const int tile = blockIdx.y, s = sub<TW>();
   if (!tile_running<TW>(w.status, tile)) return;
+  // mixed: when the tile ran the sweep, y holds the correction and dx the first solution (exactly
+  // the FP32 step in FP64); a scenario that does not refine takes dx alone, else y (the step)
+  const bool swept = kMixed && tile_refines<TW>(w, tile, iter);
   const int sc = tile * TW + s;
+  const bool own = swept && sc_refines(w, sc, iter);
   if (w.status[sc] != kRunning) return;  // no barrier follows
   const size_t bb = at<TW>(tile, t.n_bus, 0) + s;
   const size_t ub = at<TW>(tile, t.n, 0) + s;
@@ -1735,13 +1759,13 @@ __global__ __launch_bounds__(256) void k_update(DevTopo t, DevWork w) {
     const size_t o = bb + (size_t)i * TW;
     double a = w.va[o], m = w.vm[o];
     if (xa >= 0) {
-      double d = (double)y[ub + (size_t)xa * TW];
-      if (kMixed) d += w.dx[ub + (size_t)xa * TW];
+      double d = swept ? w.dx[ub + (size_t)xa * TW] : (double)y[ub + (size_t)xa * TW];
+      if (own) d += (double)y[ub + (size_t)xa * TW];
       a += d;
     }
     if (xm >= 0) {
-      double d = (double)y[ub + (size_t)xm * TW];
-      if (kMixed) d += w.dx[ub + (size_t)xm * TW];
+      double d = swept ? w.dx[ub + (size_t)xm * TW] : (double)y[ub + (size_t)xm * TW];
+      if (own) d += (double)y[ub + (size_t)xm * TW];
       m += d;
     }
     w.va[o] = a;
@@ -2043,24 +2067,24 @@ size_t lu_shmem_tile(const DevTopo& t, bool fp32, bool factor, int tile) {
   return fp32 ? lu_bytes<float>(t, tile, factor) : lu_bytes<double>(t, tile, factor);
 }
 
-void launch_refine(const DevTopo& t, const DevWork& w, double* dvr, double* dvi, hipStream_t stream) {
+void launch_refine(const DevTopo& t, const DevWork& w, double* dvr, double* dvi, int iter, hipStream_t stream) {
   with_tile(w.tile, [&](auto tw) {
     constexpr int TW = decltype(tw)::value;
     dim3 grid(elem_blocks(t.n_bus, TW), w.n_tiles);
-    hipLaunchKernelGGL((k_dv<float, TW>), grid, dim3(256), 0, stream, t, w, dvr, dvi);
-    hipLaunchKernelGGL((k_resid<float, TW>), grid, dim3(256), 0, stream, t, w, dvr, dvi);
+    hipLaunchKernelGGL((k_dv<float, TW>), grid, dim3(256), 0, stream, t, w, dvr, dvi, iter);
+    hipLaunchKernelGGL((k_resid<float, TW>), grid, dim3(256), 0, stream, t, w, dvr, dvi, iter);
   });
 }
 
-void launch_update(const DevTopo& t, const DevWork& w, bool fp32, bool mixed, hipStream_t stream) {
+void launch_update(const DevTopo& t, const DevWork& w, bool fp32, bool mixed, int iter, hipStream_t stream) {
   with_tile(w.tile, [&](auto tw) {
     constexpr int TW = decltype(tw)::value;
     dim3 grid(elem_blocks(t.n_bus, TW), w.n_tiles);
     if (fp32) {
-      if (mixed) hipLaunchKernelGGL((k_update<float, TW, true>), grid, dim3(256), 0, stream, t, w);
-      else hipLaunchKernelGGL((k_update<float, TW, false>), grid, dim3(256), 0, stream, t, w);
+      if (mixed) hipLaunchKernelGGL((k_update<float, TW, true>), grid, dim3(256), 0, stream, t, w, iter);
+      else hipLaunchKernelGGL((k_update<float, TW, false>), grid, dim3(256), 0, stream, t, w, iter);
     } else {
-      hipLaunchKernelGGL((k_update<double, TW, false>), grid, dim3(256), 0, stream, t, w);
+      hipLaunchKernelGGL((k_update<double, TW, false>), grid, dim3(256), 0, stream, t, w, iter);
     }
   });
 }
