@@ -840,17 +840,17 @@ bool valid_tile(int tile) { return tile == 2 || tile == 4 || tile == 8 || tile =
 bool valid_ring(int rows) { return rows == 0 || (rows >= 64 && rows <= 4096 && (rows & (rows - 1)) == 0); }
 
 // Scenarios per tile when the caller leaves it open: 2-scenario tiles while their wavefronts fit
-// one per SIMD (1,024), else the narrowest of 4 and 8 whose wavefronts stay within a budget per
-// launch (2048 at FP64, 1024 at FP32), else 16.  Measured: on the 10,000-bus grid with 8,192
-// scenarios, at FP64 4-scenario tiles (two wavefronts per SIMD, run one after the other at the
-// kernel's register count) beat 8-scenario tiles (134 vs 157 ms per LU launch), at FP32 the
-// 8-scenario tiles win (88 vs 134 ms); with 1,024 scenarios 2-scenario tiles beat 4-scenario tiles
-// (27.9 vs 36.4 ms at FP64), while on the 2,000-bus grid with 4,096 scenarios the 2,048
-// wavefronts of 2-scenario tiles lose to 4-scenario tiles (16.1 vs 10.3 ms).
+// one per SIMD (1,024), else the narrowest of 4 and 8 whose wavefronts stay within 2,048 per launch,
+// else 16.  Measured (LU + fill per Newton step): with 1,024 scenarios on the 10,000-bus grid
+// 2-scenario tiles beat 4-scenario tiles (27.9 vs 36.4 ms LU at FP64); on the 2,000-bus grid with
+// 4,096 scenarios the 2,048 wavefronts of 2-scenario tiles lose (16.1 vs 10.3 ms); on the
+// 10,000-bus grid with 8,192 scenarios 4-scenario tiles (two wavefronts per SIMD, run one after the
+// other at the kernel's register count) beat 8-scenario tiles at FP64 (96 + 11 vs 137 + 8 ms) and
+// at FP32 (54 + 15 vs 76 + 5 ms).
 int auto_tile(int n_scen, bool fp32) {
+  (void)fp32;
   if ((n_scen + 1) / 2 <= 1024) return 2;
-  const int resident = fp32 ? 1024 : 2048;
-  for (int tw : {4, 8}) if ((n_scen + tw - 1) / tw <= resident) return tw;
+  for (int tw : {4, 8}) if ((n_scen + tw - 1) / tw <= 2048) return tw;
   return 16;
 }
 
