@@ -484,14 +484,19 @@ __global__ __launch_bounds__(256) void k_fill_bus(DevTopo t, DevWork w) {
 // Zero into the positions that are no Jacobian entry (structural fill), ahead of each refactor.
 template <typename T, int TW>
 __global__ __launch_bounds__(256) void k_fill_zero(DevTopo t, DevWork w) {
-  const int tile = blockIdx.y * (256 / TW) + (int)threadIdx.x / TW, s = sub<TW>();
-  if (tile >= w.n_tiles) return;
-  bool run = false;
-  for (int q = 0; q < TW; ++q) run = run || w.status[tile * TW + q] == kRunning;
-  if (!run) return;
-  T* __restrict__ lu = reinterpret_cast<T*>(w.lu) + lu_at(0, t.nnz_lu, tile, TW) + s;
-  const size_t pst = (size_t)lu_block(TW) * TW;
-  for (int f = blockIdx.x; f < t.n_fz; f += gridDim.x) lu[(size_t)t.fz_pos[f] * pst] = (T)0;
+  // one block of B tiles per workgroup row (grid.y); a thread zeroes one fill position's B * TW
+  // contiguous values, so a wavefront's stores cover neighbouring positions
+  constexpr int B = lu_block(TW), V = B * TW;
+  const int blk = blockIdx.y;
+  const int lane0 = blk * V + (int)threadIdx.x;  // (the block's scenarios: threads 0..V-1)
+  const bool mine = threadIdx.x < V && lane0 < w.n_tiles * TW && w.status[lane0] == kRunning;
+  if (!__syncthreads_or(mine)) return;
+  T* __restrict__ base = reinterpret_cast<T*>(w.lu) + (size_t)blk * t.nnz_lu * V;
+  for (int f = blockIdx.x * 256 + (int)threadIdx.x; f < t.n_fz; f += gridDim.x * 256) {
+    T* __restrict__ v = base + (size_t)t.fz_pos[f] * V;
+#pragma unroll
+    for (int i = 0; i < V; ++i) v[i] = (T)0;
+  }
 }
 
 // Entries of a column a lane holds in registers (longer columns take a slower loop).
@@ -2026,7 +2031,9 @@ void launch_fill(const DevTopo& t, const DevWork& w, bool fp32, hipStream_t stre
     if (fp32) hipLaunchKernelGGL((k_fill_bus<float, TW>), grid, dim3(256), 0, stream, t, w);
     else hipLaunchKernelGGL((k_fill_bus<double, TW>), grid, dim3(256), 0, stream, t, w);
     if (t.n_fz > 0) {
-      dim3 zgrid(t.n_fz < 4096 ? t.n_fz : 4096, tb);
+      constexpr int B = lu_block(TW);
+      const int fb = (t.n_fz + 255) / 256;
+      dim3 zgrid(fb < 64 ? fb : 64, (w.n_tiles + B - 1) / B);
       if (fp32) hipLaunchKernelGGL((k_fill_zero<float, TW>), zgrid, dim3(256), 0, stream, t, w);
       else hipLaunchKernelGGL((k_fill_zero<double, TW>), zgrid, dim3(256), 0, stream, t, w);
     }
