@@ -536,16 +536,19 @@ __device__ __forceinline__ void drain() { __builtin_amdgcn_s_waitcnt(0); }
 //   items: this lane group's first item; lub/pstb: the lane's L+U base and position stride (bytes);
 //   yl: the lane's y (offsets row * TW); rl: the lane's ring (offsets slot * S); R ring rows
 template <typename T, int TW, bool kDivide>
-__device__ __forceinline__ void level_solve(const int2* __restrict__ bib, int nst, int R, int nl,
-                                            char* __restrict__ lub, unsigned pstb, T* __restrict__ yl, T* rl,
-                                            bool& ok) {
+__device__ __forceinline__ void level_solve(const int2* __restrict__ bib, int g, int nst, int R, int nl,
+                                            const char* __restrict__ lbase, unsigned pstb, char* __restrict__ ybase,
+                                            unsigned loff, T* rl, bool& ok) {
   constexpr int G = kLanes / TW;
   constexpr int S = TW + 1;
   constexpr int P = kBackItems, D = kBackAhead;
   static_assert(D == 4, "the step loop below is unrolled for four register sets");
   constexpr unsigned kPayload = (1u << 29) - 1;
   const T zero = zero_value<T>();
-  auto LU = [&](int p) -> T& { return *reinterpret_cast<T*>(lub + (size_t)(unsigned)p * pstb); };
+  // uniform bases plus 32-bit lane offsets (loff: the lane's byte offset within a position and
+  // within a row of y), so the gathers can take the scalar-base addressing form
+  auto LU = [&](unsigned p) -> const T& { return *reinterpret_cast<const T*>(lbase + (p * pstb + loff)); };
+  auto Y = [&](unsigned e) -> T& { return *reinterpret_cast<T*>(ybase + (e * (unsigned)sizeof(T) + loff)); };
   const unsigned pos_max = (unsigned)(nl - 1);
   // a set holds the values of its step and the descriptors of that step and of the step D later in
   // da / db, whose roles alternate from one turn of the set to the next (so that the loop needs no
@@ -557,15 +560,16 @@ __device__ __forceinline__ void level_solve(const int2* __restrict__ bib, int ns
   auto dsc_issue = [&](int q, int2 (&d)[P]) {
     const int qq = q < nst ? q : nst - 1;
 #pragma unroll
-    for (int i = 0; i < P; ++i) d[i] = bib[(unsigned)((qq * P + i) * G)];
+    for (int i = 0; i < P; ++i)
+      d[i] = *reinterpret_cast<const int2*>(reinterpret_cast<const char*>(bib) + (unsigned)((qq * P + i) * G + g) * 8u);
   };
   auto val_issue = [&](const int2 (&d)[P], BSet& b) {
 #pragma unroll
     for (int i = 0; i < P; ++i) {
       // (a START's payload is an LDS offset: clamped into the L+U range, the value is unused)
       const unsigned p = (unsigned)d[i].x & kPayload;
-      b.u[i] = LU((int)(p < pos_max ? p : pos_max));
-      b.y[i] = yl[(unsigned)(d[i].y >= 0 ? d[i].y : 0)];
+      b.u[i] = LU(p < pos_max ? p : pos_max);
+      b.y[i] = Y((unsigned)(d[i].y >= 0 ? d[i].y : 0));
     }
   };
   T acc = zero;
@@ -603,7 +607,7 @@ __device__ __forceinline__ void level_solve(const int2* __restrict__ bib, int ns
     if (ended) {  // the step's row end: x_k into y and the ring
       const T xk = kDivide ? pend / pend_u : pend;
       if (!finite_value(xk)) ok = false;
-      yl[pend_y] = xk;
+      Y(pend_y) = xk;
       rl[pend_s] = xk;
     }
     memory_order_point();  // the step's stores before the next loads
@@ -677,6 +681,7 @@ __device__ __forceinline__ void lu_tile(const DevTopo& t, const DevWork& w, int 
   const int nl = t.nnz_lu;
   // this tile's arrays (uniform bases); entry e of lane s at e * TW + s
   char* __restrict__ lub = reinterpret_cast<char*>(reinterpret_cast<T*>(w.lu) + lu_at(0, t.nnz_lu, tile, TW) + s);
+  const char* __restrict__ lbase = reinterpret_cast<const char*>(reinterpret_cast<T*>(w.lu) + lu_at(0, t.nnz_lu, tile, TW));
   T* __restrict__ yb = reinterpret_cast<T*>(w.y) + (size_t)tile * n * TW;
   const unsigned pstb = (unsigned)(lu_block(TW) * TW * sizeof(T));  // bytes between consecutive positions
   // (one multiply-add per address: position x stride + this lane's base)
@@ -960,7 +965,8 @@ __device__ __forceinline__ void lu_tile(const DevTopo& t, const DevWork& w, int 
 
   // ------------------------------------------------ back substitution
   if (!overflow && t.back_items) {
-    level_solve<T, TW, true>(t.back_items + g, t.back_steps, R, nl, lub, pstb, yb + s, ring + s, ok);
+    level_solve<T, TW, true>(t.back_items, g, t.back_steps, R, nl, lbase, pstb, reinterpret_cast<char*>(yb),
+                             (unsigned)(s * sizeof(T)), ring + s, ok);
   } else if (!overflow) {
     strided(n - 1 - W > 0 ? n - 1 - W : 0, g, n, [&](int r) { ring[slot(r)] = Y(r); });
     // column metadata three columns ahead; the U entries, the pivot, the entering
@@ -1431,6 +1437,7 @@ __device__ __forceinline__ void lu_tile_parts(const DevTopo& t, const DevWork& w
   const int sc = tile * TW + s;
   const int nl = t.nnz_lu;
   char* __restrict__ lub = reinterpret_cast<char*>(reinterpret_cast<T*>(w.lu) + lu_at(0, t.nnz_lu, tile, TW) + s);
+  const char* __restrict__ lbase = reinterpret_cast<const char*>(reinterpret_cast<T*>(w.lu) + lu_at(0, t.nnz_lu, tile, TW));
   T* __restrict__ yb = reinterpret_cast<T*>(w.y) + (size_t)tile * t.n * TW;
   const unsigned pstb = (unsigned)(lu_block(TW) * TW * sizeof(T));
   auto LU = [&](int p) -> T& { return *reinterpret_cast<T*>(lub + (size_t)(unsigned)p * pstb); };
@@ -1616,9 +1623,11 @@ __device__ __forceinline__ void lu_tile_parts(const DevTopo& t, const DevWork& w
   }
   if (!overflow) {
     memory_order_point();
-    level_solve<T, TW, false>(t.fwd_items + g, t.fwd_steps, R, nl, lub, pstb, yb + s, ring + s, ok);
+    level_solve<T, TW, false>(t.fwd_items, g, t.fwd_steps, R, nl, lbase, pstb, reinterpret_cast<char*>(yb),
+                             (unsigned)(s * sizeof(T)), ring + s, ok);
     memory_order_point();
-    level_solve<T, TW, true>(t.back_items + g, t.back_steps, R, nl, lub, pstb, yb + s, ring + s, ok);
+    level_solve<T, TW, true>(t.back_items, g, t.back_steps, R, nl, lbase, pstb, reinterpret_cast<char*>(yb),
+                             (unsigned)(s * sizeof(T)), ring + s, ok);
   }
   // a zero pivot seen by any group of the scenario, or a non-finite solution, makes it Singular
   ok = all_groups<TW>(ok && !overflow);
