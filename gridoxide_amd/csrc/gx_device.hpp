@@ -17,6 +17,13 @@ constexpr int kLanes = 64;        // wavefront width
 constexpr int kMaxTile = 16;      // widest narrow tile (scenarios per tile): 2, 4, 8 or 16 are compiled
 constexpr int kRingGap = 32;      // solve ring: columns between fences / rows kept behind the front
 constexpr int kChunkRounds = 16;  // refactor update rounds fetched into registers at a time
+constexpr int kPartTail = 192;   // refactor part: descriptors of entries past the registered ones in LDS
+constexpr int kPartRegs = 4;     // refactor part entries per lane group staged through registers
+// tail descriptors per part held in LDS for staging capacity cap: the entries past the registered
+// ones, at most kPartTail
+constexpr int part_tail(int cap, int tile) {
+  return cap - (64 / tile) * kPartRegs < 0 ? 0 : (cap - (64 / tile) * kPartRegs > kPartTail ? kPartTail : cap - (64 / tile) * kPartRegs);
+}
 constexpr int kMaxLds = 160 * 1024;  // LDS bytes per workgroup on gfx950
 
 constexpr int kWideTile = 64;     // one lane per scenario: every index of the LU kernel is uniform
@@ -35,11 +42,12 @@ constexpr int ring_rows(int tile) { return tile == kWideTile ? 256 : (tile <= 8 
 constexpr int ring_window_of(int rows, int tile) { return tile == kWideTile ? rows - 1 : rows - kRingGap - 1; }
 // LDS bytes of the LU kernel for a tile width: the staged column(s) of max_col + 1 rows (factoring
 // only; two for the narrow tiles) and the solve ring, each row holding tile (+1 padding on narrow
-// tiles) values of elem bytes.
+// tiles) values of elem bytes; narrow tiles factoring also hold two parts' tail descriptors.
 constexpr size_t lu_lds_bytes(int max_col, int rows, int tile, size_t elem, bool factor) {
   return tile == kWideTile
              ? ((factor ? (size_t)(max_col < 1 ? 1 : max_col) + 1 : 0) + (size_t)rows) * (size_t)tile * elem
-             : ((factor ? 2 * ((size_t)(max_col < 1 ? 1 : max_col) + 1) : 0) + (size_t)rows) * (size_t)(tile + 1) * elem;
+             : ((factor ? 2 * ((size_t)(max_col < 1 ? 1 : max_col) + 1) : 0) + (size_t)rows) * (size_t)(tile + 1) * elem +
+                   (factor ? 8 + 2 * (size_t)part_tail(max_col < 1 ? 1 : max_col, tile) * 8 : 0);
 }
 
 enum Status : int { kConverged = 0, kMaxIter = 1, kSingular = 2, kRunning = 3, kPadLane = 4, kStalled = 5 };
@@ -175,7 +183,6 @@ struct DevTopo {
 // The kernel loads a step's values at the end of the step kBackAhead before it.
 constexpr int kBackItems = 4;
 constexpr int kBackAhead = 4;
-constexpr int kPartRegs = 4;  // refactor part entries per lane group staged through registers
 constexpr int kBackEntry = 0, kBackStart = 1, kBackEnd = 2, kBackPad = 3;
 
 // The linear initial guess system of a topology (complex, m = number of PQ buses): recipes

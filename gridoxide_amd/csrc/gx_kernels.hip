@@ -1431,6 +1431,10 @@ __device__ __forceinline__ void lu_tile_parts(const DevTopo& t, const DevWork& w
   T* xcb = reinterpret_cast<T*>(smem_raw);
   const int xstride = (cap + 1) * S;
   T* ring = xcb + (kFactor ? 2 * (size_t)xstride : 0);
+  // the descriptors of a part's entries past the registered ones ("tail"), two parts' worth, after
+  // the solve ring (8-byte aligned)
+  int2* tdsc = reinterpret_cast<int2*>(smem_raw + ((((kFactor ? 2 * (size_t)xstride : 0) + (size_t)(t.window + kRingGap + 1) * (TW + 1)) *
+                                                    sizeof(T) + 7) / 8 * 8));
   const int g = grp<TW>(), s = sub<TW>();
   if (!tile_running<TW>(w.status, tile)) return;
   if (!kFactor && !tile_refines<TW>(w, tile, iter)) return;  // (the mixed correction solve)
@@ -1451,6 +1455,31 @@ __device__ __forceinline__ void lu_tile_parts(const DevTopo& t, const DevWork& w
     // staged entries o = g + G * i of a part: c* = the part being factored (values in registers),
     // x* = the next part (values in flight), y* = the one after (positions in flight)
     int cpos[MP], cdiv[MP], xpos[MP], xdiv[MP], ypos[MP], ydiv[MP];
+    // tail descriptors (entries G MP + t, t = lane + 64 i < kPartTail) of the next part, in flight
+    constexpr int MDT = kPartTail / kLanes;
+    int tpos[MDT], tdiv[MDT];
+    auto tail_issue = [&](const int4& m) {
+#pragma unroll
+      for (int i = 0; i < MDT; ++i) {
+        const int o = G * MP + lane_id() + kLanes * i;
+        const int e = m.x + (o < m.y ? o : 0);
+        tpos[i] = ppos[e];
+        tdiv[i] = pdiv[e];
+      }
+    };
+    const int ts = part_tail(cap, TW);  // tail descriptors per part held in LDS
+    auto tail_store = [&](int buf, int len) {
+#pragma unroll
+      for (int i = 0; i < MDT; ++i) {
+        const int tt = lane_id() + kLanes * i;
+        if (tt < ts && G * MP + tt < len) tdsc[buf * ts + tt] = make_int2(tpos[i], tdiv[i]);
+      }
+    };
+    // descriptor of entry o >= G MP of part p (pb: its first entry)
+    auto tail_desc = [&](int p, int pb, int o) -> int2 {
+      const int tt = o - G * MP;
+      return tt < ts ? tdsc[(p & 1) * ts + tt] : make_int2(ppos[pb + o], pdiv[pb + o]);
+    };
     T cval[MP], xval[MP];
     auto ent_issue = [&](const int4& m, int (&pp)[MP], int (&dd)[MP]) {
 #pragma unroll
@@ -1489,6 +1518,7 @@ __device__ __forceinline__ void lu_tile_parts(const DevTopo& t, const DevWork& w
     vals_issue(sA, vA);
     drain();
     int4 mk = uni4(vload4(t.part_a, 0));
+    tail_issue(mk);
     for (int p = 0; p < np; ++p) {
       const int pb = mk.x, len = mk.y, ch0 = mk.z, nr = mk.w;
       const int4 mx = uni4(m1);
@@ -1500,6 +1530,10 @@ __device__ __forceinline__ void lu_tile_parts(const DevTopo& t, const DevWork& w
         overflow = true;
         break;
       }
+      // this part's tail descriptors (loaded a part ago) into LDS; the next part's in flight
+      tail_store(p & 1, len);
+      memory_order_point();  // the descriptors in LDS before other lanes read them
+      tail_issue(mx);
       // stage A: registered entries, then (long parts) the rest
 #pragma unroll
       for (int i = 0; i < MP; ++i) {
@@ -1512,7 +1546,7 @@ __device__ __forceinline__ void lu_tile_parts(const DevTopo& t, const DevWork& w
 #pragma unroll
         for (int u = 0; u < MT; ++u) {
           const int o = b + G * u + g;
-          pp[u] = ppos[pb + (o < len ? o : 0)];
+          pp[u] = tail_desc(p, pb, o < len ? o : G * MP).x;
         }
 #pragma unroll
         for (int u = 0; u < MT; ++u) tv[u] = LU(pp[u]);
@@ -1582,8 +1616,9 @@ __device__ __forceinline__ void lu_tile_parts(const DevTopo& t, const DevWork& w
 #pragma unroll
         for (int u = 0; u < MT; ++u) {
           const int o = b + G * u + g;
-          pp[u] = ppos[pb + (o < len ? o : 0)];
-          dd[u] = pdiv[pb + (o < len ? o : 0)];
+          const int2 d = tail_desc(p, pb, o < len ? o : G * MP);
+          pp[u] = d.x;
+          dd[u] = d.y;
         }
 #pragma unroll
         for (int u = 0; u < MT; ++u) {
