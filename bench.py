@@ -102,8 +102,8 @@ def main():
                     help="first run a plain 256 MiB device copy (a known byte count for scaling counter reads)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (scenario sharding across "
-                                   "GPUs is described in DESIGN.md, not built)"}))
+        print(json.dumps({"error": "this path runs on one GPU; --gpus must be 1 (sharding scenarios across "
+                                   "GPUs is not built)"}))
         sys.exit(2)
 
     import torch

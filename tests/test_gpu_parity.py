@@ -1,7 +1,7 @@
 """GPU parity: the HIP path (through the C ABI) against the CPU oracle and the reference's
 known answers.
 
-Bars (the BASELINE parity contract): identical convergence status; identical iteration count for
+Bars: identical convergence status; identical iteration count for
 the FP64 path and at most +1 for mixed precision; converged |V| and theta within 1e-8 pu / rad of
 the FP64 oracle on the same inputs.
 """
