@@ -233,6 +233,13 @@ struct DevWork {
   int* fallback = nullptr;               // [1] scenarios handed to the FP64 re-solve
   const int* outage = nullptr;           // [n_scen] branch out of service per scenario, -1 none (or null)
   int* iter_dev = nullptr;               // [1] the Newton iteration, when kept on the device (else an argument)
+  // mixed precision: per tile, 1 when the tile runs this Newton step's refinement sweep (written by
+  // k_dv before the correction solve, read by it and by k_update; a status change made by the
+  // correction solve does not alter it)
+  int* swept = nullptr;
+  // per scenario (or null): 1 when the scenario's first Jacobian could not be factored (the
+  // topology's pivot order came from a later scenario); it is Singular after its first mismatch
+  const int* sing0 = nullptr;
 };
 
 // Launchers (gx_kernels.hip).  All run on `stream`; none synchronizes.  `fp32` selects the

@@ -994,7 +994,7 @@ struct Workspace {
   int iter_cap = 0;
   int fp32 = -1;
   DevMem vm, va, vr, vi, ps, qs, pc, qc, f, dx, y, lu, status, iters, maxmis, hist, running, dvr, dvi, stage,
-      fallback, vm0, va0, outage, iter_dev, lg_lu, lg_y, lg_status, lg_iters;
+      fallback, vm0, va0, outage, iter_dev, lg_lu, lg_y, lg_status, lg_iters, swept, sing0;
   int* h_running = nullptr;  // pinned host mirror of the running counters
   int h_running_len = 0;
   Workspace() = default;
@@ -1006,7 +1006,7 @@ struct Workspace {
   size_t bytes() const {
     const DevMem* all[] = {&vm, &va, &vr, &vi, &ps, &qs, &pc, &qc, &f, &dx, &y, &lu, &status, &iters, &maxmis,
                            &hist, &running, &dvr, &dvi, &stage, &fallback, &vm0, &va0, &outage, &iter_dev,
-                           &lg_lu, &lg_y, &lg_status, &lg_iters};
+                           &lg_lu, &lg_y, &lg_status, &lg_iters, &swept, &sing0};
     size_t b = 0;
     for (const DevMem* m : all) b += m->bytes;
     return b;
@@ -1329,7 +1329,8 @@ int ensure_workspace(const gx_topo& t, Workspace& ws, size_t lanes, int max_iter
     GX_HIP(hipHostMalloc((void**)&ws.h_running, sizeof(int) * std::max(max_iter, 1), hipHostMallocDefault));
     ws.h_running_len = std::max(max_iter, 1);
   }
-  if (fp32 && ((rc = ws.dvr.reserve(nb)) || (rc = ws.dvi.reserve(nb)))) return rc;
+  if (fp32 && ((rc = ws.dvr.reserve(nb)) || (rc = ws.dvi.reserve(nb)) || (rc = ws.swept.reserve(lanes * sizeof(int)))))
+    return rc;
   ws.lanes = lanes;
   ws.iter_cap = max_iter;
   ws.fp32 = (int)fp32;
@@ -1514,10 +1515,19 @@ struct BatchIn {
   const std::vector<int>* outage = nullptr;  // validated host copy, or null when none is out
 };
 
+// Results of a batch for which no scenario's first Jacobian could be factored (nothing runs on the
+// device): every scenario converged at its starting point or is Singular after its first mismatch,
+// with the starting point as its voltages.
+struct HostResult {
+  bool used = false;
+  std::vector<double> vm, va, mismatch;  // [n_scen][n_bus], [n_scen]
+};
+
 // Stages a batch on the device: inputs in tiles, optional linear initial guess, the topology's first
-// factorization (if not yet factored), the topology upload for this tile width, the mixed-precision
-// starting-point copy and the init kernel.  Fills w.
-int stage_batch(gx_topo& t, Workspace& ws, const BatchIn& in, int tile, gx::DevWork& w) {
+// factorization (if not yet factored), the topology upload for this tile width, the init kernel and
+// the mixed-precision starting-point copy.  Fills w.  When no scenario's first Jacobian can be
+// factored, fills *hres instead (GX_ERR_SINGULAR when hres is null).
+int stage_batch(gx_topo& t, Workspace& ws, const BatchIn& in, int tile, gx::DevWork& w, HostResult* hres) {
   const int n_scen = in.n_scen, nb = t.n_bus, max_iter = in.max_iter;
   const size_t plane = (size_t)n_scen * nb * sizeof(double);
   hipStream_t st = in.st;
@@ -1573,7 +1583,9 @@ int stage_batch(gx_topo& t, Workspace& ws, const BatchIn& in, int tile, gx::DevW
   }
   if (!t.factored) {
     // PersistentSolver's first factorization uses the first Jacobian it is asked to solve: the
-    // first scenario whose starting point is not already converged (src/solver.rs:699-713).
+    // first scenario whose starting point is not already converged (src/solver.rs:699-713).  When
+    // that Jacobian has no valid pivot order the scenario is Singular and the cache stays empty, so
+    // the next such scenario factors its own (src/solver.rs:711-716).
     gx::launch_from_tiles(w.vm, ws.stage.as<double>(), n_scen, nb, tile, st);
     std::vector<double> hvm((size_t)n_scen * nb);
     GX_HIP(hipMemcpyAsync(hvm.data(), ws.stage.p, plane, hipMemcpyDeviceToHost, st));
@@ -1582,9 +1594,10 @@ int stage_batch(gx_topo& t, Workspace& ws, const BatchIn& in, int tile, gx::DevW
     std::vector<double> hva((size_t)n_scen * nb);
     GX_HIP(hipMemcpyAsync(hva.data(), ws.stage.p, plane, hipMemcpyDeviceToHost, st));
     GX_HIP(hipStreamSynchronize(st));
-    std::vector<double> hp(nb), hq(nb), pc, qc;
-    int chosen = -1;
-    for (int s = 0; s < n_scen && chosen < 0; ++s) {
+    std::vector<double> hp(nb), hq(nb), pc, qc, mis(n_scen, 0.0);
+    std::vector<int> failed;  // scenarios whose first Jacobian could not be factored
+    int s = 0;
+    for (; s < n_scen && !t.factored; ++s) {
       const size_t off = (size_t)s * nb;
       hipMemcpyKind k = in.dev ? hipMemcpyDeviceToHost : hipMemcpyHostToHost;
       GX_HIP(hipMemcpy(hp.data(), in.p + off, nb * sizeof(double), k));
@@ -1603,10 +1616,30 @@ int stage_batch(gx_topo& t, Workspace& ws, const BatchIn& in, int tile, gx::DevW
         if (t.ns_pos[i] >= 0) mx = std::fmax(mx, std::fabs(hp[i] - pc[i]));
         if (t.pq_pos[i] >= 0) mx = std::fmax(mx, std::fabs(hq[i] - qc[i]));
       }
-      if (!(mx < in.tol) || s == n_scen - 1) chosen = s;
+      mis[s] = mx;
+      const bool converged = mx < in.tol;
+      // (a batch that starts converged still fixes a pivot order, at its last scenario, so that the
+      // device has one to run with)
+      if (converged && s < n_scen - 1) continue;
+      rc = factor_at(t, hvm.data() + off, hva.data() + off, hp.data(), hq.data());
+      if (rc != GX_OK && rc != GX_ERR_SINGULAR) return rc;
+      if (rc == GX_ERR_SINGULAR && !converged) failed.push_back(s);
     }
-    const size_t off = (size_t)chosen * nb;
-    if ((rc = factor_at(t, hvm.data() + off, hva.data() + off, hp.data(), hq.data()))) return rc;
+    if (!t.factored) {
+      // no valid pivot order in any scenario: the results are the starting points
+      if (!hres) return fail(GX_ERR_SINGULAR, "no scenario's first Jacobian has a valid pivot order");
+      hres->used = true;  // (every scenario was examined, its dead buses zeroed)
+      hres->vm.swap(hvm);
+      hres->va.swap(hva);
+      hres->mismatch.swap(mis);
+      return GX_OK;
+    }
+    if (!failed.empty()) {
+      std::vector<int> flag(lanes, 0);
+      for (int q : failed) flag[q] = 1;
+      if ((rc = ws.sing0.upload(flag))) return rc;
+      w.sing0 = ws.sing0.as<int>();
+    }
   }
   // the solve ring is sized for FP64 values: the mixed mode's FP64 re-solve uses the same lists
   const int rows = pick_ring_rows(lu_cap(t.sched.max_col, tile), tile, sizeof(double), t.ring_req);
@@ -1616,7 +1649,11 @@ int stage_batch(gx_topo& t, Workspace& ws, const BatchIn& in, int tile, gx::DevW
   // FP64-sized in both modes: the mixed mode's FP64 re-solve reuses the same buffer
   if ((rc = ws.lu.reserve(lu_lanes(lanes, tile) * (size_t)t.dt.nnz_lu * 8))) return rc;
   w.lu = ws.lu.p;
+  if (in.fp32) w.swept = ws.swept.as<int>();
+  gx::launch_init(t.dt, w, st);
+  GX_HIP(hipGetLastError());
   if (in.fp32) {
+    // the FP64 re-solve's starting point, taken after k_init pinned unreferenced islands to V = 0
     const size_t nbytes = lanes * nb * sizeof(double);
     if ((rc = ws.vm0.reserve(nbytes)) || (rc = ws.va0.reserve(nbytes))) return rc;
     w.vm0 = ws.vm0.as<double>();
@@ -1624,8 +1661,6 @@ int stage_batch(gx_topo& t, Workspace& ws, const BatchIn& in, int tile, gx::DevW
     GX_HIP(hipMemcpyAsync(w.vm0, w.vm, nbytes, hipMemcpyDeviceToDevice, st));
     GX_HIP(hipMemcpyAsync(w.va0, w.va, nbytes, hipMemcpyDeviceToDevice, st));
   }
-  gx::launch_init(t.dt, w, st);
-  GX_HIP(hipGetLastError());
   return GX_OK;
 }
 
@@ -1650,6 +1685,32 @@ int collect_batch(const gx_topo& t, Workspace& ws, const gx::DevWork& w, bool de
   if (history && w.max_iter > 0)
     GX_HIP(hipMemcpyAsync(history, w.history, sizeof(double) * n_scen * (size_t)w.max_iter, k, st));
   GX_HIP(hipStreamSynchronize(st));
+  return GX_OK;
+}
+
+// Writes a HostResult as a batch's outputs (host or device pointers as in.dev).
+int host_results(gx_topo& t, const BatchIn& in, const HostResult& h, double* vm, double* va, int* status, int* iters,
+                 double* history) {
+  const int n_scen = in.n_scen, nb = t.n_bus;
+  std::vector<int> st(n_scen), it(n_scen, 1);
+  std::vector<double> hist((size_t)n_scen * std::max(in.max_iter, 0), 0.0);
+  for (int s = 0; s < n_scen; ++s) {
+    st[s] = h.mismatch[s] < in.tol ? GX_CONVERGED : GX_SINGULAR;
+    if (in.max_iter > 0) hist[(size_t)s * in.max_iter] = h.mismatch[s];
+  }
+  if (in.max_iter == 0)
+    for (int s = 0; s < n_scen; ++s) {
+      st[s] = GX_MAX_ITER;
+      it[s] = 0;
+    }
+  const hipMemcpyKind k = in.dev ? hipMemcpyHostToDevice : hipMemcpyHostToHost;
+  const size_t plane = (size_t)n_scen * nb * sizeof(double);
+  GX_HIP(hipMemcpy(vm, h.vm.data(), plane, k));
+  GX_HIP(hipMemcpy(va, h.va.data(), plane, k));
+  GX_HIP(hipMemcpy(status, st.data(), sizeof(int) * n_scen, k));
+  GX_HIP(hipMemcpy(iters, it.data(), sizeof(int) * n_scen, k));
+  if (history && !hist.empty()) GX_HIP(hipMemcpy(history, hist.data(), sizeof(double) * hist.size(), k));
+  t.has_last = false;
   return GX_OK;
 }
 
@@ -2080,7 +2141,9 @@ int gx_batch_solve_outages(gx_topo* t, int n_scen, const double* p_spec, const d
   const int tile = t->tile_req ? t->tile_req : auto_tile(n_scen, in.fp32);
   gx::DevWork w;
   Workspace& ws = t->ws;
-  if ((rc = stage_batch(*t, ws, in, tile, w))) return rc;
+  HostResult hres;
+  if ((rc = stage_batch(*t, ws, in, tile, w, &hres))) return rc;
+  if (hres.used) return host_results(*t, in, hres, vm, va, status, iters, history);
   // optional per-launch timing of each kernel class on this stream
   auto timed = [&](int cls, auto&& launch) -> int {
     if (!t->profiling) {
@@ -2536,7 +2599,7 @@ int gx_engine_submit(gx_engine* e, gx_topo* t, int n_scen, const double* p_spec,
   int rc;
   if ((rc = read_outages(*t, n_scen, outage, in.dev, h_outage, any))) return rc;
   in.outage = any ? &h_outage : nullptr;
-  if ((rc = stage_batch(*t, r->ws, in, e->tile, r->w))) return rc;
+  if ((rc = stage_batch(*t, r->ws, in, e->tile, r->w, nullptr))) return rc;
   if ((rc = r->ws.iter_dev.reserve(sizeof(int)))) return rc;
   r->w.iter_dev = r->ws.iter_dev.as<int>();
   gx::launch_zero_ints(r->w.iter_dev, 1, e->st);

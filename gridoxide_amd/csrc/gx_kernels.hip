@@ -342,6 +342,11 @@ __global__ void k_decide(DevWork w, int iter) {
   if (m < w.tol) {
     w.status[sc] = kConverged;
     w.iters[sc] = iter + 1;
+  } else if (w.sing0 && w.sing0[sc]) {
+    // its first Jacobian had no valid pivot order: Singular at the first solve, as when
+    // PersistentSolver's first factorization fails (src/solver.rs:711-716)
+    w.status[sc] = kSingular;
+    w.iters[sc] = iter + 1;
   } else if (w.stall_check && iter >= 1 && w.history[(size_t)sc * w.max_iter + iter - 1] < kStallFloor &&
              !(m < w.history[(size_t)sc * w.max_iter + iter - 1])) {
     // mixed precision: the mismatch stopped falling although it was already small (the FP32
@@ -383,6 +388,7 @@ __global__ void k_fallback_reset(DevWork w) {
   if (sc >= w.n_tiles * w.tile) return;
   const int st = w.status[sc];
   if (st != kStalled && st != kSingular) return;
+  if (w.sing0 && w.sing0[sc]) return;  // (Singular in either precision)
   w.status[sc] = kRunning;
   w.iters[sc] = 0;
   w.maxmis[sc] = 0ull;
@@ -676,7 +682,7 @@ __device__ __forceinline__ void lu_tile(const DevTopo& t, const DevWork& w, int 
   T* ring = xcb + (kFactor ? 2 * (size_t)xstride : 0);
   const int g = grp<TW>(), s = sub<TW>();
   if (!tile_running<TW>(w.status, tile)) return;
-  if (!kFactor && !tile_refines<TW>(w, tile, iter)) return;  // (the mixed correction solve)
+  if (!kFactor && !w.swept[tile]) return;  // (the mixed correction solve)
   const int sc = tile * TW + s;
   const int n = t.n;
   const int nl = t.nnz_lu;
@@ -1115,7 +1121,7 @@ __device__ __forceinline__ void lu64_tile(const DevTopo& t, const DevWork& w, in
   constexpr int MF = kWideFar;
   const int s = threadIdx.x;
   if (!tile_running<TW>(w.status, tile)) return;
-  if (!kFactor && !tile_refines<TW>(w, tile, iter)) return;  // (the mixed correction solve)
+  if (!kFactor && !w.swept[tile]) return;  // (the mixed correction solve)
   const int sc = tile * TW + s;
   T* __restrict__ lu = reinterpret_cast<T*>(w.lu) + lu_at(0, t.nnz_lu, tile, TW) + s;
   const size_t pst = (size_t)lu_block(TW) * TW;  // elements between consecutive positions
@@ -1438,7 +1444,7 @@ __device__ __forceinline__ void lu_tile_parts(const DevTopo& t, const DevWork& w
                                                     sizeof(T) + 7) / 8 * 8));
   const int g = grp<TW>(), s = sub<TW>();
   if (!tile_running<TW>(w.status, tile)) return;
-  if (!kFactor && !tile_refines<TW>(w, tile, iter)) return;  // (the mixed correction solve)
+  if (!kFactor && !w.swept[tile]) return;  // (the mixed correction solve)
   const int sc = tile * TW + s;
   const int nl = t.nnz_lu;
   char* __restrict__ lub = reinterpret_cast<char*>(reinterpret_cast<T*>(w.lu) + lu_at(0, t.nnz_lu, tile, TW) + s);
@@ -1730,7 +1736,11 @@ template <typename T, int TW>
 __global__ __launch_bounds__(256) void k_dv(DevTopo t, DevWork w, double* __restrict__ dvr,
                                             double* __restrict__ dvi, int iter) {
   const int tile = blockIdx.y, s = sub<TW>();
-  if (!tile_running<TW>(w.status, tile) || !tile_refines<TW>(w, tile, iter)) return;
+  if (!tile_running<TW>(w.status, tile)) return;
+  // the tile's sweep decision, fixed here for the rest of the step
+  const bool rf = tile_refines<TW>(w, tile, iter);
+  if (blockIdx.x == 0 && threadIdx.x == 0) w.swept[tile] = rf ? 1 : 0;
+  if (!rf) return;
   const size_t bb = at<TW>(tile, t.n_bus, 0) + s;
   const size_t ub = at<TW>(tile, t.n, 0) + s;
   const T* __restrict__ y = reinterpret_cast<const T*>(w.y);
@@ -1759,7 +1769,7 @@ template <typename T, int TW>
 __global__ __launch_bounds__(256) void k_resid(DevTopo t, DevWork w, const double* __restrict__ dvr,
                                                const double* __restrict__ dvi, int iter) {
   const int tile = blockIdx.y, s = sub<TW>();
-  if (!tile_running<TW>(w.status, tile) || !tile_refines<TW>(w, tile, iter)) return;
+  if (!tile_running<TW>(w.status, tile) || !w.swept[tile]) return;
   const size_t bb = at<TW>(tile, t.n_bus, 0) + s;
   const size_t ub = at<TW>(tile, t.n, 0) + s;
   T* __restrict__ y = reinterpret_cast<T*>(w.y);
@@ -1795,8 +1805,10 @@ __global__ __launch_bounds__(256) void k_update(DevTopo t, DevWork w, int iter) 
   const int tile = blockIdx.y, s = sub<TW>();
   if (!tile_running<TW>(w.status, tile)) return;
   // mixed: when the tile ran the sweep, y holds the correction and dx the first solution (exactly
-  // the FP32 step in FP64); a scenario that does not refine takes dx alone, else y (the step)
-  const bool swept = kMixed && tile_refines<TW>(w, tile, iter);
+  // the FP32 step in FP64); a scenario that does not refine takes dx alone, else y (the step).
+  // The sweep decision is the one k_dv recorded: a scenario the correction solve made Singular
+  // does not change what its tile-mates take.
+  const bool swept = kMixed && w.swept[tile] != 0;
   const int sc = tile * TW + s;
   const bool own = swept && sc_refines(w, sc, iter);
   if (w.status[sc] != kRunning) return;  // no barrier follows

@@ -131,7 +131,9 @@ class Oracle:
                                                 _p(va, _f64p))
         return vm, va, rc
 
-    def batch_solve(self, p_spec, q_spec, vm0, va0, tol=1e-8, max_iter=20, init=0, threads=1):
+    def batch_solve(self, p_spec, q_spec, vm0, va0, tol=1e-8, max_iter=20, init=0, threads=1, order=None):
+        """order: optional (q, p) elimination order of the first factorization (column order and
+        preferred rows of the unknowns, e.g. the device's BTF + AMD orders); None = bus-major."""
         p_spec = c64(np.atleast_2d(p_spec))
         S = p_spec.shape[0]
         q_spec = c64(np.atleast_2d(q_spec))
@@ -142,10 +144,14 @@ class Oracle:
         status = np.zeros(S, np.int32)
         iters = np.zeros(S, np.int32)
         hist = np.zeros((S, max_iter))
-        lib().orc_batch_solve_zip(self.n, *self._ya(), _p(self.type, _i32p), _p(self.zip, _f64p), S,
-                                  _p(p_spec, _f64p), _p(q_spec, _f64p), _p(vm0, _f64p), _p(va0, _f64p), init,
-                                  ctypes.c_double(tol), max_iter, threads, _p(vm, _f64p), _p(va, _f64p),
-                                  _p(status, _i32p), _p(iters, _i32p), _p(hist, _f64p))
+        colq = rowp = None
+        if order is not None:
+            colq, rowp = c32(order[0]), c32(order[1])
+        lib().orc_batch_solve_ordered(self.n, *self._ya(), _p(self.type, _i32p), _p(self.zip, _f64p), S,
+                                      _p(p_spec, _f64p), _p(q_spec, _f64p), _p(vm0, _f64p), _p(va0, _f64p), init,
+                                      ctypes.c_double(tol), max_iter, threads, _p(colq, _i32p), _p(rowp, _i32p),
+                                      _p(vm, _f64p), _p(va, _f64p), _p(status, _i32p), _p(iters, _i32p),
+                                      _p(hist, _f64p))
         return vm, va, status, iters, hist
 
     def components(self):

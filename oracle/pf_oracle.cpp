@@ -307,6 +307,11 @@ struct Net {
   const double* yim;
   const int* type;  // 0 slack, 1 PV, 2 PQ
   const double* zip = nullptr;  // per bus (P, Q) of constant-power, -current, -impedance ZIP terms, or null
+  // optional elimination order of the Newton system's first factorization (unknowns in the
+  // reference's numbering): column order colq[k] and preferred (diagonal) row rowp[k] of step k,
+  // e.g. KLU's BTF + AMD orders; null = bus-major
+  const int* colq = nullptr;
+  const int* rowp = nullptr;
 };
 
 // effective_injection (src/network.rs:600-611) with the bus's ZIP terms summed per kind: the
@@ -413,7 +418,6 @@ struct NrCache {
   Csc<double> csc;
   SparseLU<double> lu;
   bool has_lu = false;
-  bool lu_failed = false;
 };
 
 enum { ST_CONVERGED = 0, ST_MAXITER = 1, ST_SINGULAR = 2 };
@@ -447,15 +451,16 @@ int nr_solve(const Net& net, const Unknowns& u, const double* p_spec, const doub
       cache.has_pattern = true;
     }
     pack_csc(cache.csc, vals.data(), vals.size());
-    if (!cache.has_lu && !cache.lu_failed) {
-      std::vector<int> order = bus_major_order(net, u);
-      if (cache.lu.factor(u.n, cache.csc.ap.data(), cache.csc.ai.data(), cache.csc.ax.data(), order, order, 1e-3))
+    if (!cache.has_lu) {
+      // the first factorization fixes the pivot order; when it fails the cache stays empty and the
+      // next solve factors its own first Jacobian (src/solver.rs:711-716)
+      std::vector<int> order = net.colq ? std::vector<int>(net.colq, net.colq + u.n) : bus_major_order(net, u);
+      std::vector<int> diag = net.rowp ? std::vector<int>(net.rowp, net.rowp + u.n) : order;
+      if (cache.lu.factor(u.n, cache.csc.ap.data(), cache.csc.ai.data(), cache.csc.ax.data(), order, diag, 1e-3))
         cache.has_lu = true;
-      else
-        cache.lu_failed = true;
       if (!cache.has_lu) { *iters = it + 1; return ST_SINGULAR; }
     } else {
-      if (cache.lu_failed || !cache.lu.refactor(cache.csc.ap.data(), cache.csc.ai.data(), cache.csc.ax.data())) {
+      if (!cache.lu.refactor(cache.csc.ap.data(), cache.csc.ai.data(), cache.csc.ax.data())) {
         *iters = it + 1;
         return ST_SINGULAR;
       }
@@ -627,12 +632,29 @@ int orc_batch_solve(int n, const int* rowptr, const int* col, const double* yre,
                              max_iter, n_threads, vm, va, status, iters, history);
 }
 
+int orc_batch_solve_ordered(int n, const int* rowptr, const int* col, const double* yre, const double* yim,
+                            const int* type, const double* zip, int n_scen, const double* p_spec,
+                            const double* q_spec, const double* vm0, const double* va0, int init, double tol,
+                            int max_iter, int n_threads, const int* colq, const int* rowp, double* vm, double* va,
+                            int* status, int* iters, double* history);
+
 // orc_batch_solve with ZIP terms (zip: 6 per bus as in Net, or null).
 int orc_batch_solve_zip(int n, const int* rowptr, const int* col, const double* yre, const double* yim,
                         const int* type, const double* zip, int n_scen, const double* p_spec, const double* q_spec,
                         const double* vm0, const double* va0, int init, double tol, int max_iter, int n_threads,
                         double* vm, double* va, int* status, int* iters, double* history) {
-  Net net{n, rowptr, col, yre, yim, type, zip};
+  return orc_batch_solve_ordered(n, rowptr, col, yre, yim, type, zip, n_scen, p_spec, q_spec, vm0, va0, init, tol,
+                                 max_iter, n_threads, nullptr, nullptr, vm, va, status, iters, history);
+}
+
+// orc_batch_solve_zip whose first factorizations take the elimination order colq / rowp (the
+// unknowns' column order and preferred rows, as Net; null = bus-major).
+int orc_batch_solve_ordered(int n, const int* rowptr, const int* col, const double* yre, const double* yim,
+                            const int* type, const double* zip, int n_scen, const double* p_spec,
+                            const double* q_spec, const double* vm0, const double* va0, int init, double tol,
+                            int max_iter, int n_threads, const int* colq, const int* rowp, double* vm, double* va,
+                            int* status, int* iters, double* history) {
+  Net net{n, rowptr, col, yre, yim, type, zip, colq, rowp};
   Unknowns u = unknowns(net);
   std::atomic<int> next{0};
   auto worker = [&]() {

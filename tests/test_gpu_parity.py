@@ -165,6 +165,51 @@ def test_nan_start_is_singular_and_isolated():
     assert np.all(st[[0, 1, 3, 4]] == 0)
 
 
+def test_singular_first_jacobian_factors_the_next_scenario(oracle_mod):
+    """src/solver.rs:711-716: the first Jacobian of the batch (scenario 0, NaN start) has no valid
+    pivot order; scenario 0 is Singular after one mismatch and the pivot order comes from scenario 1,
+    so scenarios 1.. equal, bit for bit, a batch that starts at scenario 1, and the oracle (whose
+    failed first factorization is not sticky either) within 1e-8."""
+    _say("case: config-1 grid, scenario 0 starts with a NaN voltage (first factorization fails)")
+    net = config_grid(1)
+    S = 5
+    p, q = scenario_specs(net, S, seed=5)
+    vm0 = np.repeat(net.vm[None], S, 0)
+    va0 = np.repeat(net.va[None], S, 0)
+    vm0[0, 17] = np.nan
+    for mode in (Mode.FP64, Mode.MIXED):
+        topo = Topology.from_network(net)
+        vm, va, st, it, hist = topo.batch_solve(p, q, vm0, va0, tol=MIS_TOL, max_iter=20, mode=mode)
+        assert st[0] == 2 and it[0] == 1 and np.all(st[1:] == 0), (mode, st, it)
+        rest = Topology.from_network(net).batch_solve(p[1:], q[1:], vm0[1:], va0[1:], tol=MIS_TOL, max_iter=20,
+                                                      mode=mode)
+        assert np.array_equal(vm[1:], rest[0]) and np.array_equal(va[1:], rest[1])
+        assert np.array_equal(it[1:], rest[3])
+        ref = oracle_mod.Oracle(net).batch_solve(p, q, vm0, va0, tol=MIS_TOL, max_iter=20, init=0, threads=1)
+        _compare((vm[1:], va[1:], st[1:], it[1:], hist[1:]), tuple(a[1:] for a in ref), mixed=mode == Mode.MIXED)
+        assert ref[2][0] == 2 and ref[3][0] == 1
+
+
+def test_no_valid_first_jacobian_returns_starting_points():
+    """Every scenario's first Jacobian is singular (NaN starts): no pivot order exists, every scenario
+    is Singular after its first mismatch and keeps its starting point; nothing is an error."""
+    _say("case: config-1 grid, every scenario starts with a NaN voltage")
+    net = config_grid(1)
+    S = 3
+    p, q = scenario_specs(net, S, seed=6)
+    vm0 = np.repeat(net.vm[None], S, 0)
+    va0 = np.repeat(net.va[None], S, 0)
+    vm0[:, 17] = np.nan
+    topo = Topology.from_network(net)
+    vm, va, st, it, hist = topo.batch_solve(p, q, vm0, va0, tol=MIS_TOL, max_iter=20)
+    assert np.all(st == 2) and np.all(it == 1)
+    assert np.array_equal(va, va0) and np.array_equal(np.isnan(vm), np.isnan(vm0))
+    assert np.array_equal(vm[~np.isnan(vm)], vm0[~np.isnan(vm0)])
+    # a later batch with valid starts factors normally
+    ok = topo.batch_solve(p, q, np.repeat(net.vm[None], S, 0), va0, tol=MIS_TOL, max_iter=20)
+    assert np.all(ok[2] == 0)
+
+
 def test_batch_equals_sequential_persistent_solver():
     """src: tests/batch_solver_test.rs:68-97 -- batch == sequential solves, bit for bit."""
     _say("case: config-1 grid, batch of 6 vs 6 sequential single-scenario solves")
@@ -344,6 +389,29 @@ def test_mixed_fallback_reproduces_fp64_for_failing_scenarios():
     ok = [s for s in range(16) if s != 3]
     assert np.all(mixed[2][ok] == 0)
     assert np.abs(mixed[0][ok] - fp64[0][ok]).max() <= V_TOL
+
+
+def test_mixed_fallback_with_unreferenced_island_equals_fp64():
+    """The FP64 re-solve of a mixed-precision fallback starts from the point the FP64 path starts
+    from: buses of an island without a Slack bus are pinned to V = 0 in both."""
+    _say("case: config-1 grid plus islands, scenario 1 overflows FP32, mixed vs FP64")
+    from test_abi_cpu import _with_islands
+    net, n0 = _with_islands()
+    S = 4
+    p = np.repeat(net.p_spec[None], S, 0)
+    q = np.repeat(net.q_spec[None], S, 0)
+    vm0 = np.repeat(net.vm[None], S, 0)
+    va0 = np.repeat(net.va[None], S, 0)
+    vm0[:, n0:n0 + 3] = 1.05  # the unreferenced island starts away from 0
+    bus = int(np.nonzero(net.bus_type[:n0] == 2)[0][0])
+    vm0[1, bus] = 1e20
+    topo = Topology.from_network(net)
+    mixed = topo.batch_solve(p, q, vm0, va0, tol=MIS_TOL, max_iter=20, mode=Mode.MIXED)
+    assert topo.fallbacks() >= 1
+    fp64 = Topology.from_network(net).batch_solve(p, q, vm0, va0, tol=MIS_TOL, max_iter=20, mode=Mode.FP64)
+    assert mixed[2][1] == fp64[2][1] and mixed[3][1] == fp64[3][1]
+    assert np.array_equal(mixed[0][1], fp64[0][1]) and np.array_equal(mixed[1][1], fp64[1][1])
+    assert np.all(mixed[0][:, n0:n0 + 3] == 0) and np.all(fp64[0][:, n0:n0 + 3] == 0)
 
 
 @pytest.mark.parametrize("shape", [(0, 0), (64, 0)])

@@ -250,3 +250,43 @@ def test_scenario_model():
         assert abs(p[s, pv].sum() - 0.95 * (-p[s, pq]).sum()) < 1e-9
         ratio = q[s, pq] / p[s, pq]
         assert np.allclose(ratio, net.q_spec[pq] / net.p_spec[pq])
+
+
+def test_failed_first_factorization_is_not_sticky(oracle_mod):
+    """src/solver.rs:711-716: when the first Jacobian a solver meets has no valid pivot order, that
+    solve is Singular after one mismatch and the cache stays empty; the next scenario factors its own
+    first Jacobian.  On one worker the later scenarios therefore equal a batch without the failed one."""
+    net = config_grid(1)
+    S = 4
+    p, q = scenario_specs(net, S, seed=21)
+    vm0 = np.repeat(net.vm[None], S, 0)
+    va0 = np.repeat(net.va[None], S, 0)
+    vm0[0, 17] = np.nan
+    o = oracle_mod.Oracle(net)
+    vm, va, st, it, hist = o.batch_solve(p, q, vm0, va0, init=0, threads=1)
+    assert st[0] == 2 and it[0] == 1
+    assert np.all(st[1:] == 0)
+    ref = o.batch_solve(p[1:], q[1:], vm0[1:], va0[1:], init=0, threads=1)
+    assert np.array_equal(vm[1:], ref[0]) and np.array_equal(va[1:], ref[1]) and np.array_equal(it[1:], ref[3])
+
+
+def test_oracle_elimination_order_option(oracle_mod):
+    """The oracle's first factorization can take a given elimination order (the device's BTF + AMD
+    orders): same statuses and iterations as bus-major order, voltages within rounding, and an L+U
+    fill no larger than bus-major's."""
+    from gridoxide_amd import Topology
+    net = config_grid(1)
+    S = 6
+    p, q = scenario_specs(net, S, seed=4)
+    vm0 = np.repeat(net.vm[None], S, 0)
+    va0 = np.repeat(net.va[None], S, 0)
+    topo = Topology.from_network(net)
+    topo.factor_at(net.vm, net.va, net.p_spec, net.q_spec)
+    pp, qq, _, _ = topo.ordering()
+    o = oracle_mod.Oracle(net)
+    a = o.batch_solve(p, q, vm0, va0, init=1)
+    b = o.batch_solve(p, q, vm0, va0, init=1, order=(qq, pp))
+    assert np.array_equal(a[2], b[2]) and np.array_equal(a[3], b[3])
+    conv = a[2] == 0
+    assert conv.all()
+    assert np.abs(a[0] - b[0]).max() < 1e-12 and np.abs(a[1] - b[1]).max() < 1e-12
