@@ -77,6 +77,7 @@ SIGNATURES = {
     "gx_sys_create": (_vp, [ctypes.c_int, ctypes.c_int, _i32p, _i32p, _f64p]),
     "gx_sys_factor_and_solve_values": (ctypes.c_int, [_vp, _f64p, _f64p, _f64p]),
     "gx_sys_destroy": (None, [_vp]),
+    "gx_sys_factor_and_solve_values_noalloc": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp]),
     "gx_sys_ordering": (ctypes.c_int, [_vp, _i32p, _i32p, _i32p, _i32p, _i32p]),
 }
 

@@ -50,6 +50,13 @@ constexpr size_t lu_lds_bytes(int max_col, int rows, int tile, size_t elem, bool
                    (factor ? 8 + 2 * (size_t)part_tail(max_col < 1 ? 1 : max_col, tile) * 8 : 0);
 }
 
+// Allocation-free system solve (gx_sys_factor_and_solve_values_noalloc): one system is lane 0 of a
+// tile of kSysTileW scenarios; its packed L+U values, right-hand side and tile status live in a
+// static device array of kScratchBytes per GPU.
+constexpr int kSysTileW = 4;
+constexpr size_t kScratchBytes = (size_t)64 << 20;
+constexpr int kMaxDevices = 64;
+
 enum Status : int { kConverged = 0, kMaxIter = 1, kSingular = 2, kRunning = 3, kPadLane = 4, kStalled = 5 };
 
 // device bus codes: the reference's BusType (0 Slack, 1 PV, 2 PQ) plus kDeadBus for a member of
@@ -275,5 +282,11 @@ size_t lu_shmem_tile(const DevTopo& t, bool fp32, bool factor, int tile);
 void launch_iter_advance(int* iter_dev, hipStream_t stream);
 void launch_zero_ints(int* p, int n, hipStream_t stream);
 size_t lu_shmem(const DevTopo& t, bool fp32, int tile);
+// Allocation-free system solve: the static device array's address on the current device (looked up
+// once per device), and the kernels that fill it from a system's values and read the solution out.
+hipError_t scratch_base(void** out);
+void launch_sys_stage(const int* pk_ptr, const int* pk_idx, const double* vals, int nlu, const int* prow,
+                      const double* rhs, int n, double* lu, double* y, int* status, int* iters, hipStream_t stream);
+void launch_sys_unpack(const double* y, const int* q, int n, double* x, hipStream_t stream);
 
 }  // namespace gx

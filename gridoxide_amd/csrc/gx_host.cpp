@@ -2225,9 +2225,17 @@ struct gx_sys {
   LuSchedule sched;
   DevSched ds;
   DevMem d_lu, d_y, d_status, d_iters, d_vals, d_rhs;
+  // tables of the allocation-free solve: per L+U position the triplets summed into it (in triplet
+  // order), the row order of the right-hand side (prow[k] = original row of pivot step k) and the
+  // column order of the solution
+  DevMem d_pk_ptr, d_pk_idx, d_prow, d_q;
   gx::DevTopo dt;
   gx::DevWork w;
 };
+
+namespace {
+int sys_to_device(gx_sys* s);
+}
 
 extern "C" {
 
@@ -2259,6 +2267,11 @@ gx_sys* gx_sys_create(int n, int nnz, const int* rows, const int* cols, const do
   for (int j = 0; j < n; ++j)
     for (int p = s->csc.ap[j]; p < s->csc.ap[j + 1]; ++p)
       s->csc2lu[p] = lu_position(s->lu, s->lu.pinv[s->csc.ai[p]], s->qinv[j]);
+  // with a GPU present the system's device tables are uploaded here, at set-up, so that its solves
+  // allocate nothing (a failed upload is retried, and reported, by the first solve)
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) == hipSuccess && ndev > 0) (void)sys_to_device(s.get());
+  else (void)hipGetLastError();
   return s.release();
 }
 
@@ -2284,6 +2297,17 @@ int sys_to_device(gx_sys* s) {
     return rc;
   GX_HIP(hipMemset(s->d_lu.p, 0, s->d_lu.bytes));
   GX_HIP(hipMemset(s->d_y.p, 0, s->d_y.bytes));
+  {
+    std::vector<int> ptr(nlu + 1, 0), idx(s->n_trip), prow(n), q(s->lu.q.begin(), s->lu.q.end());
+    for (size_t e = 0; e < s->n_trip; ++e) ++ptr[s->csc2lu[s->slot[e]] + 1];
+    for (int p = 0; p < nlu; ++p) ptr[p + 1] += ptr[p];
+    std::vector<int> fill(ptr.begin(), ptr.end() - 1);
+    for (size_t e = 0; e < s->n_trip; ++e) idx[fill[s->csc2lu[s->slot[e]]]++] = (int)e;
+    for (int r = 0; r < n; ++r) prow[s->lu.pinv[r]] = r;
+    if ((rc = s->d_pk_ptr.upload(ptr)) || (rc = s->d_pk_idx.upload(idx)) || (rc = s->d_prow.upload(prow)) ||
+        (rc = s->d_q.upload(q)))
+      return rc;
+  }
   s->w.tile = kSysTile;
   s->w.n_scen = 1;
   s->w.n_tiles = 1;
@@ -2333,6 +2357,40 @@ int gx_sys_factor_and_solve_values(gx_sys* s, const double* vals, const double* 
   if (st[0] != gx::kRunning) return GX_SINGULAR;
   for (int k = 0; k < n; ++k) x[s->lu.q[k]] = b[k];
   return GX_OK;
+}
+
+int gx_sys_factor_and_solve_values_noalloc(gx_sys* s, const double* vals, const double* rhs, double* x,
+                                           void* stream) {
+  if (!s || !vals || !rhs || !x) return fail(GX_ERR_ARG, "null");
+  static_assert(gx::kSysTileW == kSysTile, "one tile layout for both system solves");
+  if (!s->on_device) return fail(GX_ERR_STATE, "the system's device tables are not resident (gx_sys_create saw no GPU)");
+  const int n = s->n, nlu = s->lu.nnz();
+  auto up256 = [](size_t b) { return (b + 255) / 256 * 256; };
+  const size_t lu_b = up256((size_t)nlu * lu_lanes(kSysTile, kSysTile) * sizeof(double));
+  const size_t y_b = up256((size_t)n * kSysTile * sizeof(double));
+  const size_t need = lu_b + y_b + 256;
+  if (need > gx::kScratchBytes)
+    return fail(GX_ERR_CAPACITY, "system needs " + std::to_string(need) + " bytes of device scratch; the allocation-free "
+                                 "solve holds " + std::to_string(gx::kScratchBytes) +
+                                 " (64 B per L+U entry + 32 B per row): use gx_sys_factor_and_solve_values");
+  void* base = nullptr;
+  GX_HIP(gx::scratch_base(&base));
+  char* b = static_cast<char*>(base);
+  gx::DevWork w = s->w;
+  w.lu = b;
+  w.y = b + lu_b;
+  w.status = reinterpret_cast<int*>(b + lu_b + y_b);
+  w.iters = w.status + kSysTile;
+  hipStream_t st = (hipStream_t)stream;
+  gx::launch_sys_stage(s->d_pk_ptr.as<int>(), s->d_pk_idx.as<int>(), vals, nlu, s->d_prow.as<int>(), rhs, n,
+                       static_cast<double*>(w.lu), static_cast<double*>(w.y), w.status, w.iters, st);
+  gx::launch_lu(s->dt, w, false, true, 0, st);
+  gx::launch_sys_unpack(static_cast<const double*>(w.y), s->d_q.as<int>(), n, x, st);
+  GX_HIP(hipGetLastError());
+  int st0 = gx::kSingular;
+  GX_HIP(hipMemcpyAsync(&st0, w.status, sizeof(int), hipMemcpyDeviceToHost, st));
+  GX_HIP(hipStreamSynchronize(st));
+  return st0 == gx::kRunning ? GX_OK : GX_SINGULAR;
 }
 
 void gx_sys_destroy(gx_sys* s) { delete s; }

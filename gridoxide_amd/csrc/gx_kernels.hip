@@ -31,7 +31,9 @@
 // substitutions) covers the reads of older values by other lanes.
 #include "gx_device.hpp"
 
+#include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <type_traits>
 
 namespace gx {
@@ -1959,6 +1961,50 @@ __global__ void k_lane0(const double* __restrict__ src, double* __restrict__ dst
   else dst[e] = src[(size_t)e * stride];
 }
 
+// ---------------------------------------------------------------- allocation-free system solve
+// Device memory of gx_sys_factor_and_solve_values_noalloc: the packed L+U values, the right-hand
+// side and the status of the one-scenario tile.  Every GPU that loads this library holds its own copy
+// until the process ends.
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+// Packs a system's values for lane 0 of tile 0 (tile width kSysTileW): L+U position p receives the
+// sum of its triplets' values in triplet order (pk_ptr / pk_idx), the other lanes of the position's
+// block zero; y row k receives rhs[prow[k]] in lane 0, zero elsewhere; the tile's status marks lane 0
+// running and the rest padding.  Every value the LU kernel reads is written here.
+__global__ __launch_bounds__(256) void k_sys_stage(const int* __restrict__ pk_ptr, const int* __restrict__ pk_idx,
+                                                   const double* __restrict__ vals, int nlu,
+                                                   const int* __restrict__ prow, const double* __restrict__ rhs, int n,
+                                                   double* __restrict__ lu, double* __restrict__ y,
+                                                   int* __restrict__ status, int* __restrict__ iters) {
+  constexpr int V = lu_block(kSysTileW) * kSysTileW;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nlu) {
+    double acc = 0.0;
+    for (int e = pk_ptr[i]; e < pk_ptr[i + 1]; ++e) acc += vals[pk_idx[e]];
+    double* v = lu + (size_t)i * V;
+    v[0] = acc;
+#pragma unroll
+    for (int l = 1; l < V; ++l) v[l] = 0.0;
+  }
+  if (i < n) {
+    double* r = y + (size_t)i * kSysTileW;
+    r[0] = rhs[prow[i]];
+#pragma unroll
+    for (int l = 1; l < kSysTileW; ++l) r[l] = 0.0;
+  }
+  if (i < kSysTileW) {
+    status[i] = i == 0 ? kRunning : kPadLane;
+    iters[i] = 0;
+  }
+}
+
+// x[q[k]] = lane 0 of y row k.
+__global__ __launch_bounds__(256) void k_sys_unpack(const double* __restrict__ y, const int* __restrict__ q, int n,
+                                                    double* __restrict__ x) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) x[q[k]] = y[(size_t)k * kSysTileW];
+}
+
 inline int elem_blocks(int count, int tile) {
   const int per_block = 4 * (kLanes / tile) * 4;  // about 4 elements per lane group
   int b = (count + per_block - 1) / per_block;
@@ -2197,6 +2243,36 @@ void launch_linear_guess(const DevTopo& t, const DevLg& lg, const DevTopo& lgt, 
 }
 
 size_t lg_shmem(const DevTopo& lgt, int tile) { return lu_bytes<zd>(lgt, tile, true); }
+
+hipError_t scratch_base(void** out) {
+  // the array's address on each device, looked up once per device with that device current
+  static std::mutex mu;
+  static void* base[kMaxDevices] = {};
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(mu);
+  if (!base[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    base[dev] = p;
+  }
+  *out = base[dev];
+  return hipSuccess;
+}
+
+void launch_sys_stage(const int* pk_ptr, const int* pk_idx, const double* vals, int nlu, const int* prow,
+                      const double* rhs, int n, double* lu, double* y, int* status, int* iters, hipStream_t stream) {
+  const int m = std::max(std::max(nlu, n), kSysTileW);
+  hipLaunchKernelGGL(k_sys_stage, dim3((m + 255) / 256), dim3(256), 0, stream, pk_ptr, pk_idx, vals, nlu, prow, rhs, n,
+                     lu, y, status, iters);
+}
+
+void launch_sys_unpack(const double* y, const int* q, int n, double* x, hipStream_t stream) {
+  if (n > 0) hipLaunchKernelGGL(k_sys_unpack, dim3((n + 255) / 256), dim3(256), 0, stream, y, q, n, x);
+}
 
 void launch_lane0(const double* src, double* dst, int count, bool to_tile, int stride, hipStream_t stream) {
   if (count <= 0) return;

@@ -229,6 +229,17 @@ gx_sys* gx_sys_create(int n, int nnz, const int* rows, const int* cols, const do
  * Returns GX_OK, or GX_SINGULAR when a pivot is zero or the solution is not finite (None). */
 int gx_sys_factor_and_solve_values(gx_sys* s, const double* vals, const double* rhs, double* x);
 void gx_sys_destroy(gx_sys* s);
+/* No counterpart in the reference: for callers that cannot allocate device memory while they solve
+ * (code under HIP graph capture, tight serving loops).  gx_sys_factor_and_solve_values with device
+ * pointers -- vals (the creation triplets' values, positionally), rhs and x (length n) -- on `stream`
+ * (a hipStream_t, NULL = the null stream), returning after x is written: GX_OK, or GX_SINGULAR (x is
+ * then unspecified).  It allocates nothing: the system's read-only tables are uploaded by
+ * gx_sys_create (when a GPU is present; else GX_ERR_STATE), and the L+U values, right-hand side and
+ * status live in one static 64 MiB device array per GPU, so a system may hold at most about
+ * 1,000,000 L+U entries (64 B each plus 32 B per row; GX_ERR_CAPACITY beyond).  Every call on a GPU
+ * shares that array: calls that overlap on one GPU (other threads or streams) race. */
+int gx_sys_factor_and_solve_values_noalloc(gx_sys* s, const double* vals, const double* rhs, double* x,
+                                           void* stream);
 /* Symbolic/pivot orders of a system (the Symbolic p/q/r and Numeric pnum of
  * src/klu_native/analyze.rs:22-54 and factor.rs:32-40).  p, q, pnum sized n; r sized n+1;
  * *nblocks receives the BTF block count.  Returns nnz(L+U) (strictly-triangular parts plus the
