@@ -9,11 +9,11 @@
 //   * approximate minimum degree: Amestoy, Davis & Duff, SIAM J. Matrix Anal. Appl. 17(4), 1996,
 //     and ACM TOMS Algorithm 837 (quotient graph, approximate external degrees, element and
 //     aggressive absorption, hash-based supervariable detection, mass elimination, dense-row
-//     removal at 10*sqrt(n), postordering with the largest child last);
+//     removal at 10*sqrt(n), postordering with the largest child last), on per-node lists of this
+//     file's own (no shared workspace, no compaction);
 //   * left-looking LU: Gilbert & Peierls (1988), depth-first reach for each column's pattern.
-// The AMD steps are kept in the published order on purpose: the elimination order it returns
-// depends on tie-breaking inside those steps, and pivot-order parity with the reference needs
-// the same order.
+// The elimination order depends on tie-breaking (bucket and list orders), and pivot-order parity
+// with the reference needs the same order, so those orders follow the published method.
 #include "gx_symbolic.hpp"
 
 #include <algorithm>
@@ -21,13 +21,6 @@
 #include <cmath>
 
 namespace gx {
-namespace {
-
-constexpr int kEmpty = -1;
-inline int flip(int x) { return -x - 2; }
-
-}  // namespace
-
 int max_transversal(const CscPattern& a, std::vector<int>& match) {
   const int n = a.n;
   match.assign(n, -1);
@@ -141,463 +134,332 @@ int strong_components(const CscPattern& a, const std::vector<int>& colperm, std:
 
 namespace {
 
-// Visits every off-diagonal entry pair {i, j} of A + A' exactly once, in the order that the
-// adjacency lists are built: column by column, each strictly-upper entry A(j,k) followed by the
-// not-yet-seen strictly-lower entries of column j above row k; leftovers last.
-template <class F>
-void aat_pairs(const CscPattern& a, F&& emit) {
+// ---------------------------------------------------------------- A + A'
+// Neighbour lists of the pattern of A + A' (diagonal dropped, every pair once at each end), in the
+// order in which the published AMD code builds them: columns in increasing order; in column k each
+// strictly upper entry A(j, k), by increasing j, adds the pair {j, k} and then releases the strictly
+// lower entries A(i, j) of column j with i < k that are still pending (A(k, j), the mirror of
+// A(j, k), is passed over); lower entries no column releases come last, column by column.  The
+// minimum-degree order breaks ties by list position, so this order is part of its result.
+std::vector<std::vector<int>> symmetric_adjacency(const CscPattern& a) {
   const int n = a.n;
-  std::vector<int> tp(n, 0);
+  std::vector<std::vector<int>> adj(n);
+  auto link = [&adj](int u, int v) {
+    adj[u].push_back(v);
+    adj[v].push_back(u);
+  };
+  std::vector<int> pending(n, 0);  // column j: its first strictly lower entry not released yet
   for (int k = 0; k < n; ++k) {
-    int p = a.ap[k];
-    const int p2 = a.ap[k + 1];
-    while (p < p2) {
-      int j = a.ai[p];
-      if (j < k) {
-        emit(j, k);
-        ++p;
-      } else if (j == k) {
-        ++p;
-        break;
-      } else {
-        break;
-      }
-      int pj = tp[j];
-      const int pj2 = a.ap[j + 1];
-      while (pj < pj2) {
-        int i = a.ai[pj];
-        if (i < k) {
-          emit(i, j);
-          ++pj;
-        } else if (i == k) {
-          ++pj;
-          break;
-        } else {
-          break;
-        }
-      }
-      tp[j] = pj;
+    const int end = a.ap[k + 1];
+    int at = a.ap[k];
+    for (; at < end && a.ai[at] < k; ++at) {
+      const int j = a.ai[at];
+      link(j, k);
+      int& c = pending[j];
+      const int cend = a.ap[j + 1];
+      for (; c < cend && a.ai[c] < k; ++c) link(a.ai[c], j);
+      if (c < cend && a.ai[c] == k) ++c;
     }
-    tp[k] = p;
+    if (at < end && a.ai[at] == k) ++at;
+    pending[k] = at;
   }
   for (int j = 0; j < n; ++j)
-    for (int pj = tp[j]; pj < a.ap[j + 1]; ++pj) emit(a.ai[pj], j);
+    for (int c = pending[j]; c < a.ap[j + 1]; ++c) link(a.ai[c], j);
+  return adj;
 }
 
-int clear_marks(int wflg, int wbig, std::vector<int>& w) {
-  if (wflg < 2 || wflg >= wbig) {
-    for (auto& x : w)
-      if (x != 0) x = 1;
-    wflg = 2;
+// ---------------------------------------------------------------- degree buckets
+// Variables grouped by approximate degree, each group a doubly linked list: a variable enters at the
+// front, and the next pivot is the front of the lowest non-empty group.
+class DegreeBuckets {
+ public:
+  explicit DegreeBuckets(int n) : front_(n + 1, -1), before_(n, -1), after_(n, -1), group_(n, -1) {}
+  void add(int v, int d) {
+    const int f = front_[d];
+    before_[v] = -1;
+    after_[v] = f;
+    if (f >= 0) before_[f] = v;
+    front_[d] = v;
+    group_[v] = d;
   }
-  return wflg;
-}
+  void remove(int v) {
+    const int b = before_[v], a = after_[v];
+    if (a >= 0) before_[a] = b;
+    if (b >= 0) after_[b] = a;
+    else front_[group_[v]] = a;
+    group_[v] = -1;
+  }
+  int front(int d) const { return front_[d]; }
 
-// Postorder of the assembly tree; children of each node in increasing order except that the
-// child with the largest front is moved last.  order[e] = position of element e.
-void tree_postorder(int n, const std::vector<int>& parent, const std::vector<int>& nv, const std::vector<int>& fsize,
-                    std::vector<int>& order) {
-  std::vector<int> child(n, kEmpty), sibling(n, kEmpty), stack(n);
-  for (int j = n - 1; j >= 0; --j) {
-    if (nv[j] > 0 && parent[j] != kEmpty) {
-      sibling[j] = child[parent[j]];
-      child[parent[j]] = j;
-    }
+ private:
+  std::vector<int> front_, before_, after_, group_;
+};
+
+// ---------------------------------------------------------------- assembly-tree postorder
+// Position of every element in a postorder of the assembly tree: roots by increasing index, the
+// children of a node by increasing index except that the child with the largest front (the last
+// such child on ties) is visited last.
+std::vector<int> assembly_postorder(const std::vector<int>& parent, const std::vector<char>& is_node,
+                                    const std::vector<int>& front) {
+  const int n = (int)parent.size();
+  std::vector<std::vector<int>> kids(n);
+  for (int v = 0; v < n; ++v)
+    if (is_node[v] && parent[v] >= 0) kids[parent[v]].push_back(v);
+  for (auto& c : kids) {
+    if (c.size() < 2) continue;
+    size_t big = 0;
+    for (size_t t = 1; t < c.size(); ++t)
+      if (front[c[t]] >= front[c[big]]) big = t;
+    const int b = c[big];
+    c.erase(c.begin() + (long)big);
+    c.push_back(b);
   }
-  for (int i = 0; i < n; ++i) {
-    if (nv[i] > 0 && child[i] != kEmpty) {
-      int fprev = kEmpty, maxf = kEmpty, bigprev = kEmpty, big = kEmpty;
-      for (int f = child[i]; f != kEmpty; f = sibling[f]) {
-        if (fsize[f] >= maxf) {
-          maxf = fsize[f];
-          bigprev = fprev;
-          big = f;
-        }
-        fprev = f;
-      }
-      int fnext = sibling[big];
-      if (fnext != kEmpty) {
-        if (bigprev == kEmpty) child[i] = fnext;
-        else sibling[bigprev] = fnext;
-        sibling[big] = kEmpty;
-        sibling[fprev] = big;
-      }
-    }
-  }
-  order.assign(n, kEmpty);
-  int k = 0;
+  std::vector<int> pos(n, -1);
+  std::vector<std::pair<int, size_t>> walk;  // (node, next child to visit)
+  int count = 0;
   for (int root = 0; root < n; ++root) {
-    if (parent[root] != kEmpty || nv[root] <= 0) continue;
-    int head = 0;
-    stack[0] = root;
-    while (head >= 0) {
-      int i = stack[head];
-      if (child[i] != kEmpty) {
-        for (int f = child[i]; f != kEmpty; f = sibling[f]) ++head;
-        int h = head;
-        for (int f = child[i]; f != kEmpty; f = sibling[f]) stack[h--] = f;
-        child[i] = kEmpty;
+    if (!is_node[root] || parent[root] >= 0) continue;
+    walk.assign(1, {root, 0});
+    while (!walk.empty()) {
+      auto& [v, next] = walk.back();
+      if (next < kids[v].size()) {
+        const int c = kids[v][next++];
+        walk.push_back({c, 0});
       } else {
-        --head;
-        order[i] = k++;
+        pos[v] = count++;
+        walk.pop_back();
       }
     }
   }
+  return pos;
 }
 
 }  // namespace
 
+// Approximate minimum degree (Amestoy, Davis & Duff 1996) on the quotient graph of A + A'.
+// Every variable keeps its adjacent elements and its adjacent variables as two ordered lists and
+// every element its member variables; eliminating a pivot forms a new element from the members of
+// the pivot's elements and its own variables, absorbs those elements, and then, for each member:
+// approximate external degree |Le \ Lme| of its other elements (aggressive absorption of those with
+// none left), mass elimination when the new element is its only neighbour, and a hash of its
+// neighbourhood; members with equal neighbourhoods merge into one supervariable.  Rows denser than
+// max(16, 10 sqrt n) are left out and ordered last.  Ties follow the list and bucket orders of the
+// published method (new element first in each member's element list, that list's former first entry
+// and the first variable moved to the ends; buckets filled at the front), which fix the result.
 std::vector<int> amd_order(const CscPattern& a) {
   const int n = a.n;
   std::vector<int> perm(n);
   if (n == 0) return perm;
-  // degrees of A + A' and the quotient-graph adjacency lists
-  std::vector<int> len(n, 0);
-  aat_pairs(a, [&](int i, int j) { ++len[i]; ++len[j]; });
-  long long nzaat = 0;
-  for (int v : len) nzaat += v;
-  const int iwlen = (int)(nzaat + nzaat / 5 + n);
-  std::vector<int> pe(n), iw(std::max(iwlen, 1)), sp(n);
-  {
-    int s = 0;
-    for (int i = 0; i < n; ++i) {
-      pe[i] = sp[i] = s;
-      s += len[i];
-    }
-  }
-  aat_pairs(a, [&](int i, int j) {
-    iw[sp[i]++] = j;
-    iw[sp[j]++] = i;
-  });
-  int pfree = (int)nzaat;
+  std::vector<std::vector<int>> vars = symmetric_adjacency(a);  // variables adjacent to a variable
+  std::vector<std::vector<int>> elems(n);                       // elements adjacent to a variable
+  std::vector<std::vector<int>> members(n);                     // variables of an element
+  std::vector<int> weight(n, 1);      // supervariable size; 0: merged into another or dense
+  std::vector<int> degree(n, 0);      // variables: approximate degree; elements: external degree
+  std::vector<int> parent(n, -1);     // absorbing element / supervariable
+  std::vector<int> front(n, 0);       // elements: front size (pivots + external degree)
+  std::vector<char> alive(n, 0), in_front(n, 0);
+  std::vector<int> ext(n, 0), ext_stamp(n, 0), mark(n, 0), hash_of(n, 0);
+  std::vector<std::vector<int>> hash_bucket(n);
+  int stamp = 0, mark_stamp = 0;
+  DegreeBuckets buckets(n);
 
-  std::vector<int> nv(n, 1), next(n, kEmpty), last(n, kEmpty), head(n, kEmpty), elen(n, 0), degree(len), w(n, 1);
-  int dense = (int)(10.0 * std::sqrt((double)n));
-  dense = std::max(16, dense);
-  dense = std::min(n, dense);
-  const int wbig = INT_MAX - n;
-  int wflg = clear_marks(0, wbig, w);
-  int mindeg = 0, nel = 0, lemax = 0;
-
-  for (int i = 0; i < n; ++i) {
-    int deg = degree[i];
-    if (deg == 0) {
-      elen[i] = flip(1);
-      ++nel;
-      pe[i] = kEmpty;
-      w[i] = 0;
-    } else if (deg > dense) {
-      nv[i] = 0;
-      elen[i] = kEmpty;
-      ++nel;
-      pe[i] = kEmpty;
+  const int dense = std::min(n, std::max(16, (int)(10.0 * std::sqrt((double)n))));
+  int eliminated = 0;
+  for (int v = 0; v < n; ++v) {
+    const int d = (int)vars[v].size();
+    degree[v] = d;
+    if (d == 0) {  // no neighbour: an element of its own at once (a root)
+      front[v] = 1;
+      ++eliminated;
+    } else if (d > dense) {  // dense: left out, ordered last
+      weight[v] = 0;
+      ++eliminated;
     } else {
-      int inext = head[deg];
-      if (inext != kEmpty) last[inext] = i;
-      next[i] = inext;
-      head[deg] = i;
+      buckets.add(v, d);
     }
   }
 
-  auto unlink = [&](int i) {
-    int ilast = last[i], inext = next[i];
-    if (inext != kEmpty) last[inext] = ilast;
-    if (ilast != kEmpty) next[ilast] = inext;
-    else head[degree[i]] = inext;
-  };
+  std::vector<int> lme;
+  int mindeg = 0;
+  while (eliminated < n) {
+    // the pivot: a variable of least approximate degree
+    while (buckets.front(mindeg) < 0) ++mindeg;
+    const int me = buckets.front(mindeg);
+    buckets.remove(me);
+    int npiv = weight[me];
+    eliminated += npiv;
+    in_front[me] = 1;
 
-  while (nel < n) {
-    // pivot of minimum approximate degree
-    int deg = mindeg;
-    int me = kEmpty;
-    for (; deg < n; ++deg) {
-      me = head[deg];
-      if (me != kEmpty) break;
+    // the new element: the members of the pivot's elements (each absorbed into it), then the
+    // pivot's own variables
+    lme.clear();
+    long long degme = 0;
+    auto take = [&](int v) {
+      if (weight[v] > 0 && !in_front[v]) {
+        degme += weight[v];
+        in_front[v] = 1;
+        lme.push_back(v);
+        buckets.remove(v);
+      }
+    };
+    for (int e : elems[me]) {
+      for (int v : members[e]) take(v);
+      parent[e] = me;
+      alive[e] = 0;
+      std::vector<int>().swap(members[e]);
     }
-    mindeg = deg;
-    {
-      int inext = next[me];
-      if (inext != kEmpty) last[inext] = kEmpty;
-      head[deg] = inext;
-    }
-    const int elenme = elen[me];
-    int nvpiv = nv[me];
-    nel += nvpiv;
+    for (int v : vars[me]) take(v);
+    std::vector<int>().swap(elems[me]);
+    std::vector<int>().swap(vars[me]);
 
-    // form the new element Lme
-    nv[me] = -nvpiv;
-    int degme = 0;
-    int pme1, pme2;
-    if (elenme == 0) {
-      pme1 = pe[me];
-      pme2 = pme1 - 1;
-      const int pend = pme1 + len[me];
-      for (int p = pme1; p < pend; ++p) {
-        int i = iw[p];
-        int nvi = nv[i];
-        if (nvi > 0) {
-          degme += nvi;
-          nv[i] = -nvi;
-          iw[++pme2] = i;
-          unlink(i);
+    // |Le \ Lme| of every element adjacent to the new element's members
+    ++stamp;
+    for (int v : lme)
+      for (int e : elems[v]) {
+        if (!alive[e]) continue;
+        if (ext_stamp[e] != stamp) {
+          ext_stamp[e] = stamp;
+          ext[e] = degree[e];
         }
+        ext[e] -= weight[v];
       }
-    } else {
-      int p = pe[me];
-      pme1 = pfree;
-      const int slenme = len[me] - elenme;
-      for (int knt1 = 1; knt1 <= elenme + 1; ++knt1) {
-        int e, pj, ln;
-        if (knt1 > elenme) {
-          e = me;
-          pj = p;
-          ln = slenme;
-        } else {
-          e = iw[p++];
-          pj = pe[e];
-          ln = len[e];
-        }
-        for (int knt2 = 1; knt2 <= ln; ++knt2) {
-          int i = iw[pj++];
-          int nvi = nv[i];
-          if (nvi <= 0) continue;
-          if (pfree >= iwlen) {
-            // out of room: compact every live list to the front, keeping each list's order
-            pe[me] = p;
-            len[me] -= knt1;
-            if (len[me] == 0) pe[me] = kEmpty;
-            pe[e] = pj;
-            len[e] = ln - knt2;
-            if (len[e] == 0) pe[e] = kEmpty;
-            for (int j = 0; j < n; ++j) {
-              int pn = pe[j];
-              if (pn >= 0) {
-                pe[j] = iw[pn];
-                iw[pn] = flip(j);
-              }
-            }
-            int psrc = 0, pdst = 0;
-            const int pend = pme1 - 1;
-            while (psrc <= pend) {
-              int j = flip(iw[psrc++]);
-              if (j >= 0) {
-                iw[pdst] = pe[j];
-                pe[j] = pdst++;
-                for (int t = 0; t <= len[j] - 2; ++t) iw[pdst++] = iw[psrc++];
-              }
-            }
-            int p1 = pdst;
-            for (psrc = pme1; psrc <= pfree - 1; ++psrc) iw[pdst++] = iw[psrc];
-            pme1 = p1;
-            pfree = pdst;
-            pj = pe[e];
-            p = pe[me];
-          }
-          degme += nvi;
-          nv[i] = -nvi;
-          iw[pfree++] = i;
-          unlink(i);
-        }
-        if (e != me) {
-          pe[e] = flip(me);  // e is absorbed into me
-          w[e] = 0;
-        }
-      }
-      pme2 = pfree - 1;
-    }
-    degree[me] = degme;
-    pe[me] = pme1;
-    len[me] = pme2 - pme1 + 1;
-    elen[me] = flip(nvpiv + degme);
-    wflg = clear_marks(wflg, wbig, w);
 
-    // |Le \ Lme| for every element e adjacent to Lme
-    for (int pme = pme1; pme <= pme2; ++pme) {
-      int i = iw[pme];
-      int eln = elen[i];
-      if (eln <= 0) continue;
-      int nvi = -nv[i];
-      int wnvi = wflg - nvi;
-      for (int p = pe[i]; p <= pe[i] + eln - 1; ++p) {
-        int e = iw[p];
-        int we = w[e];
-        if (we >= wflg) we -= nvi;
-        else if (we != 0) we = degree[e] + wnvi;
-        w[e] = we;
-      }
-    }
-
-    // approximate degrees, element absorption, mass elimination, hashing
-    for (int pme = pme1; pme <= pme2; ++pme) {
-      int i = iw[pme];
-      int p1 = pe[i];
-      int p2 = p1 + elen[i] - 1;
-      int pn = p1;
-      unsigned int hash = 0;
-      int d = 0;
-      for (int p = p1; p <= p2; ++p) {
-        int e = iw[p];
-        int we = w[e];
-        if (we == 0) continue;
-        int dext = we - wflg;
-        if (dext > 0) {
-          d += dext;
-          iw[pn++] = e;
-          hash += (unsigned int)e;
-        } else {
-          pe[e] = flip(me);  // aggressive absorption: Le is a subset of Lme
-          w[e] = 0;
+    // degrees, absorption, mass elimination and neighbourhood hashes of the members
+    for (int v : lme) {
+      long long deg = 0;
+      unsigned long long h = 0;
+      std::vector<int>& el = elems[v];
+      size_t ke = 0;
+      for (int e : el) {
+        if (!alive[e]) continue;
+        if (ext[e] > 0) {
+          deg += ext[e];
+          h += (unsigned long long)e;
+          el[ke++] = e;
+        } else {  // no variable of e is outside the new element: absorbed into it
+          parent[e] = me;
+          alive[e] = 0;
+          std::vector<int>().swap(members[e]);
         }
       }
-      elen[i] = pn - p1 + 1;
-      int p3 = pn;
-      int p4 = p1 + len[i];
-      for (int p = p2 + 1; p < p4; ++p) {
-        int j = iw[p];
-        int nvj = nv[j];
-        if (nvj > 0) {
-          d += nvj;
-          iw[pn++] = j;
-          hash += (unsigned int)j;
+      el.resize(ke);
+      std::vector<int>& vl = vars[v];
+      size_t kv = 0;
+      for (int u : vl)
+        if (weight[u] > 0 && !in_front[u]) {
+          deg += weight[u];
+          h += (unsigned long long)u;
+          vl[kv++] = u;
         }
+      vl.resize(kv);
+      if (ke == 0 && kv == 0) {  // only the new element is left: eliminated with the pivot
+        parent[v] = me;
+        degme -= weight[v];
+        npiv += weight[v];
+        eliminated += weight[v];
+        weight[v] = 0;
+        std::vector<int>().swap(el);
+        std::vector<int>().swap(vl);
+        continue;
       }
-      if (elen[i] == 1 && p3 == pn) {
-        // i is adjacent only to me: eliminate it together with me
-        pe[i] = flip(me);
-        int nvi = -nv[i];
-        degme -= nvi;
-        nvpiv += nvi;
-        nel += nvi;
-        nv[i] = 0;
-        elen[i] = kEmpty;
+      degree[v] = (int)std::min<long long>(degree[v], deg);
+      if (ke > 0) {
+        el.push_back(el.front());
+        el.front() = me;
       } else {
-        degree[i] = std::min(degree[i], d);
-        iw[pn] = iw[p3];
-        iw[p3] = iw[p1];
-        iw[p1] = me;
-        len[i] = pn - p1 + 1;
-        hash %= (unsigned int)n;
-        int j = head[hash];
-        if (j <= kEmpty) {
-          next[i] = flip(j);
-          head[hash] = flip(i);
-        } else {
-          next[i] = last[j];
-          last[j] = i;
-        }
-        last[i] = (int)hash;
+        el.push_back(me);
       }
+      if (kv > 1) std::rotate(vl.begin(), vl.begin() + 1, vl.end());
+      hash_of[v] = (int)(h % (unsigned long long)n);
+      hash_bucket[hash_of[v]].push_back(v);
     }
-    degree[me] = degme;
-    lemax = std::max(lemax, degme);
-    wflg += lemax;
-    wflg = clear_marks(wflg, wbig, w);
+    degree[me] = (int)degme;
 
-    // supervariable detection: variables with identical adjacency merge into one
-    for (int pme = pme1; pme <= pme2; ++pme) {
-      int i = iw[pme];
-      if (nv[i] >= 0) continue;
-      unsigned int hash = (unsigned int)last[i];
-      int j = head[hash];
-      if (j == kEmpty) {
-        i = kEmpty;
-      } else if (j < kEmpty) {
-        i = flip(j);
-        head[hash] = kEmpty;
-      } else {
-        i = last[j];
-        last[j] = kEmpty;
-      }
-      while (i != kEmpty && next[i] != kEmpty) {
-        int ln = len[i];
-        int eln = elen[i];
-        for (int p = pe[i] + 1; p <= pe[i] + ln - 1; ++p) w[iw[p]] = wflg;
-        int jlast = i;
-        j = next[i];
-        while (j != kEmpty) {
-          bool same = (len[j] == ln) && (elen[j] == eln);
-          for (int p = pe[j] + 1; same && p <= pe[j] + ln - 1; ++p)
-            if (w[iw[p]] != wflg) same = false;
+    // supervariables: members with the same elements and variables (besides the new element) merge
+    // into the member hashed last among them; buckets are searched in member order
+    std::vector<int> chain;
+    for (int v : lme) {
+      if (weight[v] == 0) continue;
+      std::vector<int>& bucket = hash_bucket[hash_of[v]];
+      if (bucket.empty()) continue;
+      chain.assign(bucket.rbegin(), bucket.rend());
+      bucket.clear();
+      for (size_t x = 0; x + 1 < chain.size(); ++x) {
+        const int s = chain[x];
+        ++mark_stamp;
+        for (size_t t = 1; t < elems[s].size(); ++t) mark[elems[s][t]] = mark_stamp;
+        for (int u : vars[s]) mark[u] = mark_stamp;
+        size_t keep = x + 1;
+        for (size_t y = x + 1; y < chain.size(); ++y) {
+          const int j = chain[y];
+          bool same = elems[j].size() == elems[s].size() && vars[j].size() == vars[s].size();
+          for (size_t t = 1; same && t < elems[j].size(); ++t) same = mark[elems[j][t]] == mark_stamp;
+          for (size_t t = 0; same && t < vars[j].size(); ++t) same = mark[vars[j][t]] == mark_stamp;
           if (same) {
-            pe[j] = flip(i);
-            nv[i] += nv[j];
-            nv[j] = 0;
-            elen[j] = kEmpty;
-            j = next[j];
-            next[jlast] = j;
+            parent[j] = s;
+            weight[s] += weight[j];
+            weight[j] = 0;
+            std::vector<int>().swap(elems[j]);
+            std::vector<int>().swap(vars[j]);
           } else {
-            jlast = j;
-            j = next[j];
+            chain[keep++] = j;
           }
         }
-        ++wflg;
-        i = next[i];
+        chain.resize(keep);
       }
     }
 
-    // back into the degree lists; drop non-principal variables from Lme
-    int p = pme1;
-    const int nleft = n - nel;
-    for (int pme = pme1; pme <= pme2; ++pme) {
-      int i = iw[pme];
-      int nvi = -nv[i];
-      if (nvi <= 0) continue;
-      nv[i] = nvi;
-      int d = std::min(degree[i] + degme - nvi, nleft - nvi);
-      int inext = head[d];
-      if (inext != kEmpty) last[inext] = i;
-      next[i] = inext;
-      last[i] = kEmpty;
-      head[d] = i;
+    // the surviving members return to the buckets with their degrees including the new element
+    const int left = n - eliminated;
+    std::vector<int>& mem = members[me];
+    for (int v : lme) {
+      in_front[v] = 0;
+      const int w = weight[v];
+      if (w == 0) continue;
+      const int d = (int)std::min<long long>((long long)degree[v] + degme - w, (long long)left - w);
+      buckets.add(v, d);
       mindeg = std::min(mindeg, d);
-      degree[i] = d;
-      iw[p++] = i;
+      degree[v] = d;
+      mem.push_back(v);
     }
-    nv[me] = nvpiv;
-    len[me] = p - pme1;
-    if (len[me] == 0) {
-      pe[me] = kEmpty;
-      w[me] = 0;
-    }
-    if (elenme != 0) pfree = p;
+    in_front[me] = 0;
+    weight[me] = npiv;
+    front[me] = npiv + (int)degme;
+    alive[me] = mem.empty() ? 0 : 1;
+    if (mem.empty()) parent[me] = -1;
   }
 
-  // assembly tree: parents of elements, principal variable of every absorbed variable
-  for (int i = 0; i < n; ++i) pe[i] = flip(pe[i]);
-  for (int i = 0; i < n; ++i) elen[i] = flip(elen[i]);
-  for (int i = 0; i < n; ++i) {
-    if (nv[i] != 0) continue;
-    int j = pe[i];
-    if (j == kEmpty) continue;  // dense row: ordered last
-    while (nv[j] == 0) j = pe[j];
-    int e = j;
-    j = i;
-    while (nv[j] == 0) {
-      int jn = pe[j];
-      pe[j] = e;
-      j = jn;
+  // every merged variable joins the element its supervariable ended in
+  for (int v = 0; v < n; ++v) {
+    if (weight[v] != 0 || parent[v] < 0) continue;
+    int e = parent[v];
+    while (weight[e] == 0) e = parent[e];
+    for (int u = v; weight[u] == 0;) {
+      const int up = parent[u];
+      parent[u] = e;
+      u = up;
     }
   }
-  std::vector<int> order;
-  tree_postorder(n, pe, nv, elen, order);
-  std::fill(head.begin(), head.end(), kEmpty);
-  std::fill(next.begin(), next.end(), kEmpty);
-  for (int e = 0; e < n; ++e)
-    if (order[e] != kEmpty) head[order[e]] = e;
-  int k = 0;
-  for (int t = 0; t < n; ++t) {
-    int e = head[t];
-    if (e == kEmpty) break;
-    next[e] = k;
-    k += nv[e];
+  std::vector<char> node(n, 0);
+  for (int v = 0; v < n; ++v) node[v] = weight[v] > 0;
+  const std::vector<int> post = assembly_postorder(parent, node, front);
+  // an element's slots: its merged variables in increasing order, then the element; dense rows last
+  std::vector<int> by_post(n, -1), slot(n, 0);
+  for (int v = 0; v < n; ++v)
+    if (node[v]) by_post[post[v]] = v;
+  int next = 0;
+  for (int k = 0; k < n && by_post[k] >= 0; ++k) {
+    slot[by_post[k]] = next;
+    next += weight[by_post[k]];
   }
-  for (int i = 0; i < n; ++i) {
-    if (nv[i] != 0) continue;
-    int e = pe[i];
-    if (e != kEmpty) next[i] = next[e]++;
-    else next[i] = k++;
+  std::vector<int> where(n, 0);
+  for (int v = 0; v < n; ++v) {
+    if (node[v]) continue;
+    where[v] = parent[v] >= 0 ? slot[parent[v]]++ : next++;
   }
-  for (int i = 0; i < n; ++i) perm[next[i]] = i;
+  for (int v = 0; v < n; ++v)
+    if (node[v]) where[v] = slot[v];
+  for (int v = 0; v < n; ++v) perm[where[v]] = v;
   return perm;
 }
 

@@ -161,3 +161,40 @@ def test_external_pivot_order_reproduces_pattern():
     b.set_pivot_order(Q, Pnum)
     for x, y in zip(a.lu_pattern(), b.lu_pattern()):
         assert np.array_equal(x, y)
+
+
+def test_dense_rows_and_larger_matrices_bit_exact(oracle_mod):
+    """The ordering's dense-row rule (rows of A + A' with more than max(16, 10 sqrt n) neighbours
+    are left out and ordered last), mass elimination and supervariables on larger patterns: 2-D
+    grid Laplacians (many indistinguishable neighbourhoods), grids with a few dense rows and
+    columns, and random patterns of 300-1,500 unknowns, against the reference's C KLU."""
+    _ref(oracle_mod)
+    rng = np.random.default_rng(2024)
+    cases = []
+    for side in (12, 25):
+        n = side * side
+        A = np.zeros((n, n))
+        for i in range(side):
+            for j in range(side):
+                k = i * side + j
+                if i + 1 < side:
+                    A[k, k + side] = A[k + side, k] = -1.0
+                if j + 1 < side:
+                    A[k, k + 1] = A[k + 1, k] = -1.0
+        A += np.diag(np.abs(A).sum(1) + 1.0)
+        cases.append(A)
+        D = A.copy()
+        for d in rng.choice(n, 3, replace=False):  # dense row and column
+            D[d, :] = D[:, d] = rng.uniform(-1, 1, n)
+            D[d, d] = n
+        cases.append(D)
+    for n in (300, 800, 1500):
+        cases.append(_random_sparse(rng, n, 3.0 / n, bool(n % 2)))
+    for t, A in enumerate(cases):
+        n = A.shape[0]
+        r, c = np.nonzero(A)
+        P, Q, R, Pnum, _ = _sys_orders(n, r, c, A[r, c])
+        kP, kQ, kR, ex = _klu_orders(oracle_mod, n, r.astype(np.int32), c.astype(np.int32), A[r, c])
+        assert np.array_equal(R, kR), f"case {t} blocks"
+        assert np.array_equal(P, kP) and np.array_equal(Q, kQ), f"case {t} orders"
+        assert np.array_equal(Pnum, ex["P"]), f"case {t} pivots"
