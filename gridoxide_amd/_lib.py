@@ -24,6 +24,8 @@ GX_CONVERGED, GX_MAX_ITER, GX_SINGULAR = 0, 1, 2
 GX_MODE_FP64, GX_MODE_MIXED = 0, 1
 GX_DEVICE_PTRS = 1
 GX_INIT_LINEAR = 2
+GX_INIT_WARM = 4
+GX_KEEP_WARM = 8
 KERNEL_CLASSES = ["mismatch", "fill", "lu", "refine", "solve", "update"]
 
 _i32p = ctypes.POINTER(ctypes.c_int)

@@ -259,10 +259,17 @@ void launch_mismatch(const DevTopo& t, const DevWork& w, bool fp32, hipStream_t 
 void launch_decide(const DevTopo& t, const DevWork& w, int iter, hipStream_t stream);
 void launch_fill(const DevTopo& t, const DevWork& w, bool fp32, hipStream_t stream);
 void launch_lu(const DevTopo& t, const DevWork& w, bool fp32, bool factor, int iter, hipStream_t stream);
+// Mixed precision: the FP64 residual of the refining scenarios' step as the right-hand side of the
+// correction solve.
 void launch_refine(const DevTopo& t, const DevWork& w, double* dvr, double* dvi, int iter, hipStream_t stream);
 // (iter: the Newton step, for launches without a device iteration counter)
 void launch_update(const DevTopo& t, const DevWork& w, bool fp32, bool mixed, int iter, hipStream_t stream);
 void launch_finish(const DevWork& w, hipStream_t stream);
+// Resident warm start of a topology: warm = |V| then theta of its buses (2 n_bus doubles), ok = 1 once
+// it holds a solution.  launch_warm_start starts every scenario of w from it (when ok);
+// launch_keep_warm stores w's first converged scenario into it (when there is one).
+void launch_warm_start(const DevTopo& t, const DevWork& w, const double* warm, const int* ok, hipStream_t stream);
+void launch_keep_warm(const DevTopo& t, const DevWork& w, double* warm, int* ok, hipStream_t stream);
 // Mixed precision: restarts every scenario that did not converge from its starting point, for an
 // FP64 re-solve (status back to running, count in w.fallback[0]).
 void launch_fallback(const DevTopo& t, const DevWork& w, hipStream_t stream);

@@ -1061,6 +1061,8 @@ struct gx_topo {
   gx::DevLg dlg;
   gx::DevTopo dlgt;
   bool ybus_uploaded = false;
+  // resident warm start: |V| then theta per bus, and a device flag set once it holds a solution
+  DevMem d_warm, d_warm_ok;
   // ZIP terms (gx_topo_set_zip): 6 per bus, empty when no bus has any
   std::vector<double> zip;
   DevMem d_zip;
@@ -1527,6 +1529,16 @@ struct HostResult {
 // factorization (if not yet factored), the topology upload for this tile width, the init kernel and
 // the mixed-precision starting-point copy.  Fills w.  When no scenario's first Jacobian can be
 // factored, fills *hres instead (GX_ERR_SINGULAR when hres is null).
+// The resident warm-start buffers of a topology (allocated and marked empty on first use).
+int ensure_warm(gx_topo& t) {
+  if (t.d_warm.p) return GX_OK;
+  int rc;
+  if ((rc = t.d_warm.reserve(2 * (size_t)t.n_bus * sizeof(double))) || (rc = t.d_warm_ok.reserve(sizeof(int))))
+    return rc;
+  GX_HIP(hipMemset(t.d_warm_ok.p, 0, sizeof(int)));
+  return GX_OK;
+}
+
 int stage_batch(gx_topo& t, Workspace& ws, const BatchIn& in, int tile, gx::DevWork& w, HostResult* hres) {
   const int n_scen = in.n_scen, nb = t.n_bus, max_iter = in.max_iter;
   const size_t plane = (size_t)n_scen * nb * sizeof(double);
@@ -1562,6 +1574,8 @@ int stage_batch(gx_topo& t, Workspace& ws, const BatchIn& in, int tile, gx::DevW
     if ((rc = upload_branches(t)) || (rc = ws.outage.upload(*in.outage))) return rc;
     w.outage = ws.outage.as<int>();
   }
+  if ((in.flags & GX_INIT_WARM) && (in.flags & GX_INIT_LINEAR))
+    return fail(GX_ERR_ARG, "GX_INIT_WARM and GX_INIT_LINEAR exclude each other");
   // inputs -> interleaved tiles
   const double* src[4] = {in.p, in.q, in.vm0, in.va0};
   double* dst[4] = {w.ps, w.qs, w.vm, w.va};
@@ -1573,6 +1587,10 @@ int stage_batch(gx_topo& t, Workspace& ws, const BatchIn& in, int tile, gx::DevW
       s = ws.stage.as<double>();
     }
     gx::launch_to_tiles(s, dst[a], n_scen, n_tiles, nb, tile, st);
+  }
+  if (in.flags & GX_INIT_WARM) {
+    if ((rc = ensure_warm(t))) return rc;
+    gx::launch_warm_start(t.dt, w, t.d_warm.as<double>(), t.d_warm_ok.as<int>(), st);
   }
   if (in.flags & GX_INIT_LINEAR) {
     // BatchSolver runs linear_initial_guess on every scenario first (src/batch.rs:268)
@@ -1646,8 +1664,8 @@ int stage_batch(gx_topo& t, Workspace& ws, const BatchIn& in, int tile, gx::DevW
   if (rows < 0)
     return fail(GX_ERR_UNSUPPORTED, "longest L+U column exceeds the LDS staging limit of the device refactor");
   if ((rc = upload_topo(t, gx::ring_window_of(rows, tile), tile))) return rc;
-  // FP64-sized in both modes: the mixed mode's FP64 re-solve reuses the same buffer
-  if ((rc = ws.lu.reserve(lu_lanes(lanes, tile) * (size_t)t.dt.nnz_lu * 8))) return rc;
+  // FP32 values for the mixed mode (its FP64 re-solve of fallbacks grows the buffer first)
+  if ((rc = ws.lu.reserve(lu_lanes(lanes, tile) * (size_t)t.dt.nnz_lu * (in.fp32 ? 4 : 8)))) return rc;
   w.lu = ws.lu.p;
   if (in.fp32) w.swept = ws.swept.as<int>();
   gx::launch_init(t.dt, w, st);
@@ -2201,12 +2219,20 @@ int gx_batch_solve_outages(gx_topo* t, int n_scen, const double* p_spec, const d
     GX_HIP(hipMemcpyAsync(&ws.h_running[0], w.fallback, sizeof(int), hipMemcpyDeviceToHost, st));
     GX_HIP(hipStreamSynchronize(st));
     t->last_fallback = ws.h_running[0];
-    if (t->last_fallback > 0 && (rc = newton(false))) return rc;
+    if (t->last_fallback > 0) {
+      if ((rc = ws.lu.reserve(lu_lanes((size_t)w.n_tiles * w.tile, w.tile) * (size_t)t->dt.nnz_lu * 8))) return rc;
+      w.lu = ws.lu.p;
+      if ((rc = newton(false))) return rc;
+    }
   } else {
     if ((rc = newton(false))) return rc;
   }
   t->last = w;
   t->has_last = true;
+  if (flags & GX_KEEP_WARM) {
+    if ((rc = ensure_warm(*t))) return rc;
+    gx::launch_keep_warm(t->dt, w, t->d_warm.as<double>(), t->d_warm_ok.as<int>(), st);
+  }
   if ((rc = collect_batch(*t, ws, w, in.dev, vm, va, status, iters, history, st))) return rc;
   return fold_profile(*t);  // (the collection waited for the stream)
 }
@@ -2415,6 +2441,7 @@ struct gx_request {
   int iter = 0;   // Newton iteration of the current phase (host mirror of w.iter_dev)
   int phase = 0;  // 0 first solve, 1 FP64 re-solve of the mixed mode's fallbacks, 2 finished
   int fallbacks = 0;
+  int flags = 0;
   double cost = 0.0;  // relative LU time of one tile (queue order)
   long long reserved = 0;  // bytes admitted against the budget (the estimate at submission)
 };
@@ -2537,6 +2564,11 @@ int engine_end_phase(gx_engine& e, gx_request& r) {
     GX_HIP(hipStreamSynchronize(e.st));
     r.fallbacks = r.ws.h_running[0];
     if (r.fallbacks > 0) {
+      // the FP64 re-solve needs FP64-sized L+U values (the stream is idle here)
+      int rc;
+      if ((rc = r.ws.lu.reserve(lu_lanes((size_t)r.w.n_tiles * r.w.tile, r.w.tile) * (size_t)r.t->dt.nnz_lu * 8)))
+        return rc;
+      r.w.lu = r.ws.lu.p;
       r.phase = 1;
       r.lu32 = false;
       r.iter = 0;
@@ -2546,6 +2578,11 @@ int engine_end_phase(gx_engine& e, gx_request& r) {
   }
   r.phase = 2;
   e.dirty = true;
+  if (r.flags & GX_KEEP_WARM) {
+    int rc;
+    if ((rc = ensure_warm(*r.t))) return rc;
+    gx::launch_keep_warm(r.t->dt, r.w, r.t->d_warm.as<double>(), r.t->d_warm_ok.as<int>(), e.st);
+  }
   return GX_OK;
 }
 
@@ -2621,7 +2658,7 @@ long long gx_engine_request_bytes(const gx_topo* t, int n_scen, int max_iter, in
   const long long nb = t->n_bus, n = t->n;
   const long long nnz_lu = t->factored ? t->lu.nnz() : 4 * (long long)t->jkind.size();
   long long b = lanes * (8 * nb * 8 + 3 * n * 8 + (long long)std::max(max_iter, 1) * 8 + 16) +
-                (long long)lu_lanes((size_t)lanes, tile) * nnz_lu * 8 +
+                (long long)lu_lanes((size_t)lanes, tile) * nnz_lu * (mode == GX_MODE_MIXED ? 4 : 8) +
                 (long long)n_scen * nb * 8 + 4LL * std::max(max_iter, 1) + 64;
   if (mode == GX_MODE_MIXED) b += lanes * 4 * nb * 8;
   if (flags & GX_INIT_LINEAR) b += lanes * (4 * nnz_lu / 2 * 16 + nb * 16 + 8);
@@ -2667,6 +2704,7 @@ int gx_engine_submit(gx_engine* e, gx_topo* t, int n_scen, const double* p_spec,
   r->mixed = in.fp32;
   r->lu32 = in.fp32;
   r->w.stall_check = in.fp32 ? 1 : 0;
+  r->flags = flags;
   r->cost = (double)t->dt.n * (1.0 + (double)t->sched.n_upd / std::max(1, t->dt.n) / 16.0);
   GX_HIP(hipGetLastError());
   const int id = r->id;

@@ -32,6 +32,7 @@
 #include "gx_device.hpp"
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <mutex>
 #include <type_traits>
@@ -43,9 +44,11 @@ namespace {
 constexpr double kStallFloor = 1e-4;
 // Mixed precision: a scenario refines its step (FP64 residual, FP32 correction solve) once its
 // mismatch at that step is below this (pu); farther from the solution the FP32 solve's rounding is
-// far below the error of the Newton step itself.  (1e-3 and 1e-2 gave the same iteration counts on
-// the 2,000-bus workload; 1e-3 runs 60% fewer sweeps.)
-constexpr double kRefineBelow = 1e-3;
+// far below the error of the Newton step itself.  1e-2 rather than 1e-3: on ill-conditioned grids a
+// warm-started scenario's first step (mismatch a few 1e-3) otherwise leaves a trajectory whose
+// converged angles sit ~1e-8 rad from the FP64 ones (1e-2 and 1e-3 gave the same iteration counts on
+// the 2,000-bus workload).
+constexpr double kRefineBelow = 1e-2;
 
 template <int TW>
 __device__ __forceinline__ size_t at(int tile, int count, int e) {
@@ -1765,7 +1768,7 @@ __global__ __launch_bounds__(256) void k_dv(DevTopo t, DevWork w, double* __rest
   }
 }
 
-// Mixed precision, refinement step 2: r = f - J*dx1 in FP64, stored (in LU precision) as the
+// Mixed precision, refinement step 2: r = f - J*dx in FP64, stored (in LU precision) as the
 // right-hand side of the correction solve.
 template <typename T, int TW>
 __global__ __launch_bounds__(256) void k_resid(DevTopo t, DevWork w, const double* __restrict__ dvr,
@@ -1942,6 +1945,47 @@ __global__ __launch_bounds__(256) void k_lg_apply(DevTopo t, DevLg lg, DevWork w
     w.vm[o] = hypot(u.re, u.im);
     w.va[o] = atan2(u.im, u.re);
   }
+}
+
+// Resident warm start (src/solver.rs:395-409 keeps a topology's state between solves): every
+// scenario starts from warm[0..n_bus) (|V|) and warm[n_bus..2 n_bus) (theta) when *ok is set.
+template <int TW>
+__global__ __launch_bounds__(256) void k_warm_start(DevTopo t, DevWork w, const double* __restrict__ warm,
+                                                    const int* __restrict__ ok) {
+  if (!*ok) return;
+  const int tile = blockIdx.y, s = sub<TW>();
+  const size_t bb = at<TW>(tile, t.n_bus, 0) + s;
+  for (int i = elem_first<TW>(); i < t.n_bus; i += elem_stride<TW>()) {
+    w.vm[bb + (size_t)i * TW] = warm[i];
+    w.va[bb + (size_t)i * TW] = warm[t.n_bus + i];
+  }
+}
+
+// The solution of the first converged scenario (if any) becomes the resident warm start.  One
+// workgroup: the first converged scenario by a minimum over the workgroup, then its buses copied out
+// of the tile layout.
+template <int TW>
+__global__ __launch_bounds__(256) void k_keep_warm(DevTopo t, DevWork w, double* __restrict__ warm,
+                                                   int* __restrict__ ok) {
+  __shared__ int first;
+  if (threadIdx.x == 0) first = INT_MAX;
+  __syncthreads();
+  int mine = INT_MAX;
+  for (int sc = threadIdx.x; sc < w.n_scen; sc += blockDim.x)
+    if (w.status[sc] == kConverged) {
+      mine = sc;
+      break;
+    }
+  if (mine != INT_MAX) atomicMin(&first, mine);
+  __syncthreads();
+  const int sc = first;
+  if (sc == INT_MAX) return;
+  const size_t bb = at<TW>(sc / TW, t.n_bus, 0) + sc % TW;
+  for (int i = threadIdx.x; i < t.n_bus; i += blockDim.x) {
+    warm[i] = w.vm[bb + (size_t)i * TW];
+    warm[t.n_bus + i] = w.va[bb + (size_t)i * TW];
+  }
+  if (threadIdx.x == 0) *ok = 1;
 }
 
 __global__ void k_finish(DevWork w) {
@@ -2206,6 +2250,21 @@ void launch_fallback(const DevTopo& t, const DevWork& w, hipStream_t stream) {
   });
   const int threads = w.n_tiles * w.tile;
   hipLaunchKernelGGL(k_fallback_reset, dim3((threads + 255) / 256), dim3(256), 0, stream, w);
+}
+
+void launch_warm_start(const DevTopo& t, const DevWork& w, const double* warm, const int* ok, hipStream_t stream) {
+  with_tile(w.tile, [&](auto tw) {
+    constexpr int TW = decltype(tw)::value;
+    hipLaunchKernelGGL(k_warm_start<TW>, dim3(elem_blocks(t.n_bus, TW), w.n_tiles), dim3(256), 0, stream, t, w, warm,
+                       ok);
+  });
+}
+
+void launch_keep_warm(const DevTopo& t, const DevWork& w, double* warm, int* ok, hipStream_t stream) {
+  with_tile(w.tile, [&](auto tw) {
+    constexpr int TW = decltype(tw)::value;
+    hipLaunchKernelGGL(k_keep_warm<TW>, dim3(1), dim3(256), 0, stream, t, w, warm, ok);
+  });
 }
 
 void launch_finish(const DevWork& w, hipStream_t stream) {

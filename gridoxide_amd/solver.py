@@ -309,10 +309,11 @@ class Topology:
         check(_lib.lib().gx_topo_set_launch(self._h, int(tile), int(ring_rows)))
 
     def batch_solve(self, p_spec, q_spec, vm0, va0, tol=1e-8, max_iter=20, mode=Mode.FP64, init="given",
-                    outage=None):
+                    outage=None, keep_warm=False):
         """Host arrays [n_scen, n_bus] -> (vm, va, status, iters, history).  init: "given" starts
-        from vm0/va0; "linear" first applies linear_initial_guess on the device.  outage: optional
-        [n_scen] branch index out of service per scenario (-1: none)."""
+        from vm0/va0; "linear" first applies linear_initial_guess on the device; "warm" starts from
+        the topology's resident warm start (kept by an earlier keep_warm solve), else vm0/va0.
+        outage: optional [n_scen] branch index out of service per scenario (-1: none)."""
         p_spec = f64(np.atleast_2d(p_spec))
         S = p_spec.shape[0]
         q_spec = f64(np.broadcast_to(q_spec, (S, self.n_bus)))
@@ -326,29 +327,35 @@ class Topology:
         out = None if outage is None else i32(np.broadcast_to(outage, (S,)))
         check(_lib.lib().gx_batch_solve_outages(
             self._h, S, p_spec.ctypes.data, q_spec.ctypes.data, vm0.ctypes.data, va0.ctypes.data,
-            None if out is None else out.ctypes.data, float(tol), int(max_iter), int(mode), _init_flag(init),
+            None if out is None else out.ctypes.data, float(tol), int(max_iter), int(mode), _init_flag(init, keep_warm),
             vm.ctypes.data, va.ctypes.data, st.ctypes.data, it.ctypes.data, hist.ctypes.data, None))
         return vm, va, st, it, hist[:, :max_iter]
 
     def batch_solve_device(self, p_spec, q_spec, vm0, va0, vm, va, status, iters, history, tol=1e-8, max_iter=20,
-                           mode=Mode.FP64, stream=0, init="given", outage=None):
+                           mode=Mode.FP64, stream=0, init="given", outage=None, keep_warm=False):
         """Device-resident variant: every argument is a CUDA/HIP tensor (torch) already in HBM
         (outage: optional int32 tensor [n_scen])."""
         S = int(p_spec.shape[0])
         check(_lib.lib().gx_batch_solve_outages(self._h, S, p_spec.data_ptr(), q_spec.data_ptr(), vm0.data_ptr(),
                                         va0.data_ptr(), None if outage is None else outage.data_ptr(),
                                         float(tol), int(max_iter), int(mode),
-                                        _lib.GX_DEVICE_PTRS | _init_flag(init),
+                                        _lib.GX_DEVICE_PTRS | _init_flag(init, keep_warm),
                                         vm.data_ptr(), va.data_ptr(), status.data_ptr(), iters.data_ptr(),
                                         history.data_ptr() if history is not None else None, stream or None))
 
 
-def _init_flag(init: str) -> int:
+def _init_flag(init: str, keep_warm: bool = False) -> int:
+    """init: "given" (vm0/va0 as passed), "linear" (linear_initial_guess first) or "warm" (the
+    topology's resident warm start when it holds one, else vm0/va0); keep_warm: leave the first
+    converged scenario's solution on the device as the topology's warm start."""
+    flags = _lib.GX_KEEP_WARM if keep_warm else 0
     if init == "given":
-        return 0
+        return flags
     if init == "linear":
-        return _lib.GX_INIT_LINEAR
-    raise ValueError(f"unknown init {init!r} (expected 'given' or 'linear')")
+        return flags | _lib.GX_INIT_LINEAR
+    if init == "warm":
+        return flags | _lib.GX_INIT_WARM
+    raise ValueError(f"unknown init {init!r} (expected 'given', 'linear' or 'warm')")
 
 
 # ---------------------------------------------------------------- reference-shaped solvers
@@ -492,9 +499,10 @@ class Engine:
         return int(check(_lib.lib().gx_engine_bytes(self._h)))
 
     def submit(self, topo: "Topology", p_spec, q_spec, vm0, va0, tol=1e-8, max_iter=20, mode=Mode.MIXED,
-               init="given", outage=None):
+               init="given", outage=None, keep_warm=False):
         """Host arrays [n_scen, n_bus].  Returns the request id, or None when the request does not
-        fit the budget now (retry after collecting others)."""
+        fit the budget now (retry after collecting others).  init="warm" starts from the topology's
+        resident warm start; keep_warm leaves this request's first converged solution there."""
         p_spec = f64(np.atleast_2d(p_spec))
         S = p_spec.shape[0]
         q_spec = f64(np.broadcast_to(q_spec, (S, topo.n_bus)))
@@ -503,7 +511,7 @@ class Engine:
         out = None if outage is None else i32(np.broadcast_to(outage, (S,)))
         rc = _lib.lib().gx_engine_submit(self._h, topo._h, S, p_spec.ctypes.data, q_spec.ctypes.data,
                                          vm0.ctypes.data, va0.ctypes.data, None if out is None else out.ctypes.data,
-                                         float(tol), int(max_iter), int(mode), _init_flag(init))
+                                         float(tol), int(max_iter), int(mode), _init_flag(init, keep_warm))
         if rc == _lib.GX_ERR_BUDGET:
             return None
         rid = check(rc)

@@ -35,14 +35,22 @@ extern "C" {
 
 /* ---- numeric modes ---- */
 #define GX_MODE_FP64 0   /* Jacobian, LU and solves in FP64 (the reference's arithmetic) */
-#define GX_MODE_MIXED 1  /* FP64 mismatch/update/convergence; FP32 Jacobian/LU/solves + one FP64
-                            residual refinement sweep per Newton step */
+#define GX_MODE_MIXED 1  /* FP64 mismatch/update/convergence; FP32 Jacobian/LU/solves; a scenario whose
+                            mismatch is below 1e-2 pu refines its step with one sweep against the FP64
+                            residual (an FP32 correction solve) */
 
 /* ---- flags for gx_batch_solve ---- */
 #define GX_DEVICE_PTRS 1 /* every array argument is a device pointer (inputs already in HBM) */
 #define GX_INIT_LINEAR 2 /* start each scenario from linear_initial_guess (src/network.rs:627-678),
                             computed on the device from vm0/va0, as BatchSolver does
                             (src/batch.rs:268); without it vm0/va0 are used as given */
+#define GX_INIT_WARM 4   /* start every scenario from the topology's resident warm start (the solution
+                            a GX_KEEP_WARM solve left on the device) when it holds one, else from
+                            vm0/va0 as given; not combined with GX_INIT_LINEAR.  The reference keeps a
+                            topology's voltages between PersistentSolver solves (src/solver.rs:395-475) */
+#define GX_KEEP_WARM 8   /* after the solve (a batch, or an engine request when it finishes) the
+                            topology's resident warm start becomes the solution of its first
+                            converged scenario, on the device (unchanged when none converged) */
 
 const char* gx_version(void);
 const char* gx_last_error(void);
