@@ -19,6 +19,10 @@ constexpr int kRingGap = 32;      // solve ring: columns between fences / rows k
 constexpr int kChunkRounds = 16;  // refactor update rounds fetched into registers at a time
 constexpr int kPartTail = 192;   // refactor part: descriptors of entries past the registered ones in LDS
 constexpr int kPartRegs = 4;     // refactor part entries per lane group staged through registers
+constexpr int kPartMinChunks = 2;    // round chunks per refactor part at least (= the value-load distance)
+constexpr int kPartAhead = 4;        // chunks between loading a chunk's item positions and running it
+constexpr int kPartMaxEntries = 1023;  // refactor part entries at most (10-bit offsets in item words)
+constexpr int kStageZero = 1 << 30;    // pent_pos flag: the entry is structural fill, staged as zero
 // tail descriptors per part held in LDS for staging capacity cap: the entries past the registered
 // ones, at most kPartTail
 constexpr int part_tail(int cap, int tile) {
@@ -167,13 +171,18 @@ struct DevTopo {
   // narrow tiles, refactor by parts (null: column by column with the fused forward substitution):
   // part_a[p] = {first entry in pent_pos / pent_div, entries, first chunk of part_slots, rounds};
   // pent_div = -1 for a U entry, -2 for a pivot, else the part-buffer offset of the entry's pivot;
-  // part_slots as slots, with offsets into the part buffer; the forward substitution by levels
+  // part_x / part_w: the round items (x, word) of each chunk and lane group g at
+  // (chunk * G + g) * kChunkRounds + round; the forward substitution by levels
   // (fwd_items / fwd_steps, encoding as back_items with END keeping the sum)
   const int4* part_a = nullptr;
   const int* pent_pos = nullptr;
   const int* pent_div = nullptr;
-  const int2* part_slots = nullptr;
+  const int* part_x = nullptr;
+  const int* part_w = nullptr;
   int n_parts = 0;
+  // 1 when the refactor stages structural-fill entries as zero itself (the fill kernel then leaves
+  // those positions alone)
+  int stage_zero = 0;
   const int2* fwd_items = nullptr;
   int fwd_steps = 0;
 };

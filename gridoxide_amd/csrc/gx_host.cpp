@@ -206,21 +206,31 @@ int build_rounds(const gx::LuPattern& lu, const LuSchedule& s, int G, RoundPlan&
 // column k's U entries u in ascending order, the sequential order, so the factors are bit-identical
 // to the column-by-column refactor) are scheduled into rounds of G items with the rules of
 // build_rounds: one item per lane group, distinct targets, no target that is also read as a source,
-// a target's updates in order and after its source is final.  L(i,j) of the previous part is read
-// from that part's LDS buffer (bit 31 of the word, x = its offset there), any other from memory.
+// a target's updates in order and after its source is final.  The entries of a part past the first
+// kPartRegs per lane group are staged by the first rounds (staging items, ahead of every update of the
+// part), so their values flow through the same load stream as the L values.  Word of an item:
+// target (bits 0-9) | source (bits 10-19) | LDS offset of an L value of the previous part (bits
+// 20-29, with bit 31: read there, not from x) | bit 30: staging item (the target takes the value);
+// x: the packed position of the value to load (0 when none is needed); bit 29 of a staging item's word:
+// the entry is structural fill (kind[p] == kFill when kind is given) and is staged as zero, as is a
+// registered entry whose pent_pos carries kStageZero.  Every part spans at least
+// kPartMinChunks chunks: the kernel loads a chunk's values kPartMinChunks chunks ahead, so the values
+// of a part are loaded no earlier than during the part before it, after the part two back was written.
 struct PartPlan {
   std::vector<int> part_a;    // 4 per part: first entry (into pent_*), entries, first chunk, rounds
   std::vector<int> pent_pos;  // per part entry (in buffer order): its packed L+U position
   std::vector<int> pent_div;  // -1 U entry, -2 pivot, else the buffer offset of its column's pivot
-  std::vector<int> slots;     // pairs {x, word} as in build_rounds, chunks of kChunkRounds rounds
+  std::vector<int> slots;     // pairs {x, word} per round and lane group, chunks of kChunkRounds rounds
   int n_parts = 0;
   int cap = 0;                // entries of the largest part
 };
 
-int build_parts(const gx::LuPattern& lu, const LuSchedule& s, int G, int pcap, PartPlan& out) {
+int build_parts(const gx::LuPattern& lu, const LuSchedule& s, int G, int pcap, const std::vector<int>* kind,
+                PartPlan& out) {
   const int n = lu.n, NB = gx::kChunkRounds;
   out = PartPlan();
-  if (s.max_col > pcap || pcap >= 32767) return fail(GX_ERR_UNSUPPORTED, "a column does not fit a refactor part");
+  // (10-bit offsets in the item words; a larger part falls back to the column-by-column kernel)
+  if (s.max_col > pcap || pcap >= gx::kPartMaxEntries) return fail(GX_ERR_UNSUPPORTED, "a column does not fit a refactor part");
   std::vector<int> lev(n, 0);
   int nlev = 0;
   for (int k = 0; k < n; ++k) {
@@ -259,14 +269,15 @@ int build_parts(const gx::LuPattern& lu, const LuSchedule& s, int G, int pcap, P
     for (int k : cols) {
       const int c0 = lu.cp[k], len = lu.cp[k + 1] - c0, ul = lu.ulen[k];
       for (int o = 0; o < len; ++o) {
-        out.pent_pos.push_back(c0 + o);
+        const bool zero = kind && (*kind)[c0 + o] == gx::kFill;
+        out.pent_pos.push_back((c0 + o) | (zero ? gx::kStageZero : 0));
         out.pent_div.push_back(o < ul ? -1 : (o == ul ? -2 : base[k] + ul));
       }
     }
     const int Lp = (int)out.pent_pos.size() - pbase;
     out.cap = std::max(out.cap, Lp);
-    const int noop_w = Lp;  // empty slot: target the unused row just past the part
-    last.assign(Lp, -1);
+    const int noop_w = Lp | (Lp << 10);  // empty slot: the unused row just past the part
+    last.assign(Lp + 1, -1);
     sx.clear();
     sw.clear();
     int R = 0;
@@ -281,6 +292,17 @@ int build_parts(const gx::LuPattern& lu, const LuSchedule& s, int G, int pcap, P
       rs[R].clear();
       ++R;
     };
+    // staging items: entries past the registered ones, G per round from round 0
+    for (int o = G * gx::kPartRegs; o < Lp; ++o) {
+      const int r = (o - G * gx::kPartRegs) / G, slot = (o - G * gx::kPartRegs) % G;
+      while (r >= R) add_round();
+      const int ps = out.pent_pos[pbase + o];
+      const bool zero = (ps & gx::kStageZero) != 0;
+      sx[(size_t)r * G + slot] = zero ? 0 : ps;
+      sw[(size_t)r * G + slot] = o | (Lp << 10) | (1 << 30) | (zero ? (1 << 29) : 0);
+      rt[r].push_back(o);
+      last[o] = r;
+    }
     for (int k : cols) {
       const int c0 = lu.cp[k], ul = lu.ulen[k];
       for (int u = 0; u < ul; ++u) {
@@ -298,8 +320,9 @@ int build_parts(const gx::LuPattern& lu, const LuSchedule& s, int G, int pcap, P
             if (slot < 0) continue;
             if (std::find(rt[r].begin(), rt[r].end(), tg) != rt[r].end()) continue;
             if (std::find(rs[r].begin(), rs[r].end(), tg) != rs[r].end()) continue;
-            sx[(size_t)r * G + slot] = prev ? base[j] + (p - lu.cp[j]) : p;
-            sw[(size_t)r * G + slot] = tg | (su << 16) | (prev ? (int)0x80000000u : 0);
+            sx[(size_t)r * G + slot] = prev ? 0 : p;
+            sw[(size_t)r * G + slot] =
+                tg | (su << 10) | (prev ? (int)(0x80000000u | ((unsigned)(base[j] + (p - lu.cp[j])) << 20)) : 0);
             rt[r].push_back(tg);
             rs[r].push_back(su);
             last[tg] = r;
@@ -308,7 +331,7 @@ int build_parts(const gx::LuPattern& lu, const LuSchedule& s, int G, int pcap, P
         }
       }
     }
-    const int nch = std::max(1, (R + NB - 1) / NB);
+    const int nch = std::max(gx::kPartMinChunks, (R + NB - 1) / NB);
     out.part_a[4 * pi] = pbase;
     out.part_a[4 * pi + 1] = Lp;
     out.part_a[4 * pi + 2] = chunk;
@@ -322,8 +345,8 @@ int build_parts(const gx::LuPattern& lu, const LuSchedule& s, int G, int pcap, P
       }
     if (out.slots.size() >= (size_t)0x7fffffff) return fail(GX_ERR_UNSUPPORTED, "refactor round schedule too large");
   }
-  // two chunks past the last one, read by the kernel's prefetch and never executed
-  out.slots.insert(out.slots.end(), 2 * (size_t)2 * NB * G, 0);
+  // chunks past the last one, read by the kernel's prefetch and never executed
+  out.slots.insert(out.slots.end(), 2 * (size_t)gx::kPartAhead * NB * G, 0);
   return GX_OK;
 }
 
@@ -599,7 +622,7 @@ bool build_solve_levels(const gx::LuPattern& lu, int G, int ring_rows, bool forw
 
 struct DevSched {
   DevMem cp, ulen, row, chunk0, rounds, slots, fptr, fpos, fcol, bptr, bpos, bcol, col_a, col_f, col_b, fslot, opos, ocol;
-  DevMem uround, uitems, bslot, bitems, fitems, part_a, pent_pos, pent_div, part_slots;
+  DevMem uround, uitems, bslot, bitems, fitems, part_a, pent_pos, pent_div, part_x, part_w;
   int back_steps = 0, fwd_steps = 0, n_parts = 0, part_cap = 0;
   std::vector<int> h_chunk0, h_rounds, h_fptr, h_bptr;
   bool uploaded = false;
@@ -632,7 +655,7 @@ int lu_cap(int max_col, int tile) {
 // Uploads the pattern once, the update rounds whenever the lane-group count changes and the far
 // lists whenever the window changes.
 int upload_sched(const gx::LuPattern& lu, const LuSchedule& s, int window, int tile, DevSched& m,
-                 gx::DevTopo& d) {
+                 gx::DevTopo& d, const std::vector<int>* kind = nullptr) {
   int rc;
   const int G = gx::kLanes / tile;
   if (!m.uploaded) {
@@ -776,10 +799,22 @@ int upload_sched(const gx::LuPattern& lu, const LuSchedule& s, int window, int t
       int fsteps = 0;
       PartPlan plan;
       if (build_forward_levels(lu, G, window + gx::kRingGap + 1, fitems, fsteps) &&
-          build_parts(lu, s, G, lu_cap(s.max_col, tile), plan) == GX_OK) {
+          build_parts(lu, s, G, lu_cap(s.max_col, tile), kind, plan) == GX_OK) {
+        // the round items as two streams, x and word, each chunk's items of lane group g one run of
+        // kChunkRounds ints (the kernel loads a chunk with four 16-byte loads per lane)
+        const int NB = gx::kChunkRounds;
+        const size_t chunks = plan.slots.size() / (2 * (size_t)NB * G);
+        std::vector<int> xs(chunks * G * NB), ws(chunks * G * NB);
+        for (size_t c = 0; c < chunks; ++c)
+          for (int r = 0; r < NB; ++r)
+            for (int gg = 0; gg < G; ++gg) {
+              const size_t from = 2 * ((c * NB + r) * G + gg), to = (c * G + gg) * NB + r;
+              xs[to] = plan.slots[from];
+              ws[to] = plan.slots[from + 1];
+            }
         if ((rc = m.fitems.upload(fitems)) || (rc = m.part_a.upload(plan.part_a)) ||
             (rc = m.pent_pos.upload(plan.pent_pos)) || (rc = m.pent_div.upload(plan.pent_div)) ||
-            (rc = m.part_slots.upload(plan.slots)))
+            (rc = m.part_x.upload(xs)) || (rc = m.part_w.upload(ws)))
           return rc;
         m.fwd_steps = fsteps;
         m.n_parts = plan.n_parts;
@@ -819,8 +854,10 @@ int upload_sched(const gx::LuPattern& lu, const LuSchedule& s, int window, int t
   d.part_a = parts ? m.part_a.as<int4>() : nullptr;
   d.pent_pos = parts ? m.pent_pos.as<int>() : nullptr;
   d.pent_div = parts ? m.pent_div.as<int>() : nullptr;
-  d.part_slots = parts ? m.part_slots.as<int2>() : nullptr;
+  d.part_x = parts ? m.part_x.as<int>() : nullptr;
+  d.part_w = parts ? m.part_w.as<int>() : nullptr;
   d.n_parts = parts ? m.n_parts : 0;
+  d.stage_zero = parts && kind ? 1 : 0;
   d.fwd_items = parts ? m.fitems.as<int2>() : nullptr;
   d.fwd_steps = parts ? m.fwd_steps : 0;
   if (parts) d.max_col = m.part_cap;  // (sizes the staging buffers: lu_cap)
@@ -1276,7 +1313,7 @@ int upload_ybus(gx_topo& t);
 // far lists of the solves for the given window.
 int upload_topo(gx_topo& t, int window, int tile) {
   int rc;
-  if ((rc = upload_sched(t.lu, t.sched, window, tile, t.ds, t.dt))) return rc;
+  if ((rc = upload_sched(t.lu, t.sched, window, tile, t.ds, t.dt, &t.h_kind))) return rc;
   if (t.uploaded) return GX_OK;
   if ((rc = upload_ybus(t)) || (rc = t.d_rpos_p.upload(t.h_rp)) ||
       (rc = t.d_rpos_q.upload(t.h_rq)) || (rc = t.d_xpos_a.upload(t.h_xa)) || (rc = t.d_xpos_m.upload(t.h_xm)) ||
@@ -2062,7 +2099,7 @@ int gx_topo_refactor_parts(const gx_topo* t, int tile, long long* sizes, int* pa
   if (!t->factored) return fail(GX_ERR_STATE, "not factored yet");
   PartPlan plan;
   const int pcap = lu_cap(t->sched.max_col, tile);
-  int rc = build_parts(t->lu, t->sched, gx::kLanes / tile, pcap, plan);
+  int rc = build_parts(t->lu, t->sched, gx::kLanes / tile, pcap, &t->h_kind, plan);
   if (rc) return rc;
   sizes[0] = plan.n_parts;
   sizes[1] = (long long)plan.pent_pos.size();

@@ -1429,7 +1429,8 @@ __device__ __forceinline__ void lu64_tile(const DevTopo& t, const DevWork& w, in
 // the other in its buffer: stage A at their entries, run the part's update rounds (the chunk stream
 // of the column-by-column kernel, host-scheduled), divide each L entry by its column's pivot, write
 // back.  The first kPartRegs * G entries of a part are staged through registers loaded a part ahead
-// (their positions two parts ahead); longer parts load the rest where they are used.
+// (their positions two parts ahead); the rest are staged by the part's first rounds, whose values come
+// with the round stream's loads.
 template <typename T, int TW, bool kFactor>
 __device__ __forceinline__ void lu_tile_parts(const DevTopo& t, const DevWork& w, int tile, int cap, int iter) {
   constexpr int G = kLanes / TW;
@@ -1504,19 +1505,52 @@ __device__ __forceinline__ void lu_tile_parts(const DevTopo& t, const DevWork& w
     };
     auto val_issue_p = [&](const int (&pp)[MP], T (&vv)[MP]) {
 #pragma unroll
-      for (int i = 0; i < MP; ++i) vv[i] = LU(pp[i]);
+      for (int i = 0; i < MP; ++i) vv[i] = LU(pp[i] & ~kStageZero);
     };
-    // update-round chunk stream over all parts (as in the column-by-column kernel)
-    int2 sA[NB], sB[NB], sC[NB];
-    T vA[NB], vB[NB];
-    const int2* __restrict__ slb = t.part_slots + g;
-    auto slots_issue = [&](int q, int2 (&d)[NB]) {
+    // The update rounds of all parts form one stream of chunks of NB rounds (part by part, every
+    // part at least kPartMinChunks chunks).  The item positions of chunk gc are loaded kPartAhead
+    // chunks before it runs, its values and words kPartMinChunks chunks before; three register sets
+    // take turns by the chunk's index mod 3, so the loop copies no register a load is filling.
+    static_assert(kPartMinChunks == 2 && kPartAhead == 4, "the register roles below assume these");
+    int xs[3][NB], ws[3][NB];
+    T vs[3][NB];
+    const int* __restrict__ xsb = t.part_x + g * NB;
+    const int* __restrict__ wsb = t.part_w + g * NB;
+    auto ints_issue = [&](const int* __restrict__ base, int chunk, int (&d)[NB]) {
+      const int4* q = reinterpret_cast<const int4*>(base + (size_t)(unsigned)chunk * (G * NB));
 #pragma unroll
-      for (int r = 0; r < NB; ++r) d[r] = slb[(unsigned)((q * NB + r) * G)];
+      for (int i = 0; i < NB / 4; ++i) {
+        const int4 v = q[i];
+        d[4 * i] = v.x;
+        d[4 * i + 1] = v.y;
+        d[4 * i + 2] = v.z;
+        d[4 * i + 3] = v.w;
+      }
     };
-    auto vals_issue = [&](const int2 (&sm)[NB], T (&d)[NB]) {
+    auto vals_issue = [&](const int (&x)[NB], T (&d)[NB]) {
 #pragma unroll
-      for (int r = 0; r < NB; ++r) d[r] = LU(sm[r].x);
+      for (int r = 0; r < NB; ++r) d[r] = LU(x[r]);
+    };
+    // chunk gc in register role E: the loads of the chunks ahead, then its rounds (rleft of them
+    // belong to the part; the rest are empty).  Item word: target | source << 10 | (LDS offset of a
+    // previous-part L value) << 20 | zero << 29 | staging << 30 | previous-part << 31.
+    auto step = [&](auto role, int gc, int rleft, T* xc, const T* xprev) __attribute__((always_inline)) {
+      constexpr int E = decltype(role)::value, E1 = (E + 1) % 3, E2 = (E + 2) % 3;
+      vals_issue(xs[E2], vs[E2]);                    // values of chunk gc + 2
+      ints_issue(xsb, gc + kPartAhead, xs[E1]);      // positions of chunk gc + 4
+      ints_issue(wsb, gc + kPartMinChunks, ws[E2]);  // words of chunk gc + 2
+#pragma unroll
+      for (int r = 0; r < NB; ++r) {
+        if (r < rleft) {
+          const int wd = ws[E][r];
+          const int tg = wd & 0x3ff, su = (wd >> 10) & 0x3ff;
+          T lv = vs[E][r];
+          if (wd < 0) lv = xprev[((wd >> 20) & 0x3ff) * S + s];
+          const T xt = xc[tg * S + s], xu = xc[su * S + s];
+          const T st = (wd & (1 << 29)) ? zero_value<T>() : lv;  // (structural fill stages zero)
+          xc[tg * S + s] = (wd & (1 << 30)) ? st : xt - lv * xu;
+        }
+      }
     };
     int4 m1 = vload4(t.part_a, cl(1, np - 1)), m2 = vload4(t.part_a, cl(2, np - 1));
     {
@@ -1525,12 +1559,20 @@ __device__ __forceinline__ void lu_tile_parts(const DevTopo& t, const DevWork& w
       ent_issue(uni4(m1), xpos, xdiv);
       val_issue_p(cpos, cval);
     }
-    slots_issue(0, sA);
-    slots_issue(1, sB);
-    vals_issue(sA, vA);
+    // the stream's first chunks: positions 0-3, words 0-1, values 0-1
+    ints_issue(xsb, 0, xs[0]);
+    ints_issue(xsb, 1, xs[1]);
+    ints_issue(xsb, 2, xs[2]);
+    ints_issue(wsb, 0, ws[0]);
+    ints_issue(wsb, 1, ws[1]);
+    drain();
+    vals_issue(xs[0], vs[0]);
+    vals_issue(xs[1], vs[1]);
+    ints_issue(xsb, 3, xs[0]);
     drain();
     int4 mk = uni4(vload4(t.part_a, 0));
     tail_issue(mk);
+    int role = 0;  // the running chunk's index mod 3
     for (int p = 0; p < np; ++p) {
       const int pb = mk.x, len = mk.y, ch0 = mk.z, nr = mk.w;
       const int4 mx = uni4(m1);
@@ -1546,27 +1588,11 @@ __device__ __forceinline__ void lu_tile_parts(const DevTopo& t, const DevWork& w
       tail_store(p & 1, len);
       memory_order_point();  // the descriptors in LDS before other lanes read them
       tail_issue(mx);
-      // stage A: registered entries, then (long parts) the rest
+      // stage the registered entries (the others arrive through the part's staging rounds)
 #pragma unroll
       for (int i = 0; i < MP; ++i) {
         const int o = g + G * i;
-        if (o < len) xc[o * S + s] = cval[i];
-      }
-      for (int b = G * MP; b < len; b += G * MT) {
-        int pp[MT];
-        T tv[MT];
-#pragma unroll
-        for (int u = 0; u < MT; ++u) {
-          const int o = b + G * u + g;
-          pp[u] = tail_desc(p, pb, o < len ? o : G * MP).x;
-        }
-#pragma unroll
-        for (int u = 0; u < MT; ++u) tv[u] = LU(pp[u]);
-#pragma unroll
-        for (int u = 0; u < MT; ++u) {
-          const int o = b + G * u + g;
-          if (o < len) xc[o * S + s] = tv[u];
-        }
+        if (o < len) xc[o * S + s] = (cpos[i] & kStageZero) ? zero_value<T>() : cval[i];
       }
       // loads for the next parts
       val_issue_p(xpos, xval);
@@ -1575,29 +1601,22 @@ __device__ __forceinline__ void lu_tile_parts(const DevTopo& t, const DevWork& w
         ent_issue(my, ypos, ydiv);
       }
       memory_order_point();  // the staged values before the rounds read them
-      const int nch = nr > NB ? (nr + NB - 1) / NB : 1;
-      for (int c = 0; c < nch; ++c) {
-        slots_issue(ch0 + c + 2, sC);
-        vals_issue(sB, vB);
-        const int rleft = nr - c * NB;
-#pragma unroll
-        for (int r = 0; r < NB; ++r) {
-          if (r < rleft) {
-            const int wd = sA[r].y;
-            const int tg = wd & 0xffff, su = (wd >> 16) & 0x7fff;
-            T lv = vA[r];
-            if (wd < 0) lv = xprev[sA[r].x * S + s];
-            const T xt = xc[tg * S + s], xu = xc[su * S + s];
-            xc[tg * S + s] = xt - lv * xu;
-          }
+      const int nch = (nr + NB - 1) / NB > kPartMinChunks ? (nr + NB - 1) / NB : kPartMinChunks;
+      for (int c = 0;;) {
+        if (role == 0) {
+          step(std::integral_constant<int, 0>{}, ch0 + c, nr - c * NB, xc, xprev);
+          role = 1;
+          if (++c == nch) break;
         }
-        if (c + 1 < nch) {
-#pragma unroll
-          for (int r = 0; r < NB; ++r) {
-            sA[r] = sB[r];
-            sB[r] = sC[r];
-            vA[r] = vB[r];
-          }
+        if (role == 1) {
+          step(std::integral_constant<int, 1>{}, ch0 + c, nr - c * NB, xc, xprev);
+          role = 2;
+          if (++c == nch) break;
+        }
+        if (role == 2) {
+          step(std::integral_constant<int, 2>{}, ch0 + c, nr - c * NB, xc, xprev);
+          role = 0;
+          if (++c == nch) break;
         }
       }
       memory_order_point();  // the rounds' writes before the division reads
@@ -1619,7 +1638,7 @@ __device__ __forceinline__ void lu_tile_parts(const DevTopo& t, const DevWork& w
         const int o = g + G * i;
         if (o < len) {
           if (cdiv[i] >= 0) xc[o * S + s] = cval[i];
-          LU(cpos[i]) = cval[i];
+          LU(cpos[i] & ~kStageZero) = cval[i];
         }
       }
       for (int b = G * MP; b < len; b += G * MT) {
@@ -1629,7 +1648,7 @@ __device__ __forceinline__ void lu_tile_parts(const DevTopo& t, const DevWork& w
         for (int u = 0; u < MT; ++u) {
           const int o = b + G * u + g;
           const int2 d = tail_desc(p, pb, o < len ? o : G * MP);
-          pp[u] = d.x;
+          pp[u] = d.x & ~kStageZero;
           dd[u] = d.y;
         }
 #pragma unroll
@@ -1649,13 +1668,7 @@ __device__ __forceinline__ void lu_tile_parts(const DevTopo& t, const DevWork& w
           }
         }
       }
-      // the chunk stream moves on to the next part's first chunk; the staged registers rotate
-#pragma unroll
-      for (int r = 0; r < NB; ++r) {
-        sA[r] = sB[r];
-        sB[r] = sC[r];
-        vA[r] = vB[r];
-      }
+      // the registered sets rotate
 #pragma unroll
       for (int i = 0; i < MP; ++i) {
         cpos[i] = xpos[i];
@@ -2177,7 +2190,7 @@ void launch_fill(const DevTopo& t, const DevWork& w, bool fp32, hipStream_t stre
     dim3 grid(t.n_bus, tb);
     if (fp32) hipLaunchKernelGGL((k_fill_bus<float, TW>), grid, dim3(256), 0, stream, t, w);
     else hipLaunchKernelGGL((k_fill_bus<double, TW>), grid, dim3(256), 0, stream, t, w);
-    if (t.n_fz > 0) {
+    if (t.n_fz > 0 && !t.stage_zero) {  // (else the refactor stages the fill entries as zero)
       constexpr int B = lu_block(TW);
       const int fb = (t.n_fz + 255) / 256;
       dim3 zgrid(fb < 64 ? fb : 64, (w.n_tiles + B - 1) / B);

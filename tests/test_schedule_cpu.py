@@ -19,6 +19,9 @@ from gridoxide_amd import _lib
 from gridoxide_amd._lib import check, ptr
 
 P = 4                   # items per lane group and step (kBackItems)
+PART_REGS = 4           # refactor part entries per lane group staged through registers (kPartRegs)
+MIN_CHUNKS = 2          # chunks per part at least = chunks between a value's load and its use
+STAGE_ZERO = 1 << 30    # pent_pos flag of a structural-fill entry (kStageZero): staged as zero
 D = 4                   # steps between loading a step's values and running it (kBackAhead)
 POS = (1 << 29) - 1     # payload of the first word
 ENTRY, START, END, PAD = 0, 1, 2, 3
@@ -207,41 +210,53 @@ def _sequential_refactor(cp, ulen, row, a):
     return lu
 
 
-def _emulate_parts(pa, pos, div, slots, a, cap):
-    """The part loop of the kernel for one scenario.  A non-flagged L value of chunk c of part p is
-    loaded a chunk ahead: after the write-back of part p - 2 (c = 0) or p - 1 (c > 0); a position not
-    written back by then still holds its Jacobian value.  Staging rows hold NaN until staged."""
+def _emulate_parts(pa, pos, div, slots, a, cap, G):
+    """The part loop of the kernel for one scenario.  The first G * kPartRegs entries of a part are
+    staged before its rounds; the others by its staging items.  The values of chunk c of part p are
+    loaded kPartMinChunks = 2 chunks ahead: after the write-back of part p - 2 (c < 2) or p - 1
+    (c >= 2); a position not written back by then still holds its Jacobian value.  Staging rows hold
+    NaN until staged."""
     mem = a.copy()
     owner = np.full(a.size, np.iinfo(np.int64).max)  # the part that wrote each position back
     buf = [np.full(cap + 1, np.nan), np.full(cap + 1, np.nan)]
+    regs = G * PART_REGS
     for p, (pb, ln, ch0, nr) in enumerate(pa):
         assert 0 < ln <= cap
         xc, xprev = buf[p & 1], buf[(p & 1) ^ 1]
         xc[:] = np.nan
         ent = np.arange(pb, pb + ln)
-        assert (owner[pos[ent]] == np.iinfo(np.int64).max).all()
-        xc[:ln] = mem[pos[ent]]
-        for r in range(nr):
-            seen = p - 2 if r < NB else p - 1
+        zero = (pos[ent] & STAGE_ZERO) != 0
+        epos = pos[ent] & ~STAGE_ZERO
+        assert (owner[epos] == np.iinfo(np.int64).max).all()
+        k = min(ln, regs)
+        xc[:k] = np.where(zero[:k], 0.0, mem[epos[:k]])
+        nch = max(MIN_CHUNKS, -(-nr // NB))
+        for r in range(nch * NB):
+            seen = p - 2 if r < MIN_CHUNKS * NB else p - 1
             x = slots[ch0 * NB + r, :, 0].astype(np.int64)
             wd = slots[ch0 * NB + r, :, 1].astype(np.int64) & 0xFFFFFFFF
-            tg, su, fl = wd & 0xFFFF, (wd >> 16) & 0x7FFF, (wd >> 31) & 1
+            tg, su, off = wd & 0x3FF, (wd >> 10) & 0x3FF, (wd >> 20) & 0x3FF
+            st, fl, zr = (wd >> 30) & 1, (wd >> 31) & 1, (wd >> 29) & 1
             real = tg != ln  # empty slots target the row past the part
-            assert (tg <= ln).all() and (su[real] < ln).all()
+            if r >= nr:
+                assert not real.any()
+                continue
+            assert (tg <= ln).all() and (su[real & (st == 0)] < ln).all()
             assert np.unique(tg[real]).size == np.count_nonzero(real)
-            assert not np.isin(tg[real], su[real]).any()
-            xm = np.where(fl == 1, 0, x)
-            lv = np.where(owner[xm] <= seen, mem[xm], a[xm])
-            lv = np.where(fl == 1, xprev[np.where(fl == 1, x, 0)], lv)
+            assert not np.isin(tg[real], su[real & (st == 0)]).any()
+            assert (tg[real & (st == 1)] >= regs).all()
+            lv = np.where(owner[x] <= seen, mem[x], a[x])
+            lv = np.where(fl == 1, xprev[np.where(fl == 1, off, 0)], lv)
             xt, xu = xc[tg], xc[su]
-            xc[tg] = xt - lv * xu
+            xc[tg] = np.where(st == 1, np.where(zr == 1, 0.0, lv), xt - lv * xu)
         d = div[ent]
         v = xc[:ln].copy()
+        assert not np.isnan(v).any()
         m = d >= 0
         v[m] = v[m] / xc[d[m]]
         xc[:ln][m] = v[m]
-        mem[pos[ent]] = v
-        owner[pos[ent]] = p
+        mem[epos] = v
+        owner[epos] = p
     return mem
 
 
@@ -253,12 +268,18 @@ def test_refactor_parts_equal_column_by_column(grid, tile):
     pa, pos, div, slots, cap = _parts(topo, tile)
     n = ulen.size
     # every entry of L+U is staged by exactly one part
-    assert np.array_equal(np.sort(pos), np.arange(cp[-1]))
+    assert np.array_equal(np.sort(pos & ~STAGE_ZERO), np.arange(cp[-1]))
     rng = np.random.default_rng(13)
     a = rng.uniform(-0.2, 0.2, cp[-1])
     a[cp[:-1] + ulen] = rng.uniform(3.0, 4.0, n) * rng.choice([-1.0, 1.0], n)
+    fill = np.zeros(cp[-1], bool)
+    fill[pos[(pos & STAGE_ZERO) != 0] & ~STAGE_ZERO] = True
+    assert fill.any()  # the pattern has structural fill, staged as zero
+    a[fill] = 0.0
     want = _sequential_refactor(cp, ulen, row, a)
-    got = _emulate_parts(pa, pos, div, slots, a, cap)
+    a_dev = a.copy()
+    a_dev[fill] = rng.uniform(-9.0, 9.0, fill.sum())  # stale values the refactor must not read
+    got = _emulate_parts(pa, pos, div, slots, a_dev, cap, 64 // tile)
     assert np.array_equal(got, want)
     # fewer parts than columns: the columns of a level are factored together
     assert pa.shape[0] < n
