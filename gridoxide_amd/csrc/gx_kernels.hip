@@ -447,49 +447,63 @@ __global__ __launch_bounds__(256) void k_fill(DevTopo t, DevWork w) {
   *lu = (T)v;
 }
 
-// Jacobian fill by bus (JacobianPattern::fill_into, src/jacobian.rs:210-250, in rectangular form):
-// one row bus i and a block of tiles per workgroup; V, P and Q of bus i are read once, and each
-// neighbour k's V once for its up to four entries H, N, M, L (i, k).  The positions' stores of the
-// workgroup's tiles are contiguous.
+// Jacobian fill by bus (JacobianPattern::fill_into, src/jacobian.rs:210-250, in rectangular form),
+// block of tiles by block: the L+U values of a block's B tiles lie side by side, B * TW values per
+// position, so a lane group of B * TW lanes (the block's scenarios) writes whole runs; a wavefront
+// takes one block and 64 / (B * TW) consecutive row buses (one per lane group).  V, P and Q of a row
+// bus are read once and each neighbour k's V once for its up to four entries H, N, M, L (i, k); the
+// neighbours of a bus sit a few buses away in the same tile's arrays.
 template <typename T, int TW>
-__global__ __launch_bounds__(256) void k_fill_bus(DevTopo t, DevWork w) {
-  const int i = blockIdx.x, tile = blockIdx.y * (256 / TW) + (int)threadIdx.x / TW, s = sub<TW>();
-  if (tile >= w.n_tiles) return;
-  const int e0 = t.fb_ptr[i], e1 = t.fb_ptr[i + 1];
-  if (e0 == e1) return;
-  bool run = false;  // any scenario of the tile still iterating (the LU kernel skips the others)
-  for (int q = 0; q < TW; ++q) run = run || w.status[tile * TW + q] == kRunning;
+__global__ __launch_bounds__(256) void k_fill_tile(DevTopo t, DevWork w) {
+  // a lane group spans a whole block when it fits a wavefront, else one tile (the wide tile)
+  constexpr int B = lu_block(TW), V = B * TW, LW = V <= kLanes ? V : TW, GB = kLanes / LW;
+  const int lane = lane_id(), grp_b = lane / LW, sub_b = lane % LW;
+  // (the last block may hold tiles past n_tiles: they read the last tile's inputs, their values are
+  // never used)
+  int tile = V <= kLanes ? blockIdx.y * B + sub_b / TW : blockIdx.y;
+  tile = tile < w.n_tiles ? tile : w.n_tiles - 1;
+  const int s = sub_b % TW;
+  bool run = false;  // any scenario of the group's tiles still iterating (the LU kernel skips the others)
+  for (int q = 0; q < LW; ++q) {
+    const int sq = blockIdx.y * LW + q;
+    run = run || (sq < w.n_tiles * TW && w.status[sq] == kRunning);
+  }
   if (!run) return;
   const int sc = tile * TW + s;
   const size_t bb = at<TW>(tile, t.n_bus, 0) + s;
   const Outage og = load_outage(t, w, sc);
-  const size_t oi = bb + (size_t)i * TW;
-  const double vri = w.vr[oi], vii = w.vi[oi], pci = w.pc[oi], qci = w.qc[oi];
-  T* __restrict__ lu = reinterpret_cast<T*>(w.lu) + lu_at(0, t.nnz_lu, tile, TW) + s;
-  const size_t pst = (size_t)lu_block(TW) * TW;
-  for (int e = e0; e < e1; ++e) {
-    const int k = t.fb_k[e];
-    const int4 pos = t.fb_pos[e];
-    double g = t.fb_g[e], b = t.fb_b[e];
-    if (w.outage) y_at(t, og, t.fb_yp[e], g, b);
-    const size_t ok = bb + (size_t)k * TW;
-    const double vrk = w.vr[ok], vik = w.vi[ok], vmk = w.vm[ok];
-    const double cc = vri * vrk + vii * vik;
-    const double ss = vii * vrk - vri * vik;
-    const double hl = g * ss - b * cc;
-    const double nm = g * cc + b * ss;
-    const bool diag = k == i;
-    double vh = hl, vn = nm / vmk, vmm = -nm, vl = hl / vmk;
-    if (diag) {
-      vh += -qci;
-      vn += pci / vmk;
-      vmm += pci;
-      vl += qci / vmk;
+  T* __restrict__ lu = reinterpret_cast<T*>(w.lu) + (V <= kLanes ? (size_t)blockIdx.y * t.nnz_lu * V + sub_b
+                                                                  : lu_at(0, t.nnz_lu, tile, TW) + s);
+  const int first = (blockIdx.x * (blockDim.x / kLanes) + (int)threadIdx.x / kLanes) * GB + grp_b;
+  const int stride = gridDim.x * (blockDim.x / kLanes) * GB;
+  for (int i = first; i < t.n_bus; i += stride) {
+    const int e0 = t.fb_ptr[i], e1 = t.fb_ptr[i + 1];
+    const size_t oi = bb + (size_t)i * TW;
+    const double vri = w.vr[oi], vii = w.vi[oi], pci = w.pc[oi], qci = w.qc[oi];
+    for (int e = e0; e < e1; ++e) {
+      const int k = t.fb_k[e];
+      const int4 pos = t.fb_pos[e];
+      double g = t.fb_g[e], b = t.fb_b[e];
+      if (w.outage) y_at(t, og, t.fb_yp[e], g, b);
+      const size_t ok = bb + (size_t)k * TW;
+      const double vrk = w.vr[ok], vik = w.vi[ok], vmk = w.vm[ok];
+      const double cc = vri * vrk + vii * vik;
+      const double ss = vii * vrk - vri * vik;
+      const double hl = g * ss - b * cc;
+      const double nm = g * cc + b * ss;
+      const bool diag = k == i;
+      double vh = hl, vn = nm / vmk, vmm = -nm, vl = hl / vmk;
+      if (diag) {
+        vh += -qci;
+        vn += pci / vmk;
+        vmm += pci;
+        vl += qci / vmk;
+      }
+      if (pos.x >= 0) lu[(size_t)pos.x * V] = (T)vh;
+      if (pos.y >= 0) lu[(size_t)pos.y * V] = (T)vn;
+      if (pos.z >= 0) lu[(size_t)pos.z * V] = (T)vmm;
+      if (pos.w >= 0) lu[(size_t)pos.w * V] = (T)vl;
     }
-    if (pos.x >= 0) lu[(size_t)pos.x * pst] = (T)vh;
-    if (pos.y >= 0) lu[(size_t)pos.y * pst] = (T)vn;
-    if (pos.z >= 0) lu[(size_t)pos.z * pst] = (T)vmm;
-    if (pos.w >= 0) lu[(size_t)pos.w * pst] = (T)vl;
   }
 }
 
@@ -2187,9 +2201,11 @@ void launch_fill(const DevTopo& t, const DevWork& w, bool fp32, hipStream_t stre
       else hipLaunchKernelGGL((k_fill<double, TW>), grid, dim3(256), 0, stream, t, w);
       return;
     }
-    dim3 grid(t.n_bus, tb);
-    if (fp32) hipLaunchKernelGGL((k_fill_bus<float, TW>), grid, dim3(256), 0, stream, t, w);
-    else hipLaunchKernelGGL((k_fill_bus<double, TW>), grid, dim3(256), 0, stream, t, w);
+    (void)tb;
+    constexpr int B = lu_block(TW), V = B * TW;
+    dim3 grid(elem_blocks(t.n_bus, V <= kLanes ? V : TW), V <= kLanes ? (w.n_tiles + B - 1) / B : w.n_tiles);
+    if (fp32) hipLaunchKernelGGL((k_fill_tile<float, TW>), grid, dim3(256), 0, stream, t, w);
+    else hipLaunchKernelGGL((k_fill_tile<double, TW>), grid, dim3(256), 0, stream, t, w);
     if (t.n_fz > 0 && !t.stage_zero) {  // (else the refactor stages the fill entries as zero)
       constexpr int B = lu_block(TW);
       const int fb = (t.n_fz + 255) / 256;
