@@ -23,6 +23,7 @@ constexpr int kPartMinChunks = 2;    // round chunks per refactor part at least 
 constexpr int kPartAhead = 4;        // chunks between loading a chunk's item positions and running it
 constexpr int kPartMaxEntries = 1023;  // refactor part entries at most (10-bit offsets in item words)
 constexpr int kStageZero = 1 << 30;    // pent_pos flag: the entry is structural fill, staged as zero
+constexpr int kJointRounds = 1;        // refactor rounds whose LDS reads are issued before their writes
 // tail descriptors per part held in LDS for staging capacity cap: the entries past the registered
 // ones, at most kPartTail
 constexpr int part_tail(int cap, int tile) {
@@ -45,12 +46,13 @@ constexpr int ring_rows(int tile) { return tile == kWideTile ? 256 : (tile <= 8 
 // wide tile keeps no rows behind the front (a lane only reads its own scenario's values).
 constexpr int ring_window_of(int rows, int tile) { return tile == kWideTile ? rows - 1 : rows - kRingGap - 1; }
 // LDS bytes of the LU kernel for a tile width: the staged column(s) of max_col + 1 rows (factoring
-// only; two for the narrow tiles) and the solve ring, each row holding tile (+1 padding on narrow
-// tiles) values of elem bytes; narrow tiles factoring also hold two parts' tail descriptors.
+// only; two of max_col + 2 rows for the narrow tiles, whose last row holds the constant -1) and the
+// solve ring, each row holding tile (+1 padding on narrow tiles) values of elem bytes; narrow tiles
+// factoring also hold two parts' tail descriptors.
 constexpr size_t lu_lds_bytes(int max_col, int rows, int tile, size_t elem, bool factor) {
   return tile == kWideTile
              ? ((factor ? (size_t)(max_col < 1 ? 1 : max_col) + 1 : 0) + (size_t)rows) * (size_t)tile * elem
-             : ((factor ? 2 * ((size_t)(max_col < 1 ? 1 : max_col) + 1) : 0) + (size_t)rows) * (size_t)(tile + 1) * elem +
+             : ((factor ? 2 * ((size_t)(max_col < 1 ? 1 : max_col) + 2) : 0) + (size_t)rows) * (size_t)(tile + 1) * elem +
                    (factor ? 8 + 2 * (size_t)part_tail(max_col < 1 ? 1 : max_col, tile) * 8 : 0);
 }
 

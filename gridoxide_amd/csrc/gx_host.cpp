@@ -204,16 +204,17 @@ int build_rounds(const gx::LuPattern& lu, const LuSchedule& s, int G, RoundPlan&
 // cut into parts of at most `pcap` entries; parts run level by level.  A part's columns lie one after
 // the other in its LDS buffer (column k at base_k), and its update items (x_t -= L(i,j) * x_u for
 // column k's U entries u in ascending order, the sequential order, so the factors are bit-identical
-// to the column-by-column refactor) are scheduled into rounds of G items with the rules of
-// build_rounds: one item per lane group, distinct targets, no target that is also read as a source,
-// a target's updates in order and after its source is final.  The entries of a part past the first
-// kPartRegs per lane group are staged by the first rounds (staging items, ahead of every update of the
-// part), so their values flow through the same load stream as the L values.  Word of an item:
-// target (bits 0-9) | source (bits 10-19) | LDS offset of an L value of the previous part (bits
-// 20-29, with bit 31: read there, not from x) | bit 30: staging item (the target takes the value);
-// x: the packed position of the value to load (0 when none is needed); bit 29 of a staging item's word:
-// the entry is structural fill (kind[p] == kFill when kind is given) and is staged as zero, as is a
-// registered entry whose pent_pos carries kStageZero.  Every part spans at least
+// to the column-by-column refactor) are scheduled into joint rounds of kJointRounds * G items (the
+// kernel reads the operands of a joint round's kJointRounds rounds of G items before it writes any):
+// distinct targets, no target that is also read as a source, a target's updates in order and after
+// its source is final.  The entries of a part past the first kPartRegs per lane group start at zero
+// (the kernel clears their rows); those that are not structural fill (kind[p] == kFill when kind is
+// given) are staged by the first joint rounds with items x_o = x_o - value * x_m, m = pcap + 1 the
+// LDS row that holds -1 (ahead of every update of the part), so their values flow through the same
+// load stream as the L values.  A registered entry whose pent_pos carries kStageZero is staged as
+// zero.  Word of an item: target (bits 0-9) | source (bits 10-19) | LDS offset of an L value of the
+// previous part (bits 20-29, with bit 31: read there, not from x); x: the packed position of the
+// value to load (0 when none is needed).  Every part spans at least
 // kPartMinChunks chunks: the kernel loads a chunk's values kPartMinChunks chunks ahead, so the values
 // of a part are loaded no earlier than during the part before it, after the part two back was written.
 struct PartPlan {
@@ -227,10 +228,12 @@ struct PartPlan {
 
 int build_parts(const gx::LuPattern& lu, const LuSchedule& s, int G, int pcap, const std::vector<int>* kind,
                 PartPlan& out) {
-  const int n = lu.n, NB = gx::kChunkRounds;
+  const int n = lu.n, NB = gx::kChunkRounds, J = gx::kJointRounds * G;
+  static_assert(gx::kChunkRounds % gx::kJointRounds == 0, "a chunk holds whole joint rounds");
   out = PartPlan();
   // (10-bit offsets in the item words; a larger part falls back to the column-by-column kernel)
   if (s.max_col > pcap || pcap >= gx::kPartMaxEntries) return fail(GX_ERR_UNSUPPORTED, "a column does not fit a refactor part");
+  const int mone = pcap + 1;  // the LDS row holding -1
   std::vector<int> lev(n, 0);
   int nlev = 0;
   for (int k = 0; k < n; ++k) {
@@ -280,10 +283,10 @@ int build_parts(const gx::LuPattern& lu, const LuSchedule& s, int G, int pcap, c
     last.assign(Lp + 1, -1);
     sx.clear();
     sw.clear();
-    int R = 0;
+    int R = 0;  // joint rounds
     auto add_round = [&]() {
-      sx.insert(sx.end(), G, -1);
-      sw.insert(sw.end(), G, noop_w);
+      sx.insert(sx.end(), J, -1);
+      sw.insert(sw.end(), J, noop_w);
       if ((int)rt.size() <= R) {
         rt.emplace_back();
         rs.emplace_back();
@@ -292,15 +295,19 @@ int build_parts(const gx::LuPattern& lu, const LuSchedule& s, int G, int pcap, c
       rs[R].clear();
       ++R;
     };
-    // staging items: entries past the registered ones, G per round from round 0
+    // staging items: entries past the registered ones that are not structural fill (the kernel
+    // clears those rows), J per joint round from the first; x_o = 0 - value * (-1)
+    int staged = 0;
     for (int o = G * gx::kPartRegs; o < Lp; ++o) {
-      const int r = (o - G * gx::kPartRegs) / G, slot = (o - G * gx::kPartRegs) % G;
-      while (r >= R) add_round();
       const int ps = out.pent_pos[pbase + o];
-      const bool zero = (ps & gx::kStageZero) != 0;
-      sx[(size_t)r * G + slot] = zero ? 0 : ps;
-      sw[(size_t)r * G + slot] = o | (Lp << 10) | (1 << 30) | (zero ? (1 << 29) : 0);
+      if (ps & gx::kStageZero) continue;
+      const int r = staged / J, slot = staged % J;
+      ++staged;
+      while (r >= R) add_round();
+      sx[(size_t)r * J + slot] = ps;
+      sw[(size_t)r * J + slot] = o | (mone << 10);
       rt[r].push_back(o);
+      rs[r].push_back(mone);
       last[o] = r;
     }
     for (int k : cols) {
@@ -315,13 +322,13 @@ int build_parts(const gx::LuPattern& lu, const LuSchedule& s, int G, int pcap, c
           for (int r = std::max(last[tg], last[su]) + 1;; ++r) {
             while (r >= R) add_round();
             int slot = -1;
-            for (int gg = 0; gg < G && slot < 0; ++gg)
-              if (sx[(size_t)r * G + gg] == -1) slot = gg;
+            for (int gg = 0; gg < J && slot < 0; ++gg)
+              if (sx[(size_t)r * J + gg] == -1) slot = gg;
             if (slot < 0) continue;
             if (std::find(rt[r].begin(), rt[r].end(), tg) != rt[r].end()) continue;
             if (std::find(rs[r].begin(), rs[r].end(), tg) != rs[r].end()) continue;
-            sx[(size_t)r * G + slot] = prev ? 0 : p;
-            sw[(size_t)r * G + slot] =
+            sx[(size_t)r * J + slot] = prev ? 0 : p;
+            sw[(size_t)r * J + slot] =
                 tg | (su << 10) | (prev ? (int)(0x80000000u | ((unsigned)(base[j] + (p - lu.cp[j])) << 20)) : 0);
             rt[r].push_back(tg);
             rs[r].push_back(su);
@@ -331,6 +338,7 @@ int build_parts(const gx::LuPattern& lu, const LuSchedule& s, int G, int pcap, c
         }
       }
     }
+    R *= gx::kJointRounds;  // rounds of G items: joint round r is rounds kJointRounds r + i
     const int nch = std::max(gx::kPartMinChunks, (R + NB - 1) / NB);
     out.part_a[4 * pi] = pbase;
     out.part_a[4 * pi + 1] = Lp;
@@ -339,9 +347,10 @@ int build_parts(const gx::LuPattern& lu, const LuSchedule& s, int G, int pcap, c
     chunk += nch;
     for (int r = 0; r < nch * NB; ++r)
       for (int gg = 0; gg < G; ++gg) {
-        const bool real = r < R && sx[(size_t)r * G + gg] >= 0;
-        out.slots.push_back(real ? sx[(size_t)r * G + gg] : 0);
-        out.slots.push_back(real ? sw[(size_t)r * G + gg] : noop_w);
+        const size_t at = (size_t)r * G + gg;  // slot (r % kJointRounds) G + gg of joint round r / kJointRounds
+        const bool real = r < R && sx[at] >= 0;
+        out.slots.push_back(real ? sx[at] : 0);
+        out.slots.push_back(real ? sw[at] : noop_w);
       }
     if (out.slots.size() >= (size_t)0x7fffffff) return fail(GX_ERR_UNSUPPORTED, "refactor round schedule too large");
   }

@@ -134,6 +134,10 @@ template <typename T>
 __device__ __forceinline__ T zero_value() { return T(0); }
 template <>
 __device__ __forceinline__ zd zero_value<zd>() { return zd{0.0, 0.0}; }
+template <typename T>
+__device__ __forceinline__ T minus_one() { return T(-1); }
+template <>
+__device__ __forceinline__ zd minus_one<zd>() { return zd{-1.0, 0.0}; }
 __device__ __forceinline__ float shfl_xor_v(float v, int d) { return __shfl_xor(v, d); }
 __device__ __forceinline__ double shfl_xor_v(double v, int d) { return __shfl_xor(v, d); }
 __device__ __forceinline__ zd shfl_xor_v(zd v, int d) { return zd{__shfl_xor(v.re, d), __shfl_xor(v.im, d)}; }
@@ -1456,7 +1460,7 @@ __device__ __forceinline__ void lu_tile_parts(const DevTopo& t, const DevWork& w
   const int R = W + kRingGap + 1;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T* xcb = reinterpret_cast<T*>(smem_raw);
-  const int xstride = (cap + 1) * S;
+  const int xstride = (cap + 2) * S;  // rows: cap entries, the empty-slot row, the row holding -1
   T* ring = xcb + (kFactor ? 2 * (size_t)xstride : 0);
   // the descriptors of a part's entries past the registered ones ("tail"), two parts' worth, after
   // the solve ring (8-byte aligned)
@@ -1525,6 +1529,7 @@ __device__ __forceinline__ void lu_tile_parts(const DevTopo& t, const DevWork& w
     // part at least kPartMinChunks chunks).  The item positions of chunk gc are loaded kPartAhead
     // chunks before it runs, its values and words kPartMinChunks chunks before; three register sets
     // take turns by the chunk's index mod 3, so the loop copies no register a load is filling.
+    static_assert(NB % kJointRounds == 0, "a chunk holds whole joint rounds");
     static_assert(kPartMinChunks == 2 && kPartAhead == 4, "the register roles below assume these");
     int xs[3][NB], ws[3][NB];
     T vs[3][NB];
@@ -1546,23 +1551,31 @@ __device__ __forceinline__ void lu_tile_parts(const DevTopo& t, const DevWork& w
       for (int r = 0; r < NB; ++r) d[r] = LU(x[r]);
     };
     // chunk gc in register role E: the loads of the chunks ahead, then its rounds (rleft of them
-    // belong to the part; the rest are empty).  Item word: target | source << 10 | (LDS offset of a
-    // previous-part L value) << 20 | zero << 29 | staging << 30 | previous-part << 31.
+    // belong to the part; the rest are empty), kJointRounds at a time: no item of a joint round reads
+    // or writes another's target, so all their LDS reads go before their writes and the rounds wait
+    // for LDS once.  Item word: target | source << 10 | (LDS offset of a previous-part L value) << 20
+    // | previous-part << 31.
     auto step = [&](auto role, int gc, int rleft, T* xc, const T* xprev) __attribute__((always_inline)) {
       constexpr int E = decltype(role)::value, E1 = (E + 1) % 3, E2 = (E + 2) % 3;
       vals_issue(xs[E2], vs[E2]);                    // values of chunk gc + 2
       ints_issue(xsb, gc + kPartAhead, xs[E1]);      // positions of chunk gc + 4
       ints_issue(wsb, gc + kPartMinChunks, ws[E2]);  // words of chunk gc + 2
 #pragma unroll
-      for (int r = 0; r < NB; ++r) {
+      for (int r = 0; r < NB; r += kJointRounds) {
         if (r < rleft) {
-          const int wd = ws[E][r];
-          const int tg = wd & 0x3ff, su = (wd >> 10) & 0x3ff;
-          T lv = vs[E][r];
-          if (wd < 0) lv = xprev[((wd >> 20) & 0x3ff) * S + s];
-          const T xt = xc[tg * S + s], xu = xc[su * S + s];
-          const T st = (wd & (1 << 29)) ? zero_value<T>() : lv;  // (structural fill stages zero)
-          xc[tg * S + s] = (wd & (1 << 30)) ? st : xt - lv * xu;
+          int to[kJointRounds];
+          T nv[kJointRounds];
+#pragma unroll
+          for (int h = 0; h < kJointRounds; ++h) {
+            const int wd = ws[E][r + h];
+            const int tg = wd & 0x3ff, su = (wd >> 10) & 0x3ff;
+            T lv = vs[E][r + h];
+            if (wd < 0) lv = xprev[((wd >> 20) & 0x3ff) * S + s];
+            to[h] = tg * S + s;
+            nv[h] = xc[to[h]] - lv * xc[su * S + s];
+          }
+#pragma unroll
+          for (int h = 0; h < kJointRounds; ++h) xc[to[h]] = nv[h];
         }
       }
     };
@@ -1586,6 +1599,9 @@ __device__ __forceinline__ void lu_tile_parts(const DevTopo& t, const DevWork& w
     drain();
     int4 mk = uni4(vload4(t.part_a, 0));
     tail_issue(mk);
+    // the row after the empty-slot row holds -1 in both part buffers (staging items multiply by it)
+    xcb[(cap + 1) * S + s] = minus_one<T>();
+    xcb[xstride + (cap + 1) * S + s] = minus_one<T>();
     int role = 0;  // the running chunk's index mod 3
     for (int p = 0; p < np; ++p) {
       const int pb = mk.x, len = mk.y, ch0 = mk.z, nr = mk.w;
@@ -1602,12 +1618,14 @@ __device__ __forceinline__ void lu_tile_parts(const DevTopo& t, const DevWork& w
       tail_store(p & 1, len);
       memory_order_point();  // the descriptors in LDS before other lanes read them
       tail_issue(mx);
-      // stage the registered entries (the others arrive through the part's staging rounds)
+      // stage the registered entries; the others start at zero (their staging items add the values
+      // of those that are not structural fill)
 #pragma unroll
       for (int i = 0; i < MP; ++i) {
         const int o = g + G * i;
         if (o < len) xc[o * S + s] = (cpos[i] & kStageZero) ? zero_value<T>() : cval[i];
       }
+      for (int o = G * MP + g; o < len; o += G) xc[o * S + s] = zero_value<T>();
       // loads for the next parts
       val_issue_p(xpos, xval);
       {

@@ -22,6 +22,7 @@ P = 4                   # items per lane group and step (kBackItems)
 PART_REGS = 4           # refactor part entries per lane group staged through registers (kPartRegs)
 MIN_CHUNKS = 2          # chunks per part at least = chunks between a value's load and its use
 STAGE_ZERO = 1 << 30    # pent_pos flag of a structural-fill entry (kStageZero): staged as zero
+JOINT = 1               # rounds whose reads the kernel issues before their writes (kJointRounds)
 D = 4                   # steps between loading a step's values and running it (kBackAhead)
 POS = (1 << 29) - 1     # payload of the first word
 ENTRY, START, END, PAD = 0, 1, 2, 3
@@ -212,43 +213,49 @@ def _sequential_refactor(cp, ulen, row, a):
 
 def _emulate_parts(pa, pos, div, slots, a, cap, G):
     """The part loop of the kernel for one scenario.  The first G * kPartRegs entries of a part are
-    staged before its rounds; the others by its staging items.  The values of chunk c of part p are
-    loaded kPartMinChunks = 2 chunks ahead: after the write-back of part p - 2 (c < 2) or p - 1
-    (c >= 2); a position not written back by then still holds its Jacobian value.  Staging rows hold
-    NaN until staged."""
+    staged before its rounds, the others start at zero and take their values from staging items
+    (x_o -= value * x_m, row m = cap + 1 holding -1).  The rounds run JOINT at a time, every read
+    of the group before its writes.  The values of chunk c of part p are loaded kPartMinChunks = 2
+    chunks ahead: after the write-back of part p - 2 (c < 2) or p - 1 (c >= 2); a position not
+    written back by then still holds its Jacobian value.  The rows of the part buffers start as NaN."""
     mem = a.copy()
     owner = np.full(a.size, np.iinfo(np.int64).max)  # the part that wrote each position back
-    buf = [np.full(cap + 1, np.nan), np.full(cap + 1, np.nan)]
+    buf = [np.full(cap + 2, np.nan), np.full(cap + 2, np.nan)]
+    mone = cap + 1
+    for b in buf:
+        b[mone] = -1.0
     regs = G * PART_REGS
     for p, (pb, ln, ch0, nr) in enumerate(pa):
-        assert 0 < ln <= cap
+        assert 0 < ln <= cap and nr % JOINT == 0
         xc, xprev = buf[p & 1], buf[(p & 1) ^ 1]
-        xc[:] = np.nan
+        xc[:mone] = np.nan
         ent = np.arange(pb, pb + ln)
         zero = (pos[ent] & STAGE_ZERO) != 0
         epos = pos[ent] & ~STAGE_ZERO
         assert (owner[epos] == np.iinfo(np.int64).max).all()
         k = min(ln, regs)
         xc[:k] = np.where(zero[:k], 0.0, mem[epos[:k]])
+        xc[k:ln] = 0.0
         nch = max(MIN_CHUNKS, -(-nr // NB))
-        for r in range(nch * NB):
-            seen = p - 2 if r < MIN_CHUNKS * NB else p - 1
-            x = slots[ch0 * NB + r, :, 0].astype(np.int64)
-            wd = slots[ch0 * NB + r, :, 1].astype(np.int64) & 0xFFFFFFFF
-            tg, su, off = wd & 0x3FF, (wd >> 10) & 0x3FF, (wd >> 20) & 0x3FF
-            st, fl, zr = (wd >> 30) & 1, (wd >> 31) & 1, (wd >> 29) & 1
+        for r0 in range(0, nch * NB, JOINT):
+            seen = p - 2 if r0 < MIN_CHUNKS * NB else p - 1
+            x = slots[ch0 * NB + r0:ch0 * NB + r0 + JOINT, :, 0].astype(np.int64).ravel()
+            wd = slots[ch0 * NB + r0:ch0 * NB + r0 + JOINT, :, 1].astype(np.int64).ravel() & 0xFFFFFFFF
+            tg, su, off, fl = wd & 0x3FF, (wd >> 10) & 0x3FF, (wd >> 20) & 0x3FF, (wd >> 31) & 1
+            assert not ((wd >> 30) & 1).any()
             real = tg != ln  # empty slots target the row past the part
-            if r >= nr:
+            if r0 >= nr:
                 assert not real.any()
                 continue
-            assert (tg <= ln).all() and (su[real & (st == 0)] < ln).all()
+            assert (tg <= ln).all()
+            assert ((su[real] < ln) | (su[real] == mone)).all()
             assert np.unique(tg[real]).size == np.count_nonzero(real)
-            assert not np.isin(tg[real], su[real & (st == 0)]).any()
-            assert (tg[real & (st == 1)] >= regs).all()
+            assert not np.isin(tg[real], su[real]).any()
+            assert (tg[real & (su == mone)] >= regs).all()
             lv = np.where(owner[x] <= seen, mem[x], a[x])
             lv = np.where(fl == 1, xprev[np.where(fl == 1, off, 0)], lv)
-            xt, xu = xc[tg], xc[su]
-            xc[tg] = np.where(st == 1, np.where(zr == 1, 0.0, lv), xt - lv * xu)
+            nv = xc[tg] - lv * xc[su]  # every read of the joint round, then its writes
+            xc[tg] = nv
         d = div[ent]
         v = xc[:ln].copy()
         assert not np.isnan(v).any()
