@@ -189,19 +189,26 @@ struct DevTopo {
   int fwd_steps = 0;
 };
 
-// Back-substitution items {a, b} for a tile of TW scenarios (ring row stride S = TW + 1): a = kind << 29
+// Triangular-solve items {a, b} for a tile of TW scenarios (ring row stride S = TW + 1): a = kind << 29
 // | payload, and
-//   kBackEntry: acc -= U(k, c) * x_c; payload = position of U(k, c); b = c * TW (x_c from y), or
-//               1 << 31 | slot * S (x_c from the LDS ring row `slot`)
-//   kBackStart: acc = y_k; payload = slot * S of the ring row that will hold x_k; b = k * TW
-//   kBackEnd:   x_k = acc / U(k, k) into y and that ring row; payload = position of U(k, k); b = the
-//               zero row (at most one END per lane group and step)
-//   kBackPad:   nothing; payload 0, b = the zero row
+//   kBackEntry:   acc -= U(k, c) * x_c; payload = position of U(k, c); b = c * TW (x_c from y), or
+//                 1 << 31 | slot * S (x_c from the LDS ring row `slot`)
+//   kBackStart:   a piece of row k begins; payload = slot * S of the ring row its END writes; b = k * TW:
+//                 acc = y_k (the row's last piece), or the zero row: acc = 0 (a partial piece)
+//   kBackCombine: acc += the partial in ring row `slot`; payload 0; b = 1 << 31 | slot * S
+//   kBackEnd:     the piece ends: a partial into its ring row; the row's last piece x_k = acc / U(k, k)
+//                 (forward substitution: acc) into y and its ring row; payload = position of U(k, k)
+//                 (0 for a partial and in the forward substitution); b = the zero row (at most one
+//                 END per lane group and step)
+//   kBackPad:     nothing; payload 0, b = the zero row
 // The zero row is 1 << 31 | (R - 1) * S, the last of the R ring rows, which the kernel clears.
-// The kernel loads a step's values at the end of the step kBackAhead before it.
+// The kernel loads a step's values at the end of the step kBackAhead before it.  Rows with more than
+// kSolvePiece entries are cut into at most kSolvePieces pieces that run in different lane groups.
 constexpr int kBackItems = 4;
 constexpr int kBackAhead = 4;
-constexpr int kBackEntry = 0, kBackStart = 1, kBackEnd = 2, kBackPad = 3;
+constexpr int kBackEntry = 0, kBackStart = 1, kBackEnd = 2, kBackPad = 3, kBackCombine = 4;
+constexpr int kSolvePiece = 8;
+constexpr int kSolvePieces = 16;
 
 // The linear initial guess system of a topology (complex, m = number of PQ buses): recipes
 // for its packed L+U values and right-hand side.  Its LU pattern lives in a second DevTopo.

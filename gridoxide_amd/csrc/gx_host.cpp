@@ -9,6 +9,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <queue>
 #include <string>
 #include <tuple>
 #include <utility>
@@ -471,20 +472,27 @@ int build_uniform_rounds(const gx::LuPattern& lu, const LuSchedule& s, std::vect
 }
 
 // Device copy of a pattern's numeric schedule, pointed to by the schedule fields of a DevTopo.
-// Back substitution by levels for narrow tiles (DevTopo::back_items).  x_k = (y_k - sum_c U(k,c) x_c)
-// / U(k,k) with the subtractions in descending c, the order in which the column-oriented back
-// substitution applies them (src/klu_native/solve.rs:35-43).  The level of row k is one more than the
-// highest level among the c of its U row (0 if none), so rows of a level depend only on earlier
-// levels.  A level's rows go to the G lane groups, longest first, each to the least loaded group, as
-// START, one ENTRY per U entry (descending c) and END, a row moved to the next step where its END would
-// share a step with the group's previous END; a level fills whole steps of kBackItems items per group
-// (PAD after a group's last row), and the stream is padded to a multiple of 2 kBackAhead steps.
-// Rows are numbered in the order they end (step, then group); an END stores x_k into y and into LDS
-// ring row (number mod (ring_rows - 1)), which the row's START names; the last ring row holds zeros,
-// the x that END and PAD items multiply by.  An ENTRY of step s reads x_c from its ring row while
-// fewer than ring_rows - 1 rows ended after c before step s, else from y, which the kernel loads at the
-// end of step s - kBackAhead: c must have ended by then.  Returns false (no schedule) when the encoding's limits
-// are exceeded or a read would come too early.
+// Triangular solves by steps for narrow tiles (DevTopo::back_items / fwd_items).  Back substitution:
+// x_k = (y_k - sum_c U(k,c) x_c) / U(k,k); forward substitution (L unit lower): y_k = f_k - sum_j
+// L(k,j) y_j.  Row k's level is one more than the highest level among the rows its entries read (0
+// if none).  A row's entries, ordered by the level of the row they read (stable, so the reference
+// order within a level), are cut into pieces of at most kSolvePiece entries, at most kSolvePieces
+// pieces (contiguous runs, the last one the row's own): every piece but the last sums its entries
+// into a partial (START from zero, ENTRY..., END into a ring row only); the last piece starts from
+// y_k, takes its entries, adds the partials in piece order (COMBINE) and ends the row (END: x_k
+// into y and a ring row, divided by U(k,k) in the back substitution).  The pieces and their order
+// depend on the pattern only, so a row's arithmetic is the same for every tile width.
+// The pieces are list-scheduled onto the G lane groups, kBackItems items per group and step: at
+// each step, groups with free item slots take the pieces of the rows whose entries' rows have all
+// ended before that step, rows with the longest remaining path first; a group's pieces follow each
+// other in its item stream, a piece starting a step later where its END would share a step with the
+// group's previous END (one END per group and step), a row's COMBINEs no earlier than the step after
+// its partials ended.  Pieces are numbered in the order they end (step, then group); an END stores
+// into ring row (number mod (ring_rows - 1)), which the piece's START names; the last ring row holds
+// zeros.  An ENTRY of step s reads x_c from its ring row while fewer than ring_rows - 1 pieces ended
+// after c's row did before step s, else from y, which the kernel loads at the end of step
+// s - kBackAhead: c must have ended by then.  A COMBINE reads its partial from the ring.  Returns
+// false (no schedule) when the encoding's limits are exceeded or a read would come too early.
 bool build_solve_levels(const gx::LuPattern& lu, int G, int ring_rows, bool forward, std::vector<int>& items,
                         int& steps);
 
@@ -492,10 +500,8 @@ bool build_back_levels(const gx::LuPattern& lu, int G, int ring_rows, std::vecto
   return build_solve_levels(lu, G, ring_rows, false, items, steps);
 }
 
-// The forward substitution L y = f (L unit lower) by levels, in the same item format: row k's items
-// are START (acc = f_k), one ENTRY per L(k, j) in ascending j (the order in which the column-oriented
-// forward substitution, src/klu_native/solve.rs:23-30, applies them) and END (y_k = acc, no division;
-// payload 0).  A row's level is one more than the highest level among the j of its L row.
+// The forward substitution L y = f (L unit lower) in the same item format; END keeps the sum (no
+// division; payload 0).
 bool build_forward_levels(const gx::LuPattern& lu, int G, int ring_rows, std::vector<int>& items, int& steps) {
   return build_solve_levels(lu, G, ring_rows, true, items, steps);
 }
@@ -532,100 +538,184 @@ bool build_solve_levels(const gx::LuPattern& lu, int G, int ring_rows, bool forw
       std::reverse(rq.begin() + rptr[k], rq.begin() + rptr[k + 1]);
     }
   std::vector<int> lev(n, 0);
-  int nlev = 0;
   for (int i = 0; i < n; ++i) {
     const int k = forward ? i : n - 1 - i;  // every row after the rows it reads
     int l = 0;
     for (int e = rptr[k]; e < rptr[k + 1]; ++e) l = std::max(l, lev[rc[e]] + 1);
     lev[k] = l;
-    nlev = std::max(nlev, l + 1);
   }
-  std::vector<std::vector<int>> bucket(nlev);
-  for (int i = 0; i < n; ++i) {
-    const int k = forward ? i : n - 1 - i;
-    bucket[lev[k]].push_back(k);
+  // entries of each row by the level of the row they read (ord: indices into rc / rq)
+  std::vector<int> ord(rptr[n]);
+  for (int k = 0; k < n; ++k) {
+    for (int e = rptr[k]; e < rptr[k + 1]; ++e) ord[e] = e;
+    std::stable_sort(ord.begin() + rptr[k], ord.begin() + rptr[k + 1], [&](int a, int b) { return lev[rc[a]] < lev[rc[b]]; });
   }
-  // layout: per step, item slot, group: a row id (START / ENTRY e / END) or none
-  struct Cell {
-    int kind = gx::kBackPad, k = 0, e = 0;
+  // pieces: row k's pieces are pfirst[k] .. pfirst[k + 1] - 1, the last one its own; piece v holds
+  // ord[pe[v] .. pe[v + 1])
+  std::vector<int> pfirst(n + 1, 0), pe, prow;
+  for (int k = 0; k < n; ++k) {
+    const int L = rptr[k + 1] - rptr[k];
+    const int m = L <= gx::kSolvePiece ? 1 : std::min(gx::kSolvePieces, (L + gx::kSolvePiece - 1) / gx::kSolvePiece);
+    for (int i = 0, o = rptr[k]; i < m; ++i) {
+      pe.push_back(o);
+      prow.push_back(k);
+      o += L / m + (i < L % m ? 1 : 0);
+    }
+    pfirst[k + 1] = (int)pe.size();
+  }
+  const int nv = (int)pe.size();
+  pe.push_back(rptr[n]);
+  // priority: the longest path (in steps) from a row to the end of the solve
+  auto dur = [&](int k) {
+    int longest = 0;
+    for (int v = pfirst[k]; v < pfirst[k + 1]; ++v) longest = std::max(longest, pe[v + 1] - pe[v]);
+    return (longest + 2 + (pfirst[k + 1] - pfirst[k] - 1) + P - 1) / P;
   };
-  std::vector<Cell> cells;  // (step * P + i) * G + g
-  std::vector<int> end_step(n, -1), end_group(n, -1);
-  std::vector<std::vector<Cell>> gl(G);
-  std::vector<long long> load(G);
-  std::vector<size_t> last_end(G, 0);
-  int nst = 0;
-  for (int l = 0; l < nlev; ++l) {
-    std::vector<int> rows = bucket[l];
-    std::stable_sort(rows.begin(), rows.end(),
-                     [&](int a, int b) { return rptr[a + 1] - rptr[a] > rptr[b + 1] - rptr[b]; });
-    for (int g = 0; g < G; ++g) {
-      gl[g].clear();
-      load[g] = 0;
-      last_end[g] = 0;
-    }
-    for (int k : rows) {
-      const int g = (int)(std::min_element(load.begin(), load.end()) - load.begin());
-      // at most one END per group and step: the kernel divides once per step
-      const size_t len = (size_t)(rptr[k + 1] - rptr[k]) + 2;
-      if (!gl[g].empty() && (gl[g].size() + len - 1) / P == (last_end[g]) / P)
-        gl[g].resize((gl[g].size() + P - 1) / P * P);
-      gl[g].push_back({gx::kBackStart, k, 0});
-      for (int e = rptr[k]; e < rptr[k + 1]; ++e) gl[g].push_back({gx::kBackEntry, k, e});
-      gl[g].push_back({gx::kBackEnd, k, 0});
-      last_end[g] = gl[g].size() - 1;
-      load[g] = (long long)gl[g].size();
-    }
-    const long long M = *std::max_element(load.begin(), load.end());
-    const int ls = (int)((M + P - 1) / P);
-    cells.resize((size_t)(nst + ls) * P * G);
-    for (int g = 0; g < G; ++g)
-      for (size_t idx = 0; idx < gl[g].size(); ++idx) {
-        const int q = nst + (int)(idx / P), i = (int)(idx % P);
-        cells[((size_t)q * P + i) * G + g] = gl[g][idx];
-        if (gl[g][idx].kind == gx::kBackEnd) {
-          end_step[gl[g][idx].k] = q;
-          end_group[gl[g][idx].k] = g;
-        }
-      }
-    nst += ls;
+  std::vector<std::vector<int>> users(n);
+  for (int k = 0; k < n; ++k)
+    for (int e = rptr[k]; e < rptr[k + 1]; ++e) users[rc[e]].push_back(k);
+  std::vector<long long> bl(n, 0);
+  for (int i = n - 1; i >= 0; --i) {
+    const int k = forward ? i : n - 1 - i;
+    long long b = 0;
+    for (int u : users[k]) b = std::max(b, bl[u]);
+    bl[k] = b + dur(k);
   }
-  const int total = (nst + 2 * D - 1) / (2 * D) * (2 * D);
-  cells.resize((size_t)total * P * G);
-  // numbering in the order rows end; ended_before[s] = rows that ended in steps before s
-  std::vector<int> order(n), fin(n);
-  for (int k = 0; k < n; ++k) order[k] = k;
+  // list schedule.  Cells per group by item position; kind PAD where nothing runs.
+  struct Cell {
+    int kind = gx::kBackPad, v = 0, e = 0;
+  };
+  std::vector<std::vector<Cell>> gl(G);
+  std::vector<long long> pos(G, 0), last_end(G, -1);
+  std::vector<int> end_step(nv, -1), end_group(nv, -1), row_end(n, -1), ndep(n);
+  auto put = [&](int g, long long at, Cell c) {
+    if ((long long)gl[g].size() <= at) gl[g].resize((size_t)at + 1);
+    gl[g][(size_t)at] = c;
+  };
+  // places piece v on group g no earlier than position lo; COMBINEs (owner pieces) no earlier than
+  // step cmin; returns the END's step
+  auto place = [&](int v, int g, long long lo, long long cmin) {
+    const int k = prow[v], nent = pe[v + 1] - pe[v];
+    const bool own = v == pfirst[k + 1] - 1;
+    const int ncomb = own ? v - pfirst[k] : 0;
+    long long st = std::max(pos[g], lo);
+    long long cpos = 0, endp = 0;
+    for (;;) {
+      cpos = st + 1 + nent;  // first COMBINE (or the END)
+      if (ncomb > 0 && cpos < cmin * P) cpos = cmin * P;
+      endp = cpos + ncomb;
+      if (endp / P == last_end[g]) {
+        st = (last_end[g] + 1) * P;
+        continue;
+      }
+      break;
+    }
+    put(g, st, {gx::kBackStart, v, 0});
+    for (int i = 0; i < nent; ++i) put(g, st + 1 + i, {gx::kBackEntry, v, ord[pe[v] + i]});
+    for (int i = 0; i < ncomb; ++i) put(g, cpos + i, {gx::kBackCombine, v, pfirst[k] + i});
+    put(g, endp, {gx::kBackEnd, v, 0});
+    pos[g] = endp + 1;
+    last_end[g] = endp / P;
+    end_step[v] = (int)(endp / P);
+    end_group[v] = g;
+    return (int)(endp / P);
+  };
+  using Ent = std::pair<long long, int>;  // (-priority, row)
+  std::priority_queue<Ent, std::vector<Ent>, std::greater<Ent>> ready;
+  std::priority_queue<std::pair<int, int>, std::vector<std::pair<int, int>>, std::greater<std::pair<int, int>>> rel;
+  for (int k = 0; k < n; ++k) {
+    ndep[k] = rptr[k + 1] - rptr[k];
+    if (ndep[k] == 0) rel.push({0, k});
+  }
+  int done = 0;
+  long long s = 0;
+  while (done < n) {
+    while (!rel.empty() && rel.top().first <= s) {
+      const int k = rel.top().second;
+      rel.pop();
+      ready.push({-bl[k], k});
+    }
+    // the group whose stream is free earliest (ties: the lowest group)
+    auto least = [&]() {
+      int best = 0;
+      for (int g = 1; g < G; ++g)
+        if (std::max(pos[g], s * P) < std::max(pos[best], s * P)) best = g;
+      return best;
+    };
+    while (!ready.empty() && pos[least()] < (s + 1) * P) {
+      const int k = ready.top().second;
+      ready.pop();
+      int cmin = 0;
+      for (int v = pfirst[k]; v < pfirst[k + 1] - 1; ++v)  // partials, each where a stream is free first
+        cmin = std::max(cmin, place(v, least(), s * P, 0) + 1);
+      row_end[k] = place(pfirst[k + 1] - 1, least(), s * P, cmin);
+      ++done;
+      for (int u : users[k])
+        if (--ndep[u] == 0) {
+          int r = 0;
+          for (int e = rptr[u]; e < rptr[u + 1]; ++e) r = std::max(r, row_end[rc[e]] + 1);
+          rel.push({r, u});
+        }
+    }
+    ++s;
+    if (ready.empty() && !rel.empty()) {  // idle until the next row is released
+      bool idle = true;
+      for (int g = 0; g < G; ++g) idle = idle && pos[g] <= s * P;
+      if (idle) s = std::max<long long>(s, rel.top().first);
+    }
+    if (s >= (1LL << 26)) return false;
+  }
+  long long nst = 0;
+  for (int g = 0; g < G; ++g) nst = std::max<long long>(nst, ((long long)gl[g].size() + P - 1) / P);
+  const long long total = (nst + 2 * D - 1) / (2 * D) * (2 * D);
+  if (total * P * G * 2 >= (1LL << 31)) return false;
+  // numbering in the order pieces end; ended_before[s] = pieces that ended in steps before s
+  std::vector<int> order(nv), fin(nv);
+  for (int v = 0; v < nv; ++v) order[v] = v;
   std::sort(order.begin(), order.end(), [&](int a, int b) {
     return end_step[a] != end_step[b] ? end_step[a] < end_step[b] : end_group[a] < end_group[b];
   });
-  for (int i = 0; i < n; ++i) fin[order[i]] = i;
+  for (int i = 0; i < nv; ++i) fin[order[i]] = i;
   std::vector<int> ended_before(total + 1, 0);
-  for (int k = 0; k < n; ++k) ended_before[end_step[k] + 1]++;
-  for (int q = 0; q < total; ++q) ended_before[q + 1] += ended_before[q];
-  items.resize(2 * cells.size());
-  for (size_t x = 0; x < cells.size(); ++x) {
-    const Cell& c = cells[x];
-    const int s = (int)(x / ((size_t)P * G));
-    int a = gx::kBackPad << 29, b = zero_b;
-    if (c.kind == gx::kBackStart) {
-      a = (gx::kBackStart << 29) | (fin[c.k] % xrows) * S;
-      b = c.k * TW;
-    } else if (c.kind == gx::kBackEnd) {
-      a = (gx::kBackEnd << 29) | (forward ? 0 : lu.cp[c.k] + lu.ulen[c.k]);
-    } else if (c.kind == gx::kBackEntry) {
-      const int src = rc[c.e];
-      a = (gx::kBackEntry << 29) | rq[c.e];
-      if (ended_before[s] <= fin[src] + xrows - 1) {
-        b = (int)(0x80000000u | (unsigned)((fin[src] % xrows) * S));
-      } else {
-        if (end_step[src] > s - D) return false;  // the kernel loads it at the end of step s - D
-        b = src * TW;
+  for (int v = 0; v < nv; ++v) ended_before[end_step[v] + 1]++;
+  for (long long q = 0; q < total; ++q) ended_before[q + 1] += ended_before[q];
+  items.assign((size_t)(2 * total * P * G), 0);
+  for (long long q = 0; q < total; ++q)
+    for (int i = 0; i < P; ++i)
+      for (int g = 0; g < G; ++g) {
+        const size_t at = (size_t)(q * P + i);
+        const Cell c = at < gl[g].size() ? gl[g][at] : Cell();
+        const int sq = (int)q;
+        int a = gx::kBackPad << 29, b = zero_b;
+        const int k = prow[c.v];
+        if (c.kind == gx::kBackStart) {
+          const bool own = c.v == pfirst[k + 1] - 1;
+          a = (gx::kBackStart << 29) | (fin[c.v] % xrows) * S;
+          b = own ? k * TW : zero_b;
+        } else if (c.kind == gx::kBackEnd) {
+          const bool own = c.v == pfirst[k + 1] - 1;
+          a = (gx::kBackEnd << 29) | (forward || !own ? 0 : lu.cp[k] + lu.ulen[k]);
+        } else if (c.kind == gx::kBackEntry) {
+          const int src = rc[c.e], sv = pfirst[src + 1] - 1;  // the source row's own piece
+          if (end_step[sv] >= sq) return false;
+          a = (gx::kBackEntry << 29) | rq[c.e];
+          if (ended_before[sq] <= fin[sv] + xrows - 1) {
+            b = (int)(0x80000000u | (unsigned)((fin[sv] % xrows) * S));
+          } else {
+            if (end_step[sv] > sq - D) return false;  // the kernel loads it at the end of step s - D
+            b = src * TW;
+          }
+        } else if (c.kind == gx::kBackCombine) {
+          const int pv = c.e;  // the partial
+          if (end_step[pv] >= sq || ended_before[sq] > fin[pv] + xrows - 1) return false;
+          a = (int)((unsigned)gx::kBackCombine << 29);
+          b = (int)(0x80000000u | (unsigned)((fin[pv] % xrows) * S));
+        }
+        const size_t x = ((size_t)q * P + i) * G + g;
+        items[2 * x] = a;
+        items[2 * x + 1] = b;
       }
-    }
-    items[2 * x] = a;
-    items[2 * x + 1] = b;
-  }
-  steps = total;
+  steps = (int)total;
   return true;
 }
 

@@ -556,13 +556,14 @@ __device__ __forceinline__ int4 uni4(int4 v) {
 // columns, overflow gathers) so that the common path's waits stay exact.
 __device__ __forceinline__ void drain() { __builtin_amdgcn_s_waitcnt(0); }
 
-// A triangular solve by levels (item encoding at kBackItems) for one tile: every lane group runs its
-// rows of a level as one dependent chain in registers, the levels one after another.  Four register
+// A triangular solve by steps (item encoding at kBackItems) for one tile: every lane group runs its
+// pieces of rows one after another as a dependent chain in registers.  Four register
 // sets take turns: after a set's step has run, it loads the values of the step kBackAhead later
 // (their descriptors were loaded kBackAhead steps earlier still), so no load is waited for soon after
 // it was issued and no in-flight load is copied between registers.  x of rows that ended recently
-// comes from the LDS ring, older x from y.  kDivide: END divides by the pivot (back substitution);
-// else END keeps the sum (forward substitution with the unit L).
+// comes from the LDS ring, older x from y; partial sums of row pieces only pass through the ring.
+// kDivide: a row's END divides by the pivot (back substitution); else END keeps the sum (forward
+// substitution with the unit L).
 //   items: this lane group's first item; lub/pstb: the lane's L+U base and position stride (bytes);
 //   yl: the lane's y (offsets row * TW); rl: the lane's ring (offsets slot * S); R ring rows
 template <typename T, int TW, bool kDivide>
@@ -623,10 +624,12 @@ __device__ __forceinline__ void level_solve(const int2* __restrict__ bib, int g,
       const bool start = kind == (unsigned)kBackStart;
       xslot = start ? (a & kPayload) : xslot;
       xrow = start ? (unsigned)bb : xrow;
-      // ENTRY: acc - U(k, c) x_c; END and PAD read the zero row (acc unchanged); START: y_k
+      // ENTRY: acc - U(k, c) x_c; END and PAD read the zero row (acc unchanged); START: y_k or zero;
+      // COMBINE: acc + a partial
       const T xv = bb < 0 ? lx[i] : b.y[i];
       const T v = acc - b.u[i] * xv;
-      acc = start ? b.y[i] : v;
+      const T w = acc + xv;
+      acc = start ? xv : (kind == (unsigned)kBackCombine ? w : v);
       const bool end = kind == (unsigned)kBackEnd;  // (at most one per lane group and step)
       pend = end ? acc : pend;
       pend_u = end ? b.u[i] : pend_u;
@@ -634,10 +637,11 @@ __device__ __forceinline__ void level_solve(const int2* __restrict__ bib, int g,
       pend_s = end ? xslot : pend_s;
       ended = ended || end;
     }
-    if (ended) {  // the step's row end: x_k into y and the ring
-      const T xk = kDivide ? pend / pend_u : pend;
+    if (ended) {  // the step's piece end: a partial into the ring; x_k into y and the ring
+      const bool partial = (int)pend_y < 0;  // (its START named the zero row)
+      const T xk = kDivide && !partial ? pend / pend_u : pend;
       if (!finite_value(xk)) ok = false;
-      Y(pend_y) = xk;
+      if (!partial) Y(pend_y) = xk;
       rl[pend_s] = xk;
     }
     memory_order_point();  // the step's stores before the next loads

@@ -25,7 +25,7 @@ STAGE_ZERO = 1 << 30    # pent_pos flag of a structural-fill entry (kStageZero):
 JOINT = 1               # rounds whose reads the kernel issues before their writes (kJointRounds)
 D = 4                   # steps between loading a step's values and running it (kBackAhead)
 POS = (1 << 29) - 1     # payload of the first word
-ENTRY, START, END, PAD = 0, 1, 2, 3
+ENTRY, START, END, PAD, COMBINE = 0, 1, 2, 3, 4
 
 
 def _factored(net):
@@ -91,17 +91,64 @@ def _emulate(items, G, lu, y, ring_rows, divide=True):
                 if kind == START:
                     xslot[g] = payload[q, i, g]
                     xrow[g] = b
-                v = acc[g] - u[i, g] * (lx[i, g] if b < 0 else yv[i, g])
-                acc[g] = yv[i, g] if kind == START else v
+                xv = lx[i, g] if b < 0 else yv[i, g]
+                if kind == START:
+                    acc[g] = xv  # y_k, or zero (a partial piece starts from the zero row)
+                elif kind == COMBINE:
+                    assert b < 0 and not np.isnan(xv)  # a partial, from the ring
+                    acc[g] = acc[g] + xv
+                else:
+                    acc[g] = acc[g] - u[i, g] * xv
                 if kind == END:
                     assert g not in pend
                     pend[g] = (acc[g], u[i, g], xrow[g], xslot[g])
-        for g, (a, pv, row, sl) in pend.items():  # the step's row ends
+        for g, (a, pv, row, sl) in pend.items():  # the step's piece ends
+            if row < 0:  # a partial: the ring only
+                lds[sl] = a
+                continue
             xk = a / pv if divide else a
             mem[row // TW] = xk
             lds[sl] = xk
         loaded[q + D] = values(q + D)  # the set refilled after this step
     return mem
+
+
+def _check_pieces(items, G, n, n_entries):
+    """Every row starts from y_k once (its last piece) and ends once into y; every entry is used
+    once; each partial piece ends once and is combined once; one END per lane group and step."""
+    kinds = (items[..., 0].astype(np.int64) & 0xFFFFFFFF) >> 29
+    b = items[..., 1]
+    own = (kinds == START) & (b >= 0)
+    assert np.array_equal(np.sort(b[own] // (64 // G)), np.arange(n))
+    partial = np.count_nonzero((kinds == START) & (b < 0))
+    assert np.count_nonzero(kinds == END) == n + partial
+    assert np.count_nonzero(kinds == COMBINE) == partial
+    assert np.count_nonzero(kinds == ENTRY) == n_entries
+    assert (np.count_nonzero(kinds == END, axis=1) <= 1).all()
+
+
+@pytest.mark.parametrize("forward", [False, True])
+def test_solve_pieces_independent_of_tile(forward):
+    """A row's pieces and their order depend on the pattern only: the solves give the same bits for
+    every tile width (so a batch equals its scenarios solved one by one whatever tile each picks)."""
+    net = config_grid(2)
+    topo = _factored(net)
+    cp, ulen, row = topo.lu_pattern()
+    n = ulen.size
+    rng = np.random.default_rng(14)
+    lu = rng.uniform(-0.3, 0.3, cp[-1])
+    for k in range(n):
+        lu[cp[k] + ulen[k]] = rng.uniform(1.0, 2.0) * rng.choice([-1.0, 1.0])
+    y = rng.uniform(-1.0, 1.0, n)
+    got = []
+    for tile in (2, 4, 8, 16):
+        items, G = _schedule(topo, tile, 512, forward=forward)
+        got.append(_emulate(items, G, lu, y, 512, divide=not forward))
+        kinds = (items[..., 0].astype(np.int64) & 0xFFFFFFFF) >> 29
+        if tile == 4:
+            assert np.count_nonzero(kinds == COMBINE) > 0  # the workload grid has rows cut into pieces
+    for g in got[1:]:
+        assert np.array_equal(g, got[0])
 
 
 @pytest.mark.parametrize("grid,tile,ring_rows", [("c1", 4, 64), ("c1", 8, 512), ("c1", 16, 256), ("c2", 4, 512),
@@ -119,15 +166,9 @@ def test_back_levels_equal_column_oriented(grid, tile, ring_rows):
     y = rng.uniform(-1.0, 1.0, n)
     want = _column_oriented(cp, ulen, row, lu, y)
     got = _emulate(items, G, lu, y, ring_rows)
-    assert np.array_equal(got, want)
-    # every row ends exactly once, every U entry is used exactly once
-    kinds = (items[..., 0].astype(np.int64) & 0xFFFFFFFF) >> 29
-    starts = items[..., 1][kinds == START] // (64 // G)
-    assert np.array_equal(np.sort(starts), np.arange(n))
-    assert np.count_nonzero(kinds == END) == n
-    assert np.count_nonzero(kinds == ENTRY) == int(ulen.sum())
-    # at most one END per lane group and step (the kernel divides once per step)
-    assert (np.count_nonzero(kinds == END, axis=1) <= 1).all()
+    # (the sums of long rows are taken in pieces: the same terms, another rounding order)
+    np.testing.assert_allclose(got, want, rtol=1e-12, atol=1e-13)
+    _check_pieces(items, G, n, int(ulen.sum()))
     # on the workload grids, fewer steps than columns: the rows of a level run in parallel lane groups
     if grid != "g":
         assert items.shape[0] < n
@@ -162,11 +203,8 @@ def test_forward_levels_equal_column_oriented(grid, tile, ring_rows):
     f = rng.uniform(-1.0, 1.0, n)
     want = _forward_column_oriented(cp, ulen, row, lu, f)
     got = _emulate(items, G, lu, f, ring_rows, divide=False)
-    assert np.array_equal(got, want)
-    kinds = (items[..., 0].astype(np.int64) & 0xFFFFFFFF) >> 29
-    assert np.count_nonzero(kinds == END) == n
-    assert np.count_nonzero(kinds == ENTRY) == int((np.diff(cp) - ulen - 1).sum())
-    assert (np.count_nonzero(kinds == END, axis=1) <= 1).all()
+    np.testing.assert_allclose(got, want, rtol=1e-12, atol=1e-13)
+    _check_pieces(items, G, n, int((np.diff(cp) - ulen - 1).sum()))
 
 
 NB = 16  # update rounds per chunk (kChunkRounds)
