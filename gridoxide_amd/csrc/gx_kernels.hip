@@ -542,6 +542,11 @@ __device__ __forceinline__ int rl(int v, int lane) { return __builtin_amdgcn_rea
 // share its wait counter with the LDS operations, so every LDS wait of the column would also wait
 // for it.  The empty asm only moves the index into a vector register, which keeps the load a
 // vector load; uni4 takes the (uniform) value back into scalar registers where it is used.
+// A load the compiler is told is global memory (a generic pointer may otherwise become a flat
+// access, which also counts as an LDS access in the wait counters).
+__device__ __forceinline__ int global_int(const int* p) {
+  return *(const __attribute__((address_space(1))) int*)(p);
+}
 __device__ __forceinline__ int4 vload4(const int4* p, int i) {
   int j = i;
   asm("" : "+v"(j));
@@ -1511,9 +1516,15 @@ __device__ __forceinline__ void lu_tile_parts(const DevTopo& t, const DevWork& w
       }
     };
     // descriptor of entry o >= G MP of part p (pb: its first entry)
-    auto tail_desc = [&](int p, int pb, int o) -> int2 {
+    // (lds_only: the caller knows o is within the LDS copy; else a global read past it.  One pointer
+    // chosen between the two would make a flat access, whose wait also waits for every load in
+    // flight.)
+    auto tail_desc = [&](int p, int pb, int o, auto lds_only) -> int2 {
       const int tt = o - G * MP;
-      return tt < ts ? tdsc[(p & 1) * ts + tt] : make_int2(ppos[pb + o], pdiv[pb + o]);
+      if constexpr (decltype(lds_only)::value) return tdsc[(p & 1) * ts + tt];
+      int2 d = tdsc[(p & 1) * ts + (tt < ts ? tt : 0)];
+      if (tt >= ts) d = make_int2(global_int(ppos + pb + o), global_int(pdiv + pb + o));
+      return d;
     };
     T cval[MP], xval[MP];
     auto ent_issue = [&](const int4& m, int (&pp)[MP], int (&dd)[MP]) {
@@ -1677,13 +1688,17 @@ __device__ __forceinline__ void lu_tile_parts(const DevTopo& t, const DevWork& w
           LU(cpos[i] & ~kStageZero) = cval[i];
         }
       }
-      for (int b = G * MP; b < len; b += G * MT) {
+      // entries past the registered ones, MT per lane group and batch: a fixed number of batches
+      // (their descriptors sit in LDS), so the loads and stores of a part transition are the same
+      // on every path and the waits of the round stream stay counted; longer parts finish in a
+      // loop of their own, drained before and after
+      auto tail_batch = [&](int b, auto lds_only) __attribute__((always_inline)) {
         int pp[MT], dd[MT];
         T tv[MT];
 #pragma unroll
         for (int u = 0; u < MT; ++u) {
           const int o = b + G * u + g;
-          const int2 d = tail_desc(p, pb, o < len ? o : G * MP);
+          const int2 d = tail_desc(p, pb, o < len ? o : G * MP, lds_only);
           pp[u] = d.x & ~kStageZero;
           dd[u] = d.y;
         }
@@ -1703,6 +1718,15 @@ __device__ __forceinline__ void lu_tile_parts(const DevTopo& t, const DevWork& w
             LU(pp[u]) = tv[u];
           }
         }
+      };
+      // (the batches below cover at most kPartTail entries past the registered ones, all of them
+      // within the LDS copy of min(cap - G MP, kPartTail) descriptors)
+      constexpr int kTailBatches = kPartTail / (G * MT);
+      for (int b = G * MP; b < len && b < G * MP + kTailBatches * G * MT; b += G * MT) tail_batch(b, std::true_type{});
+      if (len > G * MP + kTailBatches * G * MT) {
+        drain();
+        for (int b = G * MP + kTailBatches * G * MT; b < len; b += G * MT) tail_batch(b, std::false_type{});
+        drain();
       }
       // the registered sets rotate
 #pragma unroll
