@@ -250,8 +250,22 @@ __global__ __launch_bounds__(256) void k_from_tiles(const double* __restrict__ s
 
 // Element-parallel kernels: grid (element blocks, tiles), 256 threads = 4 waves x G lane groups.
 template <int TW>
-__device__ __forceinline__ int elem_first() {
-  return (blockIdx.x * (blockDim.x / kLanes) + threadIdx.x / kLanes) * (kLanes / TW) + grp<TW>();
+__device__ __forceinline__ int elem_first(int bx = blockIdx.x) {
+  return (bx * (blockDim.x / kLanes) + threadIdx.x / kLanes) * (kLanes / TW) + grp<TW>();
+}
+// The workgroup's (x, y) in a grid whose rows go whole to one XCD, so the rows' shared inputs (a
+// tile's voltages, read for every neighbour of a bus) are fetched into that XCD's L2 once: the 8
+// XCDs take workgroups round-robin in linear order, XCD c running linear ids c, c + 8, ..., and is
+// given rows [c ny / 8, (c + 1) ny / 8), each row's columns in order.  A grid whose row count is
+// not a multiple of 8 keeps its own order.
+__device__ __forceinline__ void xcd_rows(int& x, int& y) {
+  const int nx = gridDim.x, ny = gridDim.y;
+  x = blockIdx.x;
+  y = blockIdx.y;
+  if (ny & 7) return;
+  const int l = blockIdx.y * nx + blockIdx.x, c = l & 7, k = l >> 3;
+  y = c * (ny >> 3) + k / nx;
+  x = k % nx;
 }
 template <int TW>
 __device__ __forceinline__ int elem_stride() {
@@ -292,7 +306,9 @@ __global__ __launch_bounds__(256) void k_init(DevTopo t, DevWork w) {
 
 template <typename T, int TW>
 __global__ __launch_bounds__(256) void k_mismatch(DevTopo t, DevWork w) {
-  const int tile = blockIdx.y, s = sub<TW>();
+  int bx, tile;
+  xcd_rows(bx, tile);
+  const int s = sub<TW>();
   if (!tile_running<TW>(w.status, tile)) return;
   const int sc = tile * TW + s;
   const size_t bb = at<TW>(tile, t.n_bus, 0) + s;
@@ -300,7 +316,7 @@ __global__ __launch_bounds__(256) void k_mismatch(DevTopo t, DevWork w) {
   T* __restrict__ y = reinterpret_cast<T*>(w.y);
   const Outage og = load_outage(t, w, sc);
   double mx = 0.0;
-  for (int i = elem_first<TW>(); i < t.n_bus; i += elem_stride<TW>()) {
+  for (int i = elem_first<TW>(bx); i < t.n_bus; i += elem_stride<TW>()) {
     double ir = 0.0, ii = 0.0;
     const int p2 = t.y_rowptr[i + 1];
     for (int p = t.y_rowptr[i]; p < p2; ++p) {
@@ -462,23 +478,25 @@ __global__ __launch_bounds__(256) void k_fill_tile(DevTopo t, DevWork w) {
   // a lane group spans a whole block when it fits a wavefront, else one tile (the wide tile)
   constexpr int B = lu_block(TW), V = B * TW, LW = V <= kLanes ? V : TW, GB = kLanes / LW;
   const int lane = lane_id(), grp_b = lane / LW, sub_b = lane % LW;
+  int bx, by;
+  xcd_rows(bx, by);
   // (the last block may hold tiles past n_tiles: they read the last tile's inputs, their values are
   // never used)
-  int tile = V <= kLanes ? blockIdx.y * B + sub_b / TW : blockIdx.y;
+  int tile = V <= kLanes ? by * B + sub_b / TW : by;
   tile = tile < w.n_tiles ? tile : w.n_tiles - 1;
   const int s = sub_b % TW;
   bool run = false;  // any scenario of the group's tiles still iterating (the LU kernel skips the others)
   for (int q = 0; q < LW; ++q) {
-    const int sq = blockIdx.y * LW + q;
+    const int sq = by * LW + q;
     run = run || (sq < w.n_tiles * TW && w.status[sq] == kRunning);
   }
   if (!run) return;
   const int sc = tile * TW + s;
   const size_t bb = at<TW>(tile, t.n_bus, 0) + s;
   const Outage og = load_outage(t, w, sc);
-  T* __restrict__ lu = reinterpret_cast<T*>(w.lu) + (V <= kLanes ? (size_t)blockIdx.y * t.nnz_lu * V + sub_b
+  T* __restrict__ lu = reinterpret_cast<T*>(w.lu) + (V <= kLanes ? (size_t)by * t.nnz_lu * V + sub_b
                                                                   : lu_at(0, t.nnz_lu, tile, TW) + s);
-  const int first = (blockIdx.x * (blockDim.x / kLanes) + (int)threadIdx.x / kLanes) * GB + grp_b;
+  const int first = (bx * (blockDim.x / kLanes) + (int)threadIdx.x / kLanes) * GB + grp_b;
   const int stride = gridDim.x * (blockDim.x / kLanes) * GB;
   for (int i = first; i < t.n_bus; i += stride) {
     const int e0 = t.fb_ptr[i], e1 = t.fb_ptr[i + 1];
