@@ -83,9 +83,7 @@ __device__ __forceinline__ bool tile_refines(const DevWork& w, int tile, int ite
 // after the other, each position the block's tiles side by side: entry p of lane s of tile t at
 // ((t / B * nnz + p) * B + t % B) * TW + s.  A wavefront's gathers stay inside its block's region
 // (nnz * B * TW values) instead of striding over all tiles' values, which keeps the address
-// translations a wavefront needs few; a block's tiles still share the cache lines of a position,
-// and the fill kernels' stores of a position stay B * TW values long.  The allocation rounds the
-// tile count up to a multiple of B.
+// translations a wavefront needs few.  The allocation rounds the tile count up to a multiple of B.
 __device__ __forceinline__ size_t lu_at(int p, int nnz, int tile, int TW) {
   const int B = lu_block(TW);
   return (((size_t)(tile / B) * nnz + p) * B + tile % B) * TW;
