@@ -46,7 +46,8 @@ HBM_PEAK_GBS = 8000.0
 
 # HBM bytes per k_lu launch measured with the FETCH_SIZE / WRITE_SIZE counters (copy-calibrated) for
 # a workload's default launch shape; the summary that holds the number also gives the commands.
-TRAFFIC_SUMMARIES = {"config2": "profiles/k_lu_parts_fp64_config2_2000bus_4096scen.md"}
+TRAFFIC_SUMMARIES = {"config2": "profiles/k_lu_parts_fp64_config2_2000bus_4096scen.md",
+                     "config3": "profiles/k_lu_parts_mixed_config3_10000bus_8192scen.md"}
 
 
 def measured_traffic(workload: str, tile: int):
