@@ -10,7 +10,8 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libgridox_hip.so")
+# GRIDOX_HIP_LIB names another build of the same library (the host-sanitizer build)
+LIB_PATH = os.environ.get("GRIDOX_HIP_LIB") or os.path.join(HERE, "libgridox_hip.so")
 
 GX_OK = 0
 GX_ERR_ARG = -1
