@@ -983,8 +983,11 @@ bool valid_ring(int rows) { return rows == 0 || (rows >= 64 && rows <= 4096 && (
 // 10,000-bus grid with 8,192 scenarios 4-scenario tiles (two wavefronts per SIMD, run one after the
 // other at the kernel's register count) beat 8-scenario tiles at FP64 (96 + 11 vs 137 + 8 ms) and
 // at FP32 (54 + 15 vs 76 + 5 ms).
+// Mixed precision with at least 1,024 tiles of 8 scenarios: pair tiles of 8 (two scenarios per
+// lane on the 4-scenario schedule, one wavefront per SIMD): 10,000-bus grid, 8,192 scenarios, LU
+// 37.0 against 39.4 ms per launch, fill 4.1 against 6.2 ms, batch 895 against 993 ms.
 int auto_tile(int n_scen, bool fp32) {
-  (void)fp32;
+  if (fp32 && (n_scen + 7) / 8 >= 1024) return 8;
   if ((n_scen + 1) / 2 <= 1024) return 2;
   for (int tw : {4, 8}) if ((n_scen + tw - 1) / tw <= 2048) return tw;
   return 16;
@@ -1675,7 +1678,8 @@ int ensure_warm(gx_topo& t) {
   return GX_OK;
 }
 
-int stage_batch(gx_topo& t, Workspace& ws, const BatchIn& in, int tile, gx::DevWork& w, HostResult* hres) {
+int stage_batch(gx_topo& t, Workspace& ws, const BatchIn& in, int tile, gx::DevWork& w, HostResult* hres,
+                bool pairs = false) {
   const int n_scen = in.n_scen, nb = t.n_bus, max_iter = in.max_iter;
   const size_t plane = (size_t)n_scen * nb * sizeof(double);
   hipStream_t st = in.st;
@@ -1795,11 +1799,13 @@ int stage_batch(gx_topo& t, Workspace& ws, const BatchIn& in, int tile, gx::DevW
       w.sing0 = ws.sing0.as<int>();
     }
   }
-  // the solve ring is sized for FP64 values: the mixed mode's FP64 re-solve uses the same lists
-  const int rows = pick_ring_rows(lu_cap(t.sched.max_col, tile), tile, sizeof(double), t.ring_req);
+  // the solve ring is sized for FP64 values: the mixed mode's FP64 re-solve uses the same lists.
+  // Pair tiles run the schedules of tiles half as wide (two scenarios per lane).
+  const int lt = pairs ? tile / 2 : tile;
+  const int rows = pick_ring_rows(lu_cap(t.sched.max_col, lt), lt, (pairs ? 2 : 1) * sizeof(double), t.ring_req);
   if (rows < 0)
     return fail(GX_ERR_UNSUPPORTED, "longest L+U column exceeds the LDS staging limit of the device refactor");
-  if ((rc = upload_topo(t, gx::ring_window_of(rows, tile), tile))) return rc;
+  if ((rc = upload_topo(t, gx::ring_window_of(rows, lt), lt))) return rc;
   // FP32 values for the mixed mode (its FP64 re-solve of fallbacks grows the buffer first)
   if ((rc = ws.lu.reserve(lu_lanes(lanes, tile) * (size_t)t.dt.nnz_lu * (in.fp32 ? 4 : 8)))) return rc;
   w.lu = ws.lu.p;
@@ -2293,10 +2299,12 @@ int gx_batch_solve_outages(gx_topo* t, int n_scen, const double* p_spec, const d
   in.outage = any_outage ? &h_outage : nullptr;
   hipStream_t st = in.st;
   const int tile = t->tile_req ? t->tile_req : auto_tile(n_scen, in.fp32);
+  // mixed precision at 8 scenarios per tile: pair tiles (4-lane groups, two scenarios per lane)
+  const bool pairs = in.fp32 && tile == 8;
   gx::DevWork w;
   Workspace& ws = t->ws;
   HostResult hres;
-  if ((rc = stage_batch(*t, ws, in, tile, w, &hres))) return rc;
+  if ((rc = stage_batch(*t, ws, in, tile, w, &hres, pairs))) return rc;
   if (hres.used) return host_results(*t, in, hres, vm, va, status, iters, history);
   // optional per-launch timing of each kernel class on this stream
   auto timed = [&](int cls, auto&& launch) -> int {
@@ -2534,7 +2542,8 @@ int gx_sys_factor_and_solve_values_noalloc(gx_sys* s, const double* vals, const 
   if (need > gx::kScratchBytes)
     return fail(GX_ERR_CAPACITY, "system needs " + std::to_string(need) + " bytes of device scratch; the allocation-free "
                                  "solve holds " + std::to_string(gx::kScratchBytes) +
-                                 " (64 B per L+U entry + 32 B per row): use gx_sys_factor_and_solve_values");
+                                 " (" + std::to_string(lu_lanes(kSysTile, kSysTile) * sizeof(double)) +
+                                 " B per L+U entry + 32 B per row): use gx_sys_factor_and_solve_values");
   void* base = nullptr;
   GX_HIP(gx::scratch_base(&base));
   char* b = static_cast<char*>(base);

@@ -65,6 +65,14 @@ template <int TW>
 __device__ __forceinline__ bool tile_running(const int* status, int tile) {
   return __any(status[tile * TW + sub<TW>()] == kRunning);
 }
+// (V scenarios per lane: the tile's TW V scenarios)
+template <int TW, int V>
+__device__ __forceinline__ bool tile_running_v(const int* status, int tile) {
+  bool r = false;
+#pragma unroll
+  for (int e = 0; e < V; ++e) r = r || status[(tile * TW + sub<TW>()) * V + e] == kRunning;
+  return __any(r);
+}
 
 // Whether scenario sc refines at this step (kRefineBelow): the step's mismatch is in the history.
 __device__ __forceinline__ bool sc_refines(const DevWork& w, int sc, int iter) {
@@ -118,12 +126,57 @@ __device__ __forceinline__ zd operator/(zd a, zd b) {
   const double ns = b.re * b.re + b.im * b.im;
   return {(a.re * b.re + a.im * b.im) / ns, (a.im * b.re - a.re * b.im) / ns};
 }
+// Two scenarios in one lane (pair tiles: a lane group of TW lanes holds 2 TW scenarios, a
+// position's 2 TW values still contiguous), element by element.  Each load, store, LDS access and
+// index computation of the LU kernel then serves two scenarios.
+template <typename S>
+struct __attribute__((aligned(2 * sizeof(S)))) pair2 {
+  S a, b;
+};
+using f2 = pair2<float>;
+using d2 = pair2<double>;
+template <typename S>
+__device__ __forceinline__ pair2<S> operator-(pair2<S> x, pair2<S> y) { return {x.a - y.a, x.b - y.b}; }
+template <typename S>
+__device__ __forceinline__ pair2<S> operator+(pair2<S> x, pair2<S> y) { return {x.a + y.a, x.b + y.b}; }
+template <typename S>
+__device__ __forceinline__ pair2<S> operator*(pair2<S> x, pair2<S> y) { return {x.a * y.a, x.b * y.b}; }
+template <typename S>
+__device__ __forceinline__ pair2<S> operator/(pair2<S> x, pair2<S> y) { return {x.a / y.a, x.b / y.b}; }
+// scenarios per lane of an LU element type
+template <typename T>
+struct lane_pack {
+  static constexpr int V = 1;
+};
+template <typename S>
+struct lane_pack<pair2<S>> {
+  static constexpr int V = 2;
+};
 __device__ __forceinline__ bool is_zero(float v) { return v == 0.0f; }
 __device__ __forceinline__ bool is_zero(double v) { return v == 0.0; }
 __device__ __forceinline__ bool is_zero(zd v) { return v.re == 0.0 && v.im == 0.0; }
 __device__ __forceinline__ bool finite_value(float v) { return isfinite(v); }
 __device__ __forceinline__ bool finite_value(double v) { return isfinite(v); }
 __device__ __forceinline__ bool finite_value(zd v) { return isfinite(v.re) && isfinite(v.im); }
+// c - a b rounded once (a fused multiply-add), whatever the element type: the refactor's updates
+// and the substitutions' multiply-subtracts give the same bits in every tile shape, pair tiles
+// included, instead of depending on where the compiler contracts a product into a sum
+__device__ __forceinline__ float fnma(float a, float b, float c) { return __builtin_fmaf(-a, b, c); }
+__device__ __forceinline__ double fnma(double a, double b, double c) { return __builtin_fma(-a, b, c); }
+__device__ __forceinline__ zd fnma(zd a, zd b, zd c) { return c - a * b; }
+template <typename S>
+__device__ __forceinline__ pair2<S> fnma(pair2<S> a, pair2<S> b, pair2<S> c) {
+  return {fnma(a.a, b.a, c.a), fnma(a.b, b.b, c.b)};
+}
+// per-scenario flags of a lane's values (bit e: its scenario e): a zero pivot, a non-finite value
+template <typename T>
+__device__ __forceinline__ int zero_mask(T v) { return is_zero(v) ? 1 : 0; }
+template <typename S>
+__device__ __forceinline__ int zero_mask(pair2<S> v) { return (v.a == S(0) ? 1 : 0) | (v.b == S(0) ? 2 : 0); }
+template <typename T>
+__device__ __forceinline__ int nonfinite_mask(T v) { return finite_value(v) ? 0 : 1; }
+template <typename S>
+__device__ __forceinline__ int nonfinite_mask(pair2<S> v) { return (isfinite(v.a) ? 0 : 1) | (isfinite(v.b) ? 0 : 2); }
 // Keeps the compiler from moving memory accesses across this point.  Lanes of a wavefront that pass
 // values through memory rely on the program order of the accesses; without it the compiler may, for
 // example, sink a load into the branch that uses it, past another lane's store.
@@ -132,10 +185,18 @@ template <typename T>
 __device__ __forceinline__ T zero_value() { return T(0); }
 template <>
 __device__ __forceinline__ zd zero_value<zd>() { return zd{0.0, 0.0}; }
+template <>
+__device__ __forceinline__ f2 zero_value<f2>() { return f2{0.0f, 0.0f}; }
+template <>
+__device__ __forceinline__ d2 zero_value<d2>() { return d2{0.0, 0.0}; }
 template <typename T>
 __device__ __forceinline__ T minus_one() { return T(-1); }
 template <>
 __device__ __forceinline__ zd minus_one<zd>() { return zd{-1.0, 0.0}; }
+template <>
+__device__ __forceinline__ f2 minus_one<f2>() { return f2{-1.0f, -1.0f}; }
+template <>
+__device__ __forceinline__ d2 minus_one<d2>() { return d2{-1.0, -1.0}; }
 __device__ __forceinline__ float shfl_xor_v(float v, int d) { return __shfl_xor(v, d); }
 __device__ __forceinline__ double shfl_xor_v(double v, int d) { return __shfl_xor(v, d); }
 __device__ __forceinline__ zd shfl_xor_v(zd v, int d) { return zd{__shfl_xor(v.re, d), __shfl_xor(v.im, d)}; }
@@ -147,6 +208,13 @@ __device__ __forceinline__ zd shfl_v(zd v, int l) { return zd{__shfl(v.re, l), _
 template <int TW, typename T>
 __device__ __forceinline__ T group_sum(T v) {
   for (int d = TW; d < kLanes; d <<= 1) v = v + shfl_xor_v(v, d);
+  return v;
+}
+
+// OR of a per-scenario flag mask over the lane groups (lanes with the same scenarios)
+template <int TW>
+__device__ __forceinline__ int any_groups(int v) {
+  for (int d = TW; d < kLanes; d <<= 1) v |= __shfl_xor(v, d);
   return v;
 }
 
@@ -586,11 +654,12 @@ __device__ __forceinline__ void drain() { __builtin_amdgcn_s_waitcnt(0); }
 // kDivide: a row's END divides by the pivot (back substitution); else END keeps the sum (forward
 // substitution with the unit L).
 //   items: this lane group's first item; lub/pstb: the lane's L+U base and position stride (bytes);
-//   yl: the lane's y (offsets row * TW); rl: the lane's ring (offsets slot * S); R ring rows
+//   yl: the lane's y (offsets row * TW); rl: the lane's ring (offsets slot * S); R ring rows;
+//   bad: the lane's scenarios with a non-finite solution are flagged (nonfinite_mask)
 template <typename T, int TW, bool kDivide>
 __device__ __forceinline__ void level_solve(const int2* __restrict__ bib, int g, int nst, int R, int nl,
                                             const char* __restrict__ lbase, unsigned pstb, char* __restrict__ ybase,
-                                            unsigned loff, T* rl, bool& ok) {
+                                            unsigned loff, T* rl, int& bad) {
   constexpr int G = kLanes / TW;
   constexpr int S = TW + 1;
   constexpr int P = kBackItems, D = kBackAhead;
@@ -648,7 +717,7 @@ __device__ __forceinline__ void level_solve(const int2* __restrict__ bib, int g,
       // ENTRY: acc - U(k, c) x_c; END and PAD read the zero row (acc unchanged); START: y_k or zero;
       // COMBINE: acc + a partial
       const T xv = bb < 0 ? lx[i] : b.y[i];
-      const T v = acc - b.u[i] * xv;
+      const T v = fnma(b.u[i], xv, acc);
       const T w = acc + xv;
       acc = start ? xv : (kind == (unsigned)kBackCombine ? w : v);
       const bool end = kind == (unsigned)kBackEnd;  // (at most one per lane group and step)
@@ -661,7 +730,7 @@ __device__ __forceinline__ void level_solve(const int2* __restrict__ bib, int g,
     if (ended) {  // the step's piece end: a partial into the ring; x_k into y and the ring
       const bool partial = (int)pend_y < 0;  // (its START named the zero row)
       const T xk = kDivide && !partial ? pend / pend_u : pend;
-      if (!finite_value(xk)) ok = false;
+      bad |= nonfinite_mask(xk);
       if (!partial) Y(pend_y) = xk;
       rl[pend_s] = xk;
     }
@@ -1020,8 +1089,10 @@ __device__ __forceinline__ void lu_tile(const DevTopo& t, const DevWork& w, int 
 
   // ------------------------------------------------ back substitution
   if (!overflow && t.back_items) {
+    int bad = 0;
     level_solve<T, TW, true>(t.back_items, g, t.back_steps, R, nl, lbase, pstb, reinterpret_cast<char*>(yb),
-                             (unsigned)(s * sizeof(T)), ring + s, ok);
+                             (unsigned)(s * sizeof(T)), ring + s, bad);
+    if (bad) ok = false;
   } else if (!overflow) {
     strided(n - 1 - W > 0 ? n - 1 - W : 0, g, n, [&](int r) { ring[slot(r)] = Y(r); });
     // column metadata three columns ahead; the U entries, the pivot, the entering
@@ -1491,17 +1562,18 @@ __device__ __forceinline__ void lu_tile_parts(const DevTopo& t, const DevWork& w
   // the solve ring (8-byte aligned)
   int2* tdsc = reinterpret_cast<int2*>(smem_raw + ((((kFactor ? 2 * (size_t)xstride : 0) + (size_t)(t.window + kRingGap + 1) * (TW + 1)) *
                                                     sizeof(T) + 7) / 8 * 8));
+  constexpr int V = lane_pack<T>::V;  // scenarios per lane (2 in pair tiles)
   const int g = grp<TW>(), s = sub<TW>();
-  if (!tile_running<TW>(w.status, tile)) return;
+  if (!tile_running_v<TW, V>(w.status, tile)) return;
   if (!kFactor && !w.swept[tile]) return;  // (the mixed correction solve)
-  const int sc = tile * TW + s;
+  const int scb = (tile * TW + s) * V;     // the lane's first scenario
   const int nl = t.nnz_lu;
   char* __restrict__ lub = reinterpret_cast<char*>(reinterpret_cast<T*>(w.lu) + lu_at(0, t.nnz_lu, tile, TW) + s);
   const char* __restrict__ lbase = reinterpret_cast<const char*>(reinterpret_cast<T*>(w.lu) + lu_at(0, t.nnz_lu, tile, TW));
   T* __restrict__ yb = reinterpret_cast<T*>(w.y) + (size_t)tile * t.n * TW;
   const unsigned pstb = (unsigned)(lu_block(TW) * TW * sizeof(T));
   auto LU = [&](int p) -> T& { return *reinterpret_cast<T*>(lub + (size_t)(unsigned)p * pstb); };
-  bool ok = true;
+  int bad = 0;  // the lane's scenarios with a zero pivot or a non-finite solution (bit per scenario)
   bool overflow = false;
   auto cl = [](int v, int hi) { return v < 0 ? 0 : (v > hi ? hi : v); };
   if (kFactor) {
@@ -1603,7 +1675,7 @@ __device__ __forceinline__ void lu_tile_parts(const DevTopo& t, const DevWork& w
             T lv = vs[E][r + h];
             if (wd < 0) lv = xprev[((wd >> 20) & 0x3ff) * S + s];
             to[h] = tg * S + s;
-            nv[h] = xc[to[h]] - lv * xc[su * S + s];
+            nv[h] = fnma(lv, xc[su * S + s], xc[to[h]]);
           }
 #pragma unroll
           for (int h = 0; h < kJointRounds; ++h) xc[to[h]] = nv[h];
@@ -1691,7 +1763,7 @@ __device__ __forceinline__ void lu_tile_parts(const DevTopo& t, const DevWork& w
           T v = xc[o * S + s];
           const int d = cdiv[i];
           if (d >= 0) v = v / xc[d * S + s];
-          if (d == -2 && is_zero(v)) ok = false;
+          if (d == -2) bad |= zero_mask(v);
           cval[i] = v;
         }
       }
@@ -1723,7 +1795,7 @@ __device__ __forceinline__ void lu_tile_parts(const DevTopo& t, const DevWork& w
           const int o = b + G * u + g;
           tv[u] = xc[(o < len ? o : 0) * S + s];
           if (dd[u] >= 0) tv[u] = tv[u] / xc[dd[u] * S + s];
-          if (o < len && dd[u] == -2 && is_zero(tv[u])) ok = false;
+          if (o < len && dd[u] == -2) bad |= zero_mask(tv[u]);
         }
         memory_order_point();
 #pragma unroll
@@ -1760,16 +1832,22 @@ __device__ __forceinline__ void lu_tile_parts(const DevTopo& t, const DevWork& w
   if (!overflow) {
     memory_order_point();
     level_solve<T, TW, false>(t.fwd_items, g, t.fwd_steps, R, nl, lbase, pstb, reinterpret_cast<char*>(yb),
-                             (unsigned)(s * sizeof(T)), ring + s, ok);
+                             (unsigned)(s * sizeof(T)), ring + s, bad);
     memory_order_point();
     level_solve<T, TW, true>(t.back_items, g, t.back_steps, R, nl, lbase, pstb, reinterpret_cast<char*>(yb),
-                             (unsigned)(s * sizeof(T)), ring + s, ok);
+                             (unsigned)(s * sizeof(T)), ring + s, bad);
   }
   // a zero pivot seen by any group of the scenario, or a non-finite solution, makes it Singular
-  ok = all_groups<TW>(ok && !overflow);
-  if (g == 0 && !ok && w.status[sc] == kRunning && (kFactor || sc_refines(w, sc, iter))) {
-    w.status[sc] = kSingular;
-    w.iters[sc] = iter + 1;
+  bad = any_groups<TW>(overflow ? (1 << V) - 1 : bad);
+  if (g == 0) {
+#pragma unroll
+    for (int e = 0; e < V; ++e) {
+      const int sc = scb + e;
+      if ((bad >> e & 1) && w.status[sc] == kRunning && (kFactor || sc_refines(w, sc, iter))) {
+        w.status[sc] = kSingular;
+        w.iters[sc] = iter + 1;
+      }
+    }
   }
 }
 
@@ -2173,6 +2251,22 @@ template <typename T, int TW>
 void run_lu(const DevTopo& t, const DevWork& w, bool factor, int iter, hipStream_t stream) {
   static size_t granted_f = 65536, granted_s = 65536, granted_pf = 65536, granted_ps = 65536;
   const int cap = t.max_col < 1 ? 1 : t.max_col;
+  if constexpr (TW == 8 && (std::is_same<T, float>::value || std::is_same<T, double>::value)) {
+    if (t.part_a && t.groups == kLanes / 4) {
+      // pair tiles: the schedule of 4-lane groups, two scenarios per lane
+      using P = pair2<T>;
+      static size_t granted_qf = 65536, granted_qs = 65536;
+      const size_t shmem = lu_bytes<P>(t, 4, factor);
+      if (factor) {
+        allow_lds(k_lu_parts<P, 4, true>, shmem, granted_qf);
+        hipLaunchKernelGGL((k_lu_parts<P, 4, true>), dim3(w.n_tiles), dim3(64), shmem, stream, t, w, cap, iter);
+      } else {
+        allow_lds(k_lu_parts<P, 4, false>, shmem, granted_qs);
+        hipLaunchKernelGGL((k_lu_parts<P, 4, false>), dim3(w.n_tiles), dim3(64), shmem, stream, t, w, cap, iter);
+      }
+      return;
+    }
+  }
   const size_t shmem = lu_bytes<T>(t, TW, factor);
   if constexpr (TW != kWideTile) {
     if (t.part_a) {

@@ -244,7 +244,7 @@ void gx_sys_destroy(gx_sys* s);
  * then unspecified).  It allocates nothing: the system's read-only tables are uploaded by
  * gx_sys_create (when a GPU is present; else GX_ERR_STATE), and the L+U values, right-hand side and
  * status live in one static 64 MiB device array per GPU, so a system may hold at most about
- * 1,000,000 L+U entries (64 B each plus 32 B per row; GX_ERR_CAPACITY beyond).  Every call on a GPU
+ * 2,000,000 L+U entries (32 B each plus 32 B per row; GX_ERR_CAPACITY beyond).  Every call on a GPU
  * shares that array: calls that overlap on one GPU (other threads or streams) race. */
 int gx_sys_factor_and_solve_values_noalloc(gx_sys* s, const double* vals, const double* rhs, double* x,
                                            void* stream);

@@ -98,7 +98,7 @@ def test_three_bus_linear_start_matches_doc_step(three_bus, known, oracle_mod):
 # launch shapes: (scenarios per tile, solve ring rows); 0 = automatic (2-scenario tiles at these
 # batch sizes).  A 64-row ring gathers every triangular-solve entry that reaches farther than 31 rows.
 SHAPES_C1 = [(0, 0), (4, 0), (8, 0), (16, 64), (64, 0), (64, 64)]
-SHAPES_C2 = [(0, 0), (2, 64), (16, 0), (4, 64), (64, 0)]
+SHAPES_C2 = [(0, 0), (2, 64), (16, 0), (4, 64), (8, 0), (64, 0)]
 
 
 @pytest.mark.parametrize("shape", SHAPES_C1)
@@ -368,12 +368,15 @@ def test_linear_initial_guess_batch_vs_oracle(oracle_mod, mode):
     _compare(gpu, ref, mixed=(mode == Mode.MIXED))
 
 
-def test_mixed_fallback_reproduces_fp64_for_failing_scenarios():
+@pytest.mark.parametrize("tile", [0, 8])
+def test_mixed_fallback_reproduces_fp64_for_failing_scenarios(tile):
     """A scenario whose FP32 factorization fails is re-solved with the FP64 path: its reported
     result is then exactly the FP64 mode's, and the others keep their mixed-precision results.
     Scenario 3 starts with |V| = 1e20 at one PQ bus: its Jacobian entries (~1e41) overflow FP32,
-    so the FP32 factor is non-finite (Singular), while FP64 represents them."""
-    _say("case: config-1 grid, 16 scenarios, scenario 3 overflows FP32, mixed vs FP64")
+    so the FP32 factor is non-finite (Singular), while FP64 represents them.  At 8 scenarios per
+    tile the mixed mode runs pair tiles (and its FP64 re-solve pairs of doubles), the FP64 mode
+    8-lane groups: the same bits."""
+    _say(f"case: config-1 grid, 16 scenarios, scenario 3 overflows FP32, mixed vs FP64, tile {tile}")
     net = config_grid(1)
     p, q = scenario_specs(net, 16, seed=13)
     vm0 = np.repeat(net.vm[None], 16, 0)
@@ -381,6 +384,7 @@ def test_mixed_fallback_reproduces_fp64_for_failing_scenarios():
     bus = int(np.nonzero(net.bus_type == 2)[0][0])
     vm0[3, bus] = 1e20
     topo = Topology.from_network(net)
+    topo.set_launch(tile, 0)
     mixed = topo.batch_solve(p, q, vm0, va0, tol=MIS_TOL, max_iter=20, mode=Mode.MIXED)
     assert topo.fallbacks() >= 1
     fp64 = topo.batch_solve(p, q, vm0, va0, tol=MIS_TOL, max_iter=20, mode=Mode.FP64)
@@ -389,6 +393,29 @@ def test_mixed_fallback_reproduces_fp64_for_failing_scenarios():
     ok = [s for s in range(16) if s != 3]
     assert np.all(mixed[2][ok] == 0)
     assert np.abs(mixed[0][ok] - fp64[0][ok]).max() <= V_TOL
+
+
+def test_pair_tiles_equal_four_scenario_tiles():
+    """Mixed precision at 8 scenarios per tile runs pair tiles: 4-lane groups with two scenarios
+    per lane, on the schedule of 4-scenario tiles.  Every scenario's arithmetic is the same, so the
+    results equal the 4-scenario tiles' bit for bit, including a scenario whose FP32 factor is
+    singular (its lane partner is unaffected) and the padding of a last, partial tile."""
+    _say("case: config-2 grid (2000 buses), 70 scenarios, mixed, tile 8 (pairs) vs tile 4")
+    net = config_grid(2)
+    p, q = scenario_specs(net, 70, seed=31)
+    vm0 = np.repeat(net.vm[None], 70, 0)
+    va0 = np.repeat(net.va[None], 70, 0)
+    vm0[5, int(np.nonzero(net.bus_type == 2)[0][0])] = np.nan
+    got = {}
+    for tile in (8, 4):
+        topo = Topology.from_network(net)
+        topo.set_launch(tile, 0)
+        got[tile] = topo.batch_solve(p, q, vm0, va0, tol=MIS_TOL, max_iter=20, mode=Mode.MIXED)
+    for a, b in zip(got[8][:4], got[4][:4]):
+        assert np.array_equal(a, b, equal_nan=True)
+    assert got[8][2][5] == 2 and (got[8][2] == 0).sum() > 0
+    # the lane partner of the singular scenario (scenario 4) converged like its tile-4 self
+    assert got[8][2][4] == got[4][2][4]
 
 
 def test_mixed_fallback_with_unreferenced_island_equals_fp64():

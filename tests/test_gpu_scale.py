@@ -104,9 +104,9 @@ def test_10k_bus_converging_loads_vs_oracle(oracle_mod, grid10k, mode, tile):
 
 def test_10k_bus_full_launch_shape_mixed(oracle_mod, grid10k):
     """8,192 scenarios at 2% of the workload's loads in one mixed-precision batch (the benchmark's
-    launch: 4-scenario FP32 tiles, 2,048 wavefronts, the LDS staging and ring sizes of the full
-    grid): 16 scenarios spread over the batch equal the oracle, and equal, bit for bit, the same
-    scenarios solved as a 16-scenario batch."""
+    launch: pair tiles of 8 FP32 scenarios, 1,024 wavefronts, the LDS staging and ring sizes of the
+    full grid): 16 scenarios spread over the batch equal the oracle, and equal, bit for bit, the same
+    scenarios solved as a 16-scenario batch of 4-scenario tiles."""
     net, topo, _, _, _, _ = grid10k
     S = 8192
     p, q = scenario_loads(net, S, seed=2026)

@@ -80,8 +80,8 @@ def test_noalloc_equals_allocating_solve(oracle_mod):
 
 
 def test_noalloc_rejects_systems_beyond_its_array():
-    _say("case: tridiagonal system of 400,000 rows (1.2 M L+U entries) -> GX_ERR_CAPACITY")
-    n = 400_000
+    _say("case: tridiagonal system of 800,000 rows (2.4 M L+U entries) -> GX_ERR_CAPACITY")
+    n = 800_000
     i = np.arange(n, dtype=np.int32)
     r = np.r_[i, i[1:], i[:-1]].astype(np.int32)
     c = np.r_[i, i[:-1], i[1:]].astype(np.int32)
@@ -90,7 +90,7 @@ def test_noalloc_rejects_systems_beyond_its_array():
     try:
         rc, _ = s.noalloc(v, np.ones(n))
         assert rc == _lib.GX_ERR_CAPACITY
-        assert b"64 B per L+U entry" in _lib.lib().gx_last_error()
+        assert b"32 B per L+U entry" in _lib.lib().gx_last_error()
         rc, x = s.host(v, np.ones(n))
         assert rc == 0 and np.all(np.isfinite(x))
     finally:
