@@ -253,7 +253,7 @@ def _emulate_parts(pa, pos, div, slots, a, cap, G):
     """The part loop of the kernel for one scenario.  The first G * kPartRegs entries of a part are
     staged before its rounds, the others start at zero and take their values from staging items
     (x_o -= value * x_m, row m = cap + 1 holding -1).  The rounds run JOINT at a time, every read
-    of the group before its writes.  The values of chunk c of part p are loaded kPartMinChunks = 2
+    of the group before its writes; a chunk that holds any of the part's rounds runs all NB of them.  The values of chunk c of part p are loaded kPartMinChunks = 2
     chunks ahead: after the write-back of part p - 2 (c < 2) or p - 1 (c >= 2); a position not
     written back by then still holds its Jacobian value.  The rows of the part buffers start as NaN."""
     mem = a.copy()
@@ -282,9 +282,10 @@ def _emulate_parts(pa, pos, div, slots, a, cap, G):
             tg, su, off, fl = wd & 0x3FF, (wd >> 10) & 0x3FF, (wd >> 20) & 0x3FF, (wd >> 31) & 1
             assert not ((wd >> 30) & 1).any()
             real = tg != ln  # empty slots target the row past the part
-            if r0 >= nr:
+            if r0 >= nr:  # (rounds past the part's last one: empty slots only)
                 assert not real.any()
-                continue
+                if r0 // NB * NB >= nr:  # the kernel skips a chunk with none of the part's rounds
+                    continue
             assert (tg <= ln).all()
             assert ((su[real] < ln) | (su[real] == mone)).all()
             assert np.unique(tg[real]).size == np.count_nonzero(real)
