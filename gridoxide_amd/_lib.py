@@ -56,6 +56,7 @@ SIGNATURES = {
     "gx_batch_island_status": (ctypes.c_int, [_vp, _i32p]),
     "gx_topo_profile": (ctypes.c_int, [_vp, ctypes.c_int, _f64p, _i32p]),
     "gx_topo_set_launch": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
+    "gx_topo_set_refactor_groups": (ctypes.c_int, [_vp, ctypes.c_int]),
     "gx_topo_set_zip": (ctypes.c_int, [_vp, _f64p]),
     "gx_topo_back_levels": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _i32p, ctypes.c_longlong, _i32p]),
     "gx_topo_forward_levels": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _i32p, ctypes.c_longlong, _i32p]),

@@ -24,6 +24,14 @@ constexpr int kPartAhead = 4;        // chunks between loading a chunk's item po
 constexpr int kPartMaxEntries = 1023;  // refactor part entries at most (10-bit offsets in item words)
 constexpr int kStageZero = 1 << 30;    // pent_pos flag: the entry is structural fill, staged as zero
 constexpr int kJointRounds = 1;        // refactor rounds whose LDS reads are issued before their writes
+// Update rounds per chunk of the refactor-by-parts round stream for G lane groups: 8 at 32 groups
+// (a quad-tile lane holds four FP32 scenarios, so a chunk's L values take 4 VGPRs per round and the
+// three chunks in flight would otherwise need 192), else 16.
+constexpr int part_chunk_rounds(int groups) { return groups >= 32 ? 8 : 16; }
+// Row stride (values) of the refactor's part buffers for lane groups of tw lanes holding v scenarios
+// each: one value of padding per row, none for four-scenario lanes (16-byte values: the buffers of
+// a 384-entry part then fit four wavefronts per CU).
+constexpr int part_row_stride(int tw, int v) { return v >= 4 ? tw : tw + 1; }
 // tail descriptors per part held in LDS for staging capacity cap: the entries past the registered
 // ones, at most kPartTail
 constexpr int part_tail(int cap, int tile) {
@@ -66,6 +74,18 @@ constexpr size_t lu_parts_lds_bytes(int max_col, int rows, int tile, size_t elem
   const size_t cap = (size_t)(max_col < 1 ? 1 : max_col), row = (size_t)(tile + 1) * elem;
   const size_t parts = factor ? (2 * (cap + 2) * row + 15) / 16 * 16 + 2 * (size_t)part_tail((int)cap, tile) * 8 : 0;
   const size_t ring = (size_t)rows * row;
+  return parts > ring ? parts : ring;
+}
+
+// Quad tiles: the refactor runs lane groups of part_tile lanes on part_elem-byte values (rows of
+// part_row_stride values), the substitutions lane groups of tile lanes on elem-byte values (ring
+// rows of tile + 1 values); the phases overlay as above.
+constexpr size_t lu_quad_lds_bytes(int max_col, int rows, int tile, size_t elem, int part_tile, size_t part_elem,
+                                   bool factor) {
+  const size_t cap = (size_t)(max_col < 1 ? 1 : max_col);
+  const size_t prow = (size_t)part_row_stride(part_tile, 4) * part_elem;
+  const size_t parts = factor ? (2 * (cap + 2) * prow + 15) / 16 * 16 + 2 * (size_t)part_tail((int)cap, part_tile) * 8 : 0;
+  const size_t ring = (size_t)rows * (size_t)(tile + 1) * elem;
   return parts > ring ? parts : ring;
 }
 
@@ -187,7 +207,8 @@ struct DevTopo {
   // part_a[p] = {first entry in pent_pos / pent_div, entries, first chunk of part_slots, rounds};
   // pent_div = -1 for a U entry, -2 for a pivot, else the part-buffer offset of the entry's pivot;
   // part_x / part_w: the round items (x, word) of each chunk and lane group g at
-  // (chunk * G + g) * kChunkRounds + round; the forward substitution by levels
+  // (chunk * PG + g) * part_chunk_rounds(PG) + round, PG = part_groups (the solves' G, or 2 G for
+  // quad tiles, whose refactor runs lane groups of two lanes); the forward substitution by levels
   // (fwd_items / fwd_steps, encoding as back_items with END keeping the sum)
   const int4* part_a = nullptr;
   const int* pent_pos = nullptr;
@@ -195,6 +216,7 @@ struct DevTopo {
   const int* part_x = nullptr;
   const int* part_w = nullptr;
   int n_parts = 0;
+  int part_groups = 0;
   // 1 when the refactor stages structural-fill entries as zero itself (the fill kernel then leaves
   // those positions alone)
   int stage_zero = 0;

@@ -222,15 +222,16 @@ struct PartPlan {
   std::vector<int> part_a;    // 4 per part: first entry (into pent_*), entries, first chunk, rounds
   std::vector<int> pent_pos;  // per part entry (in buffer order): its packed L+U position
   std::vector<int> pent_div;  // -1 U entry, -2 pivot, else the buffer offset of its column's pivot
-  std::vector<int> slots;     // pairs {x, word} per round and lane group, chunks of kChunkRounds rounds
+  std::vector<int> slots;     // pairs {x, word} per round and lane group, chunks of part_chunk_rounds(G)
   int n_parts = 0;
   int cap = 0;                // entries of the largest part
 };
 
 int build_parts(const gx::LuPattern& lu, const LuSchedule& s, int G, int pcap, const std::vector<int>* kind,
                 PartPlan& out) {
-  const int n = lu.n, NB = gx::kChunkRounds, J = gx::kJointRounds * G;
-  static_assert(gx::kChunkRounds % gx::kJointRounds == 0, "a chunk holds whole joint rounds");
+  const int n = lu.n, NB = gx::part_chunk_rounds(G), J = gx::kJointRounds * G;
+  static_assert(gx::part_chunk_rounds(32) % gx::kJointRounds == 0 && gx::part_chunk_rounds(16) % gx::kJointRounds == 0,
+                "a chunk holds whole joint rounds");
   out = PartPlan();
   // (10-bit offsets in the item words; a larger part falls back to the column-by-column kernel)
   if (s.max_col > pcap || pcap >= gx::kPartMaxEntries) return fail(GX_ERR_UNSUPPORTED, "a column does not fit a refactor part");
@@ -721,8 +722,15 @@ bool build_solve_levels(const gx::LuPattern& lu, int G, int ring_rows, bool forw
 
 struct DevSched {
   DevMem cp, ulen, row, chunk0, rounds, slots, fptr, fpos, fcol, bptr, bpos, bcol, col_a, col_f, col_b, fslot, opos, ocol;
-  DevMem uround, uitems, bslot, bitems, fitems, part_a, pent_pos, pent_div, part_x, part_w;
-  int back_steps = 0, fwd_steps = 0, n_parts = 0, part_cap = 0;
+  DevMem uround, uitems, bslot, bitems, fitems;
+  int back_steps = 0, fwd_steps = 0;
+  // refactor parts for one lane-group count (they depend on nothing else): [1] holds 32 groups
+  // (2-scenario tiles, and quad tiles, whose refactor runs at twice the solves' groups), [0] the
+  // others, so a quad-tile batch and the FP64 re-solve of its fallbacks (16 groups) keep both
+  struct PartSet {
+    DevMem part_a, pent_pos, pent_div, part_x, part_w;
+    int groups = -1, n_parts = 0, part_cap = 0;
+  } pset[2];
   std::vector<int> h_chunk0, h_rounds, h_fptr, h_bptr;
   bool uploaded = false;
   int window = -1;
@@ -731,6 +739,7 @@ struct DevSched {
     uploaded = false;
     window = -1;
     groups = -1;
+    pset[0].groups = pset[1].groups = -1;
   }
 };
 
@@ -753,12 +762,14 @@ int lu_cap(int max_col, int tile) {
   return std::max(std::max(max_col, std::min(2 * max_col, gx::kPartMaxEntries)), (gx::kLanes / tile) * gx::kPartRegs);
 }
 
-// Uploads the pattern once, the update rounds whenever the lane-group count changes and the far
-// lists whenever the window changes.
+// Uploads the pattern once, the update rounds whenever the lane-group count changes, the far
+// lists whenever the window changes, and the refactor parts of part_tile (0: tile) lanes per group
+// the first time they are asked for.
 int upload_sched(const gx::LuPattern& lu, const LuSchedule& s, int window, int tile, DevSched& m,
-                 gx::DevTopo& d, const std::vector<int>* kind = nullptr) {
+                 gx::DevTopo& d, const std::vector<int>* kind = nullptr, int part_tile = 0) {
   int rc;
   const int G = gx::kLanes / tile;
+  if (part_tile <= 0) part_tile = tile;
   if (!m.uploaded) {
     if ((rc = m.cp.upload(lu.cp)) || (rc = m.ulen.upload(lu.ulen)) || (rc = m.row.upload(lu.row))) return rc;
     m.uploaded = true;
@@ -892,36 +903,48 @@ int upload_sched(const gx::LuPattern& lu, const LuSchedule& s, int window, int t
     if (build_back_levels(lu, G, window + gx::kRingGap + 1, bitems, m.back_steps)) {
       if ((rc = m.bitems.upload(bitems))) return rc;
     }
-    // refactor by parts with the forward substitution by levels
-    m.n_parts = 0;
+    // refactor by parts (below) with the forward substitution by levels
     m.fwd_steps = 0;
     if (kUseParts && m.back_steps > 0) {
       std::vector<int> fitems;
       int fsteps = 0;
+      if (build_forward_levels(lu, G, window + gx::kRingGap + 1, fitems, fsteps)) {
+        if ((rc = m.fitems.upload(fitems))) return rc;
+        m.fwd_steps = fsteps;
+      }
+    }
+  }
+  DevSched::PartSet* ps = nullptr;
+  if (tile != gx::kWideTile && m.fwd_steps > 0) {
+    const int PG = gx::kLanes / part_tile;
+    ps = &m.pset[PG == 32 ? 1 : 0];
+    if (ps->groups != PG) {
+      ps->groups = PG;
+      ps->n_parts = 0;
       PartPlan plan;
-      if (build_forward_levels(lu, G, window + gx::kRingGap + 1, fitems, fsteps) &&
-          build_parts(lu, s, G, lu_cap(s.max_col, tile), kind, plan) == GX_OK) {
+      if (build_parts(lu, s, PG, lu_cap(s.max_col, part_tile), kind, plan) == GX_OK) {
         // the round items as two streams, x and word, each chunk's items of lane group g one run of
-        // kChunkRounds ints (the kernel loads a chunk with four 16-byte loads per lane)
-        const int NB = gx::kChunkRounds;
-        const size_t chunks = plan.slots.size() / (2 * (size_t)NB * G);
-        std::vector<int> xs(chunks * G * NB), ws(chunks * G * NB);
+        // NB ints (the kernel loads a chunk with NB / 4 16-byte loads per lane)
+        const int NB = gx::part_chunk_rounds(PG);
+        const size_t chunks = plan.slots.size() / (2 * (size_t)NB * PG);
+        std::vector<int> xs(chunks * PG * NB), ws(chunks * PG * NB);
         for (size_t c = 0; c < chunks; ++c)
           for (int r = 0; r < NB; ++r)
-            for (int gg = 0; gg < G; ++gg) {
-              const size_t from = 2 * ((c * NB + r) * G + gg), to = (c * G + gg) * NB + r;
+            for (int gg = 0; gg < PG; ++gg) {
+              const size_t from = 2 * ((c * NB + r) * PG + gg), to = (c * PG + gg) * NB + r;
               xs[to] = plan.slots[from];
               ws[to] = plan.slots[from + 1];
             }
-        if ((rc = m.fitems.upload(fitems)) || (rc = m.part_a.upload(plan.part_a)) ||
-            (rc = m.pent_pos.upload(plan.pent_pos)) || (rc = m.pent_div.upload(plan.pent_div)) ||
-            (rc = m.part_x.upload(xs)) || (rc = m.part_w.upload(ws)))
+        if ((rc = ps->part_a.upload(plan.part_a)) || (rc = ps->pent_pos.upload(plan.pent_pos)) ||
+            (rc = ps->pent_div.upload(plan.pent_div)) || (rc = ps->part_x.upload(xs)) || (rc = ps->part_w.upload(ws))) {
+          ps->groups = -1;
           return rc;
-        m.fwd_steps = fsteps;
-        m.n_parts = plan.n_parts;
-        m.part_cap = lu_cap(s.max_col, tile);
+        }
+        ps->n_parts = plan.n_parts;
+        ps->part_cap = lu_cap(s.max_col, part_tile);
       }
     }
+    if (ps->n_parts == 0) ps = nullptr;
   }
   d.n = lu.n;
   d.nnz_lu = lu.nnz();
@@ -951,17 +974,18 @@ int upload_sched(const gx::LuPattern& lu, const LuSchedule& s, int window, int t
   d.window = window;
   d.back_items = tile != gx::kWideTile && m.back_steps > 0 ? m.bitems.as<int2>() : nullptr;
   d.back_steps = d.back_items ? m.back_steps : 0;
-  const bool parts = tile != gx::kWideTile && m.n_parts > 0;
-  d.part_a = parts ? m.part_a.as<int4>() : nullptr;
-  d.pent_pos = parts ? m.pent_pos.as<int>() : nullptr;
-  d.pent_div = parts ? m.pent_div.as<int>() : nullptr;
-  d.part_x = parts ? m.part_x.as<int>() : nullptr;
-  d.part_w = parts ? m.part_w.as<int>() : nullptr;
-  d.n_parts = parts ? m.n_parts : 0;
+  const bool parts = ps != nullptr;
+  d.part_a = parts ? ps->part_a.as<int4>() : nullptr;
+  d.pent_pos = parts ? ps->pent_pos.as<int>() : nullptr;
+  d.pent_div = parts ? ps->pent_div.as<int>() : nullptr;
+  d.part_x = parts ? ps->part_x.as<int>() : nullptr;
+  d.part_w = parts ? ps->part_w.as<int>() : nullptr;
+  d.n_parts = parts ? ps->n_parts : 0;
+  d.part_groups = parts ? ps->groups : 0;
   d.stage_zero = parts && kind ? 1 : 0;
   d.fwd_items = parts ? m.fitems.as<int2>() : nullptr;
   d.fwd_steps = parts ? m.fwd_steps : 0;
-  if (parts) d.max_col = m.part_cap;  // (sizes the staging buffers: lu_cap)
+  if (parts) d.max_col = ps->part_cap;  // (sizes the staging buffers: lu_cap)
   return GX_OK;
 }
 
@@ -987,7 +1011,9 @@ bool valid_ring(int rows) { return rows == 0 || (rows >= 64 && rows <= 4096 && (
 // at FP32 (54 + 15 vs 76 + 5 ms).
 // Mixed precision with at least 1,024 tiles of 8 scenarios: pair tiles of 8 (two scenarios per
 // lane on the 4-scenario schedule, one wavefront per SIMD): 10,000-bus grid, 8,192 scenarios, LU
-// 37.0 against 39.4 ms per launch, fill 4.1 against 6.2 ms, batch 895 against 993 ms.
+// 37.0 against 39.4 ms per launch, fill 4.1 against 6.2 ms, batch 895 against 993 ms.  Their
+// refactor runs as quad tiles (four scenarios per lane on the 2-scenario part schedule): 29.4
+// against 31.4 ms per launch.
 int auto_tile(int n_scen, bool fp32) {
   if (fp32 && (n_scen + 7) / 8 >= 1024) return 8;
   if ((n_scen + 1) / 2 <= 1024) return 2;
@@ -1189,6 +1215,7 @@ struct gx_topo {
   gx::DevTopo dt;
   // launch shape: scenarios per tile and solve ring rows (0 = automatic)
   int tile_req = 0, ring_req = 0;
+  int refactor_groups = 0;  // lane groups of the FP32 8-scenario refactor: 0 automatic, 16 or 32
   // workspace of gx_batch_solve
   Workspace ws;
   int last_fallback = 0;
@@ -1415,9 +1442,9 @@ int upload_ybus(gx_topo& t);
 
 // Uploads the topology (Y-bus, recipes, LU pattern and schedule) once per factorization, and the
 // far lists of the solves for the given window.
-int upload_topo(gx_topo& t, int window, int tile) {
+int upload_topo(gx_topo& t, int window, int tile, int part_tile = 0) {
   int rc;
-  if ((rc = upload_sched(t.lu, t.sched, window, tile, t.ds, t.dt, &t.h_kind))) return rc;
+  if ((rc = upload_sched(t.lu, t.sched, window, tile, t.ds, t.dt, &t.h_kind, part_tile))) return rc;
   if (t.uploaded) return GX_OK;
   if ((rc = upload_ybus(t)) || (rc = t.d_rpos_p.upload(t.h_rp)) ||
       (rc = t.d_rpos_q.upload(t.h_rq)) || (rc = t.d_xpos_a.upload(t.h_xa)) || (rc = t.d_xpos_m.upload(t.h_xm)) ||
@@ -1681,7 +1708,7 @@ int ensure_warm(gx_topo& t) {
 }
 
 int stage_batch(gx_topo& t, Workspace& ws, const BatchIn& in, int tile, gx::DevWork& w, HostResult* hres,
-                bool pairs = false) {
+                bool pairs = false, bool quads = false) {
   const int n_scen = in.n_scen, nb = t.n_bus, max_iter = in.max_iter;
   const size_t plane = (size_t)n_scen * nb * sizeof(double);
   hipStream_t st = in.st;
@@ -1803,11 +1830,15 @@ int stage_batch(gx_topo& t, Workspace& ws, const BatchIn& in, int tile, gx::DevW
   }
   // the solve ring is sized for FP64 values: the mixed mode's FP64 re-solve uses the same lists.
   // Pair tiles run the schedules of tiles half as wide (two scenarios per lane).
+  // Quad tiles refactor on lane groups half as wide again (four scenarios per lane), when their
+  // part schedule exists; else they run as pair tiles.
   const int lt = pairs ? tile / 2 : tile;
   const int rows = pick_ring_rows(lu_cap(t.sched.max_col, lt), lt, (pairs ? 2 : 1) * sizeof(double), t.ring_req);
   if (rows < 0)
     return fail(GX_ERR_UNSUPPORTED, "longest L+U column exceeds the LDS staging limit of the device refactor");
-  if ((rc = upload_topo(t, gx::ring_window_of(rows, lt), lt))) return rc;
+  const int window = gx::ring_window_of(rows, lt);
+  if ((rc = upload_topo(t, window, lt, quads ? lt / 2 : lt))) return rc;
+  if (quads && t.dt.part_groups != 2 * (gx::kLanes / lt) && (rc = upload_topo(t, window, lt, lt))) return rc;
   // FP32 values for the mixed mode (its FP64 re-solve of fallbacks grows the buffer first)
   if ((rc = ws.lu.reserve(lu_lanes(lanes, tile) * (size_t)t.dt.nnz_lu * (in.fp32 ? 4 : 8)))) return rc;
   w.lu = ws.lu.p;
@@ -1903,6 +1934,13 @@ int gx_topo_set_launch(gx_topo* t, int tile, int ring_rows) {
   if (!valid_ring(ring_rows)) return fail(GX_ERR_ARG, "ring_rows must be 0 or a power of two in [64, 4096]");
   t->tile_req = tile;
   t->ring_req = ring_rows;
+  return GX_OK;
+}
+
+int gx_topo_set_refactor_groups(gx_topo* t, int groups) {
+  if (!t) return fail(GX_ERR_ARG, "null");
+  if (groups != 0 && groups != 16 && groups != 32) return fail(GX_ERR_ARG, "refactor groups must be 0, 16 or 32");
+  t->refactor_groups = groups;
   return GX_OK;
 }
 
@@ -2301,12 +2339,15 @@ int gx_batch_solve_outages(gx_topo* t, int n_scen, const double* p_spec, const d
   in.outage = any_outage ? &h_outage : nullptr;
   hipStream_t st = in.st;
   const int tile = t->tile_req ? t->tile_req : auto_tile(n_scen, in.fp32);
-  // mixed precision at 8 scenarios per tile: pair tiles (4-lane groups, two scenarios per lane)
+  // mixed precision at 8 scenarios per tile: pair tiles (4-lane groups, two scenarios per lane),
+  // whose refactor runs as quad tiles (2-lane groups, four scenarios per lane) unless 16 refactor
+  // groups were asked for
   const bool pairs = in.fp32 && tile == 8;
+  const bool quads = pairs && t->refactor_groups != 16;
   gx::DevWork w;
   Workspace& ws = t->ws;
   HostResult hres;
-  if ((rc = stage_batch(*t, ws, in, tile, w, &hres, pairs))) return rc;
+  if ((rc = stage_batch(*t, ws, in, tile, w, &hres, pairs, quads))) return rc;
   if (hres.used) return host_results(*t, in, hres, vm, va, status, iters, history);
   // optional per-launch timing of each kernel class on this stream
   auto timed = [&](int cls, auto&& launch) -> int {
@@ -2368,6 +2409,8 @@ int gx_batch_solve_outages(gx_topo* t, int n_scen, const double* p_spec, const d
     if (t->last_fallback > 0) {
       if ((rc = ws.lu.reserve(lu_lanes((size_t)w.n_tiles * w.tile, w.tile) * (size_t)t->dt.nnz_lu * 8))) return rc;
       w.lu = ws.lu.p;
+      // the FP64 re-solve of pair tiles refactors on the solves' lane groups
+      if (quads && (rc = upload_topo(*t, t->dt.window, tile / 2, tile / 2))) return rc;
       if ((rc = newton(false))) return rc;
     }
   } else {

@@ -128,13 +128,15 @@ __device__ __forceinline__ zd operator/(zd a, zd b) {
 }
 // Two scenarios in one lane (pair tiles: a lane group of TW lanes holds 2 TW scenarios, a
 // position's 2 TW values still contiguous), element by element.  Each load, store, LDS access and
-// index computation of the LU kernel then serves two scenarios.
+// index computation of the LU kernel then serves two scenarios; a pair of pairs serves four (the
+// refactor of quad tiles: one 16-byte access per lane).
 template <typename S>
 struct __attribute__((aligned(2 * sizeof(S)))) pair2 {
   S a, b;
 };
 using f2 = pair2<float>;
 using d2 = pair2<double>;
+using f4 = pair2<f2>;
 template <typename S>
 __device__ __forceinline__ pair2<S> operator-(pair2<S> x, pair2<S> y) { return {x.a - y.a, x.b - y.b}; }
 template <typename S>
@@ -150,7 +152,7 @@ struct lane_pack {
 };
 template <typename S>
 struct lane_pack<pair2<S>> {
-  static constexpr int V = 2;
+  static constexpr int V = 2 * lane_pack<S>::V;
 };
 __device__ __forceinline__ bool is_zero(float v) { return v == 0.0f; }
 __device__ __forceinline__ bool is_zero(double v) { return v == 0.0; }
@@ -172,31 +174,34 @@ __device__ __forceinline__ pair2<S> fnma(pair2<S> a, pair2<S> b, pair2<S> c) {
 template <typename T>
 __device__ __forceinline__ int zero_mask(T v) { return is_zero(v) ? 1 : 0; }
 template <typename S>
-__device__ __forceinline__ int zero_mask(pair2<S> v) { return (v.a == S(0) ? 1 : 0) | (v.b == S(0) ? 2 : 0); }
+__device__ __forceinline__ int zero_mask(pair2<S> v) { return zero_mask(v.a) | (zero_mask(v.b) << lane_pack<S>::V); }
 template <typename T>
 __device__ __forceinline__ int nonfinite_mask(T v) { return finite_value(v) ? 0 : 1; }
 template <typename S>
-__device__ __forceinline__ int nonfinite_mask(pair2<S> v) { return (isfinite(v.a) ? 0 : 1) | (isfinite(v.b) ? 0 : 2); }
+__device__ __forceinline__ int nonfinite_mask(pair2<S> v) {
+  return nonfinite_mask(v.a) | (nonfinite_mask(v.b) << lane_pack<S>::V);
+}
 // Keeps the compiler from moving memory accesses across this point.  Lanes of a wavefront that pass
 // values through memory rely on the program order of the accesses; without it the compiler may, for
 // example, sink a load into the branch that uses it, past another lane's store.
 __device__ __forceinline__ void memory_order_point() { asm volatile("" ::: "memory"); }
+// a real constant in every scenario of a value (zd: imaginary part zero)
 template <typename T>
-__device__ __forceinline__ T zero_value() { return T(0); }
+struct constant_of {
+  static __device__ __forceinline__ T make(double x) { return T(x); }
+};
 template <>
-__device__ __forceinline__ zd zero_value<zd>() { return zd{0.0, 0.0}; }
-template <>
-__device__ __forceinline__ f2 zero_value<f2>() { return f2{0.0f, 0.0f}; }
-template <>
-__device__ __forceinline__ d2 zero_value<d2>() { return d2{0.0, 0.0}; }
+struct constant_of<zd> {
+  static __device__ __forceinline__ zd make(double x) { return zd{x, 0.0}; }
+};
+template <typename S>
+struct constant_of<pair2<S>> {
+  static __device__ __forceinline__ pair2<S> make(double x) { return {constant_of<S>::make(x), constant_of<S>::make(x)}; }
+};
 template <typename T>
-__device__ __forceinline__ T minus_one() { return T(-1); }
-template <>
-__device__ __forceinline__ zd minus_one<zd>() { return zd{-1.0, 0.0}; }
-template <>
-__device__ __forceinline__ f2 minus_one<f2>() { return f2{-1.0f, -1.0f}; }
-template <>
-__device__ __forceinline__ d2 minus_one<d2>() { return d2{-1.0, -1.0}; }
+__device__ __forceinline__ T zero_value() { return constant_of<T>::make(0.0); }
+template <typename T>
+__device__ __forceinline__ T minus_one() { return constant_of<T>::make(-1.0); }
 __device__ __forceinline__ float shfl_xor_v(float v, int d) { return __shfl_xor(v, d); }
 __device__ __forceinline__ double shfl_xor_v(double v, int d) { return __shfl_xor(v, d); }
 __device__ __forceinline__ zd shfl_xor_v(zd v, int d) { return zd{__shfl_xor(v.re, d), __shfl_xor(v.im, d)}; }
@@ -1526,19 +1531,24 @@ __device__ __forceinline__ void lu64_tile(const DevTopo& t, const DevWork& w, in
 
 // One launch per batch: one wavefront per tile (XCD-aware numbering).
 // Refactor by parts, then the forward and back substitutions by levels; one wavefront per tile.
-//   LDS: two part buffers xc[2][cap+1][TW+1] (part p in buffer p & 1: the previous part stays
-//   readable for the updates that use its L values) and ring[R][TW+1] for the solves.
+//   LDS: two part buffers xc[2][cap+2][S] (part p in buffer p & 1: the previous part stays
+//   readable for the updates that use its L values; S = part_row_stride) while factoring, then
+//   ring[R][TW+1] for the solves from offset 0.
 // A part is a set of columns of one elimination level (independent of each other), laid one after
 // the other in its buffer: stage A at their entries, run the part's update rounds (the chunk stream
 // of the column-by-column kernel, host-scheduled), divide each L entry by its column's pivot, write
 // back.  The first kPartRegs * G entries of a part are staged through registers loaded a part ahead
 // (their positions two parts ahead); the rest are staged by the part's first rounds, whose values come
 // with the round stream's loads.
-template <typename T, int TW, bool kFactor>
+// TR / TWR: the refactor's value type and lane-group width when they differ from the substitutions'
+// (quad tiles: the refactor runs 32 lane groups of two lanes, four scenarios per lane, the
+// substitutions 16 groups of four lanes, two scenarios per lane; both address the same tile layout).
+template <typename T, int TW, bool kFactor, typename TR = T, int TWR = TW>
 __device__ __forceinline__ void lu_tile_parts(const DevTopo& t, const DevWork& w, int tile, int cap, int iter) {
-  constexpr int G = kLanes / TW;
-  constexpr int S = TW + 1;
-  constexpr int NB = kChunkRounds;
+  constexpr int G = kLanes / TWR;         // the refactor's lane groups
+  constexpr int VR = lane_pack<TR>::V;    // scenarios per refactor lane
+  constexpr int S = part_row_stride(TWR, VR);
+  constexpr int NB = part_chunk_rounds(G);
   constexpr int MP = kPartRegs;
   constexpr int MT = 4;  // long-part entries per lane group and batch
   const int W = t.window;
@@ -1547,22 +1557,25 @@ __device__ __forceinline__ void lu_tile_parts(const DevTopo& t, const DevWork& w
   // LDS (lu_parts_lds_bytes): while factoring, the two part buffers and the descriptors of a
   // part's entries past the registered ones ("tail"), two parts' worth; the solve ring from offset 0
   // once the refactor is done (the phases follow each other in the wavefront's program order)
-  T* xcb = reinterpret_cast<T*>(smem_raw);
+  TR* xcb = reinterpret_cast<TR*>(smem_raw);
   const int xstride = (cap + 2) * S;  // rows: cap entries, the empty-slot row, the row holding -1
   T* ring = reinterpret_cast<T*>(smem_raw);
-  int2* tdsc = reinterpret_cast<int2*>(smem_raw + (2 * (size_t)xstride * sizeof(T) + 15) / 16 * 16);
-  constexpr int V = lane_pack<T>::V;  // scenarios per lane (2 in pair tiles)
-  const int g = grp<TW>(), s = sub<TW>();
+  int2* tdsc = reinterpret_cast<int2*>(smem_raw + (2 * (size_t)xstride * sizeof(TR) + 15) / 16 * 16);
+  constexpr int V = lane_pack<T>::V;  // scenarios per substitution lane (2 in pair tiles)
+  static_assert(TW * V == TWR * VR && TW * sizeof(T) == TWR * sizeof(TR), "both phases cover the tile's values");
+  const int g = grp<TWR>(), s = sub<TWR>();  // the refactor's lane group and lane within it
+  const int gs = grp<TW>(), ss = sub<TW>();  // the substitutions'
   if (!tile_running_v<TW, V>(w.status, tile)) return;
   if (!kFactor && !w.swept[tile]) return;  // (the mixed correction solve)
-  const int scb = (tile * TW + s) * V;     // the lane's first scenario
   const int nl = t.nnz_lu;
-  char* __restrict__ lub = reinterpret_cast<char*>(reinterpret_cast<T*>(w.lu) + lu_at(0, t.nnz_lu, tile, TW) + s);
   const char* __restrict__ lbase = reinterpret_cast<const char*>(reinterpret_cast<T*>(w.lu) + lu_at(0, t.nnz_lu, tile, TW));
   T* __restrict__ yb = reinterpret_cast<T*>(w.y) + (size_t)tile * t.n * TW;
   const unsigned pstb = (unsigned)(lu_block(TW) * TW * sizeof(T));
-  auto LU = [&](int p) -> T& { return *reinterpret_cast<T*>(lub + (size_t)(unsigned)p * pstb); };
-  int bad = 0;  // the lane's scenarios with a zero pivot or a non-finite solution (bit per scenario)
+  // a uniform base plus a 32-bit lane offset (the scalar-base addressing form of the loads)
+  const unsigned loff = (unsigned)(s * sizeof(TR));
+  auto LU = [&](int p) -> TR& { return *reinterpret_cast<TR*>(const_cast<char*>(lbase) + ((unsigned)p * pstb + loff)); };
+  int bad = 0;   // the refactor lane's scenarios with a zero pivot (bit per scenario)
+  int sbad = 0;  // the substitution lane's scenarios with a non-finite solution
   bool overflow = false;
   auto cl = [](int v, int hi) { return v < 0 ? 0 : (v > hi ? hi : v); };
   if (kFactor) {
@@ -1584,7 +1597,7 @@ __device__ __forceinline__ void lu_tile_parts(const DevTopo& t, const DevWork& w
         tdiv[i] = pdiv[e];
       }
     };
-    const int ts = part_tail(cap, TW);  // tail descriptors per part held in LDS
+    const int ts = part_tail(cap, TWR);  // tail descriptors per part held in LDS
     auto tail_store = [&](int buf, int len) {
 #pragma unroll
       for (int i = 0; i < MDT; ++i) {
@@ -1603,7 +1616,7 @@ __device__ __forceinline__ void lu_tile_parts(const DevTopo& t, const DevWork& w
       if (tt >= ts) d = make_int2(global_int(ppos + pb + o), global_int(pdiv + pb + o));
       return d;
     };
-    T cval[MP], xval[MP];
+    TR cval[MP], xval[MP];
     auto ent_issue = [&](const int4& m, int (&pp)[MP], int (&dd)[MP]) {
 #pragma unroll
       for (int i = 0; i < MP; ++i) {
@@ -1613,7 +1626,7 @@ __device__ __forceinline__ void lu_tile_parts(const DevTopo& t, const DevWork& w
         dd[i] = pdiv[e];
       }
     };
-    auto val_issue_p = [&](const int (&pp)[MP], T (&vv)[MP]) {
+    auto val_issue_p = [&](const int (&pp)[MP], TR (&vv)[MP]) {
 #pragma unroll
       for (int i = 0; i < MP; ++i) vv[i] = LU(pp[i] & ~kStageZero);
     };
@@ -1624,7 +1637,7 @@ __device__ __forceinline__ void lu_tile_parts(const DevTopo& t, const DevWork& w
     static_assert(NB % kJointRounds == 0, "a chunk holds whole joint rounds");
     static_assert(kPartMinChunks == 2 && kPartAhead == 4, "the register roles below assume these");
     int xs[3][NB], ws[3][NB];
-    T vs[3][NB];
+    TR vs[3][NB];
     const int* __restrict__ xsb = t.part_x + g * NB;
     const int* __restrict__ wsb = t.part_w + g * NB;
     auto ints_issue = [&](const int* __restrict__ base, int chunk, int (&d)[NB]) {
@@ -1638,7 +1651,7 @@ __device__ __forceinline__ void lu_tile_parts(const DevTopo& t, const DevWork& w
         d[4 * i + 3] = v.w;
       }
     };
-    auto vals_issue = [&](const int (&x)[NB], T (&d)[NB]) {
+    auto vals_issue = [&](const int (&x)[NB], TR (&d)[NB]) {
 #pragma unroll
       for (int r = 0; r < NB; ++r) d[r] = LU(x[r]);
     };
@@ -1649,7 +1662,7 @@ __device__ __forceinline__ void lu_tile_parts(const DevTopo& t, const DevWork& w
     // the part's last one hold empty items (the unused row past the part updated by itself), so the
     // rounds need no branch between them and each round's address arithmetic is issued during the
     // previous round's LDS wait (2,059 parts on the 10,000-bus grid: 197,984 rounds run for 182,052).
-    auto step = [&](auto role, int gc, int rleft, T* xc, const T* xprev) __attribute__((always_inline)) {
+    auto step = [&](auto role, int gc, int rleft, TR* xc, const TR* xprev) __attribute__((always_inline)) {
       constexpr int E = decltype(role)::value, E1 = (E + 1) % 3, E2 = (E + 2) % 3;
       vals_issue(xs[E2], vs[E2]);                    // values of chunk gc + 2
       ints_issue(xsb, gc + kPartAhead, xs[E1]);      // positions of chunk gc + 4
@@ -1665,22 +1678,29 @@ __device__ __forceinline__ void lu_tile_parts(const DevTopo& t, const DevWork& w
 #pragma unroll
         for (int r = 0; r < NB; ++r) {
           const int wd = ws[E][r];
-          const T pv = xprev[((wd >> 20) & 0x3ff) * S + s];
+          const TR pv = xprev[((wd >> 20) & 0x3ff) * S + s];
           if (wd < 0) vs[E][r] = pv;
         }
       }
+      // LDS byte addresses as one multiply-add each: the row size as an opaque scalar keeps the
+      // compiler from turning a power-of-two multiply into a shift and a mask
+      unsigned rowb = S * sizeof(TR);
+      asm volatile("" : "+s"(rowb));
+      const unsigned lo = (unsigned)(reinterpret_cast<const char*>(xc) - reinterpret_cast<const char*>(smem_raw)) +
+                          (unsigned)(s * sizeof(TR));
+      auto X = [&](unsigned row) -> TR& { return *reinterpret_cast<TR*>(smem_raw + (__umul24(row, rowb) + lo)); };
 #pragma unroll
       for (int r = 0; r < NB; r += kJointRounds) {
-        int to[kJointRounds];
-        T nv[kJointRounds];
+        unsigned to[kJointRounds];
+        TR nv[kJointRounds];
 #pragma unroll
         for (int h = 0; h < kJointRounds; ++h) {
-          const int wd = ws[E][r + h];
-          to[h] = (wd & 0x3ff) * S + s;
-          nv[h] = fnma(vs[E][r + h], xc[((wd >> 10) & 0x3ff) * S + s], xc[to[h]]);
+          const unsigned wd = (unsigned)ws[E][r + h];
+          to[h] = wd & 0x3ffu;
+          nv[h] = fnma(vs[E][r + h], X(__builtin_amdgcn_ubfe(wd, 10, 10)), X(to[h]));
         }
 #pragma unroll
-        for (int h = 0; h < kJointRounds; ++h) xc[to[h]] = nv[h];
+        for (int h = 0; h < kJointRounds; ++h) X(to[h]) = nv[h];
       }
     };
     int4 m1 = vload4(t.part_a, cl(1, np - 1)), m2 = vload4(t.part_a, cl(2, np - 1));
@@ -1704,16 +1724,16 @@ __device__ __forceinline__ void lu_tile_parts(const DevTopo& t, const DevWork& w
     int4 mk = uni4(vload4(t.part_a, 0));
     tail_issue(mk);
     // the row after the empty-slot row holds -1 in both part buffers (staging items multiply by it)
-    xcb[(cap + 1) * S + s] = minus_one<T>();
-    xcb[xstride + (cap + 1) * S + s] = minus_one<T>();
+    xcb[(cap + 1) * S + s] = minus_one<TR>();
+    xcb[xstride + (cap + 1) * S + s] = minus_one<TR>();
     int role = 0;  // the running chunk's index mod 3
     for (int p = 0; p < np; ++p) {
       const int pb = mk.x, len = mk.y, ch0 = mk.z, nr = mk.w;
       const int4 mx = uni4(m1);
       m1 = m2;
       m2 = vload4(t.part_a, cl(p + 3, np - 1));
-      T* xc = xcb + (p & 1) * xstride;
-      const T* xprev = xcb + ((p & 1) ^ 1) * xstride;
+      TR* xc = xcb + (p & 1) * xstride;
+      const TR* xprev = xcb + ((p & 1) ^ 1) * xstride;
       if (len > cap) {  // the host sizes cap to the largest part; kept as a guard
         overflow = true;
         break;
@@ -1727,9 +1747,9 @@ __device__ __forceinline__ void lu_tile_parts(const DevTopo& t, const DevWork& w
 #pragma unroll
       for (int i = 0; i < MP; ++i) {
         const int o = g + G * i;
-        if (o < len) xc[o * S + s] = (cpos[i] & kStageZero) ? zero_value<T>() : cval[i];
+        if (o < len) xc[o * S + s] = (cpos[i] & kStageZero) ? zero_value<TR>() : cval[i];
       }
-      for (int o = G * MP + g; o < len; o += G) xc[o * S + s] = zero_value<T>();
+      for (int o = G * MP + g; o < len; o += G) xc[o * S + s] = zero_value<TR>();
       // loads for the next parts
       val_issue_p(xpos, xval);
       {
@@ -1761,7 +1781,7 @@ __device__ __forceinline__ void lu_tile_parts(const DevTopo& t, const DevWork& w
       for (int i = 0; i < MP; ++i) {
         const int o = g + G * i;
         if (o < len) {
-          T v = xc[o * S + s];
+          TR v = xc[o * S + s];
           const int d = cdiv[i];
           if (d >= 0) v = v / xc[d * S + s];
           if (d == -2) bad |= zero_mask(v);
@@ -1783,7 +1803,7 @@ __device__ __forceinline__ void lu_tile_parts(const DevTopo& t, const DevWork& w
       // loop of their own, drained before and after
       auto tail_batch = [&](int b, auto lds_only) __attribute__((always_inline)) {
         int pp[MT], dd[MT];
-        T tv[MT];
+        TR tv[MT];
 #pragma unroll
         for (int u = 0; u < MT; ++u) {
           const int o = b + G * u + g;
@@ -1832,19 +1852,35 @@ __device__ __forceinline__ void lu_tile_parts(const DevTopo& t, const DevWork& w
   }
   if (!overflow) {
     memory_order_point();
-    level_solve<T, TW, false>(t.fwd_items, g, t.fwd_steps, R, nl, lbase, pstb, reinterpret_cast<char*>(yb),
-                             (unsigned)(s * sizeof(T)), ring + s, bad);
+    level_solve<T, TW, false>(t.fwd_items, gs, t.fwd_steps, R, nl, lbase, pstb, reinterpret_cast<char*>(yb),
+                             (unsigned)(ss * sizeof(T)), ring + ss, sbad);
     memory_order_point();
-    level_solve<T, TW, true>(t.back_items, g, t.back_steps, R, nl, lbase, pstb, reinterpret_cast<char*>(yb),
-                             (unsigned)(s * sizeof(T)), ring + s, bad);
+    level_solve<T, TW, true>(t.back_items, gs, t.back_steps, R, nl, lbase, pstb, reinterpret_cast<char*>(yb),
+                             (unsigned)(ss * sizeof(T)), ring + ss, sbad);
   }
   // a zero pivot seen by any group of the scenario, or a non-finite solution, makes it Singular
-  bad = any_groups<TW>(overflow ? (1 << V) - 1 : bad);
-  if (g == 0) {
+  if constexpr (TW == TWR) {
+    bad = any_groups<TW>(overflow ? (1 << V) - 1 : (bad | sbad));
+    if (g == 0) {
 #pragma unroll
-    for (int e = 0; e < V; ++e) {
-      const int sc = scb + e;
-      if ((bad >> e & 1) && w.status[sc] == kRunning && (kFactor || sc_refines(w, sc, iter))) {
+      for (int e = 0; e < V; ++e) {
+        const int sc = (tile * TW + s) * V + e;
+        if ((bad >> e & 1) && w.status[sc] == kRunning && (kFactor || sc_refines(w, sc, iter))) {
+          w.status[sc] = kSingular;
+          w.iters[sc] = iter + 1;
+        }
+      }
+    }
+  } else {
+    // the two phases map lanes to scenarios differently: the flags in the tile's scenario order
+    const int lm = overflow ? (1 << (TW * V)) - 1 : ((bad << (s * VR)) | (sbad << (ss * V)));
+    int tm = 0;
+#pragma unroll
+    for (int e = 0; e < TW * V; ++e) tm |= __ballot((lm >> e) & 1) != 0ull ? 1 << e : 0;
+    const int l = lane_id();
+    if (l < TW * V) {
+      const int sc = tile * TW * V + l;
+      if ((tm >> l & 1) && w.status[sc] == kRunning && (kFactor || sc_refines(w, sc, iter))) {
         w.status[sc] = kSingular;
         w.iters[sc] = iter + 1;
       }
@@ -1859,9 +1895,9 @@ __global__ __launch_bounds__(64) void k_lu(DevTopo t, DevWork w, int cap, int it
   else lu_tile<T, TW, kFactor>(t, w, tile, cap, iter);
 }
 
-template <typename T, int TW, bool kFactor>
+template <typename T, int TW, bool kFactor, typename TR = T, int TWR = TW>
 __global__ __launch_bounds__(64) void k_lu_parts(DevTopo t, DevWork w, int cap, int iter) {
-  lu_tile_parts<T, TW, kFactor>(t, w, xcd_tile(blockIdx.x, gridDim.x), cap, iter);
+  lu_tile_parts<T, TW, kFactor, TR, TWR>(t, w, xcd_tile(blockIdx.x, gridDim.x), cap, iter);
 }
 
 // One persistent launch over the tiles of several requests (topologies of any size): each
@@ -1886,7 +1922,7 @@ __global__ __launch_bounds__(64) void k_lu_queue(const DevTopo* __restrict__ top
     if constexpr (TW == kWideTile) {
       lu64_tile<T, kFactor>(t, w, item.y, cap, iter);
     } else {
-      if (t.part_a) lu_tile_parts<T, TW, kFactor>(t, w, item.y, cap, iter);
+      if (t.part_a && t.part_groups == kLanes / TW) lu_tile_parts<T, TW, kFactor>(t, w, item.y, cap, iter);
       else lu_tile<T, TW, kFactor>(t, w, item.y, cap, iter);
     }
     __syncthreads();  // the next item reuses the LDS
@@ -2253,8 +2289,26 @@ template <typename T, int TW>
 void run_lu(const DevTopo& t, const DevWork& w, bool factor, int iter, hipStream_t stream) {
   static size_t granted_f = 65536, granted_s = 65536, granted_pf = 65536, granted_ps = 65536;
   const int cap = t.max_col < 1 ? 1 : t.max_col;
+  if constexpr (TW == 8 && std::is_same<T, float>::value) {
+    if (t.part_a && t.groups == kLanes / 4 && t.part_groups == kLanes / 2) {
+      // quad tiles: the refactor on 2-lane groups, four scenarios per lane; the substitutions on
+      // the 4-lane groups of pair tiles
+      using P = pair2<T>;
+      using Q = pair2<P>;
+      static size_t granted_hf = 65536, granted_hs = 65536;
+      const size_t shmem = lu_quad_lds_bytes(t.max_col, t.window + kRingGap + 1, 4, sizeof(P), 2, sizeof(Q), factor);
+      if (factor) {
+        allow_lds(k_lu_parts<P, 4, true, Q, 2>, shmem, granted_hf);
+        hipLaunchKernelGGL((k_lu_parts<P, 4, true, Q, 2>), dim3(w.n_tiles), dim3(64), shmem, stream, t, w, cap, iter);
+      } else {
+        allow_lds(k_lu_parts<P, 4, false, Q, 2>, shmem, granted_hs);
+        hipLaunchKernelGGL((k_lu_parts<P, 4, false, Q, 2>), dim3(w.n_tiles), dim3(64), shmem, stream, t, w, cap, iter);
+      }
+      return;
+    }
+  }
   if constexpr (TW == 8 && (std::is_same<T, float>::value || std::is_same<T, double>::value)) {
-    if (t.part_a && t.groups == kLanes / 4) {
+    if (t.part_a && t.groups == kLanes / 4 && t.part_groups == kLanes / 4) {
       // pair tiles: the schedule of 4-lane groups, two scenarios per lane
       using P = pair2<T>;
       static size_t granted_qf = 65536, granted_qs = 65536;
@@ -2271,7 +2325,7 @@ void run_lu(const DevTopo& t, const DevWork& w, bool factor, int iter, hipStream
   }
   const size_t shmem = lu_bytes<T>(t, TW, factor);
   if constexpr (TW != kWideTile) {
-    if (t.part_a) {
+    if (t.part_a && t.part_groups == kLanes / TW) {
       if (factor) {
         allow_lds(k_lu_parts<T, TW, true>, shmem, granted_pf);
         hipLaunchKernelGGL((k_lu_parts<T, TW, true>), dim3(w.n_tiles), dim3(64), shmem, stream, t, w, cap, iter);

@@ -308,6 +308,10 @@ class Topology:
         0 = default) for later solves."""
         check(_lib.lib().gx_topo_set_launch(self._h, int(tile), int(ring_rows)))
 
+    def set_refactor_groups(self, groups: int = 0):
+        """Lane groups of the refactor of mixed-precision 8-scenario tiles (0 = automatic, 16, 32)."""
+        check(_lib.lib().gx_topo_set_refactor_groups(self._h, int(groups)))
+
     def batch_solve(self, p_spec, q_spec, vm0, va0, tol=1e-8, max_iter=20, mode=Mode.FP64, init="given",
                     outage=None, keep_warm=False):
         """Host arrays [n_scen, n_bus] -> (vm, va, status, iters, history).  init: "given" starts

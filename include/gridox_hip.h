@@ -186,6 +186,14 @@ int gx_topo_profile(gx_topo* t, int enable, double* ms, int* launches);
  * triangular-solve sums.  GX_ERR_ARG for other values. */
 int gx_topo_set_launch(gx_topo* t, int tile, int ring_rows);
 
+/* Lane groups of the refactor of mixed-precision 8-scenario tiles in later gx_batch_solve calls on
+ * t (no counterpart in the reference).  0 (automatic) and 32: quad tiles, whose refactor runs 32
+ * lane groups of two lanes with four FP32 scenarios per lane (the substitutions keep 16 groups of
+ * four lanes, two scenarios per lane), when the topology's part schedule exists for 32 groups; 16:
+ * the refactor on the substitutions' 16 groups.  Results are the same bits either way (each entry
+ * takes its updates in the same order).  GX_ERR_ARG for other values. */
+int gx_topo_set_refactor_groups(gx_topo* t, int groups);
+
 /* Voltage-dependent ZIP injection terms of t's buses (Bus::zip_terms, src/types.rs:14-24, 40-41),
  * summed per bus and kind: zip[6*i .. 6*i+5] = (P, Q) of bus i's constant-power, constant-current
  * and constant-impedance terms in per unit at |V| = 1.  Every later solve evaluates

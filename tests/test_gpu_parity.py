@@ -395,12 +395,15 @@ def test_mixed_fallback_reproduces_fp64_for_failing_scenarios(tile):
     assert np.abs(mixed[0][ok] - fp64[0][ok]).max() <= V_TOL
 
 
-def test_pair_tiles_equal_four_scenario_tiles():
+@pytest.mark.parametrize("groups", [16, 32])
+def test_pair_tiles_equal_four_scenario_tiles(groups):
     """Mixed precision at 8 scenarios per tile runs pair tiles: 4-lane groups with two scenarios
-    per lane, on the schedule of 4-scenario tiles.  Every scenario's arithmetic is the same, so the
-    results equal the 4-scenario tiles' bit for bit, including a scenario whose FP32 factor is
-    singular (its lane partner is unaffected) and the padding of a last, partial tile."""
-    _say("case: config-2 grid (2000 buses), 70 scenarios, mixed, tile 8 (pairs) vs tile 4")
+    per lane, on the schedule of 4-scenario tiles; with 32 refactor groups (quad tiles) the refactor
+    runs 2-lane groups with four scenarios per lane on the schedule of 2-scenario tiles.  Every
+    entry takes its updates in the same order, so the results equal the 4-scenario tiles' bit for
+    bit, including a scenario whose FP32 factor is singular (its lane partners are unaffected) and
+    the padding of a last, partial tile."""
+    _say(f"case: config-2 grid (2000 buses), 70 scenarios, mixed, tile 8 ({groups} refactor groups) vs tile 4")
     net = config_grid(2)
     p, q = scenario_specs(net, 70, seed=31)
     vm0 = np.repeat(net.vm[None], 70, 0)
@@ -410,6 +413,7 @@ def test_pair_tiles_equal_four_scenario_tiles():
     for tile in (8, 4):
         topo = Topology.from_network(net)
         topo.set_launch(tile, 0)
+        topo.set_refactor_groups(groups)
         got[tile] = topo.batch_solve(p, q, vm0, va0, tol=MIS_TOL, max_iter=20, mode=Mode.MIXED)
     for a, b in zip(got[8][:4], got[4][:4]):
         assert np.array_equal(a, b, equal_nan=True)

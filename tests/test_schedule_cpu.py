@@ -207,7 +207,9 @@ def test_forward_levels_equal_column_oriented(grid, tile, ring_rows):
     _check_pieces(items, G, n, int((np.diff(cp) - ulen - 1).sum()))
 
 
-NB = 16  # update rounds per chunk (kChunkRounds)
+def _chunk_rounds(G):
+    """Update rounds per chunk of the part round stream (part_chunk_rounds): 8 at 32 lane groups."""
+    return 8 if G >= 32 else 16
 
 
 def _parts(topo, tile):
@@ -263,6 +265,7 @@ def _emulate_parts(pa, pos, div, slots, a, cap, G):
     for b in buf:
         b[mone] = -1.0
     regs = G * PART_REGS
+    NB = _chunk_rounds(G)
     for p, (pb, ln, ch0, nr) in enumerate(pa):
         assert 0 < ln <= cap and nr % JOINT == 0
         xc, xprev = buf[p & 1], buf[(p & 1) ^ 1]
@@ -306,7 +309,7 @@ def _emulate_parts(pa, pos, div, slots, a, cap, G):
     return mem
 
 
-@pytest.mark.parametrize("grid,tile", [("c1", 2), ("c1", 4), ("c1", 8), ("c1", 16), ("c2", 4), ("g", 8)])
+@pytest.mark.parametrize("grid,tile", [("c1", 2), ("c1", 4), ("c1", 8), ("c1", 16), ("c2", 2), ("c2", 4), ("g", 8)])
 def test_refactor_parts_equal_column_by_column(grid, tile):
     net = {"c1": lambda: config_grid(1), "c2": lambda: config_grid(2), "g": lambda: generate_grid(700, 900, 5)}[grid]()
     topo = _factored(net)
