@@ -626,6 +626,21 @@ constexpr int stage_regs() {
 
 __device__ __forceinline__ int rl(int v, int lane) { return __builtin_amdgcn_readlane(v, lane); }
 
+// c ? a : b on values (a conditional between two lvalues of a struct type can become a select of
+// their addresses, which keeps both in scratch memory)
+template <typename T>
+__device__ __forceinline__ T pick(bool c, T a, T b) { return c ? a : b; }
+
+// f(std::integral_constant<int, k>{}) for k = 0 .. N-1, expanded at compile time
+template <typename F, int... K>
+__device__ __forceinline__ void static_for_seq(F& f, std::integer_sequence<int, K...>) {
+  (f(std::integral_constant<int, K>{}), ...);
+}
+template <int N, typename F>
+__device__ __forceinline__ void static_for(F&& f) {
+  static_for_seq(f, std::make_integer_sequence<int, N>{});
+}
+
 // Column metadata through the vector memory path (every lane reads the same 16 bytes, a few
 // columns ahead): waits on it are counted with the other vector loads.  A scalar-cache load would
 // share its wait counter with the LDS operations, so every LDS wait of the column would also wait
@@ -682,13 +697,13 @@ __device__ __forceinline__ void level_solve(const int2* __restrict__ bib, int g,
     int2 da[P], db[P];
     T u[P], y[P];
   };
-  auto dsc_issue = [&](int q, int2 (&d)[P]) {
+  auto dsc_issue = [&](int q, int2 (&d)[P]) __attribute__((always_inline)) {
     const int qq = q < nst ? q : nst - 1;
 #pragma unroll
     for (int i = 0; i < P; ++i)
       d[i] = *reinterpret_cast<const int2*>(reinterpret_cast<const char*>(bib) + (unsigned)((qq * P + i) * G + g) * 8u);
   };
-  auto val_issue = [&](const int2 (&d)[P], BSet& b) {
+  auto val_issue = [&](const int2 (&d)[P], BSet& b) __attribute__((always_inline)) {
 #pragma unroll
     for (int i = 0; i < P; ++i) {
       // (a START's payload is an LDS offset: clamped into the L+U range, the value is unused)
@@ -720,13 +735,13 @@ __device__ __forceinline__ void level_solve(const int2* __restrict__ bib, int g,
       xrow = start ? (unsigned)bb : xrow;
       // ENTRY: acc - U(k, c) x_c; END and PAD read the zero row (acc unchanged); START: y_k or zero;
       // COMBINE: acc + a partial
-      const T xv = bb < 0 ? lx[i] : b.y[i];
+      const T xv = pick(bb < 0, lx[i], b.y[i]);
       const T v = fnma(b.u[i], xv, acc);
       const T w = acc + xv;
-      acc = start ? xv : (kind == (unsigned)kBackCombine ? w : v);
+      acc = pick(start, xv, pick(kind == (unsigned)kBackCombine, w, v));
       const bool end = kind == (unsigned)kBackEnd;  // (at most one per lane group and step)
-      pend = end ? acc : pend;
-      pend_u = end ? b.u[i] : pend_u;
+      pend = pick(end, acc, pend);
+      pend_u = pick(end, b.u[i], pend_u);
       pend_y = end ? xrow : pend_y;
       pend_s = end ? xslot : pend_s;
       ended = ended || end;
@@ -742,19 +757,16 @@ __device__ __forceinline__ void level_solve(const int2* __restrict__ bib, int g,
     val_issue(nxt, b);
     dsc_issue(q + 2 * D, cur);
   };
+  // (every index into sets is a compile-time constant, so the sets live in registers whatever the
+  // size of the unrolled loop body)
   BSet sets[D];
-#pragma unroll
-  for (int k = 0; k < D; ++k) dsc_issue(k, sets[k].da);
-#pragma unroll
-  for (int k = 0; k < D; ++k) dsc_issue(D + k, sets[k].db);
-#pragma unroll
-  for (int k = 0; k < D; ++k) val_issue(sets[k].da, sets[k]);
+  static_for<D>([&](auto k) { dsc_issue(k, sets[k].da); });
+  static_for<D>([&](auto k) { dsc_issue(D + k, sets[k].db); });
+  static_for<D>([&](auto k) { val_issue(sets[k].da, sets[k]); });
   drain();  // (the loop's waits are counted from what the loop itself issues)
   for (int q = 0; q < nst; q += 2 * D) {
-#pragma unroll
-    for (int k = 0; k < D; ++k) run(q + k, sets[k], sets[k].da, sets[k].db);
-#pragma unroll
-    for (int k = 0; k < D; ++k) run(q + D + k, sets[k], sets[k].db, sets[k].da);
+    static_for<D>([&](auto k) { run(q + k, sets[k], sets[k].da, sets[k].db); });
+    static_for<D>([&](auto k) { run(q + D + k, sets[k], sets[k].db, sets[k].da); });
   }
 }
 
