@@ -142,6 +142,8 @@ struct DevTopo {
   const double* fb_g = nullptr;
   const double* fb_b = nullptr;
   const int* fb_yp = nullptr;
+  // the same four positions by Y-bus position (-1 where absent): the fill done by the mismatch kernel
+  const int4* y_lupos = nullptr;
   const int* fz_pos = nullptr;
   int n_fz = 0;
   // branches (per-scenario outages): Y-bus positions of the ff, ft, tf, tt stamps (-1: none) and
@@ -308,7 +310,9 @@ void launch_to_tiles(const double* src, double* dst, int n_scen, int n_tiles, in
                      hipStream_t stream);
 void launch_from_tiles(const double* src, double* dst, int n_scen, int count, int tile, hipStream_t stream);
 void launch_init(const DevTopo& t, const DevWork& w, hipStream_t stream);
-void launch_mismatch(const DevTopo& t, const DevWork& w, bool fp32, hipStream_t stream);
+// fill: also write the Jacobian values into the L+U storage (when t.y_lupos exists; else only the
+// mismatch, and launch_fill writes them)
+void launch_mismatch(const DevTopo& t, const DevWork& w, bool fp32, hipStream_t stream, bool fill = false);
 void launch_decide(const DevTopo& t, const DevWork& w, int iter, hipStream_t stream);
 void launch_fill(const DevTopo& t, const DevWork& w, bool fp32, hipStream_t stream);
 void launch_lu(const DevTopo& t, const DevWork& w, bool fp32, bool factor, int iter, hipStream_t stream);

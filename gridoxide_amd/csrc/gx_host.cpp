@@ -1242,9 +1242,9 @@ struct gx_topo {
   std::vector<char> br_bridge;
   std::vector<int> h_yp;       // per L+U position: the Y CSR position its recipe reads (-1 for fill)
   // recipes by bus pair (DevTopo::fb_*) and the fill positions
-  std::vector<int> h_fb_ptr, h_fb_k, h_fb_pos, h_fb_yp, h_fz;
+  std::vector<int> h_fb_ptr, h_fb_k, h_fb_pos, h_fb_yp, h_fz, h_ylu;
   std::vector<double> h_fb_g, h_fb_b;
-  DevMem d_fb_ptr, d_fb_k, d_fb_pos, d_fb_g, d_fb_b, d_fb_yp, d_fz;
+  DevMem d_fb_ptr, d_fb_k, d_fb_pos, d_fb_g, d_fb_b, d_fb_yp, d_fz, d_ylu;
   DevMem d_br_pos, d_br_y, d_yp;
   bool br_uploaded = false;
   // the last gx_batch_solve (for island reports)
@@ -1420,6 +1420,20 @@ int finalize_factor(gx_topo& t) {
     t.h_fb_g.swap(fg);
     t.h_fb_b.swap(fb);
     t.h_fb_yp.swap(fyp);
+    // the same positions by Y-bus position, for the mismatch kernel that fills while it walks the
+    // Y rows (empty, so the separate fill runs, if a pair has no Y position)
+    std::vector<int> ylu(4 * t.ycol.size(), -1);
+    bool whole = true;
+    for (size_t e = 0; e < t.h_fb_yp.size(); ++e) {
+      const int p = t.h_fb_yp[e];
+      if (p < 0 || (size_t)p >= t.ycol.size()) {
+        whole = false;
+        break;
+      }
+      for (int c = 0; c < 4; ++c) ylu[4 * (size_t)p + c] = t.h_fb_pos[4 * e + c];
+    }
+    if (!whole) ylu.clear();
+    t.h_ylu.swap(ylu);
     t.h_fz.swap(fz);
   }
   t.h_kind.swap(kind);
@@ -1452,7 +1466,8 @@ int upload_topo(gx_topo& t, int window, int tile, int part_tile = 0) {
       (rc = t.d_g.upload(t.h_g)) || (rc = t.d_b.upload(t.h_b)) || (rc = t.d_yp.upload(t.h_yp)) ||
       (rc = t.d_fb_ptr.upload(t.h_fb_ptr)) || (rc = t.d_fb_k.upload(t.h_fb_k)) ||
       (rc = t.d_fb_pos.upload(t.h_fb_pos)) || (rc = t.d_fb_g.upload(t.h_fb_g)) ||
-      (rc = t.d_fb_b.upload(t.h_fb_b)) || (rc = t.d_fb_yp.upload(t.h_fb_yp)) || (rc = t.d_fz.upload(t.h_fz)))
+      (rc = t.d_fb_b.upload(t.h_fb_b)) || (rc = t.d_fb_yp.upload(t.h_fb_yp)) || (rc = t.d_fz.upload(t.h_fz)) ||
+      (rc = t.d_ylu.upload(t.h_ylu)))
     return rc;
   gx::DevTopo& d = t.dt;
   d.fb_ptr = t.d_fb_ptr.as<int>();
@@ -1462,6 +1477,7 @@ int upload_topo(gx_topo& t, int window, int tile, int part_tile = 0) {
   d.fb_b = t.d_fb_b.as<double>();
   d.fb_yp = t.d_fb_yp.as<int>();
   d.fz_pos = t.d_fz.as<int>();
+  d.y_lupos = t.h_ylu.empty() ? nullptr : t.d_ylu.as<int4>();
   d.n_fz = (int)t.h_fz.size();
   d.lu_yp = t.d_yp.as<int>();
   d.rpos_p = t.d_rpos_p.as<int>();
@@ -2374,13 +2390,16 @@ int gx_batch_solve_outages(gx_topo* t, int n_scen, const double* p_spec, const d
   auto newton = [&](bool lu32) -> int {
     for (int it = 0; it < max_iter; ++it) {
       int rc2;
-      if ((rc2 = timed(GX_K_MISMATCH, [&] { gx::launch_mismatch(t->dt, w, lu32, st); }))) return rc2;
+      // (the mismatch kernel also writes the Jacobian when the topology has its Y-position map and
+      // the refactor stages structural fill as zero itself; else the fill launch also zeroes those)
+      const bool fused = t->dt.y_lupos != nullptr && t->dt.stage_zero;
+      if ((rc2 = timed(GX_K_MISMATCH, [&] { gx::launch_mismatch(t->dt, w, lu32, st, fused); }))) return rc2;
       gx::launch_decide(t->dt, w, it, st);
       // stop as soon as no scenario is left running (no empty launches after convergence)
       GX_HIP(hipMemcpyAsync(ws.h_running + it, w.running + it, sizeof(int), hipMemcpyDeviceToHost, st));
       GX_HIP(hipStreamSynchronize(st));
       if (ws.h_running[it] == 0) break;
-      if ((rc2 = timed(GX_K_FILL, [&] { gx::launch_fill(t->dt, w, lu32, st); }))) return rc2;
+      if (!fused && (rc2 = timed(GX_K_FILL, [&] { gx::launch_fill(t->dt, w, lu32, st); }))) return rc2;
       if ((rc2 = timed(GX_K_LU, [&] { gx::launch_lu(t->dt, w, lu32, true, it, st); }))) return rc2;
       if (lu32) {
         if ((rc2 = timed(GX_K_REFINE, [&] {

@@ -375,7 +375,39 @@ __global__ __launch_bounds__(256) void k_init(DevTopo t, DevWork w) {
   }
 }
 
-template <typename T, int TW>
+// The four Jacobian values of bus pair (i, k) from rectangular voltages (V_i, V_k, |V_k|) and
+// Y(i, k) = g + jb, stored at their L+U positions pos = {H, N, M, L} (-1: none) with stride V; the
+// diagonal pair adds the bus's calculated injection (pci, qci).  One definition for the fill kernel
+// and the mismatch kernel that fills while it walks the Y rows, with no contraction of products into
+// fused multiply-adds (which the compiler would otherwise choose per kernel), so both store the same
+// bits.
+template <typename T>
+__device__ __forceinline__ void jac_store(T* __restrict__ lu, int V, int4 pos, double vri, double vii, double vrk,
+                                          double vik, double vmk, double g, double b, bool diag, double pci,
+                                          double qci) {
+#pragma clang fp contract(off)
+  const double cc = vri * vrk + vii * vik;
+  const double ss = vii * vrk - vri * vik;
+  const double hl = g * ss - b * cc;
+  const double nm = g * cc + b * ss;
+  double vh = hl, vn = nm / vmk, vmm = -nm, vl = hl / vmk;
+  if (diag) {
+    vh += -qci;
+    vn += pci / vmk;
+    vmm += pci;
+    vl += qci / vmk;
+  }
+  if (pos.x >= 0) lu[(size_t)pos.x * V] = (T)vh;
+  if (pos.y >= 0) lu[(size_t)pos.y * V] = (T)vn;
+  if (pos.z >= 0) lu[(size_t)pos.z * V] = (T)vmm;
+  if (pos.w >= 0) lu[(size_t)pos.w * V] = (T)vl;
+}
+
+// kFill: also the Jacobian values of every bus pair of the row into the L+U storage (what k_fill_tile
+// writes, from the same Y values and voltages; the diagonal pair once the row's injection is known),
+// so the row's voltages and Y values are read once for both.  The values of a tile whose scenarios
+// all stop at this step are not used.
+template <typename T, int TW, bool kFill = false>
 __global__ __launch_bounds__(256) void k_mismatch(DevTopo t, DevWork w) {
   int bx, tile;
   xcd_rows(bx, tile);
@@ -385,25 +417,43 @@ __global__ __launch_bounds__(256) void k_mismatch(DevTopo t, DevWork w) {
   const size_t bb = at<TW>(tile, t.n_bus, 0) + s;
   const size_t ub = at<TW>(tile, t.n, 0) + s;
   T* __restrict__ y = reinterpret_cast<T*>(w.y);
+  T* __restrict__ lu = kFill ? reinterpret_cast<T*>(w.lu) + lu_at(0, t.nnz_lu, tile, TW) + s : nullptr;
+  constexpr int V = lu_block(TW) * TW;  // L+U values per position of a tile block
   const Outage og = load_outage(t, w, sc);
   double mx = 0.0;
   for (int i = elem_first<TW>(bx); i < t.n_bus; i += elem_stride<TW>()) {
     double ir = 0.0, ii = 0.0;
+    const size_t o = bb + (size_t)i * TW;
+    const double vri = w.vr[o], vii = w.vi[o];
+    double gd = 0.0, bd = 0.0;
+    int pd = -1;
     const int p2 = t.y_rowptr[i + 1];
     for (int p = t.y_rowptr[i]; p < p2; ++p) {
-      const size_t oj = bb + (size_t)t.y_col[p] * TW;
+      const int k = t.y_col[p];
+      const size_t oj = bb + (size_t)k * TW;
       double g, b;
       y_at(t, og, p, g, b);
       const double vr = w.vr[oj], vi = w.vi[oj];
       ir += g * vr - b * vi;
       ii += g * vi + b * vr;
+      if constexpr (kFill) {
+        if (k == i) {
+          gd = g;
+          bd = b;
+          pd = p;
+        } else {
+          jac_store(lu, V, t.y_lupos[p], vri, vii, vr, vi, w.vm[oj], g, b, false, 0.0, 0.0);
+        }
+      }
     }
-    const size_t o = bb + (size_t)i * TW;
-    const double vr = w.vr[o], vi = w.vi[o];
+    const double vr = vri, vi = vii;
     const double pcal = vr * ir + vi * ii;
     const double qcal = vi * ir - vr * ii;
     w.pc[o] = pcal;
     w.qc[o] = qcal;
+    if constexpr (kFill) {
+      if (pd >= 0) jac_store(lu, V, t.y_lupos[pd], vri, vii, vri, vii, w.vm[o], gd, bd, true, pcal, qcal);
+    }
     double pe = w.ps[o], qe = w.qs[o];
     effective_injection(t, i, w.vm[o], pe, qe);
     const int rp = t.rpos_p[i];
@@ -579,23 +629,7 @@ __global__ __launch_bounds__(256) void k_fill_tile(DevTopo t, DevWork w) {
       double g = t.fb_g[e], b = t.fb_b[e];
       if (w.outage) y_at(t, og, t.fb_yp[e], g, b);
       const size_t ok = bb + (size_t)k * TW;
-      const double vrk = w.vr[ok], vik = w.vi[ok], vmk = w.vm[ok];
-      const double cc = vri * vrk + vii * vik;
-      const double ss = vii * vrk - vri * vik;
-      const double hl = g * ss - b * cc;
-      const double nm = g * cc + b * ss;
-      const bool diag = k == i;
-      double vh = hl, vn = nm / vmk, vmm = -nm, vl = hl / vmk;
-      if (diag) {
-        vh += -qci;
-        vn += pci / vmk;
-        vmm += pci;
-        vl += qci / vmk;
-      }
-      if (pos.x >= 0) lu[(size_t)pos.x * V] = (T)vh;
-      if (pos.y >= 0) lu[(size_t)pos.y * V] = (T)vn;
-      if (pos.z >= 0) lu[(size_t)pos.z * V] = (T)vmm;
-      if (pos.w >= 0) lu[(size_t)pos.w * V] = (T)vl;
+      jac_store(lu, V, pos, vri, vii, w.vr[ok], w.vi[ok], w.vm[ok], g, b, k == i, pci, qci);
     }
   }
 }
@@ -2400,10 +2434,15 @@ void launch_init(const DevTopo& t, const DevWork& w, hipStream_t stream) {
   });
 }
 
-void launch_mismatch(const DevTopo& t, const DevWork& w, bool fp32, hipStream_t stream) {
+void launch_mismatch(const DevTopo& t, const DevWork& w, bool fp32, hipStream_t stream, bool fill) {
   with_tile(w.tile, [&](auto tw) {
     constexpr int TW = decltype(tw)::value;
     dim3 grid(elem_blocks(t.n_bus, TW), w.n_tiles);
+    if (fill && t.y_lupos) {
+      if (fp32) hipLaunchKernelGGL((k_mismatch<float, TW, true>), grid, dim3(256), 0, stream, t, w);
+      else hipLaunchKernelGGL((k_mismatch<double, TW, true>), grid, dim3(256), 0, stream, t, w);
+      return;
+    }
     if (fp32) hipLaunchKernelGGL((k_mismatch<float, TW>), grid, dim3(256), 0, stream, t, w);
     else hipLaunchKernelGGL((k_mismatch<double, TW>), grid, dim3(256), 0, stream, t, w);
   });
