@@ -2738,11 +2738,14 @@ int engine_tables(gx_engine& e) {
 
 // Launches of one Newton step of every active request (no host synchronization inside).
 void engine_step_launches(gx_engine& e) {
+  // (the mismatch kernel also writes the Jacobian where the topology allows it, as in a batch solve)
+  auto fused = [](const gx_request* r) { return r->t->dt.y_lupos != nullptr && r->t->dt.stage_zero; };
   for (gx_request* r : e.active) {
-    gx::launch_mismatch(r->t->dt, r->w, r->lu32, e.st);
+    gx::launch_mismatch(r->t->dt, r->w, r->lu32, e.st, fused(r));
     gx::launch_decide(r->t->dt, r->w, 0, e.st);
   }
-  for (gx_request* r : e.active) gx::launch_fill(r->t->dt, r->w, r->lu32, e.st);
+  for (gx_request* r : e.active)
+    if (!fused(r)) gx::launch_fill(r->t->dt, r->w, r->lu32, e.st);
   for (int k = 0; k < 2; ++k) {
     gx_engine::Launch& L = e.lk[k];
     gx::launch_lu_queue(e.tile, L.topos.as<gx::DevTopo>(), L.works.as<gx::DevWork>(), L.items.as<int2>(), L.n_items,
