@@ -1001,21 +1001,23 @@ int pick_ring_rows(int max_col, int tile, size_t elem, int requested) {
 bool valid_tile(int tile) { return tile == 2 || tile == 4 || tile == 8 || tile == 16 || tile == gx::kWideTile; }
 bool valid_ring(int rows) { return rows == 0 || (rows >= 64 && rows <= 4096 && (rows & (rows - 1)) == 0); }
 
-// Scenarios per tile when the caller leaves it open: 2-scenario tiles while their wavefronts fit
-// one per SIMD (1,024), else the narrowest of 4 and 8 whose wavefronts stay within 2,048 per launch,
-// else 16.  Measured (LU + fill per Newton step): with 1,024 scenarios on the 10,000-bus grid
+// Scenarios per tile when the caller leaves it open: 8-scenario pair tiles once they make 1,024
+// wavefronts (one per SIMD); else 2-scenario tiles while their wavefronts fit one per SIMD, else
+// the narrowest of 4 and 8 whose wavefronts stay within 2,048 per launch, else 16.  Measured (LU + fill per Newton step): with 1,024 scenarios on the 10,000-bus grid
 // 2-scenario tiles beat 4-scenario tiles (27.9 vs 36.4 ms LU at FP64); on the 2,000-bus grid with
 // 4,096 scenarios the 2,048 wavefronts of 2-scenario tiles lose (16.1 vs 10.3 ms); on the
 // 10,000-bus grid with 8,192 scenarios 4-scenario tiles (two wavefronts per SIMD, run one after the
 // other at the kernel's register count) beat 8-scenario tiles at FP64 (96 + 11 vs 137 + 8 ms) and
 // at FP32 (54 + 15 vs 76 + 5 ms).
-// Mixed precision with at least 1,024 tiles of 8 scenarios: pair tiles of 8 (two scenarios per
-// lane on the 4-scenario schedule, one wavefront per SIMD): 10,000-bus grid, 8,192 scenarios, LU
+// With at least 1,024 tiles of 8 scenarios, pair tiles of 8 (two scenarios per lane on the
+// 4-scenario schedule, one wavefront per SIMD), mixed precision: 10,000-bus grid, 8,192 scenarios, LU
 // 37.0 against 39.4 ms per launch, fill 4.1 against 6.2 ms, batch 895 against 993 ms.  Their
 // refactor runs as quad tiles (four scenarios per lane on the 2-scenario part schedule): 29.4
-// against 31.4 ms per launch.
+// against 31.4 ms per launch.  FP64 with 8,192 scenarios: pair tiles 60.5 ms per LU launch against
+// 62.0 for 4-scenario tiles and 82.9 for 8-scenario tiles of one scenario per lane.
 int auto_tile(int n_scen, bool fp32) {
-  if (fp32 && (n_scen + 7) / 8 >= 1024) return 8;
+  (void)fp32;
+  if ((n_scen + 7) / 8 >= 1024) return 8;
   if ((n_scen + 1) / 2 <= 1024) return 2;
   for (int tw : {4, 8}) if ((n_scen + tw - 1) / tw <= 2048) return tw;
   return 16;
@@ -1848,13 +1850,23 @@ int stage_batch(gx_topo& t, Workspace& ws, const BatchIn& in, int tile, gx::DevW
   // Pair tiles run the schedules of tiles half as wide (two scenarios per lane).
   // Quad tiles refactor on lane groups half as wide again (four scenarios per lane), when their
   // part schedule exists; else they run as pair tiles.
-  const int lt = pairs ? tile / 2 : tile;
-  const int rows = pick_ring_rows(lu_cap(t.sched.max_col, lt), lt, (pairs ? 2 : 1) * sizeof(double), t.ring_req);
+  int lt = pairs ? tile / 2 : tile;
+  int rows = pick_ring_rows(lu_cap(t.sched.max_col, lt), lt, (pairs ? 2 : 1) * sizeof(double), t.ring_req);
   if (rows < 0)
     return fail(GX_ERR_UNSUPPORTED, "longest L+U column exceeds the LDS staging limit of the device refactor");
-  const int window = gx::ring_window_of(rows, lt);
+  int window = gx::ring_window_of(rows, lt);
   if ((rc = upload_topo(t, window, lt, quads ? lt / 2 : lt))) return rc;
   if (quads && t.dt.part_groups != 2 * (gx::kLanes / lt) && (rc = upload_topo(t, window, lt, lt))) return rc;
+  if (pairs && !t.dt.part_a) {
+    // pair tiles need the refactor by parts; without a part schedule (a column past the parts'
+    // 10-bit offsets, or solves beyond the level schedule's encoding) the tile runs as tile lanes
+    lt = tile;
+    rows = pick_ring_rows(lu_cap(t.sched.max_col, lt), lt, sizeof(double), t.ring_req);
+    if (rows < 0)
+      return fail(GX_ERR_UNSUPPORTED, "longest L+U column exceeds the LDS staging limit of the device refactor");
+    window = gx::ring_window_of(rows, lt);
+    if ((rc = upload_topo(t, window, lt))) return rc;
+  }
   // FP32 values for the mixed mode (its FP64 re-solve of fallbacks grows the buffer first)
   if ((rc = ws.lu.reserve(lu_lanes(lanes, tile) * (size_t)t.dt.nnz_lu * (in.fp32 ? 4 : 8)))) return rc;
   w.lu = ws.lu.p;
@@ -2355,15 +2367,17 @@ int gx_batch_solve_outages(gx_topo* t, int n_scen, const double* p_spec, const d
   in.outage = any_outage ? &h_outage : nullptr;
   hipStream_t st = in.st;
   const int tile = t->tile_req ? t->tile_req : auto_tile(n_scen, in.fp32);
-  // mixed precision at 8 scenarios per tile: pair tiles (4-lane groups, two scenarios per lane),
-  // whose refactor runs as quad tiles (2-lane groups, four scenarios per lane) unless 16 refactor
-  // groups were asked for
-  const bool pairs = in.fp32 && tile == 8;
-  const bool quads = pairs && t->refactor_groups != 16;
+  // 8 scenarios per tile: pair tiles (4-lane groups, two scenarios per lane), whose FP32 refactor
+  // runs as quad tiles (2-lane groups, four scenarios per lane) unless 16 refactor groups were
+  // asked for
+  const bool pairs = tile == 8;
+  const bool quads = pairs && in.fp32 && t->refactor_groups != 16;
   gx::DevWork w;
   Workspace& ws = t->ws;
   HostResult hres;
   if ((rc = stage_batch(*t, ws, in, tile, w, &hres, pairs, quads))) return rc;
+  // (quad tiles in use: the FP64 re-solve below switches the refactor to the pair groups)
+  const bool quad_run = quads && t->dt.groups == gx::kLanes / (tile / 2) && t->dt.part_groups == 2 * t->dt.groups;
   if (hres.used) return host_results(*t, in, hres, vm, va, status, iters, history);
   // optional per-launch timing of each kernel class on this stream
   auto timed = [&](int cls, auto&& launch) -> int {
@@ -2429,7 +2443,7 @@ int gx_batch_solve_outages(gx_topo* t, int n_scen, const double* p_spec, const d
       if ((rc = ws.lu.reserve(lu_lanes((size_t)w.n_tiles * w.tile, w.tile) * (size_t)t->dt.nnz_lu * 8))) return rc;
       w.lu = ws.lu.p;
       // the FP64 re-solve of pair tiles refactors on the solves' lane groups
-      if (quads && (rc = upload_topo(*t, t->dt.window, tile / 2, tile / 2))) return rc;
+      if (quad_run && (rc = upload_topo(*t, t->dt.window, tile / 2, tile / 2))) return rc;
       if ((rc = newton(false))) return rc;
     }
   } else {
@@ -2699,6 +2713,14 @@ int engine_tables(gx_engine& e) {
       const bool fp32 = r->lu32;
       if ((k == 0 && fp32) || (k > 0 && !fp32)) continue;
       const int idx = (int)topos.size();
+      if (e.tile == 8 && r->t->dt.groups == gx::kLanes / 4) {
+        // pair tiles: the refactor parts of this launch's precision (quad tiles for FP32 where the
+        // 32-group parts exist; the topology's schedule caches both part sets)
+        int rc;
+        const int window = r->t->dt.window;
+        if ((rc = upload_topo(*r->t, window, 4, fp32 ? 2 : 4))) return rc;
+        if (fp32 && r->t->dt.part_groups != gx::kLanes / 2 && (rc = upload_topo(*r->t, window, 4, 4))) return rc;
+      }
       topos.push_back(r->t->dt);
       works.push_back(r->w);
       for (int tl = 0; tl < r->w.n_tiles; ++tl) items.push_back({r->cost, make_int2(idx, tl)});
@@ -2906,7 +2928,9 @@ int gx_engine_submit(gx_engine* e, gx_topo* t, int n_scen, const double* p_spec,
   int rc;
   if ((rc = read_outages(*t, n_scen, outage, in.dev, h_outage, any))) return rc;
   in.outage = any ? &h_outage : nullptr;
-  if ((rc = stage_batch(*t, r->ws, in, e->tile, r->w, nullptr))) return rc;
+  // (at 8 scenarios per tile, pair tiles with quad-tile FP32 refactors, as in gx_batch_solve)
+  const bool pairs = e->tile == 8;
+  if ((rc = stage_batch(*t, r->ws, in, e->tile, r->w, nullptr, pairs, pairs && in.fp32))) return rc;
   if ((rc = r->ws.iter_dev.reserve(sizeof(int)))) return rc;
   r->w.iter_dev = r->ws.iter_dev.as<int>();
   gx::launch_zero_ints(r->w.iter_dev, 1, e->st);

@@ -1968,8 +1968,28 @@ __global__ __launch_bounds__(64) void k_lu_queue(const DevTopo* __restrict__ top
     if constexpr (TW == kWideTile) {
       lu64_tile<T, kFactor>(t, w, item.y, cap, iter);
     } else {
-      if (t.part_a && t.part_groups == kLanes / TW) lu_tile_parts<T, TW, kFactor>(t, w, item.y, cap, iter);
-      else lu_tile<T, TW, kFactor>(t, w, item.y, cap, iter);
+      bool done = false;
+      if constexpr (TW == 8 && (std::is_same<T, float>::value || std::is_same<T, double>::value)) {
+        // 8-scenario tiles staged as pair tiles (4-lane groups, two scenarios per lane), FP32
+        // refactors as quad tiles where the 32-group parts exist, as in run_lu
+        using P = pair2<T>;
+        if (t.part_a && t.groups == kLanes / 4) {
+          if constexpr (std::is_same<T, float>::value) {
+            if (t.part_groups == kLanes / 2) {
+              lu_tile_parts<P, 4, kFactor, pair2<P>, 2>(t, w, item.y, cap, iter);
+              done = true;
+            }
+          }
+          if (!done && t.part_groups == kLanes / 4) {
+            lu_tile_parts<P, 4, kFactor>(t, w, item.y, cap, iter);
+            done = true;
+          }
+        }
+      }
+      if (!done) {
+        if (t.part_a && t.part_groups == kLanes / TW) lu_tile_parts<T, TW, kFactor>(t, w, item.y, cap, iter);
+        else lu_tile<T, TW, kFactor>(t, w, item.y, cap, iter);
+      }
     }
     __syncthreads();  // the next item reuses the LDS
   }
@@ -2510,6 +2530,11 @@ void launch_zero_ints(int* p, int n, hipStream_t stream) {
 }
 
 size_t lu_shmem_tile(const DevTopo& t, bool fp32, bool factor, int tile) {
+  if (tile == 8 && t.part_a && t.groups == kLanes / 4) {  // pair tiles (and quad-tile refactors)
+    if (fp32 && t.part_groups == kLanes / 2)
+      return lu_quad_lds_bytes(t.max_col, t.window + kRingGap + 1, 4, sizeof(f2), 2, sizeof(f4), factor);
+    return fp32 ? lu_bytes<f2>(t, 4, factor) : lu_bytes<d2>(t, 4, factor);
+  }
   return fp32 ? lu_bytes<float>(t, tile, factor) : lu_bytes<double>(t, tile, factor);
 }
 

@@ -8,7 +8,8 @@ from typing import Iterable, List, Sequence, Tuple
 from .solver import Engine, Mode, Topology
 
 # 2-scenario tiles are left out: they pay off only while a launch has fewer tiles than SIMDs, which
-# the multi-request engine's launches exceed
+# the multi-request engine's launches exceed.  Tile 8 runs pair tiles (two scenarios per lane) whose
+# FP32 refactor runs as quad tiles (four per lane).
 DEFAULT_CANDIDATES = ((4, 0), (8, 0), (4, 256), (8, 1024))
 
 

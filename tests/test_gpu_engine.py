@@ -13,7 +13,7 @@ def _say(msg):
     print(msg, flush=True)
 
 
-def _requests():
+def _requests(tile=4):
     """Three topologies of different sizes; one with N-1 scenarios."""
     out = []
     for k, (nb, S, seed) in enumerate(((200, 40, 1), (500, 24, 2), (120, 70, 3))):
@@ -22,7 +22,7 @@ def _requests():
         vm0 = np.repeat(net.vm[None], S, 0)
         va0 = np.repeat(net.va[None], S, 0)
         topo = Topology.from_network(net)
-        topo.set_launch(4, 0)
+        topo.set_launch(tile, 0)
         outage = None
         if k == 0:
             n_br = net.line_from.size + net.tr_from.size
@@ -38,15 +38,17 @@ def _same(a, b):
         assert np.array_equal(x, y)
 
 
+@pytest.mark.parametrize("tile", [4, 8])
 @pytest.mark.parametrize("graphs", [False, True])
 @pytest.mark.parametrize("mode", [Mode.FP64, Mode.MIXED])
-def test_engine_equals_batch_solve(mode, graphs):
-    _say(f"case: engine, 3 topologies, mode={mode.name}, graphs={graphs}")
-    reqs = _requests()
+def test_engine_equals_batch_solve(mode, graphs, tile):
+    """Tile 8 runs pair tiles (and quad-tile FP32 refactors) in both paths."""
+    _say(f"case: engine, 3 topologies, mode={mode.name}, graphs={graphs}, tile={tile}")
+    reqs = _requests(tile)
     # the one-batch path first (it also fixes each topology's pivot order)
     ref = [topo.batch_solve(p, q, vm0, va0, tol=1e-8, max_iter=20, mode=mode, outage=o)
            for _, topo, p, q, vm0, va0, o in reqs]
-    eng = Engine(tile=4, graphs=graphs)
+    eng = Engine(tile=tile, graphs=graphs)
     ids = [eng.submit(topo, p, q, vm0, va0, tol=1e-8, max_iter=20, mode=mode, outage=o)
            for _, topo, p, q, vm0, va0, o in reqs]
     assert eng.run() == 0
