@@ -178,8 +178,9 @@ int gx_topo_profile(gx_topo* t, int enable, double* ms, int* launches);
 
 /* Launch shape of later gx_batch_solve calls on t (no counterpart in the reference, whose CPU
  * workers have no such knob; this is the setting a tuner sweeps).  tile: scenarios per
- * wavefront tile, 2, 4, 8, 16 (0 = automatic: 2 while the tiles fit one wavefront per SIMD, else
- * the narrowest tile whose wavefronts can all be resident at once), or 64 (one lane per scenario).
+ * wavefront tile, 2, 4, 8, 16 (0 = automatic: 8 from 8,192 scenarios, else 2 while the tiles fit
+ * one wavefront per SIMD, else the narrowest tile whose wavefronts can all be resident at once), or
+ * 64 (one lane per scenario).  8-scenario tiles run two scenarios per lane (pair tiles).
  * ring_rows: rows of the LDS window that holds the right-hand side during the triangular solves,
  * a power of two in [64, 4096] (0 = the default for the tile); it is halved while the kernel's
  * LDS would exceed 160 KB.  Results do not depend on either setting beyond rounding of the
@@ -271,7 +272,7 @@ int gx_sys_ordering(const gx_sys* s, int* p, int* q, int* r, int* pnum, int* nbl
  * admitted between steps and leave when finished; results per request equal gx_batch_solve's
  * for the same inputs, tile width and pivot order. */
 typedef struct gx_engine gx_engine;
-/* tile: 2, 4, 8, 16 or 64 (0: 4).  hbm_budget: bytes of request workspaces the engine may hold (0:
+/* tile: 2, 4, 8 (pair tiles, as gx_batch_solve's), 16 or 64 (0: 4).  hbm_budget: bytes of request workspaces the engine may hold (0:
  * unlimited).  stream: HIP stream of all its work (NULL: its own).  NULL on error (*err set). */
 gx_engine* gx_engine_create(int tile, int flags, long long hbm_budget, void* stream, int* err);
 void gx_engine_destroy(gx_engine* e);
