@@ -293,10 +293,10 @@ std::vector<int> amd_order(const CscPattern& a) {
     // the new element: the members of the pivot's elements (each absorbed into it), then the
     // pivot's own variables
     lme.clear();
-    long long degme = 0;
+    long long new_elem_degree = 0;
     auto take = [&](int v) {
       if (weight[v] > 0 && !in_front[v]) {
-        degme += weight[v];
+        new_elem_degree += weight[v];
         in_front[v] = 1;
         lme.push_back(v);
         buckets.remove(v);
@@ -354,7 +354,7 @@ std::vector<int> amd_order(const CscPattern& a) {
       vl.resize(kv);
       if (ke == 0 && kv == 0) {  // only the new element is left: eliminated with the pivot
         parent[v] = me;
-        degme -= weight[v];
+        new_elem_degree -= weight[v];
         npiv += weight[v];
         eliminated += weight[v];
         weight[v] = 0;
@@ -373,7 +373,7 @@ std::vector<int> amd_order(const CscPattern& a) {
       hash_of[v] = (int)(h % (unsigned long long)n);
       hash_bucket[hash_of[v]].push_back(v);
     }
-    degree[me] = (int)degme;
+    degree[me] = (int)new_elem_degree;
 
     // supervariables: members with the same elements and variables (besides the new element) merge
     // into the member hashed last among them; buckets are searched in member order
@@ -416,7 +416,7 @@ std::vector<int> amd_order(const CscPattern& a) {
       in_front[v] = 0;
       const int w = weight[v];
       if (w == 0) continue;
-      const int d = (int)std::min<long long>((long long)degree[v] + degme - w, (long long)left - w);
+      const int d = (int)std::min<long long>((long long)degree[v] + new_elem_degree - w, (long long)left - w);
       buckets.add(v, d);
       mindeg = std::min(mindeg, d);
       degree[v] = d;
@@ -424,7 +424,7 @@ std::vector<int> amd_order(const CscPattern& a) {
     }
     in_front[me] = 0;
     weight[me] = npiv;
-    front[me] = npiv + (int)degme;
+    front[me] = npiv + (int)new_elem_degree;
     alive[me] = mem.empty() ? 0 : 1;
     if (mem.empty()) parent[me] = -1;
   }
