@@ -1852,6 +1852,12 @@ int stage_batch(gx_topo& t, Workspace& ws, const BatchIn& in, int tile, gx::DevW
   // part schedule exists; else they run as pair tiles.
   int lt = pairs ? tile / 2 : tile;
   int rows = pick_ring_rows(lu_cap(t.sched.max_col, lt), lt, (pairs ? 2 : 1) * sizeof(double), t.ring_req);
+  if (pairs && rows < 0) {
+    // two values per lane do not fit the LDS with this topology's longest column: one per lane
+    pairs = quads = false;
+    lt = tile;
+    rows = pick_ring_rows(lu_cap(t.sched.max_col, lt), lt, sizeof(double), t.ring_req);
+  }
   if (rows < 0)
     return fail(GX_ERR_UNSUPPORTED, "longest L+U column exceeds the LDS staging limit of the device refactor");
   int window = gx::ring_window_of(rows, lt);
