@@ -165,6 +165,16 @@ def test_set_launch_validates():
         assert e.value.code == _lib.GX_ERR_ARG
 
 
+def test_set_refactor_groups_validates():
+    topo = Topology.from_network(config_grid(1))
+    for ok in (0, 16, 32):
+        topo.set_refactor_groups(ok)
+    for bad in (-1, 8, 24, 64):
+        with pytest.raises(_lib.GxError) as e:
+            topo.set_refactor_groups(bad)
+        assert e.value.code == _lib.GX_ERR_ARG
+
+
 def _connected_without(net, e):
     """Brute force: is the branch graph still one component with branch e removed?"""
     import scipy.sparse as sp
