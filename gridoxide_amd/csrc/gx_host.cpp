@@ -1002,7 +1002,7 @@ bool valid_tile(int tile) { return tile == 2 || tile == 4 || tile == 8 || tile =
 bool valid_ring(int rows) { return rows == 0 || (rows >= 64 && rows <= 4096 && (rows & (rows - 1)) == 0); }
 
 // Scenarios per tile when the caller leaves it open: 8-scenario pair tiles once they make 1,024
-// wavefronts (one per SIMD); else 2-scenario tiles while their wavefronts fit one per SIMD, else
+// wavefronts (one per SIMD; 512 in mixed precision); else 2-scenario tiles while their wavefronts fit one per SIMD, else
 // the narrowest of 4 and 8 whose wavefronts stay within 2,048 per launch, else 16.  Measured (LU + fill per Newton step): with 1,024 scenarios on the 10,000-bus grid
 // 2-scenario tiles beat 4-scenario tiles (27.9 vs 36.4 ms LU at FP64); on the 2,000-bus grid with
 // 4,096 scenarios the 2,048 wavefronts of 2-scenario tiles lose (16.1 vs 10.3 ms); on the
@@ -1014,10 +1014,11 @@ bool valid_ring(int rows) { return rows == 0 || (rows >= 64 && rows <= 4096 && (
 // 37.0 against 39.4 ms per launch, fill 4.1 against 6.2 ms, batch 895 against 993 ms.  Their
 // refactor runs as quad tiles (four scenarios per lane on the 2-scenario part schedule): 29.4
 // against 31.4 ms per launch.  FP64 with 8,192 scenarios: pair tiles 60.5 ms per LU launch against
-// 62.0 for 4-scenario tiles and 82.9 for 8-scenario tiles of one scenario per lane.
+// 62.0 for 4-scenario tiles and 82.9 for 8-scenario tiles of one scenario per lane.  Mixed precision
+// takes quad tiles from 512 of them, although half the SIMDs then idle: 2,000-bus grid, 4,096
+// scenarios, 150.0 against 162.5 ms per batch (FP64 pair tiles there: 159.8 against 154.1 ms).
 int auto_tile(int n_scen, bool fp32) {
-  (void)fp32;
-  if ((n_scen + 7) / 8 >= 1024) return 8;
+  if ((n_scen + 7) / 8 >= (fp32 ? 512 : 1024)) return 8;
   if ((n_scen + 1) / 2 <= 1024) return 2;
   for (int tw : {4, 8}) if ((n_scen + tw - 1) / tw <= 2048) return tw;
   return 16;

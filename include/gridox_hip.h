@@ -178,9 +178,10 @@ int gx_topo_profile(gx_topo* t, int enable, double* ms, int* launches);
 
 /* Launch shape of later gx_batch_solve calls on t (no counterpart in the reference, whose CPU
  * workers have no such knob; this is the setting a tuner sweeps).  tile: scenarios per
- * wavefront tile, 2, 4, 8, 16 (0 = automatic: 8 from 8,192 scenarios, else 2 while the tiles fit
- * one wavefront per SIMD, else the narrowest tile whose wavefronts can all be resident at once), or
- * 64 (one lane per scenario).  8-scenario tiles run two scenarios per lane (pair tiles).
+ * wavefront tile, 2, 4, 8, 16 (0 = automatic: 8 from 8,192 scenarios, 4,096 in mixed precision,
+ * else 2 while the tiles fit one wavefront per SIMD, else the narrowest tile whose wavefronts can
+ * all be resident at once), or 64 (one lane per scenario).  8-scenario tiles run two scenarios per
+ * lane (pair tiles).
  * ring_rows: rows of the LDS window that holds the right-hand side during the triangular solves,
  * a power of two in [64, 4096] (0 = the default for the tile); it is halved while the kernel's
  * LDS would exceed 160 KB.  Results do not depend on either setting beyond rounding of the
