@@ -390,12 +390,16 @@ __device__ __forceinline__ void jac_store(T* __restrict__ lu, int V, int4 pos, d
   const double ss = vii * vrk - vri * vik;
   const double hl = g * ss - b * cc;
   const double nm = g * cc + b * ss;
-  double vh = hl, vn = nm / vmk, vmm = -nm, vl = hl / vmk;
+  // one FP64 division per pair (the mismatch kernel's Jacobian part is paced partly by its FP64
+  // arithmetic); the products stay in FP64: in FP32 the cancellation in ss costs the FP32 Newton
+  // steps enough accuracy that many more scenarios stall
+  const double rk = 1.0 / vmk;
+  double vh = hl, vn = nm * rk, vmm = -nm, vl = hl * rk;
   if (diag) {
     vh += -qci;
-    vn += pci / vmk;
+    vn += pci * rk;
     vmm += pci;
-    vl += qci / vmk;
+    vl += qci * rk;
   }
   if (pos.x >= 0) lu[(size_t)pos.x * V] = (T)vh;
   if (pos.y >= 0) lu[(size_t)pos.y * V] = (T)vn;
