@@ -227,6 +227,9 @@ struct PartPlan {
   int cap = 0;                // entries of the largest part
 };
 
+constexpr bool kPartnerVisits = true;  // split visits of partner columns (below)
+constexpr long kPartnerMinShared = 64;
+
 int build_parts(const gx::LuPattern& lu, const LuSchedule& s, int G, int pcap, const std::vector<int>* kind,
                 PartPlan& out) {
   const int n = lu.n, NB = gx::part_chunk_rounds(G), J = gx::kJointRounds * G;
@@ -246,20 +249,62 @@ int build_parts(const gx::LuPattern& lu, const LuSchedule& s, int G, int pcap, c
   }
   std::vector<std::vector<int>> bucket(nlev);
   for (int k = 0; k < n; ++k) bucket[lev[k]].push_back(k);
-  std::vector<int> part_of(n, -1), base(n, 0);
-  std::vector<std::vector<int>> parts;
+  // Partner visits: column c = k + 1, one of whose U entries is k, takes the updates from the
+  // leading U entries whose columns lie at levels below k's (final before k's part runs) in k's part
+  // (a partial visit), each next to k's update from the same L value where k has that U entry too,
+  // so the second load finds the value in the cache; its entries are written back undivided.  Its
+  // own part stages them again -- from the previous part's buffer when that is k's part (values
+  // loaded ahead of the part would predate the write-back), else from memory -- and applies the
+  // remaining updates and the division.  Every entry keeps the sequential order of its updates.
+  // Only pairs whose shared sources carry kPartnerMinShared update products or more are split
+  // (the others would add part entries for little), and only on 32 lane groups, whose rounds the
+  // partner's items fill (10,000-bus grid: 110,200 -> 96,607 rounds); on 16 the rounds are full
+  // already, and the extra parts and staging items cost more than the cached loads save.
+  std::vector<char> split(n, 0);
+  std::vector<int> pre_u(n, 0);  // U entries of a split column taken by its partial visit
+  if (kPartnerVisits && G >= 32)
+    for (int c = 1; c < n; ++c) {
+      const int k = c - 1, ul = lu.ulen[c];
+      const int* cb = lu.row.data() + lu.cp[c];
+      if (split[k] || !std::binary_search(cb, cb + ul, k)) continue;
+      const int* kb = lu.row.data() + lu.cp[k];
+      const int* ke = kb + lu.ulen[k];
+      int m = 0;
+      long shared = 0;
+      for (; m < ul && lev[cb[m]] < lev[k]; ++m)
+        if (std::binary_search(kb, ke, cb[m])) shared += lu.cp[cb[m] + 1] - (lu.cp[cb[m]] + lu.ulen[cb[m]] + 1);
+      if (m > 0 && shared >= kPartnerMinShared) {
+        split[c] = 1;
+        pre_u[c] = m;
+      }
+    }
+  struct Visit {
+    int col, base, u0, u1, mode;  // U entries [u0, u1) of column col; mode 0 whole, 1 partial, 2 rest
+  };
+  std::vector<int> part_of(n, -1), base(n, 0), pre_part(n, -1), pre_base(n, 0);
+  std::vector<std::vector<Visit>> parts;
   for (int l = 0; l < nlev; ++l) {
     int fill = pcap + 1;
     for (int k : bucket[l]) {
-      const int len = lu.cp[k + 1] - lu.cp[k];
-      if (fill + len > pcap) {
+      const int len = lu.cp[k + 1] - lu.cp[k], ul = lu.ulen[k], c = k + 1;
+      const int clen = c < n ? lu.cp[c + 1] - lu.cp[c] : 0;
+      if (c < n && split[c] && len + clen > pcap) split[c] = 0;  // the pair does not fit a part
+      const bool pair = c < n && split[c];
+      if (fill + len + (pair ? clen : 0) > pcap) {
         parts.emplace_back();
         fill = 0;
       }
-      parts.back().push_back(k);
-      part_of[k] = (int)parts.size() - 1;
+      const int pi = (int)parts.size() - 1;
+      parts.back().push_back({k, fill, split[k] ? pre_u[k] : 0, ul, split[k] ? 2 : 0});
+      part_of[k] = pi;
       base[k] = fill;
       fill += len;
+      if (pair) {
+        parts.back().push_back({c, fill, 0, pre_u[c], 1});
+        pre_part[c] = pi;
+        pre_base[c] = fill;
+        fill += clen;
+      }
     }
   }
   out.n_parts = (int)parts.size();
@@ -269,14 +314,16 @@ int build_parts(const gx::LuPattern& lu, const LuSchedule& s, int G, int pcap, c
   std::vector<std::vector<int>> rt, rs;
   int chunk = 0;
   for (int pi = 0; pi < out.n_parts; ++pi) {
-    const std::vector<int>& cols = parts[pi];
+    const std::vector<Visit>& vis = parts[pi];
     const int pbase = (int)out.pent_pos.size();
-    for (int k : cols) {
-      const int c0 = lu.cp[k], len = lu.cp[k + 1] - c0, ul = lu.ulen[k];
+    for (const Visit& v : vis) {
+      const int k = v.col, c0 = lu.cp[k], len = lu.cp[k + 1] - c0, ul = lu.ulen[k];
+      // the rest of a split column restaged from the previous part's buffer: no value from memory
+      const bool from_prev = v.mode == 2 && pre_part[k] == pi - 1;
       for (int o = 0; o < len; ++o) {
-        const bool zero = kind && (*kind)[c0 + o] == gx::kFill;
+        const bool zero = v.mode == 2 ? from_prev : kind && (*kind)[c0 + o] == gx::kFill;
         out.pent_pos.push_back((c0 + o) | (zero ? gx::kStageZero : 0));
-        out.pent_div.push_back(o < ul ? -1 : (o == ul ? -2 : base[k] + ul));
+        out.pent_div.push_back(v.mode == 1 || o < ul ? -1 : (o == ul ? -2 : v.base + ul));
       }
     }
     const int Lp = (int)out.pent_pos.size() - pbase;
@@ -297,48 +344,79 @@ int build_parts(const gx::LuPattern& lu, const LuSchedule& s, int G, int pcap, c
       rs[R].clear();
       ++R;
     };
-    // staging items: entries past the registered ones that are not structural fill (the kernel
-    // clears those rows), J per joint round from the first; x_o = 0 - value * (-1)
+    // staging items, J per joint round from the first; x_o = 0 - value * (-1): entries past the
+    // registered ones that are not structural fill (the kernel clears those rows), then the
+    // entries restaged from the previous part's buffer
     int staged = 0;
-    for (int o = G * gx::kPartRegs; o < Lp; ++o) {
-      const int ps = out.pent_pos[pbase + o];
-      if (ps & gx::kStageZero) continue;
+    auto stage = [&](int o, int x, int w) {
       const int r = staged / J, slot = staged % J;
       ++staged;
       while (r >= R) add_round();
-      sx[(size_t)r * J + slot] = ps;
-      sw[(size_t)r * J + slot] = o | (mone << 10);
+      sx[(size_t)r * J + slot] = x;
+      sw[(size_t)r * J + slot] = w;
       rt[r].push_back(o);
       rs[r].push_back(mone);
       last[o] = r;
+    };
+    for (int o = G * gx::kPartRegs; o < Lp; ++o) {
+      const int ps = out.pent_pos[pbase + o];
+      if (!(ps & gx::kStageZero)) stage(o, ps, o | (mone << 10));
     }
-    for (int k : cols) {
-      const int c0 = lu.cp[k], ul = lu.ulen[k];
-      for (int u = 0; u < ul; ++u) {
-        const int j = lu.row[c0 + u], l0 = lu.cp[j] + lu.ulen[j] + 1, ll = lu.cp[j + 1] - l0;
-        const int q0 = s.upd_ptr[c0 + u];
-        const bool prev = part_of[j] == pi - 1;
-        const int su = base[k] + u;
-        for (int e = 0; e < ll; ++e) {
-          const int p = l0 + e, tg = base[k] + s.upd_off[q0 + e];
-          for (int r = std::max(last[tg], last[su]) + 1;; ++r) {
-            while (r >= R) add_round();
-            int slot = -1;
-            for (int gg = 0; gg < J && slot < 0; ++gg)
-              if (sx[(size_t)r * J + gg] == -1) slot = gg;
-            if (slot < 0) continue;
-            if (std::find(rt[r].begin(), rt[r].end(), tg) != rt[r].end()) continue;
-            if (std::find(rs[r].begin(), rs[r].end(), tg) != rs[r].end()) continue;
-            sx[(size_t)r * J + slot] = prev ? 0 : p;
-            sw[(size_t)r * J + slot] =
-                tg | (su << 10) | (prev ? (int)(0x80000000u | ((unsigned)(base[j] + (p - lu.cp[j])) << 20)) : 0);
-            rt[r].push_back(tg);
-            rs[r].push_back(su);
-            last[tg] = r;
-            break;
-          }
-        }
+    for (const Visit& v : vis)
+      if (v.mode == 2 && pre_part[v.col] == pi - 1)
+        for (int o = 0, len = lu.cp[v.col + 1] - lu.cp[v.col]; o < len; ++o)
+          stage(v.base + o, 0, (v.base + o) | (mone << 10) | (int)(0x80000000u | ((unsigned)(pre_base[v.col] + o) << 20)));
+    // update items, each into the earliest round after its target's previous update and its
+    // source's last one where no other item targets or reads its target
+    auto place = [&](int tg, int su, int x, int w) {
+      for (int r = std::max(last[tg], last[su]) + 1;; ++r) {
+        while (r >= R) add_round();
+        int slot = -1;
+        for (int gg = 0; gg < J && slot < 0; ++gg)
+          if (sx[(size_t)r * J + gg] == -1) slot = gg;
+        if (slot < 0) continue;
+        if (std::find(rt[r].begin(), rt[r].end(), tg) != rt[r].end()) continue;
+        if (std::find(rs[r].begin(), rs[r].end(), tg) != rs[r].end()) continue;
+        sx[(size_t)r * J + slot] = x;
+        sw[(size_t)r * J + slot] = w;
+        rt[r].push_back(tg);
+        rs[r].push_back(su);
+        last[tg] = r;
+        return;
       }
+    };
+    // the update of visit v's target e from its U entry u (L(i, j) x_u, j that entry's row)
+    auto emit = [&](const Visit& v, int u, int e) {
+      const int c0 = lu.cp[v.col], j = lu.row[c0 + u], p = lu.cp[j] + lu.ulen[j] + 1 + e;
+      const int tg = v.base + s.upd_off[s.upd_ptr[c0 + u] + e], su = v.base + u;
+      const bool prev = part_of[j] == pi - 1;
+      place(tg, su, prev ? 0 : p,
+            tg | (su << 10) | (prev ? (int)(0x80000000u | ((unsigned)(base[j] + (p - lu.cp[j])) << 20)) : 0));
+    };
+    for (size_t vi = 0; vi < vis.size(); ++vi) {
+      const Visit& v = vis[vi];
+      if (v.mode == 1) continue;  // emitted with the visit before it, its partner
+      const Visit* pv = vi + 1 < vis.size() && vis[vi + 1].mode == 1 ? &vis[vi + 1] : nullptr;
+      const int c0 = lu.cp[v.col];
+      const int* prow = pv ? lu.row.data() + lu.cp[pv->col] : nullptr;
+      auto lcount = [&](int j) { return lu.cp[j + 1] - (lu.cp[j] + lu.ulen[j] + 1); };
+      int pu = 0;
+      // the partner's updates from U entries in rows below `below`, in order
+      auto partner_to = [&](int below) {
+        for (; pv && pu < pv->u1 && prow[pu] < below; ++pu)
+          for (int e = 0, ll = lcount(prow[pu]); e < ll; ++e) emit(*pv, pu, e);
+      };
+      for (int u = v.u0; u < v.u1; ++u) {
+        const int j = lu.row[c0 + u], ll = lcount(j);
+        partner_to(j);
+        const bool both = pv && pu < pv->u1 && prow[pu] == j;
+        for (int e = 0; e < ll; ++e) {
+          emit(v, u, e);
+          if (both) emit(*pv, pu, e);
+        }
+        pu += both;
+      }
+      partner_to(n);
     }
     R *= gx::kJointRounds;  // rounds of G items: joint round r is rounds kJointRounds r + i
     const int nch = std::max(gx::kPartMinChunks, (R + NB - 1) / NB);

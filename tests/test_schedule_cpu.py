@@ -253,13 +253,21 @@ def _sequential_refactor(cp, ulen, row, a):
 
 def _emulate_parts(pa, pos, div, slots, a, cap, G):
     """The part loop of the kernel for one scenario.  The first G * kPartRegs entries of a part are
-    staged before its rounds, the others start at zero and take their values from staging items
-    (x_o -= value * x_m, row m = cap + 1 holding -1).  The rounds run JOINT at a time, every read
-    of the group before its writes; a chunk that holds any of the part's rounds runs all NB of them.  The values of chunk c of part p are loaded kPartMinChunks = 2
-    chunks ahead: after the write-back of part p - 2 (c < 2) or p - 1 (c >= 2); a position not
-    written back by then still holds its Jacobian value.  The rows of the part buffers start as NaN."""
-    mem = a.copy()
-    owner = np.full(a.size, np.iinfo(np.int64).max)  # the part that wrote each position back
+    staged from values loaded a part ahead (after the write-back of part p - 2), or as zero where
+    flagged; the others start at zero and take their values from staging items (x_o -= value * x_m,
+    row m = cap + 1 holding -1).  The rounds run JOINT at a time, every read of the group before its
+    writes; a chunk that holds any of the part's rounds runs all NB of them.  The values of chunk c
+    of part p are loaded kPartMinChunks = 2 chunks ahead: after the write-back of part p - 2 (c < 2)
+    or p - 1 (c >= 2).  A load sees the last write-back of its position by then, else the Jacobian
+    value.  A position is written back at most twice (a partner column's partial visit, undivided,
+    then its own).  The rows of the part buffers start as NaN."""
+    big = np.iinfo(np.int64).max
+    w1p, w2p = np.full(a.size, big), np.full(a.size, big)  # the parts that wrote each position back
+    w1v, w2v = np.full(a.size, np.nan), np.full(a.size, np.nan)
+
+    def load(x, seen):
+        return np.where(w2p[x] <= seen, w2v[x], np.where(w1p[x] <= seen, w1v[x], a[x]))
+
     buf = [np.full(cap + 2, np.nan), np.full(cap + 2, np.nan)]
     mone = cap + 1
     for b in buf:
@@ -273,9 +281,9 @@ def _emulate_parts(pa, pos, div, slots, a, cap, G):
         ent = np.arange(pb, pb + ln)
         zero = (pos[ent] & STAGE_ZERO) != 0
         epos = pos[ent] & ~STAGE_ZERO
-        assert (owner[epos] == np.iinfo(np.int64).max).all()
+        assert (w2p[epos] == big).all()  # a third write-back of a position would be a schedule error
         k = min(ln, regs)
-        xc[:k] = np.where(zero[:k], 0.0, mem[epos[:k]])
+        xc[:k] = np.where(zero[:k], 0.0, load(epos[:k], p - 2))
         xc[k:ln] = 0.0
         nch = max(MIN_CHUNKS, -(-nr // NB))
         for r0 in range(0, nch * NB, JOINT):
@@ -293,8 +301,9 @@ def _emulate_parts(pa, pos, div, slots, a, cap, G):
             assert ((su[real] < ln) | (su[real] == mone)).all()
             assert np.unique(tg[real]).size == np.count_nonzero(real)
             assert not np.isin(tg[real], su[real]).any()
-            assert (tg[real & (su == mone)] >= regs).all()
-            lv = np.where(owner[x] <= seen, mem[x], a[x])
+            staging = real & (su == mone)  # registered rows are staged only when flagged as zero
+            assert (zero[np.minimum(tg[staging], ln - 1)] | (tg[staging] >= regs)).all()
+            lv = load(x, seen)
             lv = np.where(fl == 1, xprev[np.where(fl == 1, off, 0)], lv)
             nv = xc[tg] - lv * xc[su]  # every read of the joint round, then its writes
             xc[tg] = nv
@@ -304,9 +313,21 @@ def _emulate_parts(pa, pos, div, slots, a, cap, G):
         m = d >= 0
         v[m] = v[m] / xc[d[m]]
         xc[:ln][m] = v[m]
-        mem[epos] = v
-        owner[epos] = p
-    return mem
+        first = w1p[epos] == big
+        w1p[epos[first]], w1v[epos[first]] = p, v[first]
+        w2p[epos[~first]], w2v[epos[~first]] = p, v[~first]
+    return np.where(w2p != big, w2v, w1v)
+
+
+def _first_staging_fill(pos, n_pos):
+    """Structural fill: the positions whose first staging (in part order) is as zero.  A later
+    staging flagged as zero takes a partner column's partial values from the previous part's
+    buffer instead of memory."""
+    p = pos & ~STAGE_ZERO
+    _, first = np.unique(p, return_index=True)
+    fill = np.zeros(n_pos, bool)
+    fill[p[first]] = (pos[first] & STAGE_ZERO) != 0
+    return fill
 
 
 @pytest.mark.parametrize("grid,tile", [("c1", 2), ("c1", 4), ("c1", 8), ("c1", 16), ("c2", 2), ("c2", 4), ("g", 8)])
@@ -316,13 +337,17 @@ def test_refactor_parts_equal_column_by_column(grid, tile):
     cp, ulen, row = topo.lu_pattern()
     pa, pos, div, slots, cap = _parts(topo, tile)
     n = ulen.size
-    # every entry of L+U is staged by exactly one part
-    assert np.array_equal(np.sort(pos & ~STAGE_ZERO), np.arange(cp[-1]))
+    # every entry of L+U is staged by one part, or by two: a partner column's partial visit, whose
+    # entries are written back undivided, then its own part
+    staged = np.bincount(pos & ~STAGE_ZERO, minlength=cp[-1])
+    assert staged.min() == 1 and staged.max() <= 2
+    twice = np.nonzero(staged == 2)[0]
+    p_first = np.array([np.nonzero((pos & ~STAGE_ZERO) == q)[0][0] for q in twice[:200]], np.int64)
+    assert (div[p_first] == -1).all()
     rng = np.random.default_rng(13)
     a = rng.uniform(-0.2, 0.2, cp[-1])
     a[cp[:-1] + ulen] = rng.uniform(3.0, 4.0, n) * rng.choice([-1.0, 1.0], n)
-    fill = np.zeros(cp[-1], bool)
-    fill[pos[(pos & STAGE_ZERO) != 0] & ~STAGE_ZERO] = True
+    fill = _first_staging_fill(pos, cp[-1])
     assert fill.any()  # the pattern has structural fill, staged as zero
     a[fill] = 0.0
     want = _sequential_refactor(cp, ulen, row, a)
