@@ -229,6 +229,7 @@ struct PartPlan {
 
 constexpr bool kPartnerVisits = true;  // split visits of partner columns (below)
 constexpr long kPartnerMinShared = 64;
+constexpr int kPartnerMax = 1;  // partner columns per whole column (2 and 3 measured the same)
 
 int build_parts(const gx::LuPattern& lu, const LuSchedule& s, int G, int pcap, const std::vector<int>* kind,
                 PartPlan& out) {
@@ -263,19 +264,23 @@ int build_parts(const gx::LuPattern& lu, const LuSchedule& s, int G, int pcap, c
   std::vector<char> split(n, 0);
   std::vector<int> pre_u(n, 0);  // U entries of a split column taken by its partial visit
   if (kPartnerVisits && G >= 32)
-    for (int c = 1; c < n; ++c) {
-      const int k = c - 1, ul = lu.ulen[c];
-      const int* cb = lu.row.data() + lu.cp[c];
-      if (split[k] || !std::binary_search(cb, cb + ul, k)) continue;
+    for (int k = 0; k < n; ++k) {
+      if (split[k]) continue;  // (a partner of the column before: k's part is that one's)
       const int* kb = lu.row.data() + lu.cp[k];
       const int* ke = kb + lu.ulen[k];
-      int m = 0;
-      long shared = 0;
-      for (; m < ul && lev[cb[m]] < lev[k]; ++m)
-        if (std::binary_search(kb, ke, cb[m])) shared += lu.cp[cb[m] + 1] - (lu.cp[cb[m]] + lu.ulen[cb[m]] + 1);
-      if (m > 0 && shared >= kPartnerMinShared) {
+      int size = lu.cp[k + 1] - lu.cp[k];
+      for (int c = k + 1; c < n && c <= k + kPartnerMax; ++c) {
+        const int ul = lu.ulen[c], clen = lu.cp[c + 1] - lu.cp[c];
+        const int* cb = lu.row.data() + lu.cp[c];
+        if (lev[c] <= lev[k] || size + clen > pcap) break;  // (c must be finished after k's part)
+        int m = 0;
+        long shared = 0;
+        for (; m < ul && lev[cb[m]] < lev[k]; ++m)
+          if (std::binary_search(kb, ke, cb[m])) shared += lu.cp[cb[m] + 1] - (lu.cp[cb[m]] + lu.ulen[cb[m]] + 1);
+        if (m == 0 || shared < kPartnerMinShared) break;
         split[c] = 1;
         pre_u[c] = m;
+        size += clen;
       }
     }
   struct Visit {
@@ -286,11 +291,12 @@ int build_parts(const gx::LuPattern& lu, const LuSchedule& s, int G, int pcap, c
   for (int l = 0; l < nlev; ++l) {
     int fill = pcap + 1;
     for (int k : bucket[l]) {
-      const int len = lu.cp[k + 1] - lu.cp[k], ul = lu.ulen[k], c = k + 1;
-      const int clen = c < n ? lu.cp[c + 1] - lu.cp[c] : 0;
-      if (c < n && split[c] && len + clen > pcap) split[c] = 0;  // the pair does not fit a part
-      const bool pair = c < n && split[c];
-      if (fill + len + (pair ? clen : 0) > pcap) {
+      const int len = lu.cp[k + 1] - lu.cp[k], ul = lu.ulen[k];
+      // k's partners: the split columns right after it (a whole column's; they fit one part)
+      int need = len, np = 0;
+      if (!split[k])
+        for (int c = k + 1; c < n && split[c]; ++c, ++np) need += lu.cp[c + 1] - lu.cp[c];
+      if (fill + need > pcap) {
         parts.emplace_back();
         fill = 0;
       }
@@ -299,11 +305,11 @@ int build_parts(const gx::LuPattern& lu, const LuSchedule& s, int G, int pcap, c
       part_of[k] = pi;
       base[k] = fill;
       fill += len;
-      if (pair) {
+      for (int c = k + 1; c <= k + np; ++c) {
         parts.back().push_back({c, fill, 0, pre_u[c], 1});
         pre_part[c] = pi;
         pre_base[c] = fill;
-        fill += clen;
+        fill += lu.cp[c + 1] - lu.cp[c];
       }
     }
   }
@@ -395,28 +401,36 @@ int build_parts(const gx::LuPattern& lu, const LuSchedule& s, int G, int pcap, c
     };
     for (size_t vi = 0; vi < vis.size(); ++vi) {
       const Visit& v = vis[vi];
-      if (v.mode == 1) continue;  // emitted with the visit before it, its partner
-      const Visit* pv = vi + 1 < vis.size() && vis[vi + 1].mode == 1 ? &vis[vi + 1] : nullptr;
+      if (v.mode == 1) continue;  // emitted with the whole visit before them, their partner
+      int np = 0;
+      while (vi + 1 + np < vis.size() && vis[vi + 1 + np].mode == 1) ++np;
+      if (np > kPartnerMax) return fail(GX_ERR_UNSUPPORTED, "too many partner visits in a part");
       const int c0 = lu.cp[v.col];
-      const int* prow = pv ? lu.row.data() + lu.cp[pv->col] : nullptr;
       auto lcount = [&](int j) { return lu.cp[j + 1] - (lu.cp[j] + lu.ulen[j] + 1); };
-      int pu = 0;
-      // the partner's updates from U entries in rows below `below`, in order
-      auto partner_to = [&](int below) {
-        for (; pv && pu < pv->u1 && prow[pu] < below; ++pu)
-          for (int e = 0, ll = lcount(prow[pu]); e < ll; ++e) emit(*pv, pu, e);
+      int pu[kPartnerMax] = {};
+      bool both[kPartnerMax] = {};
+      // partner i's updates from its U entries in rows below `below`, in order
+      auto partner_to = [&](int i, int below) {
+        const Visit& w = vis[vi + 1 + i];
+        const int* prow = lu.row.data() + lu.cp[w.col];
+        for (; pu[i] < w.u1 && prow[pu[i]] < below; ++pu[i])
+          for (int e = 0, ll = lcount(prow[pu[i]]); e < ll; ++e) emit(w, pu[i], e);
       };
       for (int u = v.u0; u < v.u1; ++u) {
         const int j = lu.row[c0 + u], ll = lcount(j);
-        partner_to(j);
-        const bool both = pv && pu < pv->u1 && prow[pu] == j;
+        for (int i = 0; i < np; ++i) {
+          partner_to(i, j);
+          const Visit& w = vis[vi + 1 + i];
+          both[i] = pu[i] < w.u1 && lu.row[lu.cp[w.col] + pu[i]] == j;
+        }
         for (int e = 0; e < ll; ++e) {
           emit(v, u, e);
-          if (both) emit(*pv, pu, e);
+          for (int i = 0; i < np; ++i)
+            if (both[i]) emit(vis[vi + 1 + i], pu[i], e);
         }
-        pu += both;
+        for (int i = 0; i < np; ++i) pu[i] += both[i];
       }
-      partner_to(n);
+      for (int i = 0; i < np; ++i) partner_to(i, n);
     }
     R *= gx::kJointRounds;  // rounds of G items: joint round r is rounds kJointRounds r + i
     const int nch = std::max(gx::kPartMinChunks, (R + NB - 1) / NB);
