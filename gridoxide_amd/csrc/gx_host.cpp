@@ -250,9 +250,10 @@ int build_parts(const gx::LuPattern& lu, const LuSchedule& s, int G, int pcap, c
   }
   std::vector<std::vector<int>> bucket(nlev);
   for (int k = 0; k < n; ++k) bucket[lev[k]].push_back(k);
-  // Partner visits: column c = k + 1, one of whose U entries is k, takes the updates from the
-  // leading U entries whose columns lie at levels below k's (final before k's part runs) in k's part
-  // (a partial visit), each next to k's update from the same L value where k has that U entry too,
+  // Partner visits: a column c = k + 1 (up to k + kPartnerMax) at a later level than k takes the
+  // updates from its leading U entries whose columns lie at levels below k's (final before k's part
+  // runs) in k's part (a partial visit), each next to k's update from the same L value where k has
+  // that U entry too,
   // so the second load finds the value in the cache; its entries are written back undivided.  Its
   // own part stages them again -- from the previous part's buffer when that is k's part (values
   // loaded ahead of the part would predate the write-back), else from memory -- and applies the
