@@ -402,7 +402,8 @@ def test_pair_tiles_equal_four_scenario_tiles(groups):
     runs 2-lane groups with four scenarios per lane on the schedule of 2-scenario tiles.  Every
     entry takes its updates in the same order, so the results equal the 4-scenario tiles' bit for
     bit, including a scenario whose FP32 factor is singular (its lane partners are unaffected) and
-    the padding of a last, partial tile."""
+    the padding of a last, partial tile.  The 32-group refactor also splits partner columns (partial
+    visits written back undivided, finished in their own parts), which the 16-group one does not."""
     _say(f"case: config-2 grid (2000 buses), 70 scenarios, mixed, tile 8 ({groups} refactor groups) vs tile 4")
     net = config_grid(2)
     p, q = scenario_specs(net, 70, seed=31)
