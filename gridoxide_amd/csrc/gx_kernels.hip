@@ -432,21 +432,38 @@ __global__ __launch_bounds__(256) void k_mismatch(DevTopo t, DevWork w) {
     double gd = 0.0, bd = 0.0;
     int pd = -1;
     const int p2 = t.y_rowptr[i + 1];
-    for (int p = t.y_rowptr[i]; p < p2; ++p) {
-      const int k = t.y_col[p];
-      const size_t oj = bb + (size_t)k * TW;
-      double g, b;
-      y_at(t, og, p, g, b);
-      const double vr = w.vr[oj], vi = w.vi[oj];
-      ir += g * vr - b * vi;
-      ii += g * vi + b * vr;
-      if constexpr (kFill) {
-        if (k == i) {
-          gd = g;
-          bd = b;
-          pd = p;
-        } else {
-          jac_store(lu, V, t.y_lupos[p], vri, vii, vr, vi, w.vm[oj], g, b, false, 0.0, 0.0);
+    // the row's entries kMisBatch at a time: their columns, then their voltages and Y values, then
+    // the sums in entry order (the loads of a batch in flight together)
+    constexpr int kMisBatch = 4;
+    for (int p0 = t.y_rowptr[i]; p0 < p2; p0 += kMisBatch) {
+      int kk[kMisBatch];
+#pragma unroll
+      for (int u = 0; u < kMisBatch; ++u) kk[u] = t.y_col[p0 + u < p2 ? p0 + u : p0];
+      double vrk[kMisBatch], vik[kMisBatch], gk[kMisBatch], bk[kMisBatch], vmk[kMisBatch];
+#pragma unroll
+      for (int u = 0; u < kMisBatch; ++u) {
+        const int p = p0 + u < p2 ? p0 + u : p0;
+        const size_t oj = bb + (size_t)kk[u] * TW;
+        y_at(t, og, p, gk[u], bk[u]);
+        vrk[u] = w.vr[oj];
+        vik[u] = w.vi[oj];
+        if constexpr (kFill) vmk[u] = w.vm[oj];
+      }
+#pragma unroll
+      for (int u = 0; u < kMisBatch; ++u) {
+        if (p0 + u >= p2) break;
+        const int p = p0 + u, k = kk[u];
+        const double g = gk[u], b = bk[u], vr = vrk[u], vi = vik[u];
+        ir += g * vr - b * vi;
+        ii += g * vi + b * vr;
+        if constexpr (kFill) {
+          if (k == i) {
+            gd = g;
+            bd = b;
+            pd = p;
+          } else {
+            jac_store(lu, V, t.y_lupos[p], vri, vii, vr, vi, vmk[u], g, b, false, 0.0, 0.0);
+          }
         }
       }
     }
