@@ -468,7 +468,7 @@ def run_serving(args, n_req=32, budget_gb=96.0, loads=(0.5, 0.8)):
     topos = [Topology.from_network(g) for g in pool]
     for topo, g in zip(topos, pool):  # resident symbolic + first factorization
         topo.factor_at(g.vm, g.va, g.p_spec, g.q_spec)
-    # 8-scenario pair tiles (quad-tile FP32 refactors): capacity 18.8 against 16.2 requests/s at tile 4
+    # 8-scenario pair tiles (quad-tile FP32 refactors): capacity 19.0 against 16.2 requests/s at tile 4
     eng = Engine(tile=args.tile or 8, budget_bytes=int(budget_gb * 2**30), graphs=True)
     # warm-up: one small request per grid builds each topology's device schedule
     for topo, g in zip(topos, pool):
